@@ -1,0 +1,874 @@
+// sphdem_capi.hip -- C ABI of the gfx950 SPH-DEM loop (include/sphdem_c.h):
+// device context, cell build orchestration (CdataLL::reset), the three
+// half-steps of CsphIncompress and the parity entry points.  All work is
+// ordered on the context's stream; the only host synchronisation inside a step
+// is the read-back of hmax in reset(), which sizes the cell grid.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "sphdem_kernels.hip"
+
+using namespace sphdem;
+
+struct sphdem_ctx {
+  sphdem_params P;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int64_t cap = 0;
+  int n = 0, ns0 = 0, nd0 = 0, ns = 0, nd = 0;
+  Store cur{}, alt{};
+  void* storeMem[2] = {nullptr, nullptr};
+  int *key = nullptr, *slot = nullptr, *tmp = nullptr, *perm = nullptr;
+  int *counts = nullptr, *start = nullptr, *tiles = nullptr, *tileOff = nullptr;
+  int Kcap = 0, tileCap = 0;
+  double* hsnap = nullptr;
+  double* partial = nullptr;
+  int partialCap = 0;
+  double* fieldbuf = nullptr;
+  int* errflag = nullptr;
+  DevGlobals* dg = nullptr;
+  long long sphStep = 0;
+  double ssr = 0.0;
+  Grid gs{}, gc{};
+  Box box{};
+  CouplingRule rule{};
+  int K = 0, reachShep = 1, reachFull = 1;
+  bool loaded = false;
+  char err[512] = {0};
+};
+
+namespace {
+
+int fail(sphdem_ctx* c, int code, const char* fmt, ...) {
+  if (c) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(c->err, sizeof(c->err), fmt, ap);
+    va_end(ap);
+  }
+  return code;
+}
+
+#define HIPC(expr)                                                                     \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(ctx, SPHDEM_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+#define LAUNCHED()                                                                       \
+  do {                                                                                   \
+    hipError_t e_ = hipGetLastError();                                                   \
+    if (e_ != hipSuccess)                                                                \
+      return fail(ctx, SPHDEM_E_HIP, "kernel launch failed: %s", hipGetErrorString(e_)); \
+  } while (0)
+#define CHECK(expr)     \
+  do {                  \
+    int r_ = (expr);    \
+    if (r_) return r_;  \
+  } while (0)
+
+constexpr int TPB = 256;
+inline int blocks(int n) { return (n + TPB - 1) / TPB; }
+
+int allocStore(sphdem_ctx* ctx, Store& s, void*& mem, int64_t cap) {
+  const size_t vec = sizeof(double4) * (size_t)cap;
+  const size_t total = vec * kNumVec + 2 * sizeof(int) * (size_t)cap;
+  HIPC(hipMalloc(&mem, total));
+  char* p = (char*)mem;
+  double4** f[kNumVec] = {&s.posh, &s.vhm, &s.dpo, &s.vel, &s.v0p, &s.vh0d,
+                          &s.frc,  &s.fp,  &s.fv,  &s.fb,  &s.fdrag, &s.fvel};
+  for (int k = 0; k < kNumVec; k++) {
+    *f[k] = (double4*)p;
+    p += vec;
+  }
+  s.iam = (int*)p;
+  p += sizeof(int) * (size_t)cap;
+  s.orig = (int*)p;
+  return 0;
+}
+
+// where each field lives: array index in the Store, component offset, width
+struct FieldLoc {
+  int arr, off, width;
+};
+bool fieldLoc(int field, FieldLoc* L) {
+  switch (field) {
+    case SPHDEM_FIELD_R: *L = {0, 0, 3}; return true;
+    case SPHDEM_FIELD_H: *L = {0, 3, 1}; return true;
+    case SPHDEM_FIELD_VHAT: *L = {1, 0, 3}; return true;
+    case SPHDEM_FIELD_MASS: *L = {1, 3, 1}; return true;
+    case SPHDEM_FIELD_DENS: *L = {2, 0, 1}; return true;
+    case SPHDEM_FIELD_PDR2: *L = {2, 1, 1}; return true;
+    case SPHDEM_FIELD_OMEGA: *L = {2, 2, 1}; return true;
+    case SPHDEM_FIELD_POROSITY: *L = {2, 3, 1}; return true;
+    case SPHDEM_FIELD_V: *L = {3, 0, 3}; return true;
+    case SPHDEM_FIELD_U: *L = {3, 3, 1}; return true;
+    case SPHDEM_FIELD_V0: *L = {4, 0, 3}; return true;
+    case SPHDEM_FIELD_PRESS: *L = {4, 3, 1}; return true;
+    case SPHDEM_FIELD_VHAT0: *L = {5, 0, 3}; return true;
+    case SPHDEM_FIELD_DDDT: *L = {5, 3, 1}; return true;
+    case SPHDEM_FIELD_F: *L = {6, 0, 3}; return true;
+    case SPHDEM_FIELD_DUDT: *L = {6, 3, 1}; return true;
+    case SPHDEM_FIELD_FP: *L = {7, 0, 3}; return true;
+    case SPHDEM_FIELD_SHEPSUM: *L = {7, 3, 1}; return true;
+    case SPHDEM_FIELD_FV: *L = {8, 0, 3}; return true;
+    case SPHDEM_FIELD_FB: *L = {9, 0, 3}; return true;
+    case SPHDEM_FIELD_FDRAG: *L = {10, 0, 3}; return true;
+    case SPHDEM_FIELD_FVEL: *L = {11, 0, 3}; return true;
+    default: return false;
+  }
+}
+double4* storeArr(Store& s, int k) {
+  double4* a[kNumVec] = {s.posh, s.vhm, s.dpo, s.vel, s.v0p, s.vh0d,
+                         s.frc,  s.fp,  s.fv,  s.fb,  s.fdrag, s.fvel};
+  return a[k];
+}
+
+bool liqDem(const sphdem_ctx* c) { return (c->P.flags & SPHDEM_F_LIQ_DEM) != 0; }
+bool varH(const sphdem_ctx* c) { return (c->P.flags & SPHDEM_F_VAR_H_CORRECTION) != 0; }
+
+int ensureKeys(sphdem_ctx* ctx, int K) {
+  if (K + 1 > ctx->Kcap) {
+    if (ctx->counts) HIPC(hipFree(ctx->counts));
+    if (ctx->start) HIPC(hipFree(ctx->start));
+    ctx->Kcap = (K + 1) + (K + 1) / 4;
+    HIPC(hipMalloc(&ctx->counts, sizeof(int) * ctx->Kcap));
+    HIPC(hipMalloc(&ctx->start, sizeof(int) * ctx->Kcap));
+  }
+  const int tiles = (K + 1 + SCAN_TILE - 1) / SCAN_TILE;
+  if (tiles > SCAN_TILE)
+    return fail(ctx, SPHDEM_E_CAPACITY, "cell grid of %d keys exceeds the scan capacity", K);
+  if (tiles + 1 > ctx->tileCap) {
+    if (ctx->tiles) HIPC(hipFree(ctx->tiles));
+    if (ctx->tileOff) HIPC(hipFree(ctx->tileOff));
+    ctx->tileCap = tiles + 1;
+    HIPC(hipMalloc(&ctx->tiles, sizeof(int) * ctx->tileCap));
+    HIPC(hipMalloc(&ctx->tileOff, sizeof(int) * ctx->tileCap));
+  }
+  return 0;
+}
+
+// exclusive scan of ctx->counts[0..m) into ctx->start
+int scanCounts(sphdem_ctx* ctx, int m) {
+  const int tiles = (m + SCAN_TILE - 1) / SCAN_TILE;
+  hipLaunchKernelGGL(k_scan_tile, dim3(tiles), dim3(SCAN_THREADS), 0, ctx->stream, ctx->counts,
+                     ctx->start, ctx->tiles, m);
+  LAUNCHED();
+  if (tiles > 1) {
+    hipLaunchKernelGGL(k_scan_tile, dim3(1), dim3(SCAN_THREADS), 0, ctx->stream, ctx->tiles,
+                       ctx->tileOff, (int*)nullptr, tiles);
+    LAUNCHED();
+    hipLaunchKernelGGL(k_scan_add, dim3(blocks(m)), dim3(TPB), 0, ctx->stream, ctx->start,
+                       ctx->tileOff, m);
+    LAUNCHED();
+  }
+  return 0;
+}
+
+void setupGrids(sphdem_ctx* ctx) {
+  const sphdem_params& P = ctx->P;
+  Grid& gs = ctx->gs;
+  for (int d = 0; d < 3; d++) gs.cmin[d] = P.proc_domain[2 * d];
+  // getCellI (dataLL.h:303-305) and the cell array size (dataLL.cpp:16, 222)
+  gs.edge = 2.0 * ctx->ssr;
+  gs.off = 3.0;
+  for (int d = 0; d < 3; d++)
+    gs.dim[d] = (int)((P.proc_domain[2 * d + 1] - P.proc_domain[2 * d]) / gs.edge) + 8;
+  gs.base = 0;
+  const int Cs = gs.dim[0] * gs.dim[1] * gs.dim[2];
+  Grid& gc = ctx->gc;
+  for (int d = 0; d < 3; d++) gc.cmin[d] = P.proc_domain[2 * d];
+  if (liqDem(ctx)) {
+    gc.edge = std::max(P.coupling_radius, 2.0 * P.dem_search_radius) * (1.0 + 1e-9);
+    gc.off = 1.0;
+    for (int d = 0; d < 3; d++)
+      gc.dim[d] = (int)((P.proc_domain[2 * d + 1] - P.proc_domain[2 * d]) / gc.edge) + 3;
+  } else {
+    gc.edge = 1.0;
+    gc.off = 0.0;
+    gc.dim[0] = gc.dim[1] = gc.dim[2] = 1;
+  }
+  gc.base = Cs;
+  const int Cc = gc.dim[0] * gc.dim[1] * gc.dim[2];
+  ctx->K = Cs + Cc + 1;  // last key: deleted particles
+  if (liqDem(ctx)) {
+    const double Rc = P.coupling_radius;
+    ctx->reachShep = std::max(1, (int)ceil(Rc * (1.0 + 1e-9) / gs.edge));
+    ctx->reachFull =
+        (int)ceil((ctx->rule.nc + 1) * ctx->rule.e / gc.edge) + 1;
+  }
+}
+
+void setupStatic(sphdem_ctx* ctx) {
+  const sphdem_params& P = ctx->P;
+  for (int d = 0; d < 3; d++) {
+    ctx->box.periodic[d] = P.periodic[d] ? 1 : 0;
+    ctx->box.Lp[d] = P.rmax[d] - P.rmin[d];
+    ctx->box.Lm[d] = P.rmin[d] - P.rmax[d];
+  }
+  CouplingRule& R = ctx->rule;
+  if (P.dem_search_radius > 0 && P.coupling_radius > 0) {
+    const double Rc = P.coupling_radius, DSR = P.dem_search_radius;
+    for (int d = 0; d < 3; d++) R.cmin[d] = P.proc_domain[2 * d];
+    R.e = 2 * DSR;
+    R.s = 2 * Rc / DSR;
+    const double t = 0.5 * Rc / DSR;
+    R.thr = t * t;
+    R.rc2 = Rc * Rc;
+    R.skip2 = R.rc2 * (1.0 + 1e-12);
+    R.nc = (int)ceil(0.5 * Rc / DSR);
+    for (int d = 0; d < 3; d++) {
+      const double dmin = P.proc_domain[2 * d];
+      const double dspace = P.proc_domain[2 * d + 1] - dmin;
+      R.gmin[d] = dmin + Rc;
+      R.gspace[d] = (dspace - Rc) - Rc;
+    }
+  }
+}
+
+// CdataLL::reset (dataLL.cpp:140-236)
+int doReset(sphdem_ctx* ctx) {
+  const sphdem_params& P = ctx->P;
+  const int n = ctx->n;
+  HIPC(hipMemsetAsync(&ctx->dg->hmax_bits, 0, sizeof(unsigned long long), ctx->stream));
+  double dmin[3], dsp[3];
+  for (int d = 0; d < 3; d++) {
+    dmin[d] = P.proc_domain[2 * d];
+    dsp[d] = P.proc_domain[2 * d + 1] - dmin[d];
+  }
+  hipLaunchKernelGGL(k_wrap, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, ctx->cur, n, dmin[0],
+                     dmin[1], dmin[2], dsp[0], dsp[1], dsp[2], ctx->box, ctx->dg);
+  LAUNCHED();
+  DevGlobals hg;
+  HIPC(hipMemcpyAsync(&hg, ctx->dg, sizeof(DevGlobals), hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  if (hg.errflags & 1u)
+    return fail(ctx, SPHDEM_E_CAPACITY, "a particle fell outside the cell grid at the last reset");
+  ctx->ns = ctx->ns0 - (int)hg.ndel_sph;
+  ctx->nd = ctx->nd0 - (int)hg.ndel_dem;
+  double hmax;
+  unsigned long long hb = hg.hmax_bits;
+  memcpy(&hmax, &hb, sizeof(hmax));
+  if (ctx->ns + ctx->nd == 0) hmax = -100000;  // calcDomainLimits on an empty container
+  if ((hmax > ctx->ssr) || (hmax < 0.9 * ctx->ssr)) ctx->ssr = hmax;
+  setupGrids(ctx);
+  CHECK(ensureKeys(ctx, ctx->K));
+  HIPC(hipMemsetAsync(ctx->counts, 0, sizeof(int) * (ctx->K + 1), ctx->stream));
+  hipLaunchKernelGGL(k_bin, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, ctx->cur, n, ctx->gs,
+                     ctx->gc, ctx->K - 1, ctx->key, ctx->slot, ctx->counts, ctx->dg);
+  LAUNCHED();
+  CHECK(scanCounts(ctx, ctx->K + 1));
+  hipLaunchKernelGGL(k_scatter, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, n, ctx->key,
+                     ctx->slot, ctx->start, ctx->tmp);
+  LAUNCHED();
+  hipLaunchKernelGGL(k_rank, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, n, ctx->key, ctx->start,
+                     ctx->tmp, ctx->cur.orig, ctx->perm);
+  LAUNCHED();
+  hipLaunchKernelGGL(k_permute, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, n, ctx->perm,
+                     ctx->cur, ctx->alt);
+  LAUNCHED();
+  std::swap(ctx->cur, ctx->alt);
+  std::swap(ctx->storeMem[0], ctx->storeMem[1]);
+  return 0;
+}
+
+int checkLoaded(sphdem_ctx* ctx) {
+  if (!ctx) return SPHDEM_E_ARG;
+  if (!ctx->loaded) return fail(ctx, SPHDEM_E_STATE, "no particles uploaded");
+  return 0;
+}
+
+// ---- passes
+int passOmega(sphdem_ctx* ctx, const double* hq) {
+  hipLaunchKernelGGL(k_omega, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
+                     ctx->gs, ctx->start, ctx->box, hq, ctx->P.wcon);
+  LAUNCHED();
+  return 0;
+}
+int passEos(sphdem_ctx* ctx) {
+  hipLaunchKernelGGL(k_eos, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
+                     ctx->P);
+  LAUNCHED();
+  return 0;
+}
+int passForce(sphdem_ctx* ctx) {
+  hipLaunchKernelGGL(k_force, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
+                     ctx->gs, ctx->start, ctx->box, ctx->P, ctx->dg);
+  LAUNCHED();
+  return 0;
+}
+int passDddt(sphdem_ctx* ctx) {
+  hipLaunchKernelGGL(k_dddt, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
+                     ctx->gs, ctx->start, ctx->box, ctx->P);
+  LAUNCHED();
+  return 0;
+}
+int passCoupling(sphdem_ctx* ctx) {
+  if (ctx->nd > 0) {
+    hipLaunchKernelGGL(k_shepsum, dim3(blocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                       ctx->ns, ctx->nd, ctx->gs, ctx->start, ctx->box, ctx->rule, ctx->reachShep,
+                       ctx->P.wcon, ctx->P.coupling_radius);
+    LAUNCHED();
+  }
+  hipLaunchKernelGGL(k_coupling_sph, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                     ctx->ns, ctx->nd, ctx->gc, ctx->start, ctx->box, ctx->rule, ctx->P);
+  LAUNCHED();
+  return 0;
+}
+int passFpi(sphdem_ctx* ctx) {
+  if (ctx->nd == 0) return 0;
+  hipLaunchKernelGGL(k_fpi, dim3(blocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
+                     ctx->nd, ctx->gs, ctx->start, ctx->box, ctx->P);
+  LAUNCHED();
+  return 0;
+}
+int passContacts(sphdem_ctx* ctx) {
+  if (ctx->nd == 0) return 0;
+  hipLaunchKernelGGL(k_contacts, dim3(blocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                     ctx->ns, ctx->nd, ctx->gc, ctx->start, ctx->box, ctx->P);
+  LAUNCHED();
+  return 0;
+}
+// one iteration of the density re-initialisation (sphIncompress.cpp:29-34);
+// `first` snapshots h as the cached neighbour lists are built then
+int passDensity(sphdem_ctx* ctx, bool first) {
+  hipLaunchKernelGGL(k_hfromdens, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                     ctx->ns, ctx->P.hfac, first ? ctx->hsnap : (double*)nullptr);
+  LAUNCHED();
+  hipLaunchKernelGGL(k_density, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                     ctx->ns, ctx->gs, ctx->start, ctx->box, (const double*)ctx->hsnap,
+                     ctx->P.wcon);
+  LAUNCHED();
+  return 0;
+}
+
+// ---- half-steps
+int stepStart(sphdem_ctx* ctx) {
+  const int na = ctx->ns + ctx->nd;
+  hipLaunchKernelGGL(k_start, dim3(blocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream, ctx->cur,
+                     na, ctx->P, ctx->dg);
+  LAUNCHED();
+  if ((ctx->sphStep % ctx->P.reinit_every == 0) || (ctx->sphStep == ctx->P.reinit_at)) {
+    CHECK(doReset(ctx));
+    for (int i = 0; i < 10; i++) CHECK(passDensity(ctx, i == 0));
+  }
+  return 0;
+}
+int stepMiddle(sphdem_ctx* ctx) {
+  hipLaunchKernelGGL(k_mid_init, dim3(1), dim3(1), 0, ctx->stream, ctx->dg);
+  LAUNCHED();
+  CHECK(doReset(ctx));
+  if (liqDem(ctx)) CHECK(passCoupling(ctx));
+  if (varH(ctx)) CHECK(passOmega(ctx, nullptr));
+  CHECK(passEos(ctx));
+  CHECK(passForce(ctx));
+  const int na = ctx->ns + ctx->nd;
+  hipLaunchKernelGGL(k_accel, dim3(blocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream, ctx->cur,
+                     na, ctx->dg);
+  LAUNCHED();
+  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1), 0, ctx->stream, ctx->P, ctx->dg);
+  LAUNCHED();
+  hipLaunchKernelGGL(k_mid_kick, dim3(blocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream,
+                     ctx->cur, na, ctx->P, ctx->dg);
+  LAUNCHED();
+  return 0;
+}
+int stepEnd(sphdem_ctx* ctx) {
+  hipLaunchKernelGGL(k_end_drift, dim3(blocks(std::max(ctx->ns, 1))), dim3(TPB), 0, ctx->stream,
+                     ctx->cur, ctx->ns, ctx->dg);
+  LAUNCHED();
+  CHECK(doReset(ctx));
+  if (varH(ctx)) CHECK(passOmega(ctx, nullptr));
+  CHECK(passDddt(ctx));
+  if (liqDem(ctx)) {
+    CHECK(passFpi(ctx));
+    if (!(ctx->P.flags & (SPHDEM_F_FIXED_DEM | SPHDEM_F_NO_DEM_CONTACTS))) CHECK(passContacts(ctx));
+    if (ctx->nd > 0) {
+      hipLaunchKernelGGL(k_dem_kick, dim3(blocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                         ctx->ns, ctx->nd, ctx->P, ctx->dg);
+      LAUNCHED();
+    }
+  }
+  hipLaunchKernelGGL(k_end_rest, dim3(blocks(std::max(ctx->ns, 1))), dim3(TPB), 0, ctx->stream,
+                     ctx->cur, ctx->ns, ctx->P, ctx->dg);
+  LAUNCHED();
+  return 0;
+}
+
+int hookBody(sphdem_ctx* ctx, uint32_t mask, const double* a) {
+  const int na = ctx->ns + ctx->nd;
+  hipLaunchKernelGGL(k_body_force, dim3(blocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream,
+                     ctx->cur, na, mask, a[0], a[1], a[2]);
+  LAUNCHED();
+  return 0;
+}
+int hookFloor(sphdem_ctx* ctx) {
+  if (ctx->nd == 0) return 0;
+  hipLaunchKernelGGL(k_dem_floor, dim3(blocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                     ctx->ns, ctx->nd, ctx->P);
+  LAUNCHED();
+  return 0;
+}
+int hookFreeze(sphdem_ctx* ctx, uint32_t mask, double t) {
+  const int na = ctx->ns + ctx->nd;
+  hipLaunchKernelGGL(k_freeze, dim3(blocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream, ctx->cur,
+                     na, mask, t, ctx->dg);
+  LAUNCHED();
+  return 0;
+}
+int hookSumDrag(sphdem_ctx* ctx) {
+  const int na = std::max(ctx->ns + ctx->nd, 1);
+  const int nb = blocks(na);
+  if (nb * 6 > ctx->partialCap) {
+    if (ctx->partial) HIPC(hipFree(ctx->partial));
+    ctx->partialCap = nb * 6;
+    HIPC(hipMalloc(&ctx->partial, sizeof(double) * ctx->partialCap));
+  }
+  hipLaunchKernelGGL(k_sum_drag, dim3(nb), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns + ctx->nd,
+                     ctx->partial);
+  LAUNCHED();
+  hipLaunchKernelGGL(k_sum_drag_final, dim3(1), dim3(64), 0, ctx->stream, ctx->partial, nb,
+                     ctx->dg);
+  LAUNCHED();
+  return 0;
+}
+
+void freeAll(sphdem_ctx* ctx) {
+  void* ptrs[] = {ctx->storeMem[0], ctx->storeMem[1], ctx->key,     ctx->slot,   ctx->tmp,
+                  ctx->perm,        ctx->counts,      ctx->start,   ctx->tiles,  ctx->tileOff,
+                  ctx->hsnap,       ctx->partial,     ctx->fieldbuf, ctx->errflag, ctx->dg};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+}
+
+}  // namespace
+
+// ============================================================ C ABI
+extern "C" {
+
+int sphdem_create(const sphdem_params* params, int device, int64_t capacity, sphdem_ctx** out) {
+  if (!params || !out || capacity <= 0 || capacity > (int64_t)0x3fffffff) return SPHDEM_E_ARG;
+  *out = nullptr;
+  sphdem_ctx* ctx = new sphdem_ctx();
+  ctx->P = *params;
+  ctx->device = device;
+  ctx->cap = capacity;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+    delete ctx;
+    return SPHDEM_E_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess ||
+      strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    delete ctx;
+    return SPHDEM_E_DEVICE;
+  }
+  int rc = 0;
+  auto setup = [&]() -> int {
+    HIPC(hipSetDevice(device));
+    HIPC(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    CHECK(allocStore(ctx, ctx->cur, ctx->storeMem[0], capacity));
+    CHECK(allocStore(ctx, ctx->alt, ctx->storeMem[1], capacity));
+    HIPC(hipMalloc(&ctx->key, sizeof(int) * capacity));
+    HIPC(hipMalloc(&ctx->slot, sizeof(int) * capacity));
+    HIPC(hipMalloc(&ctx->tmp, sizeof(int) * capacity));
+    HIPC(hipMalloc(&ctx->perm, sizeof(int) * capacity));
+    HIPC(hipMalloc(&ctx->hsnap, sizeof(double) * capacity));
+    HIPC(hipMalloc(&ctx->fieldbuf, sizeof(double) * 3 * capacity));
+    HIPC(hipMalloc(&ctx->errflag, sizeof(int)));
+    HIPC(hipMalloc(&ctx->dg, sizeof(DevGlobals)));
+    HIPC(hipMemset(ctx->dg, 0, sizeof(DevGlobals)));
+    return 0;
+  };
+  rc = setup();
+  if (rc) {
+    freeAll(ctx);
+    delete ctx;
+    return rc;
+  }
+  setupStatic(ctx);
+  *out = ctx;
+  return 0;
+}
+
+int sphdem_destroy(sphdem_ctx* ctx) {
+  if (!ctx) return SPHDEM_E_ARG;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  freeAll(ctx);
+  delete ctx;
+  return 0;
+}
+
+const char* sphdem_last_error(const sphdem_ctx* ctx) { return ctx ? ctx->err : "null context"; }
+
+int sphdem_set_params(sphdem_ctx* ctx, const sphdem_params* params) {
+  if (!ctx || !params) return SPHDEM_E_ARG;
+  for (int d = 0; d < 3; d++)
+    if (params->periodic[d] != ctx->P.periodic[d] || params->proc_domain[2 * d] != ctx->P.proc_domain[2 * d] ||
+        params->proc_domain[2 * d + 1] != ctx->P.proc_domain[2 * d + 1])
+      return fail(ctx, SPHDEM_E_ARG, "the domain cannot change on a live context");
+  ctx->P = *params;
+  setupStatic(ctx);
+  return 0;
+}
+
+int sphdem_upload(sphdem_ctx* ctx, int64_t n, const double* r, const double* v, const double* vhat,
+                  const double* mass, const double* h, const double* dens, const int32_t* iam) {
+  if (!ctx || n < 0 || !r || !mass || !h || !dens || !iam) return SPHDEM_E_ARG;
+  if (n > ctx->cap) return fail(ctx, SPHDEM_E_CAPACITY, "%lld particles exceed capacity %lld",
+                                (long long)n, (long long)ctx->cap);
+  HIPC(hipSetDevice(ctx->device));
+  std::vector<double4> a[kNumVec];
+  for (auto& x : a) x.assign((size_t)n, make_double4(0, 0, 0, 0));
+  std::vector<int> t((size_t)n), o((size_t)n);
+  int ns = 0, nd = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const int ty = iam[i];
+    if (ty == SPHDEM_SPH || ty == SPHDEM_SPH_BOUNDARY)
+      ns++;
+    else if (ty == SPHDEM_DEM)
+      nd++;
+    else
+      return fail(ctx, SPHDEM_E_ARG,
+                  "particle %lld has type %d; supported types are sph, sphBoundary and dem",
+                  (long long)i, ty);
+    a[0][i] = make_double4(r[3 * i], r[3 * i + 1], r[3 * i + 2], h[i]);
+    a[1][i] = vhat ? make_double4(vhat[3 * i], vhat[3 * i + 1], vhat[3 * i + 2], mass[i])
+                   : make_double4(0, 0, 0, mass[i]);
+    a[2][i] = make_double4(dens[i], 0.0, 0.0, 1.0);  // omega 0, porosity 1 (particle.h:77,80)
+    if (v) a[3][i] = make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], 0.0);
+    t[i] = ty;
+    o[i] = (int)i;
+  }
+  for (int k = 0; k < kNumVec; k++)
+    HIPC(hipMemcpy(storeArr(ctx->cur, k), a[k].data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(ctx->cur.iam, t.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(ctx->cur.orig, o.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  ctx->n = (int)n;
+  ctx->ns0 = ns;
+  ctx->nd0 = nd;
+  ctx->ns = ns;
+  ctx->nd = nd;
+  DevGlobals g;
+  memset(&g, 0, sizeof(g));
+  g.maxdt = ctx->P.maxdt;
+  HIPC(hipMemcpy(ctx->dg, &g, sizeof(g), hipMemcpyHostToDevice));
+  ctx->sphStep = 0;
+  // CdataLL::CdataLL (dataLL.cpp:8-40): hmax = sph_search_radius = H, then reset()
+  ctx->ssr = ctx->P.h;
+  ctx->loaded = true;
+  CHECK(doReset(ctx));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int sphdem_num_particles(const sphdem_ctx* ctx, int64_t* n) {
+  if (!ctx || !n) return SPHDEM_E_ARG;
+  *n = ctx->n;
+  return 0;
+}
+
+int sphdem_num_deleted(const sphdem_ctx* ctx, int64_t* n) {
+  if (!ctx || !n) return SPHDEM_E_ARG;
+  *n = (int64_t)(ctx->ns0 - ctx->ns) + (ctx->nd0 - ctx->nd);
+  return 0;
+}
+
+int sphdem_get_field(sphdem_ctx* ctx, int field, void* host) {
+  CHECK(checkLoaded(ctx));
+  if (!host) return SPHDEM_E_ARG;
+  HIPC(hipSetDevice(ctx->device));
+  const int n = ctx->n;
+  if (field == SPHDEM_FIELD_IAM || field == SPHDEM_FIELD_TAG) {
+    int* buf = (int*)ctx->fieldbuf;
+    hipLaunchKernelGGL(k_get_int, dim3(blocks(n)), dim3(TPB), 0, ctx->stream,
+                       field == SPHDEM_FIELD_IAM ? ctx->cur.iam : ctx->cur.orig,
+                       field == SPHDEM_FIELD_IAM ? 0 : 1, ctx->cur.orig, n, buf);
+    LAUNCHED();
+    HIPC(hipMemcpyAsync(host, buf, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    return 0;
+  }
+  FieldLoc L;
+  if (!fieldLoc(field, &L)) return fail(ctx, SPHDEM_E_ARG, "unknown field %d", field);
+  hipLaunchKernelGGL(k_get_field, dim3(blocks(n)), dim3(TPB), 0, ctx->stream,
+                     (const double*)storeArr(ctx->cur, L.arr), L.off, L.width, ctx->cur.orig, n,
+                     ctx->fieldbuf);
+  LAUNCHED();
+  HIPC(hipMemcpyAsync(host, ctx->fieldbuf, sizeof(double) * L.width * n, hipMemcpyDeviceToHost,
+                      ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int sphdem_set_field(sphdem_ctx* ctx, int field, const void* host) {
+  CHECK(checkLoaded(ctx));
+  if (!host) return SPHDEM_E_ARG;
+  HIPC(hipSetDevice(ctx->device));
+  const int n = ctx->n;
+  if (field == SPHDEM_FIELD_IAM) {
+    const int32_t* t = (const int32_t*)host;
+    for (int i = 0; i < n; i++)
+      if (t[i] != SPHDEM_SPH && t[i] != SPHDEM_SPH_BOUNDARY && t[i] != SPHDEM_DEM &&
+          t[i] != SPHDEM_DELETED)
+        return fail(ctx, SPHDEM_E_ARG, "unsupported particle type %d", t[i]);
+    return fail(ctx, SPHDEM_E_ARG, "particle types are fixed at upload");
+  }
+  FieldLoc L;
+  if (!fieldLoc(field, &L)) return fail(ctx, SPHDEM_E_ARG, "unknown or read-only field %d", field);
+  HIPC(hipMemcpyAsync(ctx->fieldbuf, host, sizeof(double) * L.width * n, hipMemcpyHostToDevice,
+                      ctx->stream));
+  hipLaunchKernelGGL(k_set_field, dim3(blocks(n)), dim3(TPB), 0, ctx->stream,
+                     (double*)storeArr(ctx->cur, L.arr), L.off, L.width, ctx->cur.orig, n,
+                     (const double*)ctx->fieldbuf);
+  LAUNCHED();
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int sphdem_set_globals(sphdem_ctx* ctx, const sphdem_globals* g) {
+  if (!ctx || !g) return SPHDEM_E_ARG;
+  HIPC(hipSetDevice(ctx->device));
+  DevGlobals d;
+  HIPC(hipMemcpyAsync(&d, ctx->dg, sizeof(d), hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  d.time = g->time;
+  d.dt = g->dt;
+  d.newDt = g->newDt;
+  d.maxdt = g->maxdt;
+  d.sphStep = g->sphStep;
+  memcpy(&d.newDt_bits, &g->newDt, sizeof(double));
+  for (int i = 0; i < 9; i++) d.custom[i] = g->custom[i];
+  HIPC(hipMemcpyAsync(ctx->dg, &d, sizeof(d), hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  ctx->sphStep = g->sphStep;
+  return 0;
+}
+
+int sphdem_get_globals(sphdem_ctx* ctx, sphdem_globals* g) {
+  if (!ctx || !g) return SPHDEM_E_ARG;
+  HIPC(hipSetDevice(ctx->device));
+  DevGlobals d;
+  HIPC(hipMemcpyAsync(&d, ctx->dg, sizeof(d), hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  g->time = d.time;
+  g->dt = d.dt;
+  g->newDt = d.newDt;
+  g->maxdt = d.maxdt;
+  g->sphStep = ctx->sphStep;
+  for (int i = 0; i < 9; i++) g->custom[i] = d.custom[i];
+  return 0;
+}
+
+int sphdem_sync(sphdem_ctx* ctx) {
+  if (!ctx) return SPHDEM_E_ARG;
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int sphdem_step_start(sphdem_ctx* ctx) {
+  CHECK(checkLoaded(ctx));
+  HIPC(hipSetDevice(ctx->device));
+  return stepStart(ctx);
+}
+int sphdem_step_middle(sphdem_ctx* ctx) {
+  CHECK(checkLoaded(ctx));
+  HIPC(hipSetDevice(ctx->device));
+  return stepMiddle(ctx);
+}
+int sphdem_step_end(sphdem_ctx* ctx) {
+  CHECK(checkLoaded(ctx));
+  HIPC(hipSetDevice(ctx->device));
+  return stepEnd(ctx);
+}
+
+int sphdem_hook_body_force(sphdem_ctx* ctx, uint32_t type_mask, const double accel[3]) {
+  CHECK(checkLoaded(ctx));
+  if (!accel) return SPHDEM_E_ARG;
+  HIPC(hipSetDevice(ctx->device));
+  return hookBody(ctx, type_mask, accel);
+}
+int sphdem_hook_dem_floor(sphdem_ctx* ctx) {
+  CHECK(checkLoaded(ctx));
+  HIPC(hipSetDevice(ctx->device));
+  return hookFloor(ctx);
+}
+int sphdem_hook_freeze(sphdem_ctx* ctx, uint32_t type_mask, double t_freeze) {
+  CHECK(checkLoaded(ctx));
+  HIPC(hipSetDevice(ctx->device));
+  return hookFreeze(ctx, type_mask, t_freeze);
+}
+int sphdem_hook_sum_drag(sphdem_ctx* ctx) {
+  CHECK(checkLoaded(ctx));
+  HIPC(hipSetDevice(ctx->device));
+  return hookSumDrag(ctx);
+}
+
+int sphdem_run_steps(sphdem_ctx* ctx, int nsteps, int hookset) {
+  CHECK(checkLoaded(ctx));
+  if (nsteps < 0 || hookset < SPHDEM_HOOKS_NONE || hookset > SPHDEM_HOOKS_SEDIMENT)
+    return fail(ctx, SPHDEM_E_ARG, "bad nsteps/hookset");
+  HIPC(hipSetDevice(ctx->device));
+  const double g[3] = {0.0, 0.0, -9.81};
+  const uint32_t sphMask = 1u << SPHDEM_SPH, demMask = 1u << SPHDEM_DEM;
+  for (int s = 0; s < nsteps; s++) {
+    ctx->sphStep++;
+    CHECK(stepStart(ctx));
+    if (hookset != SPHDEM_HOOKS_NONE) CHECK(hookBody(ctx, sphMask, g));  // beforeMiddle
+    CHECK(stepMiddle(ctx));
+    if (hookset == SPHDEM_HOOKS_SEDIMENT) {  // beforeEnd, sediment3D/customSim.cpp:97-107
+      CHECK(hookSumDrag(ctx));
+      CHECK(hookBody(ctx, demMask, g));
+      CHECK(hookFloor(ctx));
+      CHECK(hookFreeze(ctx, demMask, ctx->P.time_drop));
+    }
+    CHECK(stepEnd(ctx));
+    if (hookset == SPHDEM_HOOKS_SEDIMENT) CHECK(hookFreeze(ctx, demMask, ctx->P.time_drop));
+  }
+  return 0;
+}
+
+int sphdem_reset(sphdem_ctx* ctx) {
+  CHECK(checkLoaded(ctx));
+  HIPC(hipSetDevice(ctx->device));
+  return doReset(ctx);
+}
+
+int sphdem_get_cell_coords(sphdem_ctx* ctx, int32_t* ijk) {
+  CHECK(checkLoaded(ctx));
+  if (!ijk) return SPHDEM_E_ARG;
+  HIPC(hipSetDevice(ctx->device));
+  int* buf = (int*)ctx->fieldbuf;
+  hipLaunchKernelGGL(k_cell_coords, dim3(blocks(ctx->n)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                     ctx->n, ctx->gs, ctx->rule, buf);
+  LAUNCHED();
+  HIPC(hipMemcpyAsync(ijk, buf, sizeof(int) * 3 * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int sphdem_get_pairs(sphdem_ctx* ctx, int kind, int64_t* count, int32_t* a, int32_t* b,
+                     int32_t* image, int64_t capacity) {
+  CHECK(checkLoaded(ctx));
+  if (!count || kind < 0 || kind > 3) return SPHDEM_E_ARG;
+  if ((kind == SPHDEM_PAIRS_DEM || kind >= SPHDEM_PAIRS_SPH2DEM) && !liqDem(ctx)) {
+    *count = 0;
+    return 0;
+  }
+  HIPC(hipSetDevice(ctx->device));
+  const int nq = (kind == SPHDEM_PAIRS_SPH || kind == SPHDEM_PAIRS_SPH2DEM) ? ctx->ns : ctx->nd;
+  if (nq == 0) {
+    *count = 0;
+    return 0;
+  }
+  int* cnt = nullptr;
+  long long* off = nullptr;
+  HIPC(hipMalloc(&cnt, sizeof(int) * nq));
+  HIPC(hipMemsetAsync(cnt, 0, sizeof(int) * nq, ctx->stream));
+  hipLaunchKernelGGL(k_pairs, dim3(blocks(nq)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
+                     ctx->nd, kind, ctx->gs, ctx->gc, ctx->start, ctx->box, ctx->rule,
+                     ctx->reachFull, 0, cnt, (const long long*)nullptr, (int*)nullptr,
+                     (int*)nullptr, (int*)nullptr);
+  LAUNCHED();
+  std::vector<int> hc(nq);
+  HIPC(hipMemcpyAsync(hc.data(), cnt, sizeof(int) * nq, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  std::vector<long long> ho(nq);
+  long long tot = 0;
+  for (int i = 0; i < nq; i++) {
+    ho[i] = tot;
+    tot += hc[i];
+  }
+  *count = tot;
+  if (a && b && image && tot > 0) {
+    if (tot > capacity) {
+      (void)hipFree(cnt);
+      return fail(ctx, SPHDEM_E_CAPACITY, "pair buffer of %lld < %lld", (long long)capacity, tot);
+    }
+    int *da = nullptr, *db = nullptr, *di = nullptr;
+    HIPC(hipMalloc(&off, sizeof(long long) * nq));
+    HIPC(hipMalloc(&da, sizeof(int) * tot));
+    HIPC(hipMalloc(&db, sizeof(int) * tot));
+    HIPC(hipMalloc(&di, sizeof(int) * tot));
+    HIPC(hipMemcpyAsync(off, ho.data(), sizeof(long long) * nq, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_pairs, dim3(blocks(nq)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
+                       ctx->nd, kind, ctx->gs, ctx->gc, ctx->start, ctx->box, ctx->rule,
+                       ctx->reachFull, 1, cnt, (const long long*)off, da, db, di);
+    LAUNCHED();
+    HIPC(hipMemcpyAsync(a, da, sizeof(int) * tot, hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipMemcpyAsync(b, db, sizeof(int) * tot, hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipMemcpyAsync(image, di, sizeof(int) * tot, hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(di);
+    (void)hipFree(off);
+  }
+  (void)hipFree(cnt);
+  return 0;
+}
+
+int sphdem_pass_omega(sphdem_ctx* ctx) {
+  CHECK(checkLoaded(ctx));
+  HIPC(hipSetDevice(ctx->device));
+  return passOmega(ctx, nullptr);
+}
+int sphdem_pass_eos(sphdem_ctx* ctx) {
+  CHECK(checkLoaded(ctx));
+  HIPC(hipSetDevice(ctx->device));
+  return passEos(ctx);
+}
+int sphdem_pass_force(sphdem_ctx* ctx) {
+  CHECK(checkLoaded(ctx));
+  HIPC(hipSetDevice(ctx->device));
+  return passForce(ctx);
+}
+int sphdem_pass_dddt(sphdem_ctx* ctx) {
+  CHECK(checkLoaded(ctx));
+  HIPC(hipSetDevice(ctx->device));
+  return passDddt(ctx);
+}
+int sphdem_pass_density(sphdem_ctx* ctx) {
+  CHECK(checkLoaded(ctx));
+  HIPC(hipSetDevice(ctx->device));
+  return passDensity(ctx, true);
+}
+int sphdem_pass_porosity(sphdem_ctx* ctx) {
+  CHECK(checkLoaded(ctx));
+  if (!liqDem(ctx)) return fail(ctx, SPHDEM_E_STATE, "LIQ_DEM is off");
+  HIPC(hipSetDevice(ctx->device));
+  return passCoupling(ctx);
+}
+int sphdem_pass_fpi(sphdem_ctx* ctx) {
+  CHECK(checkLoaded(ctx));
+  if (!liqDem(ctx)) return fail(ctx, SPHDEM_E_STATE, "LIQ_DEM is off");
+  HIPC(hipSetDevice(ctx->device));
+  return passFpi(ctx);
+}
+int sphdem_pass_dem_contacts(sphdem_ctx* ctx) {
+  CHECK(checkLoaded(ctx));
+  if (!liqDem(ctx)) return fail(ctx, SPHDEM_E_STATE, "LIQ_DEM is off");
+  HIPC(hipSetDevice(ctx->device));
+  return passContacts(ctx);
+}
+int sphdem_reduce_dt(sphdem_ctx* ctx, double* newdt) {
+  CHECK(checkLoaded(ctx));
+  if (!newdt) return SPHDEM_E_ARG;
+  HIPC(hipSetDevice(ctx->device));
+  DevGlobals d;
+  HIPC(hipMemcpyAsync(&d, ctx->dg, sizeof(d), hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  memcpy(newdt, &d.newDt_bits, sizeof(double));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+// sphdem_kernels.h -- device data layout of the gfx950 SPH-DEM loop.
+//
+// All particles live in one structure of arrays, kept sorted by
+// (grid key, original index).  The SPH group (sph + sphBoundary) comes first,
+// binned in the reference's SPH cell grid (dataLL.h:303-305); the DEM group
+// follows, binned in a coarse grid whose edge covers both the DEM contact
+// distance 2R and the coupling radius; deleted particles sort last.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sphdem {
+
+enum : int {
+  T_SPH = 0,
+  T_SPHB = 1,
+  T_BOUNDARY = 3,
+  T_DEM = 11,
+  T_DELETED = 255,
+};
+
+// One particle = 12 double4 + 2 int (392 B).
+struct Store {
+  double4* posh;   // r.xyz, h
+  double4* vhm;    // vhat.xyz, mass
+  double4* dpo;    // dens, pdr2, omega, porosity
+  double4* vel;    // v.xyz, u
+  double4* v0p;    // v0.xyz, press
+  double4* vh0d;   // vhat0.xyz, dddt
+  double4* frc;    // f.xyz, dudt
+  double4* fp;     // fp.xyz, shepSum
+  double4* fv;     // fv.xyz, -
+  double4* fb;     // fb.xyz, -
+  double4* fdrag;  // fdrag.xyz, -
+  double4* fvel;   // fvel.xyz, -
+  int* iam;
+  int* orig;
+};
+constexpr int kNumVec = 12;
+
+// Globals that the kernels read or update (the CglobalVars members of the loop).
+struct DevGlobals {
+  double time, dt, newDt, maxdt, oldDt;
+  long long sphStep;
+  double custom[9];
+  unsigned long long newDt_bits;  // atomicMin target (positive doubles order as their bits)
+  unsigned long long hmax_bits;   // atomicMax target
+  unsigned long long ndel_sph, ndel_dem;
+  unsigned int errflags;  // bit 0: a particle fell outside its cell grid
+  unsigned int pad_;
+};
+
+// Uniform grid: cell_i = (int)((x_i - cmin_i)/edge + off); keys are
+// base + (cx*dim1 + cy)*dim2 + cz.
+struct Grid {
+  double cmin[3];
+  double edge;
+  double off;
+  int dim[3];
+  int base;
+};
+
+struct Box {
+  int periodic[3];
+  double Lp[3];  // RMAX - RMIN (image shift +1)
+  double Lm[3];  // RMIN - RMAX (image shift -1)
+};
+
+}  // namespace sphdem

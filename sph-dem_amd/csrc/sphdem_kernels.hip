@@ -1,0 +1,1166 @@
+// sphdem_kernels.hip -- gfx950 kernels of the SPH-DEM per-timestep loop, FP64 form.
+//
+// FP64 kernels are compiled without FMA contraction so that every a*b+c rounds
+// twice, as the reference's x86-64 build does; operand order follows the
+// reference expression by expression (cited per kernel).
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "../../include/sphdem_c.h"
+#include "sphdem_kernels.h"
+#include "sphdem_math.h"
+
+namespace sphdem {
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ bool isSphT(int t) { return t == T_SPH || t == T_SPHB; }
+__device__ __forceinline__ double imgc(double r, int s, double Lp, double Lm) {
+  return s == 1 ? (Lp + r) : (s == -1 ? (Lm + r) : r);
+}
+__device__ __forceinline__ int cellOf(double x, double cmin, double edge, double off) {
+  return (int)((x - cmin) / edge + off);
+}
+__device__ __forceinline__ double dmin2(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double dmax2(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ unsigned long long dbits(double x) {
+  return (unsigned long long)__double_as_longlong(x);
+}
+__device__ __forceinline__ double& comp(double4& v, int k) { return (&v.x)[k]; }
+
+// Wave (64-lane) and block reductions.
+__device__ __forceinline__ double waveMin(double v) {
+  for (int d = 32; d > 0; d >>= 1) v = dmin2(v, __shfl_xor(v, d, 64));
+  return v;
+}
+__device__ __forceinline__ double waveMax(double v) {
+  for (int d = 32; d > 0; d >>= 1) v = dmax2(v, __shfl_xor(v, d, 64));
+  return v;
+}
+// positive doubles order like their bit patterns, so min/max go through the
+// 64-bit integer atomics and are exact and order independent
+__device__ __forceinline__ void atomicMinPos(unsigned long long* p, double v) {
+  v = waveMin(v);
+  if ((threadIdx.x & 63) == 0) atomicMin(p, dbits(v));
+}
+__device__ __forceinline__ void atomicMaxPos(unsigned long long* p, double v) {
+  v = waveMax(v);
+  if ((threadIdx.x & 63) == 0) atomicMax(p, dbits(v));
+}
+
+// Visits every particle of grid g in the cells within `reach` of the query
+// point q, for each periodic image shift s (s = 0 first): the candidates b are
+// real particles whose image r_b + s*L may lie near q.  Cells along z of one
+// (x,y) column are contiguous in the sorted store, so each column is one range.
+template <class V>
+__device__ __forceinline__ void forCells(const Grid& g, const int* __restrict__ start,
+                                         const Box& B, double qx, double qy, double qz,
+                                         int reach, V&& visit) {
+  const int nsx = B.periodic[0] ? 3 : 1, nsy = B.periodic[1] ? 3 : 1, nsz = B.periodic[2] ? 3 : 1;
+  const double q[3] = {qx, qy, qz};
+  for (int ix = 0; ix < nsx; ix++) {
+    const int sx = ix == 0 ? 0 : (ix == 1 ? -1 : 1);
+    const double tx = sx == 0 ? q[0] : (sx == 1 ? q[0] - B.Lp[0] : q[0] - B.Lm[0]);
+    const int cx = cellOf(tx, g.cmin[0], g.edge, g.off);
+    const int x0 = max(cx - reach, 0), x1 = min(cx + reach, g.dim[0] - 1);
+    if (x0 > x1) continue;
+    for (int iy = 0; iy < nsy; iy++) {
+      const int sy = iy == 0 ? 0 : (iy == 1 ? -1 : 1);
+      const double ty = sy == 0 ? q[1] : (sy == 1 ? q[1] - B.Lp[1] : q[1] - B.Lm[1]);
+      const int cy = cellOf(ty, g.cmin[1], g.edge, g.off);
+      const int y0 = max(cy - reach, 0), y1 = min(cy + reach, g.dim[1] - 1);
+      if (y0 > y1) continue;
+      for (int iz = 0; iz < nsz; iz++) {
+        const int sz = iz == 0 ? 0 : (iz == 1 ? -1 : 1);
+        const double tz = sz == 0 ? q[2] : (sz == 1 ? q[2] - B.Lp[2] : q[2] - B.Lm[2]);
+        const int cz = cellOf(tz, g.cmin[2], g.edge, g.off);
+        const int z0 = max(cz - reach, 0), z1 = min(cz + reach, g.dim[2] - 1);
+        if (z0 > z1) continue;
+        for (int a = x0; a <= x1; a++)
+          for (int b = y0; b <= y1; b++) {
+            const int k0 = g.base + (a * g.dim[1] + b) * g.dim[2];
+            const int j0 = start[k0 + z0], j1 = start[k0 + z1 + 1];
+            for (int j = j0; j < j1; j++) visit(j, sx, sy, sz);
+          }
+      }
+    }
+  }
+}
+
+struct Img {
+  double x, y, z;
+};
+__device__ __forceinline__ Img imagePos(const double4& p, int sx, int sy, int sz, const Box& B) {
+  return Img{imgc(p.x, sx, B.Lp[0], B.Lm[0]), imgc(p.y, sy, B.Lp[1], B.Lm[1]),
+             imgc(p.z, sz, B.Lp[2], B.Lm[2])};
+}
+
+// gradW (sph.h:322-342): dr*F(q,h), divided by pa.omega under VAR_H_CORRECTION
+__device__ __forceinline__ void gradW(double wcon, bool varh, double omega_a, double dx,
+                                      double dy, double dz, double q, double h, double* g) {
+  double drF0 = 0.0, drF1 = 0.0, drF2 = 0.0;
+  if (q != 0) {
+    const double F = wendF(wcon, q, h);
+    drF0 = dx * F;
+    drF1 = dy * F;
+    drF2 = dz * F;
+  }
+  if (varh) {
+    g[0] = drF0 / omega_a;
+    g[1] = drF1 / omega_a;
+    g[2] = drF2 / omega_a;
+  } else {
+    g[0] = drF0;
+    g[1] = drF1;
+    g[2] = drF2;
+  }
+}
+
+// ============================================================ cell build
+// CdataLL::updateDomain's migration for one rank whose neighbours are itself
+// through periodic faces (dataLL.cpp:370-403, 459-497): wrap or delete, then
+// hmax (calcDomainLimits, dataLL.cpp:954-975).
+__global__ void k_wrap(Store S, int n, double dmin0, double dmin1, double dmin2_,
+                       double dsp0, double dsp1, double dsp2, Box B, DevGlobals* g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double hv = 0.0;
+  if (i < n) {
+    const int t = S.iam[i];
+    if (t != T_DELETED) {
+      double4 p = S.posh[i];
+      const double dmin[3] = {dmin0, dmin1, dmin2_};
+      const double dsp[3] = {dsp0, dsp1, dsp2};
+      int oc[3];
+      bool out = false, nonper = false, notone = false;
+      for (int k = 0; k < 3; k++) {
+        oc[k] = (int)((comp(p, k) - dmin[k]) / dsp[k] + 1);
+        if (oc[k] > 2 || oc[k] < 0) out = true;
+        if (oc[k] != 1) {
+          notone = true;
+          if (!B.periodic[k]) nonper = true;
+        }
+      }
+      if (out || nonper) {
+        S.iam[i] = T_DELETED;
+        atomicAdd(t == T_DEM ? &g->ndel_dem : &g->ndel_sph, 1ull);
+      } else {
+        if (notone) {
+          for (int k = 0; k < 3; k++) {
+            if (oc[k] == 0) comp(p, k) = B.Lp[k] + comp(p, k);
+            if (oc[k] == 2) comp(p, k) = B.Lm[k] + comp(p, k);
+          }
+          S.posh[i] = p;
+        }
+        hv = p.w;
+      }
+    }
+  }
+  atomicMaxPos(&g->hmax_bits, hv);
+}
+
+__global__ void k_bin(Store S, int n, Grid gs, Grid gc, int keyDeleted, int* __restrict__ key,
+                      int* __restrict__ slot, int* __restrict__ counts, DevGlobals* g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int t = S.iam[i];
+  int k;
+  if (t == T_DELETED) {
+    k = keyDeleted;
+  } else {
+    const double4 p = S.posh[i];
+    const Grid& gr = (t == T_DEM) ? gc : gs;
+    int c[3];
+    for (int d = 0; d < 3; d++) {
+      c[d] = cellOf(comp(const_cast<double4&>(p), d), gr.cmin[d], gr.edge, gr.off);
+      if (c[d] < 0 || c[d] >= gr.dim[d]) {
+        atomicOr(&g->errflags, 1u);
+        c[d] = c[d] < 0 ? 0 : gr.dim[d] - 1;
+      }
+    }
+    k = gr.base + (c[0] * gr.dim[1] + c[1]) * gr.dim[2] + c[2];
+  }
+  key[i] = k;
+  slot[i] = atomicAdd(&counts[k], 1);
+}
+
+// Exclusive scan, 4096 elements per 1024-thread block: per-thread serial scan
+// of 4 items, 64-lane wavefront scan, then across the 16 wavefronts in LDS.
+constexpr int SCAN_THREADS = 1024;
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_tile(const int* __restrict__ in,
+                                                            int* __restrict__ out,
+                                                            int* __restrict__ tileSums, int n) {
+  __shared__ int waveTot[SCAN_THREADS / 64];
+  const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  int v[SCAN_ITEMS];
+  int run = 0;
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    const int x = (base + k < n) ? in[base + k] : 0;
+    v[k] = run;
+    run += x;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int incl = run;
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  if (lane == 63) waveTot[wave] = incl;
+  __syncthreads();
+  if (wave == 0) {
+    int w = (lane < SCAN_THREADS / 64) ? waveTot[lane] : 0;
+    int wi = w;
+    for (int d = 1; d < 64; d <<= 1) {
+      const int y = __shfl_up(wi, d, 64);
+      if (lane >= d) wi += y;
+    }
+    if (lane < SCAN_THREADS / 64) waveTot[lane] = wi - w;
+    if (lane == SCAN_THREADS / 64 - 1 && tileSums) tileSums[blockIdx.x] = wi;
+  }
+  __syncthreads();
+  const int excl = waveTot[wave] + incl - run;
+  for (int k = 0; k < SCAN_ITEMS; k++)
+    if (base + k < n) out[base + k] = excl + v[k];
+}
+__global__ void k_scan_add(int* __restrict__ out, const int* __restrict__ tileOff, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += tileOff[i / SCAN_TILE];
+}
+
+__global__ void k_scatter(int n, const int* __restrict__ key, const int* __restrict__ slot,
+                          const int* __restrict__ start, int* __restrict__ tmp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) tmp[start[key[i]] + slot[i]] = i;
+}
+
+// Order inside a cell = original index (the reference inserts particles in
+// container order, dataLL.cpp:226-231): each entry's rank is the number of
+// cell members with a smaller original index.
+__global__ void k_rank(int n, const int* __restrict__ key, const int* __restrict__ start,
+                       const int* __restrict__ tmp, const int* __restrict__ orig,
+                       int* __restrict__ perm) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int i = tmp[p];
+  const int k = key[i];
+  const int s0 = start[k], s1 = start[k + 1];
+  const int o = orig[i];
+  int rank = 0;
+  for (int q = s0; q < s1; q++) rank += (orig[tmp[q]] < o) ? 1 : 0;
+  perm[s0 + rank] = i;
+}
+
+__global__ void k_permute(int n, const int* __restrict__ perm, Store src, Store dst) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int i = perm[p];
+  dst.posh[p] = src.posh[i];
+  dst.vhm[p] = src.vhm[i];
+  dst.dpo[p] = src.dpo[i];
+  dst.vel[p] = src.vel[i];
+  dst.v0p[p] = src.v0p[i];
+  dst.vh0d[p] = src.vh0d[i];
+  dst.frc[p] = src.frc[i];
+  dst.fp[p] = src.fp[i];
+  dst.fv[p] = src.fv[i];
+  dst.fb[p] = src.fb[i];
+  dst.fdrag[p] = src.fdrag[i];
+  dst.fvel[p] = src.fvel[i];
+  dst.iam[p] = src.iam[i];
+  dst.orig[p] = src.orig[i];
+}
+
+// ============================================================ SPH pair passes
+// The neighbour predicate is Cparticle::is_neighbr (particle.h:236-238):
+// b != a && len2(r_a - r_b) <= pow(2*max(h_a,h_b), 2); hq holds the h the
+// cached lists were built with.
+
+// Nsph::calcOmega sph.h:437-452
+__global__ void k_omega(Store S, int ns, Grid gs, const int* __restrict__ start, Box B,
+                        const double* __restrict__ hq, double wcon) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= ns) return;
+  if (!isSphT(S.iam[a])) return;
+  const double4 pa = S.posh[a];
+  const double ha = pa.w;
+  const double hqa = hq ? hq[a] : ha;
+  const double ma = S.vhm[a].w;
+  double om = ma * wendPdWh(wcon, 0, ha);
+  forCells(gs, start, B, pa.x, pa.y, pa.z, 1, [&](int b, int sx, int sy, int sz) {
+    if (b == a && (sx | sy | sz) == 0) return;
+    const double4 pb = S.posh[b];
+    const Img im = imagePos(pb, sx, sy, sz, B);
+    const double dx = pa.x - im.x, dy = pa.y - im.y, dz = pa.z - im.z;
+    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+    const double hb = hq ? hq[b] : pb.w;
+    const double lim = 2.0 * dmax2(hqa, hb);
+    if (!(l2 <= lim * lim)) return;
+    if (isSphT(S.iam[b])) {
+      const double r = sqrt(l2);
+      om += S.vhm[b].w * wendPdWh(wcon, r / ha, ha);
+    }
+  });
+  om = 1 + ha * om / (S.dpo[a].x * 3);
+  S.dpo[a].z = om;
+}
+
+// CsphIncompress::calcPressSpsoundPdr2 sphIncompress.h:863-896
+__global__ void k_eos(Store S, int ns, sphdem_params P) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= ns) return;
+  if (!isSphT(S.iam[a])) return;
+  double4 d = S.dpo[a];
+  double press;
+  if (P.flags & SPHDEM_F_LIQ_DEM)
+    press = P.prb * (pow(d.x / (P.refd * d.w), P.gamma) - 1.0);
+  else
+    press = P.prb * (pow(d.x / P.refd, 7.0) - 1.0);
+  d.y = press / (d.x * d.x);
+  S.dpo[a] = d;
+  S.v0p[a].w = press;
+}
+
+// CsphIncompress::calcForce sphIncompress.h:899-972 with calcViscForce
+// (sph.h:579-666) and calcPressForce (sph.h:692-734); the per-pair dt
+// conditions (sph.h:545-560) are min-reduced into g->newDt_bits.
+__global__ void k_force(Store S, int ns, Grid gs, const int* __restrict__ start, Box B,
+                        sphdem_params P, DevGlobals* g) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  double newDt = __longlong_as_double(0x7ff0000000000000ll);  // +inf
+  if (a < ns && isSphT(S.iam[a])) {
+    const int ta = S.iam[a];
+    const double4 pa = S.posh[a];
+    const double4 va = S.vhm[a];
+    const double4 da = S.dpo[a];
+    const double ha = pa.w;
+    const bool varh = (P.flags & SPHDEM_F_VAR_H_CORRECTION) != 0;
+    const bool morris = (P.flags & SPHDEM_F_VISC_MORRIS) != 0;
+    const bool artif = (P.flags & SPHDEM_F_VISC_ARTIFICIAL) != 0;
+    const bool liqdem = (P.flags & SPHDEM_F_LIQ_DEM) != 0;
+    double4 f = S.frc[a], fpa = S.fp[a], fva = S.fv[a];
+    forCells(gs, start, B, pa.x, pa.y, pa.z, 1, [&](int b, int sx, int sy, int sz) {
+      if (b == a && (sx | sy | sz) == 0) return;
+      const double4 pb = S.posh[b];
+      const Img im = imagePos(pb, sx, sy, sz, B);
+      const double dx = pa.x - im.x, dy = pa.y - im.y, dz = pa.z - im.z;
+      const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+      const double hb = pb.w;
+      const double lim = 2.0 * dmax2(ha, hb);
+      if (!(l2 <= lim * lim)) return;
+      const int tb = S.iam[b];
+      const double4 vb = S.vhm[b];
+      const double4 db = S.dpo[b];
+      const double r = sqrt(l2);
+      double gWa[3], gWb[3];
+      if (varh) {
+        gradW(P.wcon, true, da.z, dx, dy, dz, r / ha, ha, gWa);
+        gradW(P.wcon, true, da.z, dx, dy, dz, r / hb, hb, gWb);
+      } else {
+        double g1[3], g2[3];
+        gradW(P.wcon, false, da.z, dx, dy, dz, r / ha, ha, g1);
+        gradW(P.wcon, false, da.z, dx, dy, dz, r / hb, hb, g2);
+        for (int k = 0; k < 3; k++) gWa[k] = gWb[k] = 0.5 * (g1[k] + g2[k]);
+      }
+      if (ta == T_SPH && tb != T_DEM) {
+        const double dv0 = va.x - vb.x, dv1 = va.y - vb.y, dv2 = va.z - vb.z;
+        const double gb0 = 0.5 * (gWa[0] + gWb[0]), gb1 = 0.5 * (gWa[1] + gWb[1]),
+                     gb2 = 0.5 * (gWa[2] + gWb[2]);
+        const double vdr = dot3(dx, dy, dz, dv0, dv1, dv2);
+        const double hav = 0.5 * (ha + hb);
+        double fv0 = 0.0, fv1 = 0.0, fv2 = 0.0;
+        double vsig = 0.0;
+        if (morris) {
+          double visc = P.viscosity * (da.x + db.x) / (da.x * db.x);
+          vsig = 2.0 * P.spsound;
+          if (liqdem) visc = 2.0 * P.viscosity * P.dens / (da.x * db.x);
+          const double d = dot3(dx, dy, dz, gb0, gb1, gb2);
+          const double den = r * r + 0.01 * (hav * hav);
+          fv0 = dv0 * vb.w * visc * d / den;
+          fv1 = dv1 * vb.w * visc * d / den;
+          fv2 = dv2 * vb.w * visc * d / den;
+        }
+        if (artif) {
+          double avsig = 0, avisc = 0.0;
+          if (vdr < 0) {
+            double rr = 0.0;
+            if (r != 0.0) rr = 1 / r;
+            const double viss = vdr * rr;
+            avsig = 2.0 * P.spsound - 4.0 * viss;
+            avisc = viss * avsig * P.alpha_artificial / (da.x + db.x);
+          }
+          const double s = vb.w * avisc;
+          f.x += s * gb0;
+          f.y += s * gb1;
+          f.z += s * gb2;
+          vsig = dmax2(vsig, avsig);
+        }
+        newDt = dmin2(newDt, 0.6 * hav / vsig);
+        newDt = dmin2(newDt, 0.125 * (hav * hav) / P.viscosity);
+        fva.x += fv0;
+        fva.y += fv1;
+        fva.z += fv2;
+        f.x += fv0;
+        f.y += fv1;
+        f.z += fv2;
+      }
+      if (tb == T_SPH || tb == T_SPHB) {
+        const double antic = 0.01 * fabs(da.y + db.y) * 0.0;  // NO_ANTICLUMPING: kdwPowNeps = 0
+        double fp0 = da.y * gWa[0] + db.y * gWb[0];
+        double fp1 = da.y * gWa[1] + db.y * gWb[1];
+        double fp2 = da.y * gWa[2] + db.y * gWb[2];
+        fp0 = -vb.w * (fp0 + antic);
+        fp1 = -vb.w * (fp1 + antic);
+        fp2 = -vb.w * (fp2 + antic);
+        fpa.x += fp0;
+        fpa.y += fp1;
+        fpa.z += fp2;
+        f.x += fp0;
+        f.y += fp1;
+        f.z += fp2;
+      }
+    });
+    S.frc[a] = f;
+    S.fp[a] = fpa;
+    S.fv[a] = fva;
+  }
+  atomicMinPos(&g->newDt_bits, newDt);
+}
+
+// Nsph::accelCondition sph.h:566-568 over sph, sphBoundary and dem
+__global__ void k_accel(Store S, int n, DevGlobals* g) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = __longlong_as_double(0x7ff0000000000000ll);
+  if (a < n) {
+    const int t = S.iam[a];
+    if (isSphT(t) || t == T_DEM) {
+      const double4 f = S.frc[a];
+      v = 0.25 * sqrt(S.posh[a].w / sqrt(dot3(f.x, f.y, f.z, f.x, f.y, f.z)));
+    }
+  }
+  atomicMinPos(&g->newDt_bits, v);
+}
+
+// CsphIncompress::calcDddtDudt sphIncompress.h:783-851
+__global__ void k_dddt(Store S, int ns, Grid gs, const int* __restrict__ start, Box B,
+                       sphdem_params P) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= ns || !isSphT(S.iam[a])) return;
+  const double4 pa = S.posh[a];
+  const double4 va = S.vhm[a];
+  const double om = S.dpo[a].z;
+  const double ha = pa.w;
+  const bool varh = (P.flags & SPHDEM_F_VAR_H_CORRECTION) != 0;
+  double dddt = S.vh0d[a].w;
+  forCells(gs, start, B, pa.x, pa.y, pa.z, 1, [&](int b, int sx, int sy, int sz) {
+    if (b == a && (sx | sy | sz) == 0) return;
+    const double4 pb = S.posh[b];
+    const Img im = imagePos(pb, sx, sy, sz, B);
+    const double dx = pa.x - im.x, dy = pa.y - im.y, dz = pa.z - im.z;
+    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+    const double hb = pb.w;
+    const double lim = 2.0 * dmax2(ha, hb);
+    if (!(l2 <= lim * lim)) return;
+    const double4 vb = S.vhm[b];
+    const double dv0 = va.x - vb.x, dv1 = va.y - vb.y, dv2 = va.z - vb.z;
+    const double r = sqrt(l2);
+    double gW[3];
+    if (varh) {
+      gradW(P.wcon, true, om, dx, dy, dz, r / ha, ha, gW);
+    } else {
+      double g1[3], g2[3];
+      gradW(P.wcon, false, om, dx, dy, dz, r / ha, ha, g1);
+      gradW(P.wcon, false, om, dx, dy, dz, r / hb, hb, g2);
+      for (int k = 0; k < 3; k++) gW[k] = 0.5 * (g1[k] + g2[k]);
+    }
+    const double vdr = dot3(dv0, dv1, dv2, gW[0], gW[1], gW[2]);
+    dddt += vb.w * vdr;
+  });
+  S.vh0d[a].w = dddt;
+}
+
+// Nsph::calcHFromDens sph.h:1823-1825
+__global__ void k_hfromdens(Store S, int ns, double hfac, double* hsnap) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= ns || !isSphT(S.iam[a])) return;
+  const double h = hfac * pow_third(S.vhm[a].w / S.dpo[a].x);
+  S.posh[a].w = h;
+  if (hsnap) hsnap[a] = h;
+}
+
+// Nsph::calcDensity sph.h:1149-1191 (kernel-sum branch)
+__global__ void k_density(Store S, int ns, Grid gs, const int* __restrict__ start, Box B,
+                          const double* __restrict__ hq, double wcon) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= ns || !isSphT(S.iam[a])) return;
+  const double4 pa = S.posh[a];
+  const double ha = pa.w;
+  const double hqa = hq[a];
+  double rho = S.vhm[a].w * wendW(wcon, 0, ha);
+  forCells(gs, start, B, pa.x, pa.y, pa.z, 1, [&](int b, int sx, int sy, int sz) {
+    if (b == a && (sx | sy | sz) == 0) return;
+    const double4 pb = S.posh[b];
+    const Img im = imagePos(pb, sx, sy, sz, B);
+    const double dx = pa.x - im.x, dy = pa.y - im.y, dz = pa.z - im.z;
+    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+    const double lim = 2.0 * dmax2(hqa, hq[b]);
+    if (!(l2 <= lim * lim)) return;
+    if (!isSphT(S.iam[b])) return;
+    const double r = sqrt(l2);
+    rho += S.vhm[b].w * wendW(wcon, r / ha, ha);
+  });
+  S.dpo[a].x = rho;
+}
+
+// ============================================================ SPH <-> DEM coupling
+// calcNeighboursCoupling's SPH-side rule (dataLL.cpp:1154-1190) as a predicate
+// on (DEM cell of the SPH particle, DEM cell of the DEM image, distance).
+struct CouplingRule {
+  double cmin[3];
+  double e;       // 2*DEM_SEARCH_RADIUS
+  double s;       // 2*LIQ_DEM_COUPLING_RADIUS/DEM_SEARCH_RADIUS
+  double thr;     // pow(0.5*Rc/DSR, 2)
+  double rc2;     // pow(Rc, 2)
+  double skip2;   // pairs beyond this squared distance have W == 0 exactly
+  int nc;         // ceil(0.5*Rc/DSR)
+  // DEM ghost band (dataLL.cpp:337-364): images exist only within Rc of a face
+  double gmin[3], gspace[3];
+};
+__device__ __forceinline__ int demCell(const CouplingRule& R, double x, int d) {
+  return (int)((x - R.cmin[d]) / R.e + R.s - 1);
+}
+// accepted(l2) with the whole-cell (addAll) branch; `full` keeps addAll pairs
+// whatever their distance (the reference's list), otherwise only pairs that
+// can carry a nonzero kernel value are reported.
+__device__ __forceinline__ bool couplingAccept(const CouplingRule& R, const int* ca, double ix,
+                                               double iy, double iz, double l2, bool full) {
+  const int d0 = demCell(R, ix, 0) - ca[0];
+  const int d1 = demCell(R, iy, 1) - ca[1];
+  const int d2 = demCell(R, iz, 2) - ca[2];
+  if (d0 < -R.nc || d0 > R.nc || d1 < -R.nc || d1 > R.nc || d2 < -R.nc || d2 > R.nc) return false;
+  const int plus = (d0 + 1) * (d0 + 1) + (d1 + 1) * (d1 + 1) + (d2 + 1) * (d2 + 1);
+  const int minus = (d0 - 1) * (d0 - 1) + (d1 - 1) * (d1 - 1) + (d2 - 1) * (d2 - 1);
+  if (plus < R.thr) return full || l2 <= R.skip2;
+  if (minus < R.thr) return l2 < R.rc2;
+  return false;
+}
+__device__ __forceinline__ bool demImageExists(const CouplingRule& R, const Box& B,
+                                               const double4& p, int sx, int sy, int sz) {
+  const int s[3] = {sx, sy, sz};
+  for (int d = 0; d < 3; d++) {
+    if (s[d] == 0) continue;
+    if (!B.periodic[d]) return false;
+    int ic = (int)((comp(const_cast<double4&>(p), d) - R.gmin[d]) / R.gspace[d] + 1);
+    ic = ic > 2 ? 2 : (ic < 0 ? 0 : ic);
+    if (ic != (s[d] == 1 ? 0 : 2)) return false;
+  }
+  return true;
+}
+
+// Shepard sum of each DEM particle (calcShepSum, sph.h:1303-1320, a scatter
+// from SPH in the reference, followed by reverseSync) as a gather over the SPH
+// particles whose coupling list holds this DEM particle or one of its images.
+__global__ void k_shepsum(Store S, int ns, int nd, Grid gs, const int* __restrict__ start,
+                          Box B, CouplingRule R, int reach, double wcon, double Rc) {
+  const int j = ns + blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns + nd) return;
+  const double4 pj = S.posh[j];
+  double sum = 0.0;
+  // candidates a see the image r_j + s L; enumerate them around that image
+  const int nsx = B.periodic[0] ? 3 : 1, nsy = B.periodic[1] ? 3 : 1, nsz = B.periodic[2] ? 3 : 1;
+  for (int ix = 0; ix < nsx; ix++)
+    for (int iy = 0; iy < nsy; iy++)
+      for (int iz = 0; iz < nsz; iz++) {
+        const int sx = ix == 0 ? 0 : (ix == 1 ? -1 : 1);
+        const int sy = iy == 0 ? 0 : (iy == 1 ? -1 : 1);
+        const int sz = iz == 0 ? 0 : (iz == 1 ? -1 : 1);
+        if ((sx | sy | sz) != 0 && !demImageExists(R, B, pj, sx, sy, sz)) continue;
+        const Img im = imagePos(pj, sx, sy, sz, B);
+        Box B0 = B;
+        B0.periodic[0] = B0.periodic[1] = B0.periodic[2] = 0;
+        forCells(gs, start, B0, im.x, im.y, im.z, reach, [&](int a, int, int, int) {
+          if (!isSphT(S.iam[a])) return;
+          const double4 pa = S.posh[a];
+          const double dx = pa.x - im.x, dy = pa.y - im.y, dz = pa.z - im.z;
+          const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+          if (l2 > R.skip2) return;
+          const int ca[3] = {demCell(R, pa.x, 0), demCell(R, pa.y, 1), demCell(R, pa.z, 2)};
+          if (!couplingAccept(R, ca, im.x, im.y, im.z, l2, false)) return;
+          const double dv = S.vhm[a].w / S.dpo[a].x;
+          const double r = sqrt(l2);
+          sum += dv * wendW(wcon, r * (2.0 / Rc), Rc / 2.0);
+        });
+      }
+  S.fp[j].w = sum;
+}
+
+// SPH side of middle()'s coupling block (sphIncompress.cpp:64-78): calcPorosity
+// (sph.h:1323-1341), calcDEMtoSPHDrag (sph.h:1343-1362) and finalisePorosity
+// (sph.h:1289-1296), one gather over the DEM partners.
+__global__ void k_coupling_sph(Store S, int ns, int nd, Grid gc, const int* __restrict__ start,
+                               Box B, CouplingRule R, sphdem_params P) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= ns || !isSphT(S.iam[a])) return;
+  const double4 pa = S.posh[a];
+  const double Rc = P.coupling_radius;
+  const double rdens = 1.0 / S.dpo[a].x;
+  const int ca[3] = {demCell(R, pa.x, 0), demCell(R, pa.y, 1), demCell(R, pa.z, 2)};
+  double por = 0.0;
+  double fd0 = 0.0, fd1 = 0.0, fd2 = 0.0;
+  forCells(gc, start, B, pa.x, pa.y, pa.z, 1, [&](int j, int sx, int sy, int sz) {
+    const double4 pj = S.posh[j];
+    if ((sx | sy | sz) != 0 && !demImageExists(R, B, pj, sx, sy, sz)) return;
+    const Img im = imagePos(pj, sx, sy, sz, B);
+    const double dx = pa.x - im.x, dy = pa.y - im.y, dz = pa.z - im.z;
+    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+    if (l2 > R.skip2) return;
+    if (!couplingAccept(R, ca, im.x, im.y, im.z, l2, false)) return;
+    const double r = sqrt(l2);
+    const double w = wendW(P.wcon, r * (2.0 / Rc), Rc / 2.0);
+    por += P.porosity_incr * w;
+    const double4 fdj = S.fdrag[j];
+    const double s = S.vhm[j].w * rdens;
+    const double shep = S.fp[j].w;
+    fd0 -= s * fdj.x * w / shep;
+    fd1 -= s * fdj.y * w / shep;
+    fd2 -= s * fdj.z * w / shep;
+  });
+  S.fdrag[a] = make_double4(fd0, fd1, fd2, 0.0);
+  double4 f = S.frc[a];
+  f.x += fd0;
+  f.y += fd1;
+  f.z += fd2;
+  S.frc[a] = f;
+  por = 1.0 - por;
+  if (por < 0.37) por = 0.37;
+  S.dpo[a].w = por;
+}
+
+// calcFPI (sph.h:1708-1792) with calcParticleDrag (sph.h:1575-1631, Di Felice):
+// each DEM particle gathers its SPH partners (the 27 SPH cells around it,
+// dataLL.cpp:1106-1126).
+__global__ void k_fpi(Store S, int ns, int nd, Grid gs, const int* __restrict__ start, Box B,
+                      sphdem_params P) {
+  const int j = ns + blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns + nd) return;
+  const double4 pj = S.posh[j];
+  const double4 vj = S.vhm[j];
+  double sum = 0, fdens = 0, por = 0;
+  double ff[3] = {0, 0, 0}, fpfb[3] = {0, 0, 0}, fv[3] = {0, 0, 0}, fdr[3] = {0, 0, 0},
+         fvel[3] = {0, 0, 0};
+  forCells(gs, start, B, pj.x, pj.y, pj.z, 1, [&](int b, int sx, int sy, int sz) {
+    const double4 pb = S.posh[b];
+    const Img im = imagePos(pb, sx, sy, sz, B);
+    const double dx = pj.x - im.x, dy = pj.y - im.y, dz = pj.z - im.z;
+    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+    const double lim = 2.0 * dmax2(pj.w, pb.w);
+    if (!(l2 <= lim * lim)) return;
+    if (S.iam[b] == T_BOUNDARY) return;
+    const double r = sqrt(l2);
+    const double4 vb = S.vhm[b];
+    const double4 db = S.dpo[b];
+    const double dvol = vb.w / db.x;
+    const double q = r / pb.w;
+    const double Wab = wendW(P.wcon, q, pb.w);
+    const double dvWab = dvol * Wab;
+    const double mW = vb.w * Wab;
+    const double4 fpb = S.fp[b], fbb = S.fb[b], fvb = S.fv[b], fb_ = S.frc[b], fdb = S.fdrag[b];
+    fpfb[0] += mW * (fpb.x + fbb.x);
+    fpfb[1] += mW * (fpb.y + fbb.y);
+    fpfb[2] += mW * (fpb.z + fbb.z);
+    fv[0] += mW * (fvb.x);
+    fv[1] += mW * (fvb.y);
+    fv[2] += mW * (fvb.z);
+    fv[0] += dvWab * (fvb.x);
+    fv[1] += dvWab * (fvb.y);
+    fv[2] += dvWab * (fvb.z);
+    ff[0] += dvWab * (fb_.x);
+    ff[1] += dvWab * (fb_.y);
+    ff[2] += dvWab * (fb_.z);
+    fdr[0] += dvWab * fdb.x;
+    fdr[1] += dvWab * fdb.y;
+    fdr[2] += dvWab * fdb.z;
+    fvel[0] += dvWab * vb.x;
+    fvel[1] += dvWab * vb.y;
+    fvel[2] += dvWab * vb.z;
+    por += dvWab * db.w;
+    fdens += Wab * vb.w;
+    sum += dvWab;
+  });
+  double4 fdrag_out = make_double4(0.0, 0.0, 0.0, 0.0);
+  double4 f = S.frc[j];
+  if (sum > 0.5) {
+    const double rsum = 1.0 / sum;
+    for (int k = 0; k < 3; k++) {
+      ff[k] *= rsum;
+      fpfb[k] *= rsum;
+      fv[k] *= rsum;
+      fdr[k] *= rsum;
+      fvel[k] *= rsum;
+    }
+    fdens *= rsum;
+    por *= rsum;
+    S.fb[j] = make_double4(fpfb[0], fpfb[1], fpfb[2], 0.0);
+    S.fv[j] = make_double4(fv[0], fv[1], fv[2], 0.0);
+    // calcParticleDrag
+    const double dv0 = por * (fvel[0] - vj.x), dv1 = por * (fvel[1] - vj.y),
+                 dv2 = por * (fvel[2] - vj.z);
+    const double vdiff = sqrt(dot3(dv0, dv1, dv2, dv0, dv1, dv2));
+    double fd0 = 0.0, fd1 = 0.0, fd2 = 0.0;
+    if (vdiff != 0) {
+      const double A = P.pi * (P.dem_radius * P.dem_radius);
+      const double Re = (2.0 * P.dem_radius / P.viscosity) * vdiff;
+      const double t = 1.5 - log10(Re);
+      const double dragGamma = P.drag_a - P.drag_b * exp(-(t * t) / 2.0);
+      const double c = 0.63 + 4.8 / sqrt(Re);
+      const double C = c * c;
+      const double s = (1.0 / 2.0) * A * C * vdiff;
+      const double pw = pow(por, -dragGamma);
+      fd0 = s * dv0 * pw;
+      fd1 = s * dv1 * pw;
+      fd2 = s * dv2 * pw;
+    }
+    const double dm = P.dens / vj.w;
+    const double sv = P.dem_vol / vj.w;
+    fdrag_out.x = sv * (fv[0] + fpfb[0]) + dm * fd0;
+    fdrag_out.y = sv * (fv[1] + fpfb[1]) + dm * fd1;
+    fdrag_out.z = sv * (fv[2] + fpfb[2]) + dm * fd2;
+    f.x += fdrag_out.x;
+    f.y += fdrag_out.y;
+    f.z += fdrag_out.z;
+    S.frc[j] = f;
+  }
+  S.fdrag[j] = fdrag_out;
+  S.fvel[j] = make_double4(fvel[0], fvel[1], fvel[2], 0.0);
+  S.dpo[j].w = por;
+  S.fp[j].w = sum;
+}
+
+// calcDEMContacts (sph.h:1676-1706) with demNormal LINEAR (sph.h:1513-1515)
+__global__ void k_contacts(Store S, int ns, int nd, Grid gc, const int* __restrict__ start,
+                           Box B, sphdem_params P) {
+  const int j = ns + blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns + nd) return;
+  const double4 pj = S.posh[j];
+  const double4 vj = S.vhm[j];
+  double4 fp = S.fp[j], f = S.frc[j];
+  bool any = false;
+  forCells(gc, start, B, pj.x, pj.y, pj.z, 1, [&](int b, int sx, int sy, int sz) {
+    if (b == j && (sx | sy | sz) == 0) return;
+    const double4 pb = S.posh[b];
+    const Img im = imagePos(pb, sx, sy, sz, B);
+    const double dx = pj.x - im.x, dy = pj.y - im.y, dz = pj.z - im.z;
+    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+    const double lim = 2.0 * dmax2(pj.w, pb.w);
+    if (!(l2 <= lim * lim)) return;
+    const double4 vb = S.vhm[b];
+    const double r = sqrt(l2);
+    const double dv0 = vj.x - vb.x, dv1 = vj.y - vb.y, dv2 = vj.z - vb.z;
+    const double n0 = dx / r, n1 = dy / r, n2 = dz / r;
+    const double overlap = 2.0 * P.dem_radius - r;
+    if (overlap > 0) {
+      const double od = -dot3(dv0, dv1, dv2, n0, n1, n2);
+      const double s = P.dem_k * overlap + P.dem_gamma * od;
+      const double f0 = s * n0 / vj.w, f1 = s * n1 / vj.w, f2 = s * n2 / vj.w;
+      fp.x += f0;
+      fp.y += f1;
+      fp.z += f2;
+      f.x += f0;
+      f.y += f1;
+      f.z += f2;
+      any = true;
+    }
+  });
+  if (any) {
+    S.fp[j] = fp;
+    S.frc[j] = f;
+  }
+}
+
+// ============================================================ integrator
+// start(): driftRAndKick + driftRest + initSumsMiddle on sph/sphBoundary,
+// driftR on dem (sphIncompress.cpp:4-16,37; sphIncompress.h:620-704, 744-763);
+// time += dt/2.
+__global__ void k_start(Store S, int n, sphdem_params P, DevGlobals* g) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const double dt = g->dt / 2;
+  if (a == 0) g->time += g->dt / 2;
+  if (a >= n) return;
+  const int t = S.iam[a];
+  if (isSphT(t)) {
+    double4 p = S.posh[a];
+    double4 vh = S.vhm[a];
+    p.x += dt * vh.x;
+    p.y += dt * vh.y;
+    p.z += dt * vh.z;
+    if (t != T_SPHB) {
+      double4 v = S.vel[a];
+      const double4 f = S.frc[a];
+      S.v0p[a] = make_double4(v.x, v.y, v.z, S.v0p[a].w);
+      S.vh0d[a] = make_double4(vh.x, vh.y, vh.z, S.vh0d[a].w);
+      v.x += dt * f.x;
+      v.y += dt * f.y;
+      v.z += dt * f.z;
+      vh.x += dt * f.x;
+      vh.y += dt * f.y;
+      vh.z += dt * f.z;
+      S.vel[a] = v;
+      S.vhm[a] = vh;
+    }
+    // driftRest
+    double4 d = S.dpo[a];
+    d.x += dt * S.vh0d[a].w;
+    S.dpo[a] = d;
+    if (!(P.flags & SPHDEM_F_CONST_H)) p.w = P.hfac * pow_third(vh.w / d.x);
+    S.posh[a] = p;
+    // initSumsMiddle
+    const double4 z = make_double4(0.0, 0.0, 0.0, 0.0);
+    S.frc[a] = z;
+    S.fp[a] = make_double4(0.0, 0.0, 0.0, S.fp[a].w);
+    S.fv[a] = z;
+    S.fb[a] = z;
+  } else if (t == T_DEM) {
+    double4 p = S.posh[a];
+    const double4 vh = S.vhm[a];
+    p.x += dt * vh.x;
+    p.y += dt * vh.y;
+    p.z += dt * vh.z;
+    S.posh[a] = p;
+  }
+}
+
+__global__ void k_mid_init(DevGlobals* g) { g->newDt_bits = (unsigned long long)__double_as_longlong(g->maxdt); }
+
+// dt rule (sphIncompress.cpp:126-154): min with LIQ_DEM_TIMESTEP and, after
+// TIME_DROP_PARTICLE, DEM_TIMESTEP; both branches set dt = newDt.
+__global__ void k_dt_final(sphdem_params P, DevGlobals* g) {
+  double nd = __longlong_as_double((long long)g->newDt_bits);
+  if (P.flags & SPHDEM_F_LIQ_DEM) {
+    nd = dmin2(nd, P.liq_dem_timestep);
+    if (!(P.flags & (SPHDEM_F_FIXED_DEM | SPHDEM_F_NO_DEM_CONTACTS)))
+      if (g->time > P.time_drop) nd = dmin2(nd, P.dem_timestep);
+  }
+  g->newDt = nd;
+  g->oldDt = g->dt;
+  g->dt = nd;
+}
+
+// kick (sphIncompress.h:706-742) for sph with (oldDt/2, dt/2), initSumsEnd on
+// sph/sphBoundary, then driftRAndKick + initSumsMiddle on dem
+// (sphIncompress.cpp:157-163).
+__global__ void k_mid_kick(Store S, int n, sphdem_params P, DevGlobals* g) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const int t = S.iam[a];
+  if (isSphT(t)) {
+    if (t == T_SPH) {
+      const double dt = 0.5 * g->oldDt + 0.5 * g->dt;
+      const double4 f = S.frc[a];
+      const double4 v0 = S.v0p[a];
+      const double4 vh0 = S.vh0d[a];
+      double4 v = S.vel[a];
+      double4 vh = S.vhm[a];
+      v.x = v0.x + dt * f.x;
+      v.y = v0.y + dt * f.y;
+      v.z = v0.z + dt * f.z;
+      vh.x = vh0.x + dt * f.x;
+      vh.y = vh0.y + dt * f.y;
+      vh.z = vh0.z + dt * f.z;
+      if (g->time < P.damptime) {
+        v.x *= 0.98;
+        v.y *= 0.98;
+        v.z *= 0.98;
+        vh.x *= 0.98;
+        vh.y *= 0.98;
+        vh.z *= 0.98;
+      }
+      S.vel[a] = v;
+      S.vhm[a] = vh;
+    }
+    S.vh0d[a].w = 0.0;  // initSumsEnd
+  } else if (t == T_DEM && (P.flags & SPHDEM_F_LIQ_DEM)) {
+    const double dt = g->dt / 2;
+    double4 p = S.posh[a];
+    double4 vh = S.vhm[a];
+    double4 v = S.vel[a];
+    const double4 f = S.frc[a];
+    p.x += dt * vh.x;
+    p.y += dt * vh.y;
+    p.z += dt * vh.z;
+    S.v0p[a] = make_double4(v.x, v.y, v.z, S.v0p[a].w);
+    S.vh0d[a] = make_double4(vh.x, vh.y, vh.z, S.vh0d[a].w);
+    if (P.flags & SPHDEM_F_FIXED_DEM) {
+      // v and vhat stay at v0, vhat0
+    } else {
+      v.x += dt * f.x;
+      v.y += dt * f.y;
+      v.z += dt * f.z;
+      vh.x += dt * f.x;
+      vh.y += dt * f.y;
+      vh.z += dt * f.z;
+    }
+    S.posh[a] = p;
+    S.vel[a] = v;
+    S.vhm[a] = vh;
+    const double4 z = make_double4(0.0, 0.0, 0.0, 0.0);
+    S.frc[a] = make_double4(0.0, 0.0, 0.0, 0.0);  // dudt too
+    S.fp[a] = make_double4(0.0, 0.0, 0.0, S.fp[a].w);
+    S.fv[a] = z;
+    S.fb[a] = z;
+  }
+}
+
+// end(): driftR on sph/sphBoundary (sphIncompress.cpp:174), time += dt/2
+__global__ void k_end_drift(Store S, int ns, DevGlobals* g) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const double dt = g->dt / 2;
+  if (a == 0) g->time += g->dt / 2;
+  if (a >= ns || !isSphT(S.iam[a])) return;
+  double4 p = S.posh[a];
+  const double4 vh = S.vhm[a];
+  p.x += dt * vh.x;
+  p.y += dt * vh.y;
+  p.z += dt * vh.z;
+  S.posh[a] = p;
+}
+
+// kick on dem with (dt/2, dt/2) (sphIncompress.cpp:239)
+__global__ void k_dem_kick(Store S, int ns, int nd, sphdem_params P, DevGlobals* g) {
+  const int a = ns + blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= ns + nd || S.iam[a] != T_DEM) return;
+  const double dt = 0.5 * g->dt + 0.5 * g->dt;
+  const double4 f = S.frc[a];
+  const double4 v0 = S.v0p[a];
+  const double4 vh0 = S.vh0d[a];
+  double4 v = S.vel[a];
+  double4 vh = S.vhm[a];
+  v.x = v0.x + dt * f.x;
+  v.y = v0.y + dt * f.y;
+  v.z = v0.z + dt * f.z;
+  vh.x = vh0.x + dt * f.x;
+  vh.y = vh0.y + dt * f.y;
+  vh.z = vh0.z + dt * f.z;
+  if (g->time < P.damptime) {
+    v.x *= 0.98;
+    v.y *= 0.98;
+    v.z *= 0.98;
+    vh.x *= 0.98;
+    vh.y *= 0.98;
+    vh.z *= 0.98;
+  }
+  if (P.flags & SPHDEM_F_FIXED_DEM) {
+    v.x = v0.x;
+    v.y = v0.y;
+    v.z = v0.z;
+    vh.x = vh0.x;
+    vh.y = vh0.y;
+    vh.z = vh0.z;
+  }
+  S.vel[a] = v;
+  S.vhm[a] = vh;
+}
+
+// driftRest + calcEnergies on sph/sphBoundary (sphIncompress.cpp:245-249;
+// sphIncompress.h:678-704, 769-781)
+__global__ void k_end_rest(Store S, int ns, sphdem_params P, DevGlobals* g) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= ns || !isSphT(S.iam[a])) return;
+  const double dt = g->dt / 2;
+  double4 d = S.dpo[a];
+  const double dddt = S.vh0d[a].w;
+  d.x += dt * dddt;
+  if (!(P.flags & SPHDEM_F_CONST_H)) S.posh[a].w = P.hfac * pow_third(S.vhm[a].w / d.x);
+  double press;
+  if (P.flags & SPHDEM_F_LIQ_DEM)
+    press = P.prb * (pow(d.x / (P.refd * d.w), P.gamma) - 1.0);
+  else
+    press = P.prb * (pow(d.x / P.refd, 7.0) - 1.0);
+  d.y = press / (d.x * d.x);
+  S.dpo[a] = d;
+  S.v0p[a].w = press;
+  const double dudt = d.y * dddt;
+  S.frc[a].w = dudt;
+  S.vel[a].w += g->dt * dudt;
+}
+
+// ============================================================ hooks
+__global__ void k_body_force(Store S, int n, unsigned mask, double ax, double ay, double az) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const int t = S.iam[a];
+  if (t >= 32 || !((mask >> t) & 1u)) return;
+  double4 f = S.frc[a];
+  if (ax != 0.0) f.x += ax;
+  if (ay != 0.0) f.y += ay;
+  if (az != 0.0) f.z += az;
+  S.frc[a] = f;
+}
+// addCustomBoundaries sediment3D/customSim.cpp:67-81
+__global__ void k_dem_floor(Store S, int ns, int nd, sphdem_params P) {
+  const int a = ns + blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= ns + nd || S.iam[a] != T_DEM) return;
+  const double4 p = S.posh[a];
+  const double overlap = P.rmin[2] + P.dem_radius - p.z;
+  if (overlap > 0.0) {
+    const double od = -S.vhm[a].z;
+    const double s = P.dem_k * overlap + P.dem_gamma * od;
+    const double fz = s * 1.0 / S.vhm[a].w;
+    S.fb[a].z += fz;
+    S.frc[a].z += fz;
+  }
+}
+// freezeParticles sediment3D/customSim.cpp:12-25, applied while time <= tfreeze
+__global__ void k_freeze(Store S, int n, unsigned mask, double tfreeze, const DevGlobals* g) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n || !(g->time <= tfreeze)) return;
+  const int t = S.iam[a];
+  if (t >= 32 || !((mask >> t) & 1u)) return;
+  S.frc[a] = make_double4(0.0, 0.0, 0.0, S.frc[a].w);
+  S.vel[a] = make_double4(0.0, 0.0, 0.0, S.vel[a].w);
+  S.vhm[a] = make_double4(0.0, 0.0, 0.0, S.vhm[a].w);
+}
+// sumDragForces sediment3D/customSim.cpp:27-42: per-block partial sums of
+// m*fdrag for sph (0..2) and dem (3..5); k_sum_drag_final adds them in block order.
+__global__ void k_sum_drag(Store S, int n, double* partial) {
+  __shared__ double red[6][256];
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  if (a < n) {
+    const int t = S.iam[a];
+    const double m = S.vhm[a].w;
+    const double4 fd = S.fdrag[a];
+    if (t == T_SPH) {
+      v[0] = m * fd.x;
+      v[1] = m * fd.y;
+      v[2] = m * fd.z;
+    } else if (t == T_DEM) {
+      v[3] = m * fd.x;
+      v[4] = m * fd.y;
+      v[5] = m * fd.z;
+    }
+  }
+  for (int k = 0; k < 6; k++) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int k = 0; k < 6; k++) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 6; k++) partial[6 * blockIdx.x + k] = red[k][0];
+}
+__global__ void k_sum_drag_final(const double* partial, int nb, DevGlobals* g) {
+  if (threadIdx.x != 0) return;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < nb; b++)
+    for (int k = 0; k < 6; k++) s[k] += partial[6 * b + k];
+  for (int k = 0; k < 6; k++) g->custom[k] = s[k];
+}
+
+// ============================================================ parity helpers
+__global__ void k_get_field(const double* __restrict__ base, int off, int width,
+                            const int* __restrict__ orig, int n, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int o = orig[i];
+  for (int c = 0; c < width; c++) out[(size_t)o * width + c] = base[(size_t)4 * i + off + c];
+}
+__global__ void k_set_field(double* __restrict__ base, int off, int width,
+                            const int* __restrict__ orig, int n, const double* __restrict__ in) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int o = orig[i];
+  for (int c = 0; c < width; c++) base[(size_t)4 * i + off + c] = in[(size_t)o * width + c];
+}
+__global__ void k_get_int(const int* __restrict__ src, int add, const int* __restrict__ orig,
+                          int n, int* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[orig[i]] = src[i] + add;
+}
+__global__ void k_set_int(int* __restrict__ dst, const int* __restrict__ orig, int n,
+                          const int* __restrict__ in) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = in[orig[i]];
+}
+// getCellI / dem_getCellI (dataLL.h:303-311) per original index
+__global__ void k_cell_coords(Store S, int n, Grid gs, CouplingRule R, int* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int t = S.iam[i];
+  double4 p = S.posh[i];
+  const int o = S.orig[i];
+  for (int d = 0; d < 3; d++) {
+    int c;
+    if (t == T_DEM)
+      c = demCell(R, comp(p, d), d);
+    else
+      c = cellOf(comp(p, d), gs.cmin[d], gs.edge, gs.off);
+    out[3 * o + d] = c;
+  }
+}
+
+// Pair enumeration of one search kind; mode 0 counts, mode 1 writes.
+__global__ void k_pairs(Store S, int ns, int nd, int kind, Grid gs, Grid gc,
+                        const int* __restrict__ start, Box B, CouplingRule R, int reachFull,
+                        int mode, int* __restrict__ cnt, const long long* __restrict__ off,
+                        int* __restrict__ oa, int* __restrict__ ob, int* __restrict__ oimg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int a;
+  if (kind == SPHDEM_PAIRS_SPH || kind == SPHDEM_PAIRS_SPH2DEM) {
+    if (i >= ns) return;
+    a = i;
+    if (!isSphT(S.iam[a])) return;
+  } else {
+    if (i >= nd) return;
+    a = ns + i;
+  }
+  const double4 pa = S.posh[a];
+  long long w = (mode == 1) ? off[i] : 0;
+  int c = 0;
+  auto emit = [&](int b, int sx, int sy, int sz) {
+    if (mode == 1) {
+      oa[w] = S.orig[a];
+      ob[w] = S.orig[b];
+      oimg[w] = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+      w++;
+    }
+    c++;
+  };
+  if (kind == SPHDEM_PAIRS_SPH || kind == SPHDEM_PAIRS_DEM2SPH) {
+    forCells(gs, start, B, pa.x, pa.y, pa.z, 1, [&](int b, int sx, int sy, int sz) {
+      if (b == a && (sx | sy | sz) == 0) return;
+      const double4 pb = S.posh[b];
+      const Img im = imagePos(pb, sx, sy, sz, B);
+      const double dx = pa.x - im.x, dy = pa.y - im.y, dz = pa.z - im.z;
+      const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+      const double lim = 2.0 * dmax2(pa.w, pb.w);
+      if (l2 <= lim * lim) emit(b, sx, sy, sz);
+    });
+  } else if (kind == SPHDEM_PAIRS_DEM) {
+    forCells(gc, start, B, pa.x, pa.y, pa.z, 1, [&](int b, int sx, int sy, int sz) {
+      if (b == a && (sx | sy | sz) == 0) return;
+      const double4 pb = S.posh[b];
+      const Img im = imagePos(pb, sx, sy, sz, B);
+      const double dx = pa.x - im.x, dy = pa.y - im.y, dz = pa.z - im.z;
+      const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+      const double lim = 2.0 * dmax2(pa.w, pb.w);
+      if (l2 <= lim * lim) emit(b, sx, sy, sz);
+    });
+  } else {
+    const int ca[3] = {demCell(R, pa.x, 0), demCell(R, pa.y, 1), demCell(R, pa.z, 2)};
+    forCells(gc, start, B, pa.x, pa.y, pa.z, reachFull, [&](int j, int sx, int sy, int sz) {
+      const double4 pj = S.posh[j];
+      if ((sx | sy | sz) != 0 && !demImageExists(R, B, pj, sx, sy, sz)) return;
+      const Img im = imagePos(pj, sx, sy, sz, B);
+      const double dx = pa.x - im.x, dy = pa.y - im.y, dz = pa.z - im.z;
+      const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+      if (couplingAccept(R, ca, im.x, im.y, im.z, l2, true)) emit(j, sx, sy, sz);
+    });
+  }
+  if (mode == 0) cnt[i] = c;
+}
+
+}  // namespace sphdem

@@ -1,0 +1,259 @@
+"""Parity of the gfx950 library (through its C ABI) with the CPU oracle on
+identical seeded inputs.
+
+Bar (FP64 form): cell coordinates and neighbour-pair sets bit-exact; every
+field of a pass within 1e-12 normwise relative error
+(max_i |gpu_i - oracle_i| / max_i |oracle_i| over the particles the pass
+writes); multi-step runs within the tolerances stated per test."""
+import numpy as np
+import pytest
+
+import oracle
+import sphdem as sd
+
+pytestmark = pytest.mark.gpu
+
+TOL_PASS = 1e-12
+
+
+def normwise(a, b):
+    a = np.asarray(a, float)
+    b = np.asarray(b, float)
+    scale = np.abs(b).max() if b.size else 0.0
+    if scale == 0.0:
+        return float(np.abs(a).max()) if a.size else 0.0
+    return float(np.abs(a - b).max() / scale)
+
+
+def as_set(p):
+    return set(map(tuple, np.asarray(p, dtype=np.int64).tolist()))
+
+
+def dam_case():
+    return sd.dambreak_case(nf=12)
+
+
+def sed_case():
+    return sd.sediment_case(nf=12, n_dem=60)
+
+
+def contact_case():
+    """sediment with DEM spheres pushed into overlapping pairs."""
+    c = sd.sediment_case(nf=12, n_dem=40, seed=77)
+    dem = np.where(c.iam == sd.DEM)[0]
+    R = c.params.dem_radius
+    r = c.r.copy()
+    rng = np.random.default_rng(2)
+    for k in range(0, len(dem) - 1, 2):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        r[dem[k + 1]] = r[dem[k]] + 1.9 * R * d
+    c.r = r
+    return c
+
+
+CASES = {"dambreak": dam_case, "sediment": sed_case}
+
+
+@pytest.fixture(scope="module", params=list(CASES))
+def case(request):
+    return CASES[request.param]()
+
+
+def both(case):
+    return sd.Sim(case), oracle.Oracle(case)
+
+
+def perturb(case, sim, orc, seed=11):
+    """Identical nontrivial state in both: velocities, densities, previous DEM drag."""
+    rng = np.random.default_rng(seed)
+    n = case.n
+    vh = rng.normal(size=(n, 3)) * 0.05
+    vh[case.iam == sd.SPH_BOUNDARY] = 0.0
+    dens = case.dens * (1 + 0.002 * rng.normal(size=n))
+    dens[case.iam == sd.DEM] = case.dens[case.iam == sd.DEM]
+    fd = np.zeros((n, 3))
+    fd[case.iam == sd.DEM] = rng.normal(size=((case.iam == sd.DEM).sum(), 3))
+    for x in (sim, orc):
+        x.set("vhat", vh)
+        x.set("v", vh)
+        x.set("dens", dens)
+        x.set("fdrag", fd)
+    orc.refresh_ghosts()
+
+
+def fresh_newdt(sim, orc, value):
+    for x in (sim, orc):
+        g = x.globals()
+        g.newDt = value
+        x.set_globals(g)
+
+
+def check_fields(sim, orc, names, sel, tol=TOL_PASS):
+    errs = {}
+    for nm in names:
+        errs[nm] = normwise(sim.get(nm)[sel], orc.get(nm)[sel])
+    bad = {k: v for k, v in errs.items() if not v <= tol}
+    assert not bad, f"fields above {tol}: {bad} (all: {errs})"
+    return errs
+
+
+# ------------------------------------------------------------ neighbour search
+def test_cell_coordinates_bit_exact(case):
+    sim, orc = both(case)
+    assert np.array_equal(sim.cells(), orc.cells())
+
+
+@pytest.mark.parametrize("kind", [sd.PAIRS_SPH, sd.PAIRS_DEM, sd.PAIRS_SPH2DEM, sd.PAIRS_DEM2SPH])
+def test_pair_sets_bit_exact(case, kind):
+    if kind != sd.PAIRS_SPH and not (case.params.flags & sd.F_LIQ_DEM):
+        pytest.skip("no DEM in this case")
+    sim, orc = both(case)
+    g, o = sim.pairs(kind), orc.pairs(kind)
+    assert len(g) == len(o)
+    assert as_set(g) == as_set(o)
+
+
+def test_contact_pairs_bit_exact():
+    c = contact_case()
+    sim, orc = both(c)
+    g, o = sim.pairs(sd.PAIRS_DEM), orc.pairs(sd.PAIRS_DEM)
+    assert len(o) > 0
+    assert as_set(g) == as_set(o)
+
+
+# ------------------------------------------------------------ per-pass parity
+def test_middle_passes(case):
+    """middle()'s passes on identical state: coupling (porosity, Shepard sum,
+    DEM->SPH drag), omega, EOS, force and the pair dt conditions."""
+    sim, orc = both(case)
+    perturb(case, sim, orc)
+    fresh_newdt(sim, orc, case.params.maxdt)
+    liq = bool(case.params.flags & sd.F_LIQ_DEM)
+    sph = (case.iam == sd.SPH) | (case.iam == sd.SPH_BOUNDARY)
+    dem = case.iam == sd.DEM
+    if liq:
+        sim.pass_("porosity")
+        orc.pass_("porosity")
+        check_fields(sim, orc, ["shepSum"], dem)
+        check_fields(sim, orc, ["porosity", "fdrag", "f"], sph)
+    if case.params.flags & sd.F_VAR_H_CORRECTION:
+        sim.pass_("omega")
+        orc.pass_("omega")
+        check_fields(sim, orc, ["omega"], sph)
+    sim.pass_("eos")
+    orc.pass_("eos")
+    check_fields(sim, orc, ["press", "pdr2"], sph)
+    sim.pass_("force")
+    orc.pass_("force")
+    check_fields(sim, orc, ["f", "fp", "fv"], sph)
+    assert sim.reduce_dt() == pytest.approx(orc.new_dt(), rel=1e-15)
+
+
+def test_end_passes(case):
+    """end()'s passes: omega, dddt, then FPI with the Di Felice drag."""
+    sim, orc = both(case)
+    perturb(case, sim, orc, seed=12)
+    sph = (case.iam == sd.SPH) | (case.iam == sd.SPH_BOUNDARY)
+    dem = case.iam == sd.DEM
+    liq = bool(case.params.flags & sd.F_LIQ_DEM)
+    if liq:  # forces and porosity the SPH side hands to the DEM gather
+        rng = np.random.default_rng(4)
+        for nm in ("f", "fp", "fv", "fb"):
+            x = rng.normal(size=(case.n, 3))
+            sim.set(nm, x)
+            orc.set(nm, x)
+        por = np.clip(1 - 0.2 * rng.random(case.n), 0.37, 1)
+        sim.set("porosity", por)
+        orc.set("porosity", por)
+        orc.refresh_ghosts()
+    if case.params.flags & sd.F_VAR_H_CORRECTION:
+        sim.pass_("omega")
+        orc.pass_("omega")
+    sim.pass_("dddt")
+    orc.pass_("dddt")
+    check_fields(sim, orc, ["dddt"], sph)
+    if liq:
+        sim.pass_("fpi")
+        orc.pass_("fpi")
+        check_fields(sim, orc, ["fdrag", "f", "fvel", "porosity", "shepSum", "fb", "fv"], dem)
+
+
+def test_dem_contacts_pass():
+    c = contact_case()
+    sim, orc = both(c)
+    rng = np.random.default_rng(9)
+    vh = rng.normal(size=(c.n, 3)) * 0.01
+    for x in (sim, orc):
+        x.set("vhat", vh)
+    orc.refresh_ghosts()
+    dem = c.iam == sd.DEM
+    sim.pass_("dem_contacts")
+    orc.pass_("dem_contacts")
+    errs = check_fields(sim, orc, ["fp", "f"], dem)
+    assert np.abs(orc.get("fp")[dem]).max() > 0
+
+
+def test_density_reinit_pass():
+    c = dam_case()
+    sim, orc = both(c)
+    perturb(c, sim, orc, seed=13)
+    sim.pass_("density")
+    orc.pass_("density")
+    sph = c.iam != sd.DEM
+    check_fields(sim, orc, ["h", "dens"], sph)
+
+
+# ------------------------------------------------------------ whole steps
+@pytest.mark.parametrize("nsteps", [1, 3])
+def test_steps(case, nsteps):
+    """run.cpp's loop (sphStep++, hooks, start/middle/end) for a few steps."""
+    sim, orc = both(case)
+    sim.run(nsteps)
+    orc.run(nsteps)
+    gs, go = sim.globals(), orc.globals()
+    assert gs.dt == pytest.approx(go.dt, rel=1e-12)
+    assert gs.time == pytest.approx(go.time, rel=1e-12)
+    alive = orc.get("iam") != -1
+    tol = 1e-11 if nsteps == 1 else 1e-9
+    names = ["r", "v", "vhat", "dens", "h", "f", "press"]
+    if case.params.flags & sd.F_LIQ_DEM:
+        names += ["porosity", "fdrag", "shepSum"]
+    check_fields(sim, orc, names, alive, tol)
+    # the neighbour sets after the run are still identical
+    assert as_set(sim.pairs(sd.PAIRS_SPH)) == as_set(orc.pairs(sd.PAIRS_SPH))
+
+
+def test_reinit_every_step():
+    """REINIT_DENS_EVERY_N_STEPS = 1 (as damBreak sets): 10 density passes per step."""
+    c = dam_case()
+    c.params.reinit_every = 1
+    sim, orc = both(c)
+    sim.run(2)
+    orc.run(2)
+    alive = orc.get("iam") != -1
+    check_fields(sim, orc, ["dens", "h", "r", "f"], alive, 1e-10)
+
+
+def test_out_of_range_deletion_matches():
+    c = sd.dambreak_case(nf=8)
+    sim, orc = both(c)
+    assert sim.num_deleted() == orc.num_deleted() > 0
+    alive = orc.get("iam") != -1
+    assert np.array_equal(sim.get("iam") != sd.DELETED, alive)
+    sim.run(1)
+    orc.run(1)
+    check_fields(sim, orc, ["r", "dens", "f"], alive, 1e-11)
+
+
+def test_periodic_wrap_matches():
+    c = sd.sediment_case(nf=12, n_dem=10)
+    sim, orc = both(c)
+    r = c.r.copy()
+    r[0, 0] = 1.0 + 0.3 / 12
+    r[1, 1] = -0.2 / 12
+    for x in (sim, orc):
+        x.set("r", r)
+        x.reset()
+    assert np.array_equal(sim.get("r"), orc.get("r"))
+    assert np.array_equal(sim.cells(), orc.cells())
