@@ -1,0 +1,170 @@
+"""Benchmark: SPH-DEM particle-steps/s of the gfx950 timestep loop.
+
+Workload (one GPU): SPH-DEM sedimentation, 64^3 = 262,144 fluid particles
+(dx = 1/64 m, periodic in x and y), 2 x 3 layers of sphBoundary walls
+(24,576) and 4,096 DEM spheres (diameter dx/2, density 2500), the sediment3D
+flag set, FP64.  A step is one run.cpp iteration (sphStep++, the sediment3D
+hooks, start/middle/end) over all particles with the inputs already resident
+in HBM.  particle-steps/s = (N_fluid + N_DEM) * steps / wall time of the
+timed steps (walls are processed but not counted).
+
+Alongside the metric the JSON line carries
+  roofline      : the dominant kernel's algorithmic bytes per launch divided by
+                  its mean duration, timed with HIP events on the library's own
+                  stream over the timed steps, against the 8 TB/s HBM peak;
+  step_roofline : whole-step algorithmic bytes (1152 B per SPH/wall particle +
+                  650 B per DEM particle per step) x steps/s against 8 TB/s;
+  cpu_baseline  : the CPU FP64 restatement timed on this host on a bounded
+                  sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "sph-dem_amd"))
+
+import numpy as np  # noqa: E402
+
+import sphdem as sd  # noqa: E402
+
+METRIC = "SPH-DEM particle-steps/s, mixed vs FP64 at stated error; % HBM roofline"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per particle per launch (FP64 widths), from the pass table:
+# each input field read once, each output field written once per pass.
+B_ALG_KERNEL = {
+    # calcForce: r vhat h mass dens pdr2 iam omega porosity -> f fp fv fb
+    "force": 84 + 16 + 24 + 72,
+    # calcOmega: r h mass dens iam -> omega
+    "omega": 60,
+    # calcDddtDudt: r vhat h mass iam omega -> dddt
+    "dddt": 76 + 8,
+    # porosity (28 -> 8) + DEM->SPH drag (56 -> 48)
+    "coupling": 36 + 104,
+}
+B_ALG_STEP_SPH = 1152  # per SPH / wall particle per step (SPH-DEM flag set)
+B_ALG_STEP_DEM = 650   # per DEM particle per step
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--nf", type=int, default=64, help="fluid particles per axis")
+    p.add_argument("--ndem", type=int, default=4096)
+    p.add_argument("--roofline-kernel", default="force")
+    p.add_argument("--cpu-seconds", type=float, default=12.0,
+                   help="budget of the CPU baseline sample")
+    p.add_argument("--no-cpu", action="store_true")
+    return p.parse_args()
+
+
+def cpu_baseline(case, budget_s):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle
+
+    orc = oracle.Oracle(case)
+    done = 0
+    t0 = time.perf_counter()
+    while True:
+        orc.run(1)
+        done += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or done >= 5:
+            break
+    counted = int(np.sum((case.iam == sd.SPH) | (case.iam == sd.DEM)))
+    orc.close()
+    return {
+        "value": counted * done / el,
+        "unit": "particle-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{done} step(s) of the same {case.n}-particle case "
+                  f"({el:.1f} s, single-threaded CPU FP64 restatement)",
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(the path shards by space, see the design notes)", file=sys.stderr)
+        sys.exit(2)
+    case = sd.sediment_case(nf=a.nf, n_dem=a.ndem, seed=1001)
+    counts = case.counts()
+    counted = counts["fluid"] + counts["dem"]
+    sim = sd.Sim(case, device=0)
+    sim.run(a.warmup)
+    sim.sync()
+    sim.timing_begin(a.roofline_kernel)
+    t0 = time.perf_counter()
+    sim.run(a.steps)
+    sim.sync()
+    t1 = time.perf_counter()
+    kms, launches = sim.timing_end()
+    el = t1 - t0
+    # sanity: the state is finite
+    for nm in ("r", "dens"):
+        if not np.all(np.isfinite(sim.get(nm))):
+            raise RuntimeError(f"non-finite {nm} after the run")
+    g = sim.globals()
+    sim.close()
+    value = counted * a.steps / el
+    units = counts["fluid"] + counts["wall"]  # SPH-group particles a pair kernel processes
+    if a.roofline_kernel in ("fpi", "contacts", "shepsum"):
+        units = counts["dem"]
+    b_launch = B_ALG_KERNEL.get(a.roofline_kernel, 0) * units
+    mean_s = (kms / 1e3) / max(launches, 1)
+    achieved = b_launch / mean_s / 1e9 if launches else None
+    step_bytes = B_ALG_STEP_SPH * (counts["fluid"] + counts["wall"]) + B_ALG_STEP_DEM * counts["dem"]
+    step_gbs = step_bytes * (a.steps / el) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * el / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"SPH-DEM sedimentation {a.nf}^3 fluid + {counts['wall']} wall + "
+                        f"{counts['dem']} DEM, periodic x/y, sediment3D flags, FP64",
+            "fluid": counts["fluid"], "wall": counts["wall"], "dem": counts["dem"],
+            "seed": 1001, "sim_time_s": g.time, "dt_last": g.dt,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": a.roofline_kernel,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": None,
+            "mean_launch_ms": mean_s * 1e3,
+            "launches": launches,
+            "alg_bytes_per_launch": b_launch,
+        },
+        "step_roofline": {
+            "alg_bytes_per_step": step_bytes,
+            "achieved": step_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": step_gbs / HBM_PEAK_GBS,
+        },
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(case, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -226,6 +226,16 @@ int sphdem_reduce_dt(sphdem_ctx* ctx, double* newdt);
 /* Number of particles deleted as out of range since the context was created. */
 int sphdem_num_deleted(const sphdem_ctx* ctx, int64_t* n);
 
+/* ---- measurement (no counterpart in the reference; its timing is the
+ * wtTotal* wall-clock fields of CglobalVars, globalVars.h:163-188) ------------
+ * While a timing window is open, every launch of the named kernel ("force",
+ * "omega", "dddt", "coupling", "shepsum", "fpi", "contacts", "eos", "start",
+ * "kick", "rest", "bin", "rank", "permute", "wrap") is bracketed by HIP events
+ * on the context's stream; sphdem_timing_end synchronises and returns the
+ * summed kernel time and the number of launches timed. */
+int sphdem_timing_begin(sphdem_ctx* ctx, const char* kernel);
+int sphdem_timing_end(sphdem_ctx* ctx, double* total_ms, int64_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,8 +40,44 @@ struct sphdem_ctx {
   CouplingRule rule{};
   int K = 0, reachShep = 1, reachFull = 1;
   bool loaded = false;
+  // measurement window: events bracketing each launch of one kernel
+  int timedKernel = -1;
+  std::vector<hipEvent_t> evPool;
+  size_t evUsed = 0;
   char err[512] = {0};
 };
+
+namespace {
+enum KernelId {
+  KID_FORCE = 0, KID_OMEGA, KID_DDDT, KID_COUPLING, KID_SHEPSUM, KID_FPI, KID_CONTACTS,
+  KID_EOS, KID_START, KID_KICK, KID_REST, KID_BIN, KID_RANK, KID_PERMUTE, KID_WRAP, KID_COUNT
+};
+const char* kKernelNames[KID_COUNT] = {"force", "omega", "dddt", "coupling", "shepsum",
+                                       "fpi",   "contacts", "eos", "start", "kick",
+                                       "rest",  "bin",   "rank",  "permute", "wrap"};
+constexpr size_t kMaxTimedEvents = 40000;
+
+bool recordEvent(sphdem_ctx* c) {
+  if (c->evUsed == c->evPool.size()) {
+    if (c->evPool.size() >= kMaxTimedEvents) return false;
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return false;
+    c->evPool.push_back(e);
+  }
+  return hipEventRecord(c->evPool[c->evUsed++], c->stream) == hipSuccess;
+}
+// RAII bracket of one kernel launch (records only inside an open window)
+struct Timed {
+  sphdem_ctx* c;
+  bool on;
+  Timed(sphdem_ctx* c_, int kid) : c(c_), on(c_->timedKernel == kid && c_->evUsed % 2 == 0) {
+    if (on) on = recordEvent(c);
+  }
+  ~Timed() {
+    if (on && !recordEvent(c)) c->evUsed--;  // keep start/stop events paired
+  }
+};
+}  // namespace
 
 namespace {
 
@@ -242,9 +278,12 @@ int doReset(sphdem_ctx* ctx) {
     dmin[d] = P.proc_domain[2 * d];
     dsp[d] = P.proc_domain[2 * d + 1] - dmin[d];
   }
-  hipLaunchKernelGGL(k_wrap, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, ctx->cur, n, dmin[0],
-                     dmin[1], dmin[2], dsp[0], dsp[1], dsp[2], ctx->box, ctx->dg);
-  LAUNCHED();
+  {
+    Timed tm(ctx, KID_WRAP);
+    hipLaunchKernelGGL(k_wrap, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, ctx->cur, n, dmin[0],
+                       dmin[1], dmin[2], dsp[0], dsp[1], dsp[2], ctx->box, ctx->dg);
+    LAUNCHED();
+  }
   DevGlobals hg;
   HIPC(hipMemcpyAsync(&hg, ctx->dg, sizeof(DevGlobals), hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
@@ -260,19 +299,28 @@ int doReset(sphdem_ctx* ctx) {
   setupGrids(ctx);
   CHECK(ensureKeys(ctx, ctx->K));
   HIPC(hipMemsetAsync(ctx->counts, 0, sizeof(int) * (ctx->K + 1), ctx->stream));
-  hipLaunchKernelGGL(k_bin, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, ctx->cur, n, ctx->gs,
-                     ctx->gc, ctx->K - 1, ctx->key, ctx->slot, ctx->counts, ctx->dg);
-  LAUNCHED();
+  {
+    Timed tm(ctx, KID_BIN);
+    hipLaunchKernelGGL(k_bin, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, ctx->cur, n, ctx->gs,
+                       ctx->gc, ctx->K - 1, ctx->key, ctx->slot, ctx->counts, ctx->dg);
+    LAUNCHED();
+  }
   CHECK(scanCounts(ctx, ctx->K + 1));
   hipLaunchKernelGGL(k_scatter, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, n, ctx->key,
                      ctx->slot, ctx->start, ctx->tmp);
   LAUNCHED();
-  hipLaunchKernelGGL(k_rank, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, n, ctx->key, ctx->start,
-                     ctx->tmp, ctx->cur.orig, ctx->perm);
-  LAUNCHED();
-  hipLaunchKernelGGL(k_permute, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, n, ctx->perm,
-                     ctx->cur, ctx->alt);
-  LAUNCHED();
+  {
+    Timed tm(ctx, KID_RANK);
+    hipLaunchKernelGGL(k_rank, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, n, ctx->key,
+                       ctx->start, ctx->tmp, ctx->cur.orig, ctx->perm);
+    LAUNCHED();
+  }
+  {
+    Timed tm(ctx, KID_PERMUTE);
+    hipLaunchKernelGGL(k_permute, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, n, ctx->perm,
+                       ctx->cur, ctx->alt);
+    LAUNCHED();
+  }
   std::swap(ctx->cur, ctx->alt);
   std::swap(ctx->storeMem[0], ctx->storeMem[1]);
   return 0;
@@ -286,24 +334,28 @@ int checkLoaded(sphdem_ctx* ctx) {
 
 // ---- passes
 int passOmega(sphdem_ctx* ctx, const double* hq) {
+  Timed tm(ctx, KID_OMEGA);
   hipLaunchKernelGGL(k_omega, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
                      ctx->gs, ctx->start, ctx->box, hq, ctx->P.wcon);
   LAUNCHED();
   return 0;
 }
 int passEos(sphdem_ctx* ctx) {
+  Timed tm(ctx, KID_EOS);
   hipLaunchKernelGGL(k_eos, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
                      ctx->P);
   LAUNCHED();
   return 0;
 }
 int passForce(sphdem_ctx* ctx) {
+  Timed tm(ctx, KID_FORCE);
   hipLaunchKernelGGL(k_force, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
                      ctx->gs, ctx->start, ctx->box, ctx->P, ctx->dg);
   LAUNCHED();
   return 0;
 }
 int passDddt(sphdem_ctx* ctx) {
+  Timed tm(ctx, KID_DDDT);
   hipLaunchKernelGGL(k_dddt, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
                      ctx->gs, ctx->start, ctx->box, ctx->P);
   LAUNCHED();
@@ -311,11 +363,13 @@ int passDddt(sphdem_ctx* ctx) {
 }
 int passCoupling(sphdem_ctx* ctx) {
   if (ctx->nd > 0) {
+    Timed tm(ctx, KID_SHEPSUM);
     hipLaunchKernelGGL(k_shepsum, dim3(blocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur,
                        ctx->ns, ctx->nd, ctx->gs, ctx->start, ctx->box, ctx->rule, ctx->reachShep,
                        ctx->P.wcon, ctx->P.coupling_radius);
     LAUNCHED();
   }
+  Timed tm(ctx, KID_COUPLING);
   hipLaunchKernelGGL(k_coupling_sph, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
                      ctx->ns, ctx->nd, ctx->gc, ctx->start, ctx->box, ctx->rule, ctx->P);
   LAUNCHED();
@@ -323,6 +377,7 @@ int passCoupling(sphdem_ctx* ctx) {
 }
 int passFpi(sphdem_ctx* ctx) {
   if (ctx->nd == 0) return 0;
+  Timed tm(ctx, KID_FPI);
   hipLaunchKernelGGL(k_fpi, dim3(blocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
                      ctx->nd, ctx->gs, ctx->start, ctx->box, ctx->P);
   LAUNCHED();
@@ -330,6 +385,7 @@ int passFpi(sphdem_ctx* ctx) {
 }
 int passContacts(sphdem_ctx* ctx) {
   if (ctx->nd == 0) return 0;
+  Timed tm(ctx, KID_CONTACTS);
   hipLaunchKernelGGL(k_contacts, dim3(blocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur,
                      ctx->ns, ctx->nd, ctx->gc, ctx->start, ctx->box, ctx->P);
   LAUNCHED();
@@ -351,9 +407,12 @@ int passDensity(sphdem_ctx* ctx, bool first) {
 // ---- half-steps
 int stepStart(sphdem_ctx* ctx) {
   const int na = ctx->ns + ctx->nd;
-  hipLaunchKernelGGL(k_start, dim3(blocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream, ctx->cur,
-                     na, ctx->P, ctx->dg);
-  LAUNCHED();
+  {
+    Timed tm(ctx, KID_START);
+    hipLaunchKernelGGL(k_start, dim3(blocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream,
+                       ctx->cur, na, ctx->P, ctx->dg);
+    LAUNCHED();
+  }
   if ((ctx->sphStep % ctx->P.reinit_every == 0) || (ctx->sphStep == ctx->P.reinit_at)) {
     CHECK(doReset(ctx));
     for (int i = 0; i < 10; i++) CHECK(passDensity(ctx, i == 0));
@@ -374,6 +433,7 @@ int stepMiddle(sphdem_ctx* ctx) {
   LAUNCHED();
   hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1), 0, ctx->stream, ctx->P, ctx->dg);
   LAUNCHED();
+  Timed tm(ctx, KID_KICK);
   hipLaunchKernelGGL(k_mid_kick, dim3(blocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream,
                      ctx->cur, na, ctx->P, ctx->dg);
   LAUNCHED();
@@ -395,6 +455,7 @@ int stepEnd(sphdem_ctx* ctx) {
       LAUNCHED();
     }
   }
+  Timed tm(ctx, KID_REST);
   hipLaunchKernelGGL(k_end_rest, dim3(blocks(std::max(ctx->ns, 1))), dim3(TPB), 0, ctx->stream,
                      ctx->cur, ctx->ns, ctx->P, ctx->dg);
   LAUNCHED();
@@ -445,6 +506,8 @@ void freeAll(sphdem_ctx* ctx) {
                   ctx->hsnap,       ctx->partial,     ctx->fieldbuf, ctx->errflag, ctx->dg};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (hipEvent_t e : ctx->evPool) (void)hipEventDestroy(e);
+  ctx->evPool.clear();
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
 }
 
@@ -860,6 +923,35 @@ int sphdem_pass_dem_contacts(sphdem_ctx* ctx) {
   HIPC(hipSetDevice(ctx->device));
   return passContacts(ctx);
 }
+int sphdem_timing_begin(sphdem_ctx* ctx, const char* kernel) {
+  if (!ctx || !kernel) return SPHDEM_E_ARG;
+  for (int k = 0; k < KID_COUNT; k++)
+    if (strcmp(kernel, kKernelNames[k]) == 0) {
+      ctx->timedKernel = k;
+      ctx->evUsed = 0;
+      return 0;
+    }
+  return fail(ctx, SPHDEM_E_ARG, "unknown kernel name '%s'", kernel);
+}
+
+int sphdem_timing_end(sphdem_ctx* ctx, double* total_ms, int64_t* launches) {
+  if (!ctx || !total_ms || !launches) return SPHDEM_E_ARG;
+  HIPC(hipSetDevice(ctx->device));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  double tot = 0.0;
+  const size_t pairs = ctx->evUsed / 2;
+  for (size_t i = 0; i < pairs; i++) {
+    float ms = 0.0f;
+    HIPC(hipEventElapsedTime(&ms, ctx->evPool[2 * i], ctx->evPool[2 * i + 1]));
+    tot += ms;
+  }
+  *total_ms = tot;
+  *launches = (int64_t)pairs;
+  ctx->timedKernel = -1;
+  ctx->evUsed = 0;
+  return 0;
+}
+
 int sphdem_reduce_dt(sphdem_ctx* ctx, double* newdt) {
   CHECK(checkLoaded(ctx));
   if (!newdt) return SPHDEM_E_ARG;

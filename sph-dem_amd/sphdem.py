@@ -455,6 +455,8 @@ def load_library(path: str = LIB_PATH):
     lib.sphdem_get_pairs.argtypes = [vp, ctypes.c_int, ctypes.POINTER(i64), ip, ip, ip, i64]
     lib.sphdem_reduce_dt.argtypes = [vp, dp]
     lib.sphdem_num_deleted.argtypes = [vp, ctypes.POINTER(i64)]
+    lib.sphdem_timing_begin.argtypes = [vp, ctypes.c_char_p]
+    lib.sphdem_timing_end.argtypes = [vp, dp, ctypes.POINTER(i64)]
     _lib = lib
     return lib
 
@@ -603,3 +605,13 @@ class Sim:
         v = ctypes.c_int64()
         self._check(self.lib.sphdem_num_deleted(self.ctx, ctypes.byref(v)))
         return v.value
+
+    # ---- measurement: HIP events around each launch of one kernel
+    def timing_begin(self, kernel: str):
+        self._check(self.lib.sphdem_timing_begin(self.ctx, kernel.encode()))
+
+    def timing_end(self) -> tuple[float, int]:
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        self._check(self.lib.sphdem_timing_end(self.ctx, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value

@@ -29,6 +29,15 @@ def as_set(p):
     return set(map(tuple, np.asarray(p, dtype=np.int64).tolist()))
 
 
+def assert_same_pairs(g, o):
+    gs, os_ = as_set(g), as_set(o)
+    extra, missing = gs - os_, os_ - gs
+    assert not extra and not missing, (
+        f"{len(extra)} pairs only on the GPU (e.g. {sorted(extra)[:5]}), "
+        f"{len(missing)} only in the oracle (e.g. {sorted(missing)[:5]})")
+    assert len(g) == len(o)
+
+
 def dam_case():
     return sd.dambreak_case(nf=12)
 
@@ -109,9 +118,7 @@ def test_pair_sets_bit_exact(case, kind):
     if kind != sd.PAIRS_SPH and not (case.params.flags & sd.F_LIQ_DEM):
         pytest.skip("no DEM in this case")
     sim, orc = both(case)
-    g, o = sim.pairs(kind), orc.pairs(kind)
-    assert len(g) == len(o)
-    assert as_set(g) == as_set(o)
+    assert_same_pairs(sim.pairs(kind), orc.pairs(kind))
 
 
 def test_contact_pairs_bit_exact():
@@ -119,7 +126,7 @@ def test_contact_pairs_bit_exact():
     sim, orc = both(c)
     g, o = sim.pairs(sd.PAIRS_DEM), orc.pairs(sd.PAIRS_DEM)
     assert len(o) > 0
-    assert as_set(g) == as_set(o)
+    assert_same_pairs(g, o)
 
 
 # ------------------------------------------------------------ per-pass parity
@@ -220,8 +227,11 @@ def test_steps(case, nsteps):
     if case.params.flags & sd.F_LIQ_DEM:
         names += ["porosity", "fdrag", "shepSum"]
     check_fields(sim, orc, names, alive, tol)
-    # the neighbour sets after the run are still identical
-    assert as_set(sim.pairs(sd.PAIRS_SPH)) == as_set(orc.pairs(sd.PAIRS_SPH))
+    # the neighbour sets of the evolved state (lists rebuilt by a reset: the
+    # cached ones predate end()'s driftRest, which changes h) are identical
+    sim.reset()
+    orc.reset()
+    assert_same_pairs(sim.pairs(sd.PAIRS_SPH), orc.pairs(sd.PAIRS_SPH))
 
 
 def test_reinit_every_step():
