@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "sphdem_kernels.hip"
+#include "sphdem_tiles.hip"
 
 using namespace sphdem;
 
@@ -27,6 +28,10 @@ struct sphdem_ctx {
   int *key = nullptr, *slot = nullptr, *tmp = nullptr, *perm = nullptr;
   int *counts = nullptr, *start = nullptr, *tiles = nullptr, *tileOff = nullptr;
   int Kcap = 0, tileCap = 0;
+  // segments of the SPH grid (the wavefront work units of the pair passes)
+  int *segCnt = nullptr, *segOff = nullptr;
+  Segment* segs = nullptr;
+  int segColCap = 0, segCap = 0;
   double* hsnap = nullptr;
   double* partial = nullptr;
   int partialCap = 0;
@@ -111,10 +116,12 @@ int fail(sphdem_ctx* c, int code, const char* fmt, ...) {
 
 constexpr int TPB = 256;
 inline int blocks(int n) { return (n + TPB - 1) / TPB; }
+// kernels that give each item a whole 64-lane wavefront
+inline int waveBlocks(int n) { return (n + TPB / 64 - 1) / (TPB / 64); }
 
 int allocStore(sphdem_ctx* ctx, Store& s, void*& mem, int64_t cap) {
   const size_t vec = sizeof(double4) * (size_t)cap;
-  const size_t total = vec * kNumVec + 2 * sizeof(int) * (size_t)cap;
+  const size_t total = vec * kNumVec + 3 * sizeof(int) * (size_t)cap;
   HIPC(hipMalloc(&mem, total));
   char* p = (char*)mem;
   double4** f[kNumVec] = {&s.posh, &s.vhm, &s.dpo, &s.vel, &s.v0p, &s.vh0d,
@@ -126,6 +133,8 @@ int allocStore(sphdem_ctx* ctx, Store& s, void*& mem, int64_t cap) {
   s.iam = (int*)p;
   p += sizeof(int) * (size_t)cap;
   s.orig = (int*)p;
+  p += sizeof(int) * (size_t)cap;
+  s.cell = (int*)p;
   return 0;
 }
 
@@ -190,20 +199,66 @@ int ensureKeys(sphdem_ctx* ctx, int K) {
   return 0;
 }
 
-// exclusive scan of ctx->counts[0..m) into ctx->start
-int scanCounts(sphdem_ctx* ctx, int m) {
+// exclusive scan of in[0..m) into out (m <= SCAN_TILE * (tileCap - 1))
+int scanInts(sphdem_ctx* ctx, const int* in, int* out, int m) {
   const int tiles = (m + SCAN_TILE - 1) / SCAN_TILE;
-  hipLaunchKernelGGL(k_scan_tile, dim3(tiles), dim3(SCAN_THREADS), 0, ctx->stream, ctx->counts,
-                     ctx->start, ctx->tiles, m);
+  if (tiles + 1 > ctx->tileCap) return fail(ctx, SPHDEM_E_CAPACITY, "scan of %d items", m);
+  hipLaunchKernelGGL(k_scan_tile, dim3(tiles), dim3(SCAN_THREADS), 0, ctx->stream, in, out,
+                     ctx->tiles, m);
   LAUNCHED();
   if (tiles > 1) {
     hipLaunchKernelGGL(k_scan_tile, dim3(1), dim3(SCAN_THREADS), 0, ctx->stream, ctx->tiles,
                        ctx->tileOff, (int*)nullptr, tiles);
     LAUNCHED();
-    hipLaunchKernelGGL(k_scan_add, dim3(blocks(m)), dim3(TPB), 0, ctx->stream, ctx->start,
+    hipLaunchKernelGGL(k_scan_add, dim3(blocks(m)), dim3(TPB), 0, ctx->stream, out,
                        ctx->tileOff, m);
     LAUNCHED();
   }
+  return 0;
+}
+int scanCounts(sphdem_ctx* ctx, int m) { return scanInts(ctx, ctx->counts, ctx->start, m); }
+
+// Segments of the SPH grid after a rebuild: per-column counts, a scan, then
+// the segment records in column order (so segment order follows space).
+int buildSegments(sphdem_ctx* ctx) {
+  const Grid& g = ctx->gs;
+  const int ncol = g.dim[0] * g.dim[1];
+  if (ncol + 1 > ctx->segColCap) {
+    if (ctx->segCnt) HIPC(hipFree(ctx->segCnt));
+    if (ctx->segOff) HIPC(hipFree(ctx->segOff));
+    ctx->segColCap = (ncol + 1) + (ncol + 1) / 4;
+    HIPC(hipMalloc(&ctx->segCnt, sizeof(int) * ctx->segColCap));
+    HIPC(hipMalloc(&ctx->segOff, sizeof(int) * ctx->segColCap));
+  }
+  // a column holds at most ceil(n_col/64) + (nonempty cells) segments
+  const int need = ctx->ns + ncol + 1;
+  if (need > ctx->segCap) {
+    if (ctx->segs) HIPC(hipFree(ctx->segs));
+    ctx->segCap = need + need / 4;
+    HIPC(hipMalloc(&ctx->segs, sizeof(Segment) * ctx->segCap));
+  }
+  HIPC(hipMemsetAsync(ctx->segCnt + ncol, 0, sizeof(int), ctx->stream));
+  hipLaunchKernelGGL(k_seg_count, dim3(blocks(ncol)), dim3(TPB), 0, ctx->stream, ctx->start, g,
+                     ctx->segCnt);
+  LAUNCHED();
+  CHECK(scanInts(ctx, ctx->segCnt, ctx->segOff, ncol + 1));
+  hipLaunchKernelGGL(k_seg_write, dim3(blocks(ncol)), dim3(TPB), 0, ctx->stream, ctx->start, g,
+                     ctx->segOff, ctx->segs);
+  LAUNCHED();
+  return 0;
+}
+// workgroups of the tile kernels: a multiple of 8 (one slab per XCD)
+int tileBlocks(const sphdem_ctx* ctx) {
+  int est = (ctx->ns / 32 + kWavesPerBlock - 1) / kWavesPerBlock;
+  est = (est + 7) & ~7;
+  return std::min(std::max(est, 8), 2048);
+}
+template <class Op>
+int launchTiles(sphdem_ctx* ctx, const Op& op) {
+  hipLaunchKernelGGL(k_tile<Op>, dim3(tileBlocks(ctx)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                     ctx->gs, ctx->start, ctx->box, ctx->segs,
+                     ctx->segOff + ctx->gs.dim[0] * ctx->gs.dim[1], op);
+  LAUNCHED();
   return 0;
 }
 
@@ -297,6 +352,10 @@ int doReset(sphdem_ctx* ctx) {
   if (ctx->ns + ctx->nd == 0) hmax = -100000;  // calcDomainLimits on an empty container
   if ((hmax > ctx->ssr) || (hmax < 0.9 * ctx->ssr)) ctx->ssr = hmax;
   setupGrids(ctx);
+  for (const Grid* gr : {&ctx->gs, &ctx->gc})
+    if (gr->dim[0] > 2048 || gr->dim[1] > 2048 || gr->dim[2] > 1024)
+      return fail(ctx, SPHDEM_E_CAPACITY, "cell grid %d x %d x %d exceeds 2048 x 2048 x 1024",
+                  gr->dim[0], gr->dim[1], gr->dim[2]);
   CHECK(ensureKeys(ctx, ctx->K));
   HIPC(hipMemsetAsync(ctx->counts, 0, sizeof(int) * (ctx->K + 1), ctx->stream));
   {
@@ -323,7 +382,7 @@ int doReset(sphdem_ctx* ctx) {
   }
   std::swap(ctx->cur, ctx->alt);
   std::swap(ctx->storeMem[0], ctx->storeMem[1]);
-  return 0;
+  return buildSegments(ctx);
 }
 
 int checkLoaded(sphdem_ctx* ctx) {
@@ -335,10 +394,7 @@ int checkLoaded(sphdem_ctx* ctx) {
 // ---- passes
 int passOmega(sphdem_ctx* ctx, const double* hq) {
   Timed tm(ctx, KID_OMEGA);
-  hipLaunchKernelGGL(k_omega, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
-                     ctx->gs, ctx->start, ctx->box, hq, ctx->P.wcon);
-  LAUNCHED();
-  return 0;
+  return launchTiles(ctx, OmegaOp{hq, ctx->P.wcon});
 }
 int passEos(sphdem_ctx* ctx) {
   Timed tm(ctx, KID_EOS);
@@ -349,22 +405,16 @@ int passEos(sphdem_ctx* ctx) {
 }
 int passForce(sphdem_ctx* ctx) {
   Timed tm(ctx, KID_FORCE);
-  hipLaunchKernelGGL(k_force, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
-                     ctx->gs, ctx->start, ctx->box, ctx->P, ctx->dg);
-  LAUNCHED();
-  return 0;
+  return launchTiles(ctx, ForceOp{ctx->P, ctx->dg});
 }
 int passDddt(sphdem_ctx* ctx) {
   Timed tm(ctx, KID_DDDT);
-  hipLaunchKernelGGL(k_dddt, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
-                     ctx->gs, ctx->start, ctx->box, ctx->P);
-  LAUNCHED();
-  return 0;
+  return launchTiles(ctx, DddtOp{ctx->P.wcon, varH(ctx)});
 }
 int passCoupling(sphdem_ctx* ctx) {
   if (ctx->nd > 0) {
     Timed tm(ctx, KID_SHEPSUM);
-    hipLaunchKernelGGL(k_shepsum, dim3(blocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur,
+    hipLaunchKernelGGL(k_shepsum, dim3(waveBlocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur,
                        ctx->ns, ctx->nd, ctx->gs, ctx->start, ctx->box, ctx->rule, ctx->reachShep,
                        ctx->P.wcon, ctx->P.coupling_radius);
     LAUNCHED();
@@ -378,7 +428,7 @@ int passCoupling(sphdem_ctx* ctx) {
 int passFpi(sphdem_ctx* ctx) {
   if (ctx->nd == 0) return 0;
   Timed tm(ctx, KID_FPI);
-  hipLaunchKernelGGL(k_fpi, dim3(blocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
+  hipLaunchKernelGGL(k_fpi, dim3(waveBlocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
                      ctx->nd, ctx->gs, ctx->start, ctx->box, ctx->P);
   LAUNCHED();
   return 0;
@@ -397,11 +447,7 @@ int passDensity(sphdem_ctx* ctx, bool first) {
   hipLaunchKernelGGL(k_hfromdens, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
                      ctx->ns, ctx->P.hfac, first ? ctx->hsnap : (double*)nullptr);
   LAUNCHED();
-  hipLaunchKernelGGL(k_density, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
-                     ctx->ns, ctx->gs, ctx->start, ctx->box, (const double*)ctx->hsnap,
-                     ctx->P.wcon);
-  LAUNCHED();
-  return 0;
+  return launchTiles(ctx, DensityOp{ctx->hsnap, ctx->P.wcon});
 }
 
 // ---- half-steps
@@ -494,7 +540,7 @@ int hookSumDrag(sphdem_ctx* ctx) {
   hipLaunchKernelGGL(k_sum_drag, dim3(nb), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns + ctx->nd,
                      ctx->partial);
   LAUNCHED();
-  hipLaunchKernelGGL(k_sum_drag_final, dim3(1), dim3(64), 0, ctx->stream, ctx->partial, nb,
+  hipLaunchKernelGGL(k_sum_drag_final, dim3(1), dim3(256), 0, ctx->stream, ctx->partial, nb,
                      ctx->dg);
   LAUNCHED();
   return 0;
@@ -503,7 +549,8 @@ int hookSumDrag(sphdem_ctx* ctx) {
 void freeAll(sphdem_ctx* ctx) {
   void* ptrs[] = {ctx->storeMem[0], ctx->storeMem[1], ctx->key,     ctx->slot,   ctx->tmp,
                   ctx->perm,        ctx->counts,      ctx->start,   ctx->tiles,  ctx->tileOff,
-                  ctx->hsnap,       ctx->partial,     ctx->fieldbuf, ctx->errflag, ctx->dg};
+                  ctx->hsnap,       ctx->partial,     ctx->fieldbuf, ctx->errflag, ctx->dg,
+                  ctx->segCnt,      ctx->segOff,      ctx->segs};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : ctx->evPool) (void)hipEventDestroy(e);
@@ -837,11 +884,16 @@ int sphdem_get_pairs(sphdem_ctx* ctx, int kind, int64_t* count, int32_t* a, int3
   long long* off = nullptr;
   HIPC(hipMalloc(&cnt, sizeof(int) * nq));
   HIPC(hipMemsetAsync(cnt, 0, sizeof(int) * nq, ctx->stream));
-  hipLaunchKernelGGL(k_pairs, dim3(blocks(nq)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
-                     ctx->nd, kind, ctx->gs, ctx->gc, ctx->start, ctx->box, ctx->rule,
-                     ctx->reachFull, 0, cnt, (const long long*)nullptr, (int*)nullptr,
-                     (int*)nullptr, (int*)nullptr);
-  LAUNCHED();
+  // the SPH set is enumerated by the same tile machinery as the SPH passes
+  auto enumerate = [&](int mode, const long long* o, int* da, int* db, int* di) -> int {
+    if (kind == SPHDEM_PAIRS_SPH) return launchTiles(ctx, PairsOp{mode, cnt, o, da, db, di});
+    hipLaunchKernelGGL(k_pairs, dim3(blocks(nq)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
+                       ctx->nd, kind, ctx->gs, ctx->gc, ctx->start, ctx->box, ctx->rule,
+                       ctx->reachFull, mode, cnt, o, da, db, di);
+    LAUNCHED();
+    return 0;
+  };
+  CHECK(enumerate(0, nullptr, nullptr, nullptr, nullptr));
   std::vector<int> hc(nq);
   HIPC(hipMemcpyAsync(hc.data(), cnt, sizeof(int) * nq, hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
@@ -863,10 +915,7 @@ int sphdem_get_pairs(sphdem_ctx* ctx, int kind, int64_t* count, int32_t* a, int3
     HIPC(hipMalloc(&db, sizeof(int) * tot));
     HIPC(hipMalloc(&di, sizeof(int) * tot));
     HIPC(hipMemcpyAsync(off, ho.data(), sizeof(long long) * nq, hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_pairs, dim3(blocks(nq)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
-                       ctx->nd, kind, ctx->gs, ctx->gc, ctx->start, ctx->box, ctx->rule,
-                       ctx->reachFull, 1, cnt, (const long long*)off, da, db, di);
-    LAUNCHED();
+    CHECK(enumerate(1, off, da, db, di));
     HIPC(hipMemcpyAsync(a, da, sizeof(int) * tot, hipMemcpyDeviceToHost, ctx->stream));
     HIPC(hipMemcpyAsync(b, db, sizeof(int) * tot, hipMemcpyDeviceToHost, ctx->stream));
     HIPC(hipMemcpyAsync(image, di, sizeof(int) * tot, hipMemcpyDeviceToHost, ctx->stream));

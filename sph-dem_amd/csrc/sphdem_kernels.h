@@ -36,8 +36,25 @@ struct Store {
   double4* fvel;   // fvel.xyz, -
   int* iam;
   int* orig;
+  int* cell;       // packed cell coordinates of the particle's grid (packCell)
 };
 constexpr int kNumVec = 12;
+
+// cell coordinates packed as x | y << 11 | z << 22 (x, y < 2048, z < 1024)
+__host__ __device__ __forceinline__ int packCell(int x, int y, int z) {
+  return x | (y << 11) | (z << 22);
+}
+__host__ __device__ __forceinline__ int cellX(int c) { return c & 2047; }
+__host__ __device__ __forceinline__ int cellY(int c) { return (c >> 11) & 2047; }
+__host__ __device__ __forceinline__ int cellZ(int c) { return (c >> 22) & 1023; }
+
+// A run of whole cells of one (x,y) column of the SPH grid holding at most 64
+// particles (a cell with more is split): the unit of work of one wavefront.
+struct Segment {
+  int p0, p1;  // particle range in the sorted store
+  int col;     // x * dim1 + y
+  int zz;      // z0 | z1 << 16
+};
 
 // Globals that the kernels read or update (the CglobalVars members of the loop).
 struct DevGlobals {

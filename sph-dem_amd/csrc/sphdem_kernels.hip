@@ -39,25 +39,45 @@ __device__ __forceinline__ double waveMax(double v) {
   for (int d = 32; d > 0; d >>= 1) v = dmax2(v, __shfl_xor(v, d, 64));
   return v;
 }
+// Deterministic wavefront sum: a fixed butterfly, so the result does not
+// depend on timing (every lane ends with the same value).
+__device__ __forceinline__ double waveSum(double v) {
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
 // positive doubles order like their bit patterns, so min/max go through the
-// 64-bit integer atomics and are exact and order independent
+// 64-bit integer atomics and are exact and order independent; one atomic per
+// block (blockDim.x <= 1024)
 __device__ __forceinline__ void atomicMinPos(unsigned long long* p, double v) {
+  __shared__ double red[16];
   v = waveMin(v);
-  if ((threadIdx.x & 63) == 0) atomicMin(p, dbits(v));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) v = dmin2(v, red[w]);
+    atomicMin(p, dbits(v));
+  }
 }
 __device__ __forceinline__ void atomicMaxPos(unsigned long long* p, double v) {
+  __shared__ double red[16];
   v = waveMax(v);
-  if ((threadIdx.x & 63) == 0) atomicMax(p, dbits(v));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) v = dmax2(v, red[w]);
+    atomicMax(p, dbits(v));
+  }
 }
 
 // Visits every particle of grid g in the cells within `reach` of the query
 // point q, for each periodic image shift s (s = 0 first): the candidates b are
 // real particles whose image r_b + s*L may lie near q.  Cells along z of one
 // (x,y) column are contiguous in the sorted store, so each column is one range.
+// `first`/`stride` split each contiguous range over the lanes of a wavefront.
 template <class V>
 __device__ __forceinline__ void forCells(const Grid& g, const int* __restrict__ start,
                                          const Box& B, double qx, double qy, double qz,
-                                         int reach, V&& visit) {
+                                         int reach, V&& visit, int first = 0, int stride = 1) {
   const int nsx = B.periodic[0] ? 3 : 1, nsy = B.periodic[1] ? 3 : 1, nsz = B.periodic[2] ? 3 : 1;
   const double q[3] = {qx, qy, qz};
   for (int ix = 0; ix < nsx; ix++) {
@@ -82,7 +102,7 @@ __device__ __forceinline__ void forCells(const Grid& g, const int* __restrict__ 
           for (int b = y0; b <= y1; b++) {
             const int k0 = g.base + (a * g.dim[1] + b) * g.dim[2];
             const int j0 = start[k0 + z0], j1 = start[k0 + z1 + 1];
-            for (int j = j0; j < j1; j++) visit(j, sx, sy, sz);
+            for (int j = j0 + first; j < j1; j += stride) visit(j, sx, sy, sz);
           }
       }
     }
@@ -168,6 +188,7 @@ __global__ void k_bin(Store S, int n, Grid gs, Grid gc, int keyDeleted, int* __r
   int k;
   if (t == T_DELETED) {
     k = keyDeleted;
+    S.cell[i] = 0;
   } else {
     const double4 p = S.posh[i];
     const Grid& gr = (t == T_DEM) ? gc : gs;
@@ -180,6 +201,7 @@ __global__ void k_bin(Store S, int n, Grid gs, Grid gc, int keyDeleted, int* __r
       }
     }
     k = gr.base + (c[0] * gr.dim[1] + c[1]) * gr.dim[2] + c[2];
+    S.cell[i] = packCell(c[0], c[1], c[2]);
   }
   key[i] = k;
   slot[i] = atomicAdd(&counts[k], 1);
@@ -272,41 +294,13 @@ __global__ void k_permute(int n, const int* __restrict__ perm, Store src, Store 
   dst.fvel[p] = src.fvel[i];
   dst.iam[p] = src.iam[i];
   dst.orig[p] = src.orig[i];
+  dst.cell[p] = src.cell[i];
 }
 
 // ============================================================ SPH pair passes
 // The neighbour predicate is Cparticle::is_neighbr (particle.h:236-238):
 // b != a && len2(r_a - r_b) <= pow(2*max(h_a,h_b), 2); hq holds the h the
 // cached lists were built with.
-
-// Nsph::calcOmega sph.h:437-452
-__global__ void k_omega(Store S, int ns, Grid gs, const int* __restrict__ start, Box B,
-                        const double* __restrict__ hq, double wcon) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= ns) return;
-  if (!isSphT(S.iam[a])) return;
-  const double4 pa = S.posh[a];
-  const double ha = pa.w;
-  const double hqa = hq ? hq[a] : ha;
-  const double ma = S.vhm[a].w;
-  double om = ma * wendPdWh(wcon, 0, ha);
-  forCells(gs, start, B, pa.x, pa.y, pa.z, 1, [&](int b, int sx, int sy, int sz) {
-    if (b == a && (sx | sy | sz) == 0) return;
-    const double4 pb = S.posh[b];
-    const Img im = imagePos(pb, sx, sy, sz, B);
-    const double dx = pa.x - im.x, dy = pa.y - im.y, dz = pa.z - im.z;
-    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
-    const double hb = hq ? hq[b] : pb.w;
-    const double lim = 2.0 * dmax2(hqa, hb);
-    if (!(l2 <= lim * lim)) return;
-    if (isSphT(S.iam[b])) {
-      const double r = sqrt(l2);
-      om += S.vhm[b].w * wendPdWh(wcon, r / ha, ha);
-    }
-  });
-  om = 1 + ha * om / (S.dpo[a].x * 3);
-  S.dpo[a].z = om;
-}
 
 // CsphIncompress::calcPressSpsoundPdr2 sphIncompress.h:863-896
 __global__ void k_eos(Store S, int ns, sphdem_params P) {
@@ -324,112 +318,6 @@ __global__ void k_eos(Store S, int ns, sphdem_params P) {
   S.v0p[a].w = press;
 }
 
-// CsphIncompress::calcForce sphIncompress.h:899-972 with calcViscForce
-// (sph.h:579-666) and calcPressForce (sph.h:692-734); the per-pair dt
-// conditions (sph.h:545-560) are min-reduced into g->newDt_bits.
-__global__ void k_force(Store S, int ns, Grid gs, const int* __restrict__ start, Box B,
-                        sphdem_params P, DevGlobals* g) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  double newDt = __longlong_as_double(0x7ff0000000000000ll);  // +inf
-  if (a < ns && isSphT(S.iam[a])) {
-    const int ta = S.iam[a];
-    const double4 pa = S.posh[a];
-    const double4 va = S.vhm[a];
-    const double4 da = S.dpo[a];
-    const double ha = pa.w;
-    const bool varh = (P.flags & SPHDEM_F_VAR_H_CORRECTION) != 0;
-    const bool morris = (P.flags & SPHDEM_F_VISC_MORRIS) != 0;
-    const bool artif = (P.flags & SPHDEM_F_VISC_ARTIFICIAL) != 0;
-    const bool liqdem = (P.flags & SPHDEM_F_LIQ_DEM) != 0;
-    double4 f = S.frc[a], fpa = S.fp[a], fva = S.fv[a];
-    forCells(gs, start, B, pa.x, pa.y, pa.z, 1, [&](int b, int sx, int sy, int sz) {
-      if (b == a && (sx | sy | sz) == 0) return;
-      const double4 pb = S.posh[b];
-      const Img im = imagePos(pb, sx, sy, sz, B);
-      const double dx = pa.x - im.x, dy = pa.y - im.y, dz = pa.z - im.z;
-      const double l2 = dot3(dx, dy, dz, dx, dy, dz);
-      const double hb = pb.w;
-      const double lim = 2.0 * dmax2(ha, hb);
-      if (!(l2 <= lim * lim)) return;
-      const int tb = S.iam[b];
-      const double4 vb = S.vhm[b];
-      const double4 db = S.dpo[b];
-      const double r = sqrt(l2);
-      double gWa[3], gWb[3];
-      if (varh) {
-        gradW(P.wcon, true, da.z, dx, dy, dz, r / ha, ha, gWa);
-        gradW(P.wcon, true, da.z, dx, dy, dz, r / hb, hb, gWb);
-      } else {
-        double g1[3], g2[3];
-        gradW(P.wcon, false, da.z, dx, dy, dz, r / ha, ha, g1);
-        gradW(P.wcon, false, da.z, dx, dy, dz, r / hb, hb, g2);
-        for (int k = 0; k < 3; k++) gWa[k] = gWb[k] = 0.5 * (g1[k] + g2[k]);
-      }
-      if (ta == T_SPH && tb != T_DEM) {
-        const double dv0 = va.x - vb.x, dv1 = va.y - vb.y, dv2 = va.z - vb.z;
-        const double gb0 = 0.5 * (gWa[0] + gWb[0]), gb1 = 0.5 * (gWa[1] + gWb[1]),
-                     gb2 = 0.5 * (gWa[2] + gWb[2]);
-        const double vdr = dot3(dx, dy, dz, dv0, dv1, dv2);
-        const double hav = 0.5 * (ha + hb);
-        double fv0 = 0.0, fv1 = 0.0, fv2 = 0.0;
-        double vsig = 0.0;
-        if (morris) {
-          double visc = P.viscosity * (da.x + db.x) / (da.x * db.x);
-          vsig = 2.0 * P.spsound;
-          if (liqdem) visc = 2.0 * P.viscosity * P.dens / (da.x * db.x);
-          const double d = dot3(dx, dy, dz, gb0, gb1, gb2);
-          const double den = r * r + 0.01 * (hav * hav);
-          fv0 = dv0 * vb.w * visc * d / den;
-          fv1 = dv1 * vb.w * visc * d / den;
-          fv2 = dv2 * vb.w * visc * d / den;
-        }
-        if (artif) {
-          double avsig = 0, avisc = 0.0;
-          if (vdr < 0) {
-            double rr = 0.0;
-            if (r != 0.0) rr = 1 / r;
-            const double viss = vdr * rr;
-            avsig = 2.0 * P.spsound - 4.0 * viss;
-            avisc = viss * avsig * P.alpha_artificial / (da.x + db.x);
-          }
-          const double s = vb.w * avisc;
-          f.x += s * gb0;
-          f.y += s * gb1;
-          f.z += s * gb2;
-          vsig = dmax2(vsig, avsig);
-        }
-        newDt = dmin2(newDt, 0.6 * hav / vsig);
-        newDt = dmin2(newDt, 0.125 * (hav * hav) / P.viscosity);
-        fva.x += fv0;
-        fva.y += fv1;
-        fva.z += fv2;
-        f.x += fv0;
-        f.y += fv1;
-        f.z += fv2;
-      }
-      if (tb == T_SPH || tb == T_SPHB) {
-        const double antic = 0.01 * fabs(da.y + db.y) * 0.0;  // NO_ANTICLUMPING: kdwPowNeps = 0
-        double fp0 = da.y * gWa[0] + db.y * gWb[0];
-        double fp1 = da.y * gWa[1] + db.y * gWb[1];
-        double fp2 = da.y * gWa[2] + db.y * gWb[2];
-        fp0 = -vb.w * (fp0 + antic);
-        fp1 = -vb.w * (fp1 + antic);
-        fp2 = -vb.w * (fp2 + antic);
-        fpa.x += fp0;
-        fpa.y += fp1;
-        fpa.z += fp2;
-        f.x += fp0;
-        f.y += fp1;
-        f.z += fp2;
-      }
-    });
-    S.frc[a] = f;
-    S.fp[a] = fpa;
-    S.fv[a] = fva;
-  }
-  atomicMinPos(&g->newDt_bits, newDt);
-}
-
 // Nsph::accelCondition sph.h:566-568 over sph, sphBoundary and dem
 __global__ void k_accel(Store S, int n, DevGlobals* g) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
@@ -444,44 +332,6 @@ __global__ void k_accel(Store S, int n, DevGlobals* g) {
   atomicMinPos(&g->newDt_bits, v);
 }
 
-// CsphIncompress::calcDddtDudt sphIncompress.h:783-851
-__global__ void k_dddt(Store S, int ns, Grid gs, const int* __restrict__ start, Box B,
-                       sphdem_params P) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= ns || !isSphT(S.iam[a])) return;
-  const double4 pa = S.posh[a];
-  const double4 va = S.vhm[a];
-  const double om = S.dpo[a].z;
-  const double ha = pa.w;
-  const bool varh = (P.flags & SPHDEM_F_VAR_H_CORRECTION) != 0;
-  double dddt = S.vh0d[a].w;
-  forCells(gs, start, B, pa.x, pa.y, pa.z, 1, [&](int b, int sx, int sy, int sz) {
-    if (b == a && (sx | sy | sz) == 0) return;
-    const double4 pb = S.posh[b];
-    const Img im = imagePos(pb, sx, sy, sz, B);
-    const double dx = pa.x - im.x, dy = pa.y - im.y, dz = pa.z - im.z;
-    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
-    const double hb = pb.w;
-    const double lim = 2.0 * dmax2(ha, hb);
-    if (!(l2 <= lim * lim)) return;
-    const double4 vb = S.vhm[b];
-    const double dv0 = va.x - vb.x, dv1 = va.y - vb.y, dv2 = va.z - vb.z;
-    const double r = sqrt(l2);
-    double gW[3];
-    if (varh) {
-      gradW(P.wcon, true, om, dx, dy, dz, r / ha, ha, gW);
-    } else {
-      double g1[3], g2[3];
-      gradW(P.wcon, false, om, dx, dy, dz, r / ha, ha, g1);
-      gradW(P.wcon, false, om, dx, dy, dz, r / hb, hb, g2);
-      for (int k = 0; k < 3; k++) gW[k] = 0.5 * (g1[k] + g2[k]);
-    }
-    const double vdr = dot3(dv0, dv1, dv2, gW[0], gW[1], gW[2]);
-    dddt += vb.w * vdr;
-  });
-  S.vh0d[a].w = dddt;
-}
-
 // Nsph::calcHFromDens sph.h:1823-1825
 __global__ void k_hfromdens(Store S, int ns, double hfac, double* hsnap) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
@@ -489,30 +339,6 @@ __global__ void k_hfromdens(Store S, int ns, double hfac, double* hsnap) {
   const double h = hfac * pow_third(S.vhm[a].w / S.dpo[a].x);
   S.posh[a].w = h;
   if (hsnap) hsnap[a] = h;
-}
-
-// Nsph::calcDensity sph.h:1149-1191 (kernel-sum branch)
-__global__ void k_density(Store S, int ns, Grid gs, const int* __restrict__ start, Box B,
-                          const double* __restrict__ hq, double wcon) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= ns || !isSphT(S.iam[a])) return;
-  const double4 pa = S.posh[a];
-  const double ha = pa.w;
-  const double hqa = hq[a];
-  double rho = S.vhm[a].w * wendW(wcon, 0, ha);
-  forCells(gs, start, B, pa.x, pa.y, pa.z, 1, [&](int b, int sx, int sy, int sz) {
-    if (b == a && (sx | sy | sz) == 0) return;
-    const double4 pb = S.posh[b];
-    const Img im = imagePos(pb, sx, sy, sz, B);
-    const double dx = pa.x - im.x, dy = pa.y - im.y, dz = pa.z - im.z;
-    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
-    const double lim = 2.0 * dmax2(hqa, hq[b]);
-    if (!(l2 <= lim * lim)) return;
-    if (!isSphT(S.iam[b])) return;
-    const double r = sqrt(l2);
-    rho += S.vhm[b].w * wendW(wcon, r / ha, ha);
-  });
-  S.dpo[a].x = rho;
 }
 
 // ============================================================ SPH <-> DEM coupling
@@ -563,10 +389,15 @@ __device__ __forceinline__ bool demImageExists(const CouplingRule& R, const Box&
 // Shepard sum of each DEM particle (calcShepSum, sph.h:1303-1320, a scatter
 // from SPH in the reference, followed by reverseSync) as a gather over the SPH
 // particles whose coupling list holds this DEM particle or one of its images.
-__global__ void k_shepsum(Store S, int ns, int nd, Grid gs, const int* __restrict__ start,
-                          Box B, CouplingRule R, int reach, double wcon, double Rc) {
-  const int j = ns + blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ns + nd) return;
+// One wavefront per DEM particle; the lanes split the candidates and the
+// partial sums meet in a fixed butterfly.
+__global__ void __launch_bounds__(256) k_shepsum(Store S, int ns, int nd, Grid gs,
+                                                 const int* __restrict__ start, Box B,
+                                                 CouplingRule R, int reach, double wcon,
+                                                 double Rc) {
+  const int lane = threadIdx.x & 63;
+  const int j = ns + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (j >= ns + nd) return;  // uniform per wavefront
   const double4 pj = S.posh[j];
   double sum = 0.0;
   // candidates a see the image r_j + s L; enumerate them around that image
@@ -592,9 +423,10 @@ __global__ void k_shepsum(Store S, int ns, int nd, Grid gs, const int* __restric
           const double dv = S.vhm[a].w / S.dpo[a].x;
           const double r = sqrt(l2);
           sum += dv * wendW(wcon, r * (2.0 / Rc), Rc / 2.0);
-        });
+        }, lane, 64);
       }
-  S.fp[j].w = sum;
+  sum = waveSum(sum);
+  if (lane == 0) S.fp[j].w = sum;
 }
 
 // SPH side of middle()'s coupling block (sphIncompress.cpp:64-78): calcPorosity
@@ -642,10 +474,13 @@ __global__ void k_coupling_sph(Store S, int ns, int nd, Grid gc, const int* __re
 // calcFPI (sph.h:1708-1792) with calcParticleDrag (sph.h:1575-1631, Di Felice):
 // each DEM particle gathers its SPH partners (the 27 SPH cells around it,
 // dataLL.cpp:1106-1126).
-__global__ void k_fpi(Store S, int ns, int nd, Grid gs, const int* __restrict__ start, Box B,
-                      sphdem_params P) {
-  const int j = ns + blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ns + nd) return;
+// One wavefront per DEM particle, as k_shepsum.
+__global__ void __launch_bounds__(256) k_fpi(Store S, int ns, int nd, Grid gs,
+                                             const int* __restrict__ start, Box B,
+                                             sphdem_params P) {
+  const int lane = threadIdx.x & 63;
+  const int j = ns + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (j >= ns + nd) return;  // uniform per wavefront
   const double4 pj = S.posh[j];
   const double4 vj = S.vhm[j];
   double sum = 0, fdens = 0, por = 0;
@@ -689,7 +524,18 @@ __global__ void k_fpi(Store S, int ns, int nd, Grid gs, const int* __restrict__ 
     por += dvWab * db.w;
     fdens += Wab * vb.w;
     sum += dvWab;
-  });
+  }, lane, 64);
+  sum = waveSum(sum);
+  fdens = waveSum(fdens);
+  por = waveSum(por);
+  for (int k = 0; k < 3; k++) {
+    ff[k] = waveSum(ff[k]);
+    fpfb[k] = waveSum(fpfb[k]);
+    fv[k] = waveSum(fv[k]);
+    fdr[k] = waveSum(fdr[k]);
+    fvel[k] = waveSum(fvel[k]);
+  }
+  if (lane != 0) return;
   double4 fdrag_out = make_double4(0.0, 0.0, 0.0, 0.0);
   double4 f = S.frc[j];
   if (sum > 0.5) {
@@ -1052,12 +898,22 @@ __global__ void k_sum_drag(Store S, int n, double* partial) {
   if (threadIdx.x == 0)
     for (int k = 0; k < 6; k++) partial[6 * blockIdx.x + k] = red[k][0];
 }
-__global__ void k_sum_drag_final(const double* partial, int nb, DevGlobals* g) {
-  if (threadIdx.x != 0) return;
+// one 256-thread block: strided partial sums in a fixed order, then a tree
+__global__ void __launch_bounds__(256) k_sum_drag_final(const double* partial, int nb,
+                                                        DevGlobals* g) {
+  __shared__ double red[6][256];
   double s[6] = {0, 0, 0, 0, 0, 0};
-  for (int b = 0; b < nb; b++)
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
     for (int k = 0; k < 6; k++) s[k] += partial[6 * b + k];
-  for (int k = 0; k < 6; k++) g->custom[k] = s[k];
+  for (int k = 0; k < 6; k++) red[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int k = 0; k < 6; k++) red[k][threadIdx.x] += red[k][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 6; k++) g->custom[k] = red[k][0];
 }
 
 // ============================================================ parity helpers

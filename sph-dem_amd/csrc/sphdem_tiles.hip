@@ -1,0 +1,548 @@
+// sphdem_tiles.hip -- SPH pair passes of the gfx950 loop as wave-uniform,
+// LDS-staged candidate tiles.
+//
+// Work unit: one 64-lane wavefront per Segment (whole z-adjacent cells of one
+// (x,y) column of the SPH grid, at most 64 particles; lane = target particle).
+// The wavefront walks the candidate columns of the segment (the 3x3 columns
+// around it, z range extended by one cell, plus periodic images), loads each
+// column range 64 particles at a time into its own LDS slice (coalesced reads
+// of the cell-sorted store) and every lane tests all staged candidates against
+// its particle.  Per lane a candidate is admitted only if the cell of its
+// (image) position lies within +-1 of the lane's cell on every axis -- the
+// reference's 27-cell stencil (dataLL.cpp:1049-1071) -- and passes
+// Cparticle::is_neighbr (particle.h:236-238).  Columns are walked x-outer,
+// then y, then z, particles in original-index order, so each lane sums its
+// neighbours in the order of the reference's cached list.
+#pragma once
+
+namespace sphdem {
+
+constexpr int kWavesPerBlock = 4;
+
+__device__ __forceinline__ void waveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---- segments -------------------------------------------------------------------
+// Greedy packing of one column's cells into segments of <= 64 particles.
+template <class E>
+__device__ __forceinline__ void walkColumn(const int* __restrict__ start, const Grid& g, int col,
+                                           E&& emit) {
+  const int base = g.base + col * g.dim[2];
+  int z = 0;
+  while (z < g.dim[2]) {
+    const int c0 = start[base + z], c1 = start[base + z + 1];
+    if (c1 == c0) {
+      z++;
+      continue;
+    }
+    if (c1 - c0 > 64) {  // an over-full cell is split into several segments
+      for (int p = c0; p < c1; p += 64) emit(p, min(p + 64, c1), z, z);
+      z++;
+      continue;
+    }
+    const int z0 = z;
+    int last = z, acc = c1 - c0;
+    z++;
+    while (z < g.dim[2]) {
+      const int k = start[base + z + 1] - start[base + z];
+      if (acc + k > 64) break;
+      acc += k;
+      if (k) last = z;
+      z++;
+    }
+    emit(c0, start[base + last + 1], z0, last);
+  }
+}
+__global__ void k_seg_count(const int* __restrict__ start, Grid g, int* __restrict__ cnt) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= g.dim[0] * g.dim[1]) return;
+  int c = 0;
+  walkColumn(start, g, col, [&](int, int, int, int) { c++; });
+  cnt[col] = c;
+}
+__global__ void k_seg_write(const int* __restrict__ start, Grid g, const int* __restrict__ off,
+                            Segment* __restrict__ segs) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= g.dim[0] * g.dim[1]) return;
+  int w = off[col];
+  walkColumn(start, g, col, [&](int p0, int p1, int z0, int z1) {
+    segs[w++] = Segment{p0, p1, col, z0 | (z1 << 16)};
+  });
+}
+
+// Real-cell range along one axis holding every particle whose image (shift s)
+// can fall into the target cells [lo-1, hi+1]; a superset, clipped to the grid.
+__device__ __forceinline__ void axisRange(const Grid& g, const Box& B, int ax, int s, int lo,
+                                          int hi, int& r0, int& r1) {
+  if (s == 0) {
+    r0 = lo - 1;
+    r1 = hi + 1;
+  } else {
+    const double L = (s == 1) ? B.Lp[ax] : B.Lm[ax];
+    const double xlo = g.cmin[ax] + ((lo - 1) - g.off) * g.edge - L;
+    const double xhi = g.cmin[ax] + ((hi + 2) - g.off) * g.edge - L;
+    r0 = cellOf(xlo, g.cmin[ax], g.edge, g.off) - 1;
+    r1 = cellOf(xhi, g.cmin[ax], g.edge, g.off) + 1;
+  }
+  r0 = max(r0, 0);
+  r1 = min(r1, g.dim[ax] - 1);
+}
+
+// Staged candidate common part.
+struct TileBase {
+  double x[64], y[64], z[64];
+  int cell[64];  // cell of the (image) position
+  int self[64];  // index in the store, or -1 for a periodic image
+};
+__device__ __forceinline__ void stageBase(TileBase& T, int l, const Store& S, const Grid& g,
+                                          const Box& B, int j, int sx, int sy, int sz,
+                                          double4& p) {
+  p = S.posh[j];
+  if ((sx | sy | sz) == 0) {
+    T.x[l] = p.x;
+    T.y[l] = p.y;
+    T.z[l] = p.z;
+    T.cell[l] = S.cell[j];
+    T.self[l] = j;
+  } else {
+    const Img im = imagePos(p, sx, sy, sz, B);
+    T.x[l] = im.x;
+    T.y[l] = im.y;
+    T.z[l] = im.z;
+    T.cell[l] = packCell(cellOf(im.x, g.cmin[0], g.edge, g.off),
+                         cellOf(im.y, g.cmin[1], g.edge, g.off),
+                         cellOf(im.z, g.cmin[2], g.edge, g.off));
+    T.self[l] = -1;
+  }
+}
+__device__ __forceinline__ bool inStencil(int a, int b) {
+  return abs(cellX(a) - cellX(b)) <= 1 && abs(cellY(a) - cellY(b)) <= 1 &&
+         abs(cellZ(a) - cellZ(b)) <= 1;
+}
+
+// The driver shared by every pass.  Each wavefront takes a contiguous run of
+// segments; consecutive workgroups land on different XCDs, so workgroup b is
+// renumbered to give each XCD one contiguous (spatially coherent) slab.
+template <class Op>
+__global__ void __launch_bounds__(256) k_tile(Store S, Grid g, const int* __restrict__ start,
+                                              Box B, const Segment* __restrict__ segs,
+                                              const int* __restrict__ nsegPtr, Op op) {
+  __shared__ typename Op::Tile tiles[kWavesPerBlock];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  typename Op::Tile& T = tiles[w];
+  const int nseg = *nsegPtr;
+  const int nb = gridDim.x;  // a multiple of 8
+  const int vb = (blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3);
+  const int nw = nb * kWavesPerBlock;
+  const int gw = vb * kWavesPerBlock + w;
+  const int chunk = (nseg + nw - 1) / nw;
+  const int s0 = gw * chunk, s1 = min(s0 + chunk, nseg);
+  typename Op::Acc acc;
+  op.begin(acc);
+  for (int si = s0; si < s1; si++) {
+    const Segment sg = segs[si];
+    const int t = sg.p0 + lane;
+    const bool act = t < sg.p1;
+    const int cx = sg.col / g.dim[1], cy = sg.col - cx * g.dim[1];
+    const int z0 = sg.zz & 0xffff, z1 = sg.zz >> 16;
+    typename Op::Lane L;
+    int tcell = 0;
+    if (act) {
+      op.load(S, t, L);
+      tcell = S.cell[t];
+    }
+    const int nsx = B.periodic[0] ? 3 : 1, nsy = B.periodic[1] ? 3 : 1, nsz = B.periodic[2] ? 3 : 1;
+    for (int ix = 0; ix < nsx; ix++) {
+      const int sx = ix == 0 ? 0 : (ix == 1 ? -1 : 1);
+      int X0, X1;
+      axisRange(g, B, 0, sx, cx, cx, X0, X1);
+      if (X0 > X1) continue;
+      for (int iy = 0; iy < nsy; iy++) {
+        const int sy = iy == 0 ? 0 : (iy == 1 ? -1 : 1);
+        int Y0, Y1;
+        axisRange(g, B, 1, sy, cy, cy, Y0, Y1);
+        if (Y0 > Y1) continue;
+        for (int iz = 0; iz < nsz; iz++) {
+          const int sz = iz == 0 ? 0 : (iz == 1 ? -1 : 1);
+          int Z0, Z1;
+          axisRange(g, B, 2, sz, z0, z1, Z0, Z1);
+          if (Z0 > Z1) continue;
+          for (int X = X0; X <= X1; X++)
+            for (int Y = Y0; Y <= Y1; Y++) {
+              const int k0 = g.base + (X * g.dim[1] + Y) * g.dim[2];
+              const int j0 = start[k0 + Z0], j1 = start[k0 + Z1 + 1];
+              for (int b0 = j0; b0 < j1; b0 += 64) {
+                const int m = min(64, j1 - b0);
+                waveSync();
+                if (lane < m) op.stage(S, g, B, b0 + lane, sx, sy, sz, T, lane);
+                waveSync();
+                if (act) {
+                  for (int k = 0; k < m; k++) {
+                    if (!inStencil(tcell, T.cell[k])) continue;
+                    if (T.self[k] == t) continue;
+                    op.pair(L, T, k);
+                  }
+                }
+              }
+            }
+        }
+      }
+    }
+    if (act) op.store(S, t, L, acc);
+  }
+  op.end(acc);
+}
+
+// ---- calcOmega (sph.h:437-452) ---------------------------------------------------
+struct OmegaOp {
+  const double* hq;  // h of the cached lists (null: current h)
+  double wcon;
+  struct Tile : TileBase {
+    double hq[64], m[64];
+    int iam[64];
+  };
+  struct Lane {
+    double x, y, z, h, hq, om;
+  };
+  struct Acc {};
+  __device__ void begin(Acc&) const {}
+  __device__ void end(Acc&) const {}
+  __device__ void load(const Store& S, int t, Lane& L) const {
+    const double4 p = S.posh[t];
+    L.x = p.x;
+    L.y = p.y;
+    L.z = p.z;
+    L.h = p.w;
+    L.hq = hq ? hq[t] : p.w;
+    L.om = S.vhm[t].w * wendPdWh(wcon, 0, L.h);
+  }
+  __device__ void stage(const Store& S, const Grid& g, const Box& B, int j, int sx, int sy, int sz,
+                        Tile& T, int l) const {
+    double4 p;
+    stageBase(T, l, S, g, B, j, sx, sy, sz, p);
+    T.hq[l] = hq ? hq[j] : p.w;
+    T.m[l] = S.vhm[j].w;
+    T.iam[l] = S.iam[j];
+  }
+  __device__ void pair(Lane& L, const Tile& T, int k) const {
+    const double dx = L.x - T.x[k], dy = L.y - T.y[k], dz = L.z - T.z[k];
+    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+    const double lim = 2.0 * dmax2(L.hq, T.hq[k]);
+    if (!(l2 <= lim * lim)) return;
+    if (isSphT(T.iam[k])) {
+      const double r = sqrt(l2);
+      L.om += T.m[k] * wendPdWh(wcon, r / L.h, L.h);
+    }
+  }
+  __device__ void store(const Store& S, int t, Lane& L, Acc&) const {
+    S.dpo[t].z = 1 + L.h * L.om / (S.dpo[t].x * 3);
+  }
+};
+
+// ---- calcDensity (sph.h:1149-1191) ------------------------------------------------
+struct DensityOp {
+  const double* hq;
+  double wcon;
+  struct Tile : TileBase {
+    double hq[64], m[64];
+    int iam[64];
+  };
+  struct Lane {
+    double x, y, z, h, hq, rho;
+  };
+  struct Acc {};
+  __device__ void begin(Acc&) const {}
+  __device__ void end(Acc&) const {}
+  __device__ void load(const Store& S, int t, Lane& L) const {
+    const double4 p = S.posh[t];
+    L.x = p.x;
+    L.y = p.y;
+    L.z = p.z;
+    L.h = p.w;
+    L.hq = hq[t];
+    L.rho = S.vhm[t].w * wendW(wcon, 0, L.h);
+  }
+  __device__ void stage(const Store& S, const Grid& g, const Box& B, int j, int sx, int sy, int sz,
+                        Tile& T, int l) const {
+    double4 p;
+    stageBase(T, l, S, g, B, j, sx, sy, sz, p);
+    T.hq[l] = hq[j];
+    T.m[l] = S.vhm[j].w;
+    T.iam[l] = S.iam[j];
+  }
+  __device__ void pair(Lane& L, const Tile& T, int k) const {
+    const double dx = L.x - T.x[k], dy = L.y - T.y[k], dz = L.z - T.z[k];
+    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+    const double lim = 2.0 * dmax2(L.hq, T.hq[k]);
+    if (!(l2 <= lim * lim)) return;
+    if (!isSphT(T.iam[k])) return;
+    const double r = sqrt(l2);
+    L.rho += T.m[k] * wendW(wcon, r / L.h, L.h);
+  }
+  __device__ void store(const Store& S, int t, Lane& L, Acc&) const { S.dpo[t].x = L.rho; }
+};
+
+// ---- calcDddtDudt (sphIncompress.h:783-851) --------------------------------------
+struct DddtOp {
+  double wcon;
+  bool varh;
+  struct Tile : TileBase {
+    double h[64], vx[64], vy[64], vz[64], m[64];
+  };
+  struct Lane {
+    double x, y, z, h, vx, vy, vz, om, dddt;
+  };
+  struct Acc {};
+  __device__ void begin(Acc&) const {}
+  __device__ void end(Acc&) const {}
+  __device__ void load(const Store& S, int t, Lane& L) const {
+    const double4 p = S.posh[t];
+    const double4 v = S.vhm[t];
+    L.x = p.x;
+    L.y = p.y;
+    L.z = p.z;
+    L.h = p.w;
+    L.vx = v.x;
+    L.vy = v.y;
+    L.vz = v.z;
+    L.om = S.dpo[t].z;
+    L.dddt = S.vh0d[t].w;
+  }
+  __device__ void stage(const Store& S, const Grid& g, const Box& B, int j, int sx, int sy, int sz,
+                        Tile& T, int l) const {
+    double4 p;
+    stageBase(T, l, S, g, B, j, sx, sy, sz, p);
+    const double4 v = S.vhm[j];
+    T.h[l] = p.w;
+    T.vx[l] = v.x;
+    T.vy[l] = v.y;
+    T.vz[l] = v.z;
+    T.m[l] = v.w;
+  }
+  __device__ void pair(Lane& L, const Tile& T, int k) const {
+    const double dx = L.x - T.x[k], dy = L.y - T.y[k], dz = L.z - T.z[k];
+    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+    const double hb = T.h[k];
+    const double lim = 2.0 * dmax2(L.h, hb);
+    if (!(l2 <= lim * lim)) return;
+    const double dv0 = L.vx - T.vx[k], dv1 = L.vy - T.vy[k], dv2 = L.vz - T.vz[k];
+    const double r = sqrt(l2);
+    double gW[3];
+    if (varh) {
+      gradW(wcon, true, L.om, dx, dy, dz, r / L.h, L.h, gW);
+    } else {
+      double g1[3], g2[3];
+      gradW(wcon, false, L.om, dx, dy, dz, r / L.h, L.h, g1);
+      gradW(wcon, false, L.om, dx, dy, dz, r / hb, hb, g2);
+      for (int c = 0; c < 3; c++) gW[c] = 0.5 * (g1[c] + g2[c]);
+    }
+    const double vdr = dot3(dv0, dv1, dv2, gW[0], gW[1], gW[2]);
+    L.dddt += T.m[k] * vdr;
+  }
+  __device__ void store(const Store& S, int t, Lane& L, Acc&) const { S.vh0d[t].w = L.dddt; }
+};
+
+// ---- calcForce (sphIncompress.h:899-972; sph.h:579-666, 692-734) ------------------
+struct ForceOp {
+  sphdem_params P;
+  DevGlobals* g;
+  struct Tile : TileBase {
+    double h[64], vx[64], vy[64], vz[64], m[64], dens[64], pdr2[64];
+    int iam[64];
+  };
+  struct Lane {
+    double x, y, z, h, vx, vy, vz, dens, pdr2, om;
+    int iam;
+    double f0, f1, f2, fp0, fp1, fp2, fv0, fv1, fv2, newDt;
+  };
+  struct Acc {
+    double newDt;
+  };
+  __device__ void begin(Acc& a) const { a.newDt = __longlong_as_double(0x7ff0000000000000ll); }
+  __device__ void end(Acc& a) const { atomicMinPos(&g->newDt_bits, a.newDt); }
+  __device__ void load(const Store& S, int t, Lane& L) const {
+    const double4 p = S.posh[t];
+    const double4 v = S.vhm[t];
+    const double4 d = S.dpo[t];
+    const double4 f = S.frc[t], fp = S.fp[t], fv = S.fv[t];
+    L.x = p.x;
+    L.y = p.y;
+    L.z = p.z;
+    L.h = p.w;
+    L.vx = v.x;
+    L.vy = v.y;
+    L.vz = v.z;
+    L.dens = d.x;
+    L.pdr2 = d.y;
+    L.om = d.z;
+    L.iam = S.iam[t];
+    L.f0 = f.x;
+    L.f1 = f.y;
+    L.f2 = f.z;
+    L.fp0 = fp.x;
+    L.fp1 = fp.y;
+    L.fp2 = fp.z;
+    L.fv0 = fv.x;
+    L.fv1 = fv.y;
+    L.fv2 = fv.z;
+    L.newDt = __longlong_as_double(0x7ff0000000000000ll);
+  }
+  __device__ void stage(const Store& S, const Grid& gg, const Box& B, int j, int sx, int sy,
+                        int sz, Tile& T, int l) const {
+    double4 p;
+    stageBase(T, l, S, gg, B, j, sx, sy, sz, p);
+    const double4 v = S.vhm[j];
+    const double4 d = S.dpo[j];
+    T.h[l] = p.w;
+    T.vx[l] = v.x;
+    T.vy[l] = v.y;
+    T.vz[l] = v.z;
+    T.m[l] = v.w;
+    T.dens[l] = d.x;
+    T.pdr2[l] = d.y;
+    T.iam[l] = S.iam[j];
+  }
+  __device__ void pair(Lane& L, const Tile& T, int k) const {
+    const double dx = L.x - T.x[k], dy = L.y - T.y[k], dz = L.z - T.z[k];
+    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+    const double hb = T.h[k];
+    const double lim = 2.0 * dmax2(L.h, hb);
+    if (!(l2 <= lim * lim)) return;
+    const int tb = T.iam[k];
+    const double mb = T.m[k], rhob = T.dens[k];
+    const double r = sqrt(l2);
+    const bool varh = (P.flags & SPHDEM_F_VAR_H_CORRECTION) != 0;
+    double gWa[3], gWb[3];
+    if (varh) {
+      gradW(P.wcon, true, L.om, dx, dy, dz, r / L.h, L.h, gWa);
+      gradW(P.wcon, true, L.om, dx, dy, dz, r / hb, hb, gWb);
+    } else {
+      double g1[3], g2[3];
+      gradW(P.wcon, false, L.om, dx, dy, dz, r / L.h, L.h, g1);
+      gradW(P.wcon, false, L.om, dx, dy, dz, r / hb, hb, g2);
+      for (int c = 0; c < 3; c++) gWa[c] = gWb[c] = 0.5 * (g1[c] + g2[c]);
+    }
+    if (L.iam == T_SPH && tb != T_DEM) {
+      const double dv0 = L.vx - T.vx[k], dv1 = L.vy - T.vy[k], dv2 = L.vz - T.vz[k];
+      const double gb0 = 0.5 * (gWa[0] + gWb[0]), gb1 = 0.5 * (gWa[1] + gWb[1]),
+                   gb2 = 0.5 * (gWa[2] + gWb[2]);
+      const double vdr = dot3(dx, dy, dz, dv0, dv1, dv2);
+      const double hav = 0.5 * (L.h + hb);
+      double fv0 = 0.0, fv1 = 0.0, fv2 = 0.0;
+      double vsig = 0.0;
+      if (P.flags & SPHDEM_F_VISC_MORRIS) {
+        double visc = P.viscosity * (L.dens + rhob) / (L.dens * rhob);
+        vsig = 2.0 * P.spsound;
+        if (P.flags & SPHDEM_F_LIQ_DEM) visc = 2.0 * P.viscosity * P.dens / (L.dens * rhob);
+        const double d = dot3(dx, dy, dz, gb0, gb1, gb2);
+        const double den = r * r + 0.01 * (hav * hav);
+        fv0 = dv0 * mb * visc * d / den;
+        fv1 = dv1 * mb * visc * d / den;
+        fv2 = dv2 * mb * visc * d / den;
+      }
+      if (P.flags & SPHDEM_F_VISC_ARTIFICIAL) {
+        double avsig = 0, avisc = 0.0;
+        if (vdr < 0) {
+          double rr = 0.0;
+          if (r != 0.0) rr = 1 / r;
+          const double viss = vdr * rr;
+          avsig = 2.0 * P.spsound - 4.0 * viss;
+          avisc = viss * avsig * P.alpha_artificial / (L.dens + rhob);
+        }
+        const double s = mb * avisc;
+        L.f0 += s * gb0;
+        L.f1 += s * gb1;
+        L.f2 += s * gb2;
+        vsig = dmax2(vsig, avsig);
+      }
+      L.newDt = dmin2(L.newDt, 0.6 * hav / vsig);
+      L.newDt = dmin2(L.newDt, 0.125 * (hav * hav) / P.viscosity);
+      L.fv0 += fv0;
+      L.fv1 += fv1;
+      L.fv2 += fv2;
+      L.f0 += fv0;
+      L.f1 += fv1;
+      L.f2 += fv2;
+    }
+    if (tb == T_SPH || tb == T_SPHB) {
+      const double antic = 0.01 * fabs(L.pdr2 + T.pdr2[k]) * 0.0;  // NO_ANTICLUMPING
+      double p0 = L.pdr2 * gWa[0] + T.pdr2[k] * gWb[0];
+      double p1 = L.pdr2 * gWa[1] + T.pdr2[k] * gWb[1];
+      double p2 = L.pdr2 * gWa[2] + T.pdr2[k] * gWb[2];
+      p0 = -mb * (p0 + antic);
+      p1 = -mb * (p1 + antic);
+      p2 = -mb * (p2 + antic);
+      L.fp0 += p0;
+      L.fp1 += p1;
+      L.fp2 += p2;
+      L.f0 += p0;
+      L.f1 += p1;
+      L.f2 += p2;
+    }
+  }
+  __device__ void store(const Store& S, int t, Lane& L, Acc& a) const {
+    S.frc[t] = make_double4(L.f0, L.f1, L.f2, S.frc[t].w);
+    S.fp[t] = make_double4(L.fp0, L.fp1, L.fp2, S.fp[t].w);
+    S.fv[t] = make_double4(L.fv0, L.fv1, L.fv2, S.fv[t].w);
+    a.newDt = dmin2(a.newDt, L.newDt);
+  }
+};
+
+// ---- the SPH pair set (parity): count, then write (a, b, image) triples -------------
+struct PairsOp {
+  int mode;                // 0 count, 1 write
+  int* cnt;                // per target particle
+  const long long* off;    // per target particle (mode 1)
+  int *oa, *ob, *oimg;
+  struct Tile : TileBase {
+    double h[64];
+    int orig[64], code[64];
+  };
+  struct Lane {
+    double x, y, z, h;
+    int orig, c;
+    long long w;
+  };
+  struct Acc {};
+  __device__ void begin(Acc&) const {}
+  __device__ void end(Acc&) const {}
+  __device__ void load(const Store& S, int t, Lane& L) const {
+    const double4 p = S.posh[t];
+    L.x = p.x;
+    L.y = p.y;
+    L.z = p.z;
+    L.h = p.w;
+    L.orig = S.orig[t];
+    L.c = 0;
+    L.w = mode == 1 ? off[t] : 0;
+  }
+  __device__ void stage(const Store& S, const Grid& g, const Box& B, int j, int sx, int sy, int sz,
+                        Tile& T, int l) const {
+    double4 p;
+    stageBase(T, l, S, g, B, j, sx, sy, sz, p);
+    T.h[l] = p.w;
+    T.orig[l] = S.orig[j];
+    T.code[l] = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+  }
+  __device__ void pair(Lane& L, const Tile& T, int k) const {
+    const double dx = L.x - T.x[k], dy = L.y - T.y[k], dz = L.z - T.z[k];
+    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+    const double lim = 2.0 * dmax2(L.h, T.h[k]);
+    if (!(l2 <= lim * lim)) return;
+    if (mode == 1) {
+      oa[L.w] = L.orig;
+      ob[L.w] = T.orig[k];
+      oimg[L.w] = T.code[k];
+      L.w++;
+    }
+    L.c++;
+  }
+  __device__ void store(const Store&, int t, Lane& L, Acc&) const {
+    if (mode == 0) cnt[t] = L.c;
+  }
+};
+
+}  // namespace sphdem
