@@ -32,6 +32,9 @@ struct sphdem_ctx {
   int *segCnt = nullptr, *segOff = nullptr;
   Segment* segs = nullptr;
   int segColCap = 0, segCap = 0;
+  // cached SPH neighbour lists (valid from their first use until the next reset)
+  int *nbl = nullptr, *nbcnt = nullptr;
+  bool listsValid = false;
   double* hsnap = nullptr;
   double* partial = nullptr;
   int partialCap = 0;
@@ -55,11 +58,13 @@ struct sphdem_ctx {
 namespace {
 enum KernelId {
   KID_FORCE = 0, KID_OMEGA, KID_DDDT, KID_COUPLING, KID_SHEPSUM, KID_FPI, KID_CONTACTS,
-  KID_EOS, KID_START, KID_KICK, KID_REST, KID_BIN, KID_RANK, KID_PERMUTE, KID_WRAP, KID_COUNT
+  KID_EOS, KID_START, KID_KICK, KID_REST, KID_BIN, KID_RANK, KID_PERMUTE, KID_WRAP, KID_LISTS,
+  KID_COUNT
 };
 const char* kKernelNames[KID_COUNT] = {"force", "omega", "dddt", "coupling", "shepsum",
                                        "fpi",   "contacts", "eos", "start", "kick",
-                                       "rest",  "bin",   "rank",  "permute", "wrap"};
+                                       "rest",  "bin",   "rank",  "permute", "wrap",
+                                       "lists"};
 constexpr size_t kMaxTimedEvents = 40000;
 
 bool recordEvent(sphdem_ctx* c) {
@@ -344,6 +349,8 @@ int doReset(sphdem_ctx* ctx) {
   HIPC(hipStreamSynchronize(ctx->stream));
   if (hg.errflags & 1u)
     return fail(ctx, SPHDEM_E_CAPACITY, "a particle fell outside the cell grid at the last reset");
+  if (hg.errflags & 2u)
+    return fail(ctx, SPHDEM_E_CAPACITY, "a particle had more than %d SPH neighbours", kMaxNb);
   ctx->ns = ctx->ns0 - (int)hg.ndel_sph;
   ctx->nd = ctx->nd0 - (int)hg.ndel_dem;
   double hmax;
@@ -382,7 +389,18 @@ int doReset(sphdem_ctx* ctx) {
   }
   std::swap(ctx->cur, ctx->alt);
   std::swap(ctx->storeMem[0], ctx->storeMem[1]);
+  ctx->listsValid = false;  // updateDomain clears every neighbour list (dataLL.cpp:374)
   return buildSegments(ctx);
+}
+
+// The cached lists are built by the first pair pass after a reset, with the h
+// of that moment (dataLL.impl.h:186-188).
+int ensureLists(sphdem_ctx* ctx) {
+  if (ctx->listsValid) return 0;
+  Timed tm(ctx, KID_LISTS);
+  CHECK(launchTiles(ctx, ListOp{nullptr, ctx->ns, ctx->nbl, ctx->nbcnt, ctx->dg}));
+  ctx->listsValid = true;
+  return 0;
 }
 
 int checkLoaded(sphdem_ctx* ctx) {
@@ -392,9 +410,13 @@ int checkLoaded(sphdem_ctx* ctx) {
 }
 
 // ---- passes
-int passOmega(sphdem_ctx* ctx, const double* hq) {
+int passOmega(sphdem_ctx* ctx) {
+  CHECK(ensureLists(ctx));
   Timed tm(ctx, KID_OMEGA);
-  return launchTiles(ctx, OmegaOp{hq, ctx->P.wcon});
+  hipLaunchKernelGGL(k_omega_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                     ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->P.wcon);
+  LAUNCHED();
+  return 0;
 }
 int passEos(sphdem_ctx* ctx) {
   Timed tm(ctx, KID_EOS);
@@ -404,12 +426,20 @@ int passEos(sphdem_ctx* ctx) {
   return 0;
 }
 int passForce(sphdem_ctx* ctx) {
+  CHECK(ensureLists(ctx));
   Timed tm(ctx, KID_FORCE);
-  return launchTiles(ctx, ForceOp{ctx->P, ctx->dg});
+  hipLaunchKernelGGL(k_force_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                     ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->P, ctx->dg);
+  LAUNCHED();
+  return 0;
 }
 int passDddt(sphdem_ctx* ctx) {
+  CHECK(ensureLists(ctx));
   Timed tm(ctx, KID_DDDT);
-  return launchTiles(ctx, DddtOp{ctx->P.wcon, varH(ctx)});
+  hipLaunchKernelGGL(k_dddt_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                     ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->P.wcon, varH(ctx) ? 1 : 0);
+  LAUNCHED();
+  return 0;
 }
 int passCoupling(sphdem_ctx* ctx) {
   if (ctx->nd > 0) {
@@ -443,11 +473,15 @@ int passContacts(sphdem_ctx* ctx) {
 }
 // one iteration of the density re-initialisation (sphIncompress.cpp:29-34);
 // `first` snapshots h as the cached neighbour lists are built then
-int passDensity(sphdem_ctx* ctx, bool first) {
+int passDensity(sphdem_ctx* ctx) {
   hipLaunchKernelGGL(k_hfromdens, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
-                     ctx->ns, ctx->P.hfac, first ? ctx->hsnap : (double*)nullptr);
+                     ctx->ns, ctx->P.hfac, (double*)nullptr);
   LAUNCHED();
-  return launchTiles(ctx, DensityOp{ctx->hsnap, ctx->P.wcon});
+  CHECK(ensureLists(ctx));  // the first iteration builds them with the new h
+  hipLaunchKernelGGL(k_density_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                     ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->P.wcon);
+  LAUNCHED();
+  return 0;
 }
 
 // ---- half-steps
@@ -461,7 +495,7 @@ int stepStart(sphdem_ctx* ctx) {
   }
   if ((ctx->sphStep % ctx->P.reinit_every == 0) || (ctx->sphStep == ctx->P.reinit_at)) {
     CHECK(doReset(ctx));
-    for (int i = 0; i < 10; i++) CHECK(passDensity(ctx, i == 0));
+    for (int i = 0; i < 10; i++) CHECK(passDensity(ctx));
   }
   return 0;
 }
@@ -470,7 +504,7 @@ int stepMiddle(sphdem_ctx* ctx) {
   LAUNCHED();
   CHECK(doReset(ctx));
   if (liqDem(ctx)) CHECK(passCoupling(ctx));
-  if (varH(ctx)) CHECK(passOmega(ctx, nullptr));
+  if (varH(ctx)) CHECK(passOmega(ctx));
   CHECK(passEos(ctx));
   CHECK(passForce(ctx));
   const int na = ctx->ns + ctx->nd;
@@ -490,7 +524,7 @@ int stepEnd(sphdem_ctx* ctx) {
                      ctx->cur, ctx->ns, ctx->dg);
   LAUNCHED();
   CHECK(doReset(ctx));
-  if (varH(ctx)) CHECK(passOmega(ctx, nullptr));
+  if (varH(ctx)) CHECK(passOmega(ctx));
   CHECK(passDddt(ctx));
   if (liqDem(ctx)) {
     CHECK(passFpi(ctx));
@@ -550,7 +584,7 @@ void freeAll(sphdem_ctx* ctx) {
   void* ptrs[] = {ctx->storeMem[0], ctx->storeMem[1], ctx->key,     ctx->slot,   ctx->tmp,
                   ctx->perm,        ctx->counts,      ctx->start,   ctx->tiles,  ctx->tileOff,
                   ctx->hsnap,       ctx->partial,     ctx->fieldbuf, ctx->errflag, ctx->dg,
-                  ctx->segCnt,      ctx->segOff,      ctx->segs};
+                  ctx->segCnt,      ctx->segOff,      ctx->segs,    ctx->nbl,    ctx->nbcnt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : ctx->evPool) (void)hipEventDestroy(e);
@@ -592,6 +626,8 @@ int sphdem_create(const sphdem_params* params, int device, int64_t capacity, sph
     HIPC(hipMalloc(&ctx->tmp, sizeof(int) * capacity));
     HIPC(hipMalloc(&ctx->perm, sizeof(int) * capacity));
     HIPC(hipMalloc(&ctx->hsnap, sizeof(double) * capacity));
+    HIPC(hipMalloc(&ctx->nbl, sizeof(int) * (size_t)kMaxNb * capacity));
+    HIPC(hipMalloc(&ctx->nbcnt, sizeof(int) * capacity));
     HIPC(hipMalloc(&ctx->fieldbuf, sizeof(double) * 3 * capacity));
     HIPC(hipMalloc(&ctx->errflag, sizeof(int)));
     HIPC(hipMalloc(&ctx->dg, sizeof(DevGlobals)));
@@ -932,7 +968,7 @@ int sphdem_get_pairs(sphdem_ctx* ctx, int kind, int64_t* count, int32_t* a, int3
 int sphdem_pass_omega(sphdem_ctx* ctx) {
   CHECK(checkLoaded(ctx));
   HIPC(hipSetDevice(ctx->device));
-  return passOmega(ctx, nullptr);
+  return passOmega(ctx);
 }
 int sphdem_pass_eos(sphdem_ctx* ctx) {
   CHECK(checkLoaded(ctx));
@@ -952,7 +988,7 @@ int sphdem_pass_dddt(sphdem_ctx* ctx) {
 int sphdem_pass_density(sphdem_ctx* ctx) {
   CHECK(checkLoaded(ctx));
   HIPC(hipSetDevice(ctx->device));
-  return passDensity(ctx, true);
+  return passDensity(ctx);
 }
 int sphdem_pass_porosity(sphdem_ctx* ctx) {
   CHECK(checkLoaded(ctx));

@@ -1,7 +1,7 @@
-// sphdem_tiles.hip -- SPH pair passes of the gfx950 loop as wave-uniform,
-// LDS-staged candidate tiles.
+// sphdem_tiles.hip -- SPH neighbour search of the gfx950 loop and the pair
+// passes that walk its cached lists.
 //
-// Work unit: one 64-lane wavefront per Segment (whole z-adjacent cells of one
+// Search: one 64-lane wavefront per Segment (whole z-adjacent cells of one
 // (x,y) column of the SPH grid, at most 64 particles; lane = target particle).
 // The wavefront walks the candidate columns of the segment (the 3x3 columns
 // around it, z range extended by one cell, plus periodic images), loads each
@@ -11,8 +11,12 @@
 // (image) position lies within +-1 of the lane's cell on every axis -- the
 // reference's 27-cell stencil (dataLL.cpp:1049-1071) -- and passes
 // Cparticle::is_neighbr (particle.h:236-238).  Columns are walked x-outer,
-// then y, then z, particles in original-index order, so each lane sums its
+// then y, then z, particles in original-index order, so each list holds the
 // neighbours in the order of the reference's cached list.
+//
+// Passes: one thread per particle walks its list (every lane busy with an
+// accepted pair, instead of the ~1/6 of lanes a divergent candidate test
+// leaves active), summing in list order.
 #pragma once
 
 namespace sphdem {
@@ -196,300 +200,267 @@ __global__ void __launch_bounds__(256) k_tile(Store S, Grid g, const int* __rest
   op.end(acc);
 }
 
-// ---- calcOmega (sph.h:437-452) ---------------------------------------------------
-struct OmegaOp {
-  const double* hq;  // h of the cached lists (null: current h)
-  double wcon;
-  struct Tile : TileBase {
-    double hq[64], m[64];
-    int iam[64];
-  };
-  struct Lane {
-    double x, y, z, h, hq, om;
-  };
-  struct Acc {};
-  __device__ void begin(Acc&) const {}
-  __device__ void end(Acc&) const {}
-  __device__ void load(const Store& S, int t, Lane& L) const {
-    const double4 p = S.posh[t];
-    L.x = p.x;
-    L.y = p.y;
-    L.z = p.z;
-    L.h = p.w;
-    L.hq = hq ? hq[t] : p.w;
-    L.om = S.vhm[t].w * wendPdWh(wcon, 0, L.h);
-  }
-  __device__ void stage(const Store& S, const Grid& g, const Box& B, int j, int sx, int sy, int sz,
-                        Tile& T, int l) const {
-    double4 p;
-    stageBase(T, l, S, g, B, j, sx, sy, sz, p);
-    T.hq[l] = hq ? hq[j] : p.w;
-    T.m[l] = S.vhm[j].w;
-    T.iam[l] = S.iam[j];
-  }
-  __device__ void pair(Lane& L, const Tile& T, int k) const {
-    const double dx = L.x - T.x[k], dy = L.y - T.y[k], dz = L.z - T.z[k];
-    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
-    const double lim = 2.0 * dmax2(L.hq, T.hq[k]);
-    if (!(l2 <= lim * lim)) return;
-    if (isSphT(T.iam[k])) {
-      const double r = sqrt(l2);
-      L.om += T.m[k] * wendPdWh(wcon, r / L.h, L.h);
-    }
-  }
-  __device__ void store(const Store& S, int t, Lane& L, Acc&) const {
-    S.dpo[t].z = 1 + L.h * L.om / (S.dpo[t].x * 3);
-  }
-};
+// ---- cached neighbour lists (CpInfo::neighbrs, built by calcNeighbours,
+// dataLL.cpp:1026-1075, and kept until the next reset, dataLL.impl.h:186-188) --
+// Entry-major layout: entry k of particle t at nbl[k * ns + t], so a wavefront
+// reading entry k of 64 consecutive particles reads 256 contiguous bytes.
+// An entry is the store index of the neighbour with its periodic image code in
+// the top bits.
+constexpr int kMaxNb = 256;
+constexpr int kEntryShift = 26;  // store index < 2^26
+__device__ __forceinline__ int nbIndex(int e) { return e & ((1 << kEntryShift) - 1); }
+__device__ __forceinline__ int nbCode(int e) { return (int)((unsigned)e >> kEntryShift); }
+__device__ __forceinline__ void codeShift(int code, int& sx, int& sy, int& sz) {
+  sx = code % 3 - 1;
+  sy = (code / 3) % 3 - 1;
+  sz = code / 9 - 1;
+}
 
-// ---- calcDensity (sph.h:1149-1191) ------------------------------------------------
-struct DensityOp {
-  const double* hq;
-  double wcon;
-  struct Tile : TileBase {
-    double hq[64], m[64];
-    int iam[64];
-  };
-  struct Lane {
-    double x, y, z, h, hq, rho;
-  };
-  struct Acc {};
-  __device__ void begin(Acc&) const {}
-  __device__ void end(Acc&) const {}
-  __device__ void load(const Store& S, int t, Lane& L) const {
-    const double4 p = S.posh[t];
-    L.x = p.x;
-    L.y = p.y;
-    L.z = p.z;
-    L.h = p.w;
-    L.hq = hq[t];
-    L.rho = S.vhm[t].w * wendW(wcon, 0, L.h);
-  }
-  __device__ void stage(const Store& S, const Grid& g, const Box& B, int j, int sx, int sy, int sz,
-                        Tile& T, int l) const {
-    double4 p;
-    stageBase(T, l, S, g, B, j, sx, sy, sz, p);
-    T.hq[l] = hq[j];
-    T.m[l] = S.vhm[j].w;
-    T.iam[l] = S.iam[j];
-  }
-  __device__ void pair(Lane& L, const Tile& T, int k) const {
-    const double dx = L.x - T.x[k], dy = L.y - T.y[k], dz = L.z - T.z[k];
-    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
-    const double lim = 2.0 * dmax2(L.hq, T.hq[k]);
-    if (!(l2 <= lim * lim)) return;
-    if (!isSphT(T.iam[k])) return;
-    const double r = sqrt(l2);
-    L.rho += T.m[k] * wendW(wcon, r / L.h, L.h);
-  }
-  __device__ void store(const Store& S, int t, Lane& L, Acc&) const { S.dpo[t].x = L.rho; }
-};
-
-// ---- calcDddtDudt (sphIncompress.h:783-851) --------------------------------------
-struct DddtOp {
-  double wcon;
-  bool varh;
-  struct Tile : TileBase {
-    double h[64], vx[64], vy[64], vz[64], m[64];
-  };
-  struct Lane {
-    double x, y, z, h, vx, vy, vz, om, dddt;
-  };
-  struct Acc {};
-  __device__ void begin(Acc&) const {}
-  __device__ void end(Acc&) const {}
-  __device__ void load(const Store& S, int t, Lane& L) const {
-    const double4 p = S.posh[t];
-    const double4 v = S.vhm[t];
-    L.x = p.x;
-    L.y = p.y;
-    L.z = p.z;
-    L.h = p.w;
-    L.vx = v.x;
-    L.vy = v.y;
-    L.vz = v.z;
-    L.om = S.dpo[t].z;
-    L.dddt = S.vh0d[t].w;
-  }
-  __device__ void stage(const Store& S, const Grid& g, const Box& B, int j, int sx, int sy, int sz,
-                        Tile& T, int l) const {
-    double4 p;
-    stageBase(T, l, S, g, B, j, sx, sy, sz, p);
-    const double4 v = S.vhm[j];
-    T.h[l] = p.w;
-    T.vx[l] = v.x;
-    T.vy[l] = v.y;
-    T.vz[l] = v.z;
-    T.m[l] = v.w;
-  }
-  __device__ void pair(Lane& L, const Tile& T, int k) const {
-    const double dx = L.x - T.x[k], dy = L.y - T.y[k], dz = L.z - T.z[k];
-    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
-    const double hb = T.h[k];
-    const double lim = 2.0 * dmax2(L.h, hb);
-    if (!(l2 <= lim * lim)) return;
-    const double dv0 = L.vx - T.vx[k], dv1 = L.vy - T.vy[k], dv2 = L.vz - T.vz[k];
-    const double r = sqrt(l2);
-    double gW[3];
-    if (varh) {
-      gradW(wcon, true, L.om, dx, dy, dz, r / L.h, L.h, gW);
-    } else {
-      double g1[3], g2[3];
-      gradW(wcon, false, L.om, dx, dy, dz, r / L.h, L.h, g1);
-      gradW(wcon, false, L.om, dx, dy, dz, r / hb, hb, g2);
-      for (int c = 0; c < 3; c++) gW[c] = 0.5 * (g1[c] + g2[c]);
-    }
-    const double vdr = dot3(dv0, dv1, dv2, gW[0], gW[1], gW[2]);
-    L.dddt += T.m[k] * vdr;
-  }
-  __device__ void store(const Store& S, int t, Lane& L, Acc&) const { S.vh0d[t].w = L.dddt; }
-};
-
-// ---- calcForce (sphIncompress.h:899-972; sph.h:579-666, 692-734) ------------------
-struct ForceOp {
-  sphdem_params P;
+struct ListOp {
+  const double* hq;  // h to build with (null: current h)
+  int ns;
+  int* nbl;
+  int* nbcnt;
   DevGlobals* g;
   struct Tile : TileBase {
-    double h[64], vx[64], vy[64], vz[64], m[64], dens[64], pdr2[64];
-    int iam[64];
+    double hq[64];
+    int ent[64];
   };
   struct Lane {
-    double x, y, z, h, vx, vy, vz, dens, pdr2, om;
-    int iam;
-    double f0, f1, f2, fp0, fp1, fp2, fv0, fv1, fv2, newDt;
+    double x, y, z, hq;
+    int t, c;
   };
-  struct Acc {
-    double newDt;
-  };
-  __device__ void begin(Acc& a) const { a.newDt = __longlong_as_double(0x7ff0000000000000ll); }
-  __device__ void end(Acc& a) const { atomicMinPos(&g->newDt_bits, a.newDt); }
+  struct Acc {};
+  __device__ void begin(Acc&) const {}
+  __device__ void end(Acc&) const {}
   __device__ void load(const Store& S, int t, Lane& L) const {
     const double4 p = S.posh[t];
-    const double4 v = S.vhm[t];
-    const double4 d = S.dpo[t];
-    const double4 f = S.frc[t], fp = S.fp[t], fv = S.fv[t];
     L.x = p.x;
     L.y = p.y;
     L.z = p.z;
-    L.h = p.w;
-    L.vx = v.x;
-    L.vy = v.y;
-    L.vz = v.z;
-    L.dens = d.x;
-    L.pdr2 = d.y;
-    L.om = d.z;
-    L.iam = S.iam[t];
-    L.f0 = f.x;
-    L.f1 = f.y;
-    L.f2 = f.z;
-    L.fp0 = fp.x;
-    L.fp1 = fp.y;
-    L.fp2 = fp.z;
-    L.fv0 = fv.x;
-    L.fv1 = fv.y;
-    L.fv2 = fv.z;
-    L.newDt = __longlong_as_double(0x7ff0000000000000ll);
+    L.hq = hq ? hq[t] : p.w;
+    L.t = t;
+    L.c = 0;
   }
-  __device__ void stage(const Store& S, const Grid& gg, const Box& B, int j, int sx, int sy,
-                        int sz, Tile& T, int l) const {
+  __device__ void stage(const Store& S, const Grid& gg, const Box& B, int j, int sx, int sy, int sz,
+                        Tile& T, int l) const {
     double4 p;
     stageBase(T, l, S, gg, B, j, sx, sy, sz, p);
-    const double4 v = S.vhm[j];
-    const double4 d = S.dpo[j];
-    T.h[l] = p.w;
-    T.vx[l] = v.x;
-    T.vy[l] = v.y;
-    T.vz[l] = v.z;
-    T.m[l] = v.w;
-    T.dens[l] = d.x;
-    T.pdr2[l] = d.y;
-    T.iam[l] = S.iam[j];
+    T.hq[l] = hq ? hq[j] : p.w;
+    T.ent[l] = j | (((sx + 1) + 3 * (sy + 1) + 9 * (sz + 1)) << kEntryShift);
   }
   __device__ void pair(Lane& L, const Tile& T, int k) const {
     const double dx = L.x - T.x[k], dy = L.y - T.y[k], dz = L.z - T.z[k];
     const double l2 = dot3(dx, dy, dz, dx, dy, dz);
-    const double hb = T.h[k];
-    const double lim = 2.0 * dmax2(L.h, hb);
+    const double lim = 2.0 * dmax2(L.hq, T.hq[k]);
     if (!(l2 <= lim * lim)) return;
-    const int tb = T.iam[k];
-    const double mb = T.m[k], rhob = T.dens[k];
-    const double r = sqrt(l2);
-    const bool varh = (P.flags & SPHDEM_F_VAR_H_CORRECTION) != 0;
-    double gWa[3], gWb[3];
-    if (varh) {
-      gradW(P.wcon, true, L.om, dx, dy, dz, r / L.h, L.h, gWa);
-      gradW(P.wcon, true, L.om, dx, dy, dz, r / hb, hb, gWb);
-    } else {
-      double g1[3], g2[3];
-      gradW(P.wcon, false, L.om, dx, dy, dz, r / L.h, L.h, g1);
-      gradW(P.wcon, false, L.om, dx, dy, dz, r / hb, hb, g2);
-      for (int c = 0; c < 3; c++) gWa[c] = gWb[c] = 0.5 * (g1[c] + g2[c]);
-    }
-    if (L.iam == T_SPH && tb != T_DEM) {
-      const double dv0 = L.vx - T.vx[k], dv1 = L.vy - T.vy[k], dv2 = L.vz - T.vz[k];
-      const double gb0 = 0.5 * (gWa[0] + gWb[0]), gb1 = 0.5 * (gWa[1] + gWb[1]),
-                   gb2 = 0.5 * (gWa[2] + gWb[2]);
-      const double vdr = dot3(dx, dy, dz, dv0, dv1, dv2);
-      const double hav = 0.5 * (L.h + hb);
-      double fv0 = 0.0, fv1 = 0.0, fv2 = 0.0;
-      double vsig = 0.0;
-      if (P.flags & SPHDEM_F_VISC_MORRIS) {
-        double visc = P.viscosity * (L.dens + rhob) / (L.dens * rhob);
-        vsig = 2.0 * P.spsound;
-        if (P.flags & SPHDEM_F_LIQ_DEM) visc = 2.0 * P.viscosity * P.dens / (L.dens * rhob);
-        const double d = dot3(dx, dy, dz, gb0, gb1, gb2);
-        const double den = r * r + 0.01 * (hav * hav);
-        fv0 = dv0 * mb * visc * d / den;
-        fv1 = dv1 * mb * visc * d / den;
-        fv2 = dv2 * mb * visc * d / den;
-      }
-      if (P.flags & SPHDEM_F_VISC_ARTIFICIAL) {
-        double avsig = 0, avisc = 0.0;
-        if (vdr < 0) {
-          double rr = 0.0;
-          if (r != 0.0) rr = 1 / r;
-          const double viss = vdr * rr;
-          avsig = 2.0 * P.spsound - 4.0 * viss;
-          avisc = viss * avsig * P.alpha_artificial / (L.dens + rhob);
-        }
-        const double s = mb * avisc;
-        L.f0 += s * gb0;
-        L.f1 += s * gb1;
-        L.f2 += s * gb2;
-        vsig = dmax2(vsig, avsig);
-      }
-      L.newDt = dmin2(L.newDt, 0.6 * hav / vsig);
-      L.newDt = dmin2(L.newDt, 0.125 * (hav * hav) / P.viscosity);
-      L.fv0 += fv0;
-      L.fv1 += fv1;
-      L.fv2 += fv2;
-      L.f0 += fv0;
-      L.f1 += fv1;
-      L.f2 += fv2;
-    }
-    if (tb == T_SPH || tb == T_SPHB) {
-      const double antic = 0.01 * fabs(L.pdr2 + T.pdr2[k]) * 0.0;  // NO_ANTICLUMPING
-      double p0 = L.pdr2 * gWa[0] + T.pdr2[k] * gWb[0];
-      double p1 = L.pdr2 * gWa[1] + T.pdr2[k] * gWb[1];
-      double p2 = L.pdr2 * gWa[2] + T.pdr2[k] * gWb[2];
-      p0 = -mb * (p0 + antic);
-      p1 = -mb * (p1 + antic);
-      p2 = -mb * (p2 + antic);
-      L.fp0 += p0;
-      L.fp1 += p1;
-      L.fp2 += p2;
-      L.f0 += p0;
-      L.f1 += p1;
-      L.f2 += p2;
-    }
+    if (L.c < kMaxNb) nbl[(size_t)L.c * ns + L.t] = T.ent[k];
+    L.c++;
   }
-  __device__ void store(const Store& S, int t, Lane& L, Acc& a) const {
-    S.frc[t] = make_double4(L.f0, L.f1, L.f2, S.frc[t].w);
-    S.fp[t] = make_double4(L.fp0, L.fp1, L.fp2, S.fp[t].w);
-    S.fv[t] = make_double4(L.fv0, L.fv1, L.fv2, S.fv[t].w);
-    a.newDt = dmin2(a.newDt, L.newDt);
+  __device__ void store(const Store&, int t, Lane& L, Acc&) const {
+    if (L.c > kMaxNb) {
+      atomicOr(&g->errflags, 2u);
+      L.c = kMaxNb;
+    }
+    nbcnt[t] = L.c;
   }
 };
+
+// Loads neighbour e of the list: store index and image position.
+__device__ __forceinline__ int nbLoad(const Store& S, const Box& B, int e, double4& p) {
+  const int j = nbIndex(e);
+  p = S.posh[j];
+  const int code = nbCode(e);
+  if (code != 13) {
+    int sx, sy, sz;
+    codeShift(code, sx, sy, sz);
+    const Img im = imagePos(p, sx, sy, sz, B);
+    p.x = im.x;
+    p.y = im.y;
+    p.z = im.z;
+  }
+  return j;
+}
+
+// ---- calcOmega (sph.h:437-452) over the cached list -----------------------------
+__global__ void __launch_bounds__(256) k_omega_l(Store S, int ns, Box B,
+                                                 const int* __restrict__ nbl,
+                                                 const int* __restrict__ nbcnt, double wcon) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= ns) return;
+  const double4 pa = S.posh[a];
+  const double ha = pa.w;
+  double om = S.vhm[a].w * wendPdWh(wcon, 0, ha);
+  const int n = nbcnt[a];
+  for (int k = 0; k < n; k++) {
+    double4 pb;
+    const int b = nbLoad(S, B, nbl[(size_t)k * ns + a], pb);
+    if (!isSphT(S.iam[b])) continue;
+    const double dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+    const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
+    om += S.vhm[b].w * wendPdWh(wcon, r / ha, ha);
+  }
+  S.dpo[a].z = 1 + ha * om / (S.dpo[a].x * 3);
+}
+
+// ---- calcDensity (sph.h:1149-1191) over the cached list -------------------------
+__global__ void __launch_bounds__(256) k_density_l(Store S, int ns, Box B,
+                                                   const int* __restrict__ nbl,
+                                                   const int* __restrict__ nbcnt, double wcon) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= ns) return;
+  const double4 pa = S.posh[a];
+  const double ha = pa.w;
+  double rho = S.vhm[a].w * wendW(wcon, 0, ha);
+  const int n = nbcnt[a];
+  for (int k = 0; k < n; k++) {
+    double4 pb;
+    const int b = nbLoad(S, B, nbl[(size_t)k * ns + a], pb);
+    if (!isSphT(S.iam[b])) continue;
+    const double dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+    const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
+    rho += S.vhm[b].w * wendW(wcon, r / ha, ha);
+  }
+  S.dpo[a].x = rho;
+}
+
+// ---- calcDddtDudt (sphIncompress.h:783-851) over the cached list ----------------
+__global__ void __launch_bounds__(256) k_dddt_l(Store S, int ns, Box B,
+                                                const int* __restrict__ nbl,
+                                                const int* __restrict__ nbcnt, double wcon,
+                                                int varh) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= ns) return;
+  const double4 pa = S.posh[a];
+  const double4 va = S.vhm[a];
+  const double ha = pa.w;
+  const double om = S.dpo[a].z;
+  double dddt = S.vh0d[a].w;
+  const int n = nbcnt[a];
+  for (int k = 0; k < n; k++) {
+    double4 pb;
+    const int b = nbLoad(S, B, nbl[(size_t)k * ns + a], pb);
+    const double4 vb = S.vhm[b];
+    const double dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+    const double dv0 = va.x - vb.x, dv1 = va.y - vb.y, dv2 = va.z - vb.z;
+    const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
+    double gW[3];
+    if (varh) {
+      gradW(wcon, true, om, dx, dy, dz, r / ha, ha, gW);
+    } else {
+      double g1[3], g2[3];
+      gradW(wcon, false, om, dx, dy, dz, r / ha, ha, g1);
+      gradW(wcon, false, om, dx, dy, dz, r / pb.w, pb.w, g2);
+      for (int c = 0; c < 3; c++) gW[c] = 0.5 * (g1[c] + g2[c]);
+    }
+    dddt += vb.w * dot3(dv0, dv1, dv2, gW[0], gW[1], gW[2]);
+  }
+  S.vh0d[a].w = dddt;
+}
+
+// ---- calcForce (sphIncompress.h:899-972; sph.h:579-666, 692-734) over the list --
+// The per-pair dt conditions (sph.h:545-560) are min-reduced into newDt_bits.
+__global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
+                                                 const int* __restrict__ nbl,
+                                                 const int* __restrict__ nbcnt, sphdem_params P,
+                                                 DevGlobals* g) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  double newDt = __longlong_as_double(0x7ff0000000000000ll);
+  if (a < ns) {
+    const int ta = S.iam[a];
+    const double4 pa = S.posh[a];
+    const double4 va = S.vhm[a];
+    const double4 da = S.dpo[a];
+    const double ha = pa.w;
+    const bool varh = (P.flags & SPHDEM_F_VAR_H_CORRECTION) != 0;
+    const bool morris = (P.flags & SPHDEM_F_VISC_MORRIS) != 0;
+    const bool artif = (P.flags & SPHDEM_F_VISC_ARTIFICIAL) != 0;
+    const bool liqdem = (P.flags & SPHDEM_F_LIQ_DEM) != 0;
+    double4 f = S.frc[a], fpa = S.fp[a], fva = S.fv[a];
+    const int n = nbcnt[a];
+    for (int k = 0; k < n; k++) {
+      double4 pb;
+      const int b = nbLoad(S, B, nbl[(size_t)k * ns + a], pb);
+      const int tb = S.iam[b];
+      const double4 vb = S.vhm[b];
+      const double4 db = S.dpo[b];
+      const double hb = pb.w;
+      const double dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+      const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
+      double gWa[3], gWb[3];
+      if (varh) {
+        gradW(P.wcon, true, da.z, dx, dy, dz, r / ha, ha, gWa);
+        gradW(P.wcon, true, da.z, dx, dy, dz, r / hb, hb, gWb);
+      } else {
+        double g1[3], g2[3];
+        gradW(P.wcon, false, da.z, dx, dy, dz, r / ha, ha, g1);
+        gradW(P.wcon, false, da.z, dx, dy, dz, r / hb, hb, g2);
+        for (int c = 0; c < 3; c++) gWa[c] = gWb[c] = 0.5 * (g1[c] + g2[c]);
+      }
+      if (ta == T_SPH && tb != T_DEM) {
+        const double dv0 = va.x - vb.x, dv1 = va.y - vb.y, dv2 = va.z - vb.z;
+        const double gb0 = 0.5 * (gWa[0] + gWb[0]), gb1 = 0.5 * (gWa[1] + gWb[1]),
+                     gb2 = 0.5 * (gWa[2] + gWb[2]);
+        const double vdr = dot3(dx, dy, dz, dv0, dv1, dv2);
+        const double hav = 0.5 * (ha + hb);
+        double fv0 = 0.0, fv1 = 0.0, fv2 = 0.0;
+        double vsig = 0.0;
+        if (morris) {
+          double visc = P.viscosity * (da.x + db.x) / (da.x * db.x);
+          vsig = 2.0 * P.spsound;
+          if (liqdem) visc = 2.0 * P.viscosity * P.dens / (da.x * db.x);
+          const double d = dot3(dx, dy, dz, gb0, gb1, gb2);
+          const double den = r * r + 0.01 * (hav * hav);
+          fv0 = dv0 * vb.w * visc * d / den;
+          fv1 = dv1 * vb.w * visc * d / den;
+          fv2 = dv2 * vb.w * visc * d / den;
+        }
+        if (artif) {
+          double avsig = 0, avisc = 0.0;
+          if (vdr < 0) {
+            double rr = 0.0;
+            if (r != 0.0) rr = 1 / r;
+            const double viss = vdr * rr;
+            avsig = 2.0 * P.spsound - 4.0 * viss;
+            avisc = viss * avsig * P.alpha_artificial / (da.x + db.x);
+          }
+          const double s = vb.w * avisc;
+          f.x += s * gb0;
+          f.y += s * gb1;
+          f.z += s * gb2;
+          vsig = dmax2(vsig, avsig);
+        }
+        newDt = dmin2(newDt, 0.6 * hav / vsig);
+        newDt = dmin2(newDt, 0.125 * (hav * hav) / P.viscosity);
+        fva.x += fv0;
+        fva.y += fv1;
+        fva.z += fv2;
+        f.x += fv0;
+        f.y += fv1;
+        f.z += fv2;
+      }
+      if (tb == T_SPH || tb == T_SPHB) {
+        const double antic = 0.01 * fabs(da.y + db.y) * 0.0;  // NO_ANTICLUMPING
+        double p0 = da.y * gWa[0] + db.y * gWb[0];
+        double p1 = da.y * gWa[1] + db.y * gWb[1];
+        double p2 = da.y * gWa[2] + db.y * gWb[2];
+        p0 = -vb.w * (p0 + antic);
+        p1 = -vb.w * (p1 + antic);
+        p2 = -vb.w * (p2 + antic);
+        fpa.x += p0;
+        fpa.y += p1;
+        fpa.z += p2;
+        f.x += p0;
+        f.y += p1;
+        f.z += p2;
+      }
+    }
+    S.frc[a] = f;
+    S.fp[a] = fpa;
+    S.fv[a] = fva;
+  }
+  atomicMinPos(&g->newDt_bits, newDt);
+}
 
 // ---- the SPH pair set (parity): count, then write (a, b, image) triples -------------
 struct PairsOp {
