@@ -127,9 +127,10 @@ __device__ __forceinline__ bool inStencil(int a, int b) {
          abs(cellZ(a) - cellZ(b)) <= 1;
 }
 
-// The driver shared by every pass.  Each wavefront takes a contiguous run of
-// segments; consecutive workgroups land on different XCDs, so workgroup b is
-// renumbered to give each XCD one contiguous (spatially coherent) slab.
+// The driver shared by every pass.  Consecutive workgroups land on different
+// XCDs (round robin over 8), so XCD x = blockIdx % 8 gets the x-th eighth of
+// the segments (a spatially coherent slab for its L2) and its wavefronts split
+// that slab into contiguous runs.
 template <class Op>
 __global__ void __launch_bounds__(256) k_tile(Store S, Grid g, const int* __restrict__ start,
                                               Box B, const Segment* __restrict__ segs,
@@ -138,12 +139,13 @@ __global__ void __launch_bounds__(256) k_tile(Store S, Grid g, const int* __rest
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   typename Op::Tile& T = tiles[w];
   const int nseg = *nsegPtr;
-  const int nb = gridDim.x;  // a multiple of 8
-  const int vb = (blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3);
-  const int nw = nb * kWavesPerBlock;
-  const int gw = vb * kWavesPerBlock + w;
-  const int chunk = (nseg + nw - 1) / nw;
-  const int s0 = gw * chunk, s1 = min(s0 + chunk, nseg);
+  const int xcd = blockIdx.x & 7;                          // gridDim.x is a multiple of 8
+  const int slab0 = (int)((long long)nseg * xcd / 8);
+  const int slab1 = (int)((long long)nseg * (xcd + 1) / 8);
+  const int wpx = (gridDim.x >> 3) * kWavesPerBlock;       // wavefronts per XCD
+  const int lw = (blockIdx.x >> 3) * kWavesPerBlock + w;
+  const int chunk = (slab1 - slab0 + wpx - 1) / wpx;
+  const int s0 = slab0 + lw * chunk, s1 = min(s0 + chunk, slab1);
   typename Op::Acc acc;
   op.begin(acc);
   for (int si = s0; si < s1; si++) {
@@ -184,10 +186,20 @@ __global__ void __launch_bounds__(256) k_tile(Store S, Grid g, const int* __rest
                 if (lane < m) op.stage(S, g, B, b0 + lane, sx, sy, sz, T, lane);
                 waveSync();
                 if (act) {
-                  for (int k = 0; k < m; k++) {
-                    if (!inStencil(tcell, T.cell[k])) continue;
-                    if (T.self[k] == t) continue;
-                    op.pair(L, T, k);
+                  // four staged candidates per step: the tests are evaluated
+                  // without branches so their LDS reads issue together, then
+                  // the admitted ones are taken in order
+                  for (int k = 0; k < m; k += 4) {
+                    bool ok[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                      const int kk = min(k + u, 63);
+                      ok[u] = (int)(k + u < m) & (int)inStencil(tcell, T.cell[kk]) &
+                              (int)(T.self[kk] != t) & (int)op.test(L, T, kk);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++)
+                      if (ok[u]) op.accept(L, T, k + u);
                   }
                 }
               }
@@ -249,11 +261,13 @@ struct ListOp {
     T.hq[l] = hq ? hq[j] : p.w;
     T.ent[l] = j | (((sx + 1) + 3 * (sy + 1) + 9 * (sz + 1)) << kEntryShift);
   }
-  __device__ void pair(Lane& L, const Tile& T, int k) const {
+  __device__ bool test(const Lane& L, const Tile& T, int k) const {
     const double dx = L.x - T.x[k], dy = L.y - T.y[k], dz = L.z - T.z[k];
     const double l2 = dot3(dx, dy, dz, dx, dy, dz);
     const double lim = 2.0 * dmax2(L.hq, T.hq[k]);
-    if (!(l2 <= lim * lim)) return;
+    return l2 <= lim * lim;
+  }
+  __device__ void accept(Lane& L, const Tile& T, int k) const {
     if (L.c < kMaxNb) nbl[(size_t)L.c * ns + L.t] = T.ent[k];
     L.c++;
   }
@@ -266,10 +280,8 @@ struct ListOp {
   }
 };
 
-// Loads neighbour e of the list: store index and image position.
-__device__ __forceinline__ int nbLoad(const Store& S, const Box& B, int e, double4& p) {
-  const int j = nbIndex(e);
-  p = S.posh[j];
+// Loads neighbour entry e: store index and (image) position.
+__device__ __forceinline__ void imageFix(const Box& B, int e, double4& p) {
   const int code = nbCode(e);
   if (code != 13) {
     int sx, sy, sz;
@@ -279,8 +291,15 @@ __device__ __forceinline__ int nbLoad(const Store& S, const Box& B, int e, doubl
     p.y = im.y;
     p.z = im.z;
   }
-  return j;
 }
+// Batched list walk: U entries are fetched, then their gathers are issued
+// together (one memory round trip per batch instead of two per neighbour), and
+// the arithmetic runs over the batch in list order.  Every neighbour in the
+// SPH group is sph or sphBoundary, so no type needs to be read.
+#define SPHDEM_FETCH_ENTRIES(U)                                                   \
+  int e[U];                                                                       \
+  _Pragma("unroll") for (int u = 0; u < U; u++) e[u] =                             \
+      (k0 + u < n) ? nbl[(size_t)(k0 + u) * ns + a] : (a | (13 << kEntryShift));
 
 // ---- calcOmega (sph.h:437-452) over the cached list -----------------------------
 __global__ void __launch_bounds__(256) k_omega_l(Store S, int ns, Box B,
@@ -292,13 +311,25 @@ __global__ void __launch_bounds__(256) k_omega_l(Store S, int ns, Box B,
   const double ha = pa.w;
   double om = S.vhm[a].w * wendPdWh(wcon, 0, ha);
   const int n = nbcnt[a];
-  for (int k = 0; k < n; k++) {
-    double4 pb;
-    const int b = nbLoad(S, B, nbl[(size_t)k * ns + a], pb);
-    if (!isSphT(S.iam[b])) continue;
-    const double dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-    const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
-    om += S.vhm[b].w * wendPdWh(wcon, r / ha, ha);
+  constexpr int U = 8;
+  for (int k0 = 0; k0 < n; k0 += U) {
+    SPHDEM_FETCH_ENTRIES(U)
+    double4 p[U];
+    double m[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int j = nbIndex(e[u]);
+      p[u] = S.posh[j];
+      m[u] = S.vhm[j].w;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (k0 + u >= n) break;
+      imageFix(B, e[u], p[u]);
+      const double dx = pa.x - p[u].x, dy = pa.y - p[u].y, dz = pa.z - p[u].z;
+      const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
+      om += m[u] * wendPdWh(wcon, r / ha, ha);
+    }
   }
   S.dpo[a].z = 1 + ha * om / (S.dpo[a].x * 3);
 }
@@ -313,13 +344,25 @@ __global__ void __launch_bounds__(256) k_density_l(Store S, int ns, Box B,
   const double ha = pa.w;
   double rho = S.vhm[a].w * wendW(wcon, 0, ha);
   const int n = nbcnt[a];
-  for (int k = 0; k < n; k++) {
-    double4 pb;
-    const int b = nbLoad(S, B, nbl[(size_t)k * ns + a], pb);
-    if (!isSphT(S.iam[b])) continue;
-    const double dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-    const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
-    rho += S.vhm[b].w * wendW(wcon, r / ha, ha);
+  constexpr int U = 8;
+  for (int k0 = 0; k0 < n; k0 += U) {
+    SPHDEM_FETCH_ENTRIES(U)
+    double4 p[U];
+    double m[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int j = nbIndex(e[u]);
+      p[u] = S.posh[j];
+      m[u] = S.vhm[j].w;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (k0 + u >= n) break;
+      imageFix(B, e[u], p[u]);
+      const double dx = pa.x - p[u].x, dy = pa.y - p[u].y, dz = pa.z - p[u].z;
+      const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
+      rho += m[u] * wendW(wcon, r / ha, ha);
+    }
   }
   S.dpo[a].x = rho;
 }
@@ -337,23 +380,34 @@ __global__ void __launch_bounds__(256) k_dddt_l(Store S, int ns, Box B,
   const double om = S.dpo[a].z;
   double dddt = S.vh0d[a].w;
   const int n = nbcnt[a];
-  for (int k = 0; k < n; k++) {
-    double4 pb;
-    const int b = nbLoad(S, B, nbl[(size_t)k * ns + a], pb);
-    const double4 vb = S.vhm[b];
-    const double dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-    const double dv0 = va.x - vb.x, dv1 = va.y - vb.y, dv2 = va.z - vb.z;
-    const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
-    double gW[3];
-    if (varh) {
-      gradW(wcon, true, om, dx, dy, dz, r / ha, ha, gW);
-    } else {
-      double g1[3], g2[3];
-      gradW(wcon, false, om, dx, dy, dz, r / ha, ha, g1);
-      gradW(wcon, false, om, dx, dy, dz, r / pb.w, pb.w, g2);
-      for (int c = 0; c < 3; c++) gW[c] = 0.5 * (g1[c] + g2[c]);
+  constexpr int U = 8;
+  for (int k0 = 0; k0 < n; k0 += U) {
+    SPHDEM_FETCH_ENTRIES(U)
+    double4 p[U], v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int j = nbIndex(e[u]);
+      p[u] = S.posh[j];
+      v[u] = S.vhm[j];
     }
-    dddt += vb.w * dot3(dv0, dv1, dv2, gW[0], gW[1], gW[2]);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (k0 + u >= n) break;
+      imageFix(B, e[u], p[u]);
+      const double dx = pa.x - p[u].x, dy = pa.y - p[u].y, dz = pa.z - p[u].z;
+      const double dv0 = va.x - v[u].x, dv1 = va.y - v[u].y, dv2 = va.z - v[u].z;
+      const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
+      double gW[3];
+      if (varh) {
+        gradW(wcon, true, om, dx, dy, dz, r / ha, ha, gW);
+      } else {
+        double g1[3], g2[3];
+        gradW(wcon, false, om, dx, dy, dz, r / ha, ha, g1);
+        gradW(wcon, false, om, dx, dy, dz, r / p[u].w, p[u].w, g2);
+        for (int c = 0; c < 3; c++) gW[c] = 0.5 * (g1[c] + g2[c]);
+      }
+      dddt += v[u].w * dot3(dv0, dv1, dv2, gW[0], gW[1], gW[2]);
+    }
   }
   S.vh0d[a].w = dddt;
 }
@@ -367,7 +421,7 @@ __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   double newDt = __longlong_as_double(0x7ff0000000000000ll);
   if (a < ns) {
-    const int ta = S.iam[a];
+    const bool fluid = S.iam[a] == T_SPH;  // viscosity only acts on sph (sphIncompress.h:927)
     const double4 pa = S.posh[a];
     const double4 va = S.vhm[a];
     const double4 da = S.dpo[a];
@@ -378,75 +432,85 @@ __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
     const bool liqdem = (P.flags & SPHDEM_F_LIQ_DEM) != 0;
     double4 f = S.frc[a], fpa = S.fp[a], fva = S.fv[a];
     const int n = nbcnt[a];
-    for (int k = 0; k < n; k++) {
-      double4 pb;
-      const int b = nbLoad(S, B, nbl[(size_t)k * ns + a], pb);
-      const int tb = S.iam[b];
-      const double4 vb = S.vhm[b];
-      const double4 db = S.dpo[b];
-      const double hb = pb.w;
-      const double dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-      const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
-      double gWa[3], gWb[3];
-      if (varh) {
-        gradW(P.wcon, true, da.z, dx, dy, dz, r / ha, ha, gWa);
-        gradW(P.wcon, true, da.z, dx, dy, dz, r / hb, hb, gWb);
-      } else {
-        double g1[3], g2[3];
-        gradW(P.wcon, false, da.z, dx, dy, dz, r / ha, ha, g1);
-        gradW(P.wcon, false, da.z, dx, dy, dz, r / hb, hb, g2);
-        for (int c = 0; c < 3; c++) gWa[c] = gWb[c] = 0.5 * (g1[c] + g2[c]);
+    constexpr int U = 4;
+    for (int k0 = 0; k0 < n; k0 += U) {
+      SPHDEM_FETCH_ENTRIES(U)
+      double4 p[U], v[U];
+      double2 d[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int j = nbIndex(e[u]);
+        p[u] = S.posh[j];
+        v[u] = S.vhm[j];
+        d[u] = *reinterpret_cast<const double2*>(&S.dpo[j]);  // dens, pdr2
       }
-      if (ta == T_SPH && tb != T_DEM) {
-        const double dv0 = va.x - vb.x, dv1 = va.y - vb.y, dv2 = va.z - vb.z;
-        const double gb0 = 0.5 * (gWa[0] + gWb[0]), gb1 = 0.5 * (gWa[1] + gWb[1]),
-                     gb2 = 0.5 * (gWa[2] + gWb[2]);
-        const double vdr = dot3(dx, dy, dz, dv0, dv1, dv2);
-        const double hav = 0.5 * (ha + hb);
-        double fv0 = 0.0, fv1 = 0.0, fv2 = 0.0;
-        double vsig = 0.0;
-        if (morris) {
-          double visc = P.viscosity * (da.x + db.x) / (da.x * db.x);
-          vsig = 2.0 * P.spsound;
-          if (liqdem) visc = 2.0 * P.viscosity * P.dens / (da.x * db.x);
-          const double d = dot3(dx, dy, dz, gb0, gb1, gb2);
-          const double den = r * r + 0.01 * (hav * hav);
-          fv0 = dv0 * vb.w * visc * d / den;
-          fv1 = dv1 * vb.w * visc * d / den;
-          fv2 = dv2 * vb.w * visc * d / den;
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (k0 + u >= n) break;
+        imageFix(B, e[u], p[u]);
+        const double hb = p[u].w, mb = v[u].w, rhob = d[u].x, pdr2b = d[u].y;
+        const double dx = pa.x - p[u].x, dy = pa.y - p[u].y, dz = pa.z - p[u].z;
+        const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
+        double gWa[3], gWb[3];
+        if (varh) {
+          gradW(P.wcon, true, da.z, dx, dy, dz, r / ha, ha, gWa);
+          gradW(P.wcon, true, da.z, dx, dy, dz, r / hb, hb, gWb);
+        } else {
+          double g1[3], g2[3];
+          gradW(P.wcon, false, da.z, dx, dy, dz, r / ha, ha, g1);
+          gradW(P.wcon, false, da.z, dx, dy, dz, r / hb, hb, g2);
+          for (int c = 0; c < 3; c++) gWa[c] = gWb[c] = 0.5 * (g1[c] + g2[c]);
         }
-        if (artif) {
-          double avsig = 0, avisc = 0.0;
-          if (vdr < 0) {
-            double rr = 0.0;
-            if (r != 0.0) rr = 1 / r;
-            const double viss = vdr * rr;
-            avsig = 2.0 * P.spsound - 4.0 * viss;
-            avisc = viss * avsig * P.alpha_artificial / (da.x + db.x);
+        if (fluid) {
+          const double dv0 = va.x - v[u].x, dv1 = va.y - v[u].y, dv2 = va.z - v[u].z;
+          const double gb0 = 0.5 * (gWa[0] + gWb[0]), gb1 = 0.5 * (gWa[1] + gWb[1]),
+                       gb2 = 0.5 * (gWa[2] + gWb[2]);
+          const double vdr = dot3(dx, dy, dz, dv0, dv1, dv2);
+          const double hav = 0.5 * (ha + hb);
+          double fv0 = 0.0, fv1 = 0.0, fv2 = 0.0;
+          double vsig = 0.0;
+          if (morris) {
+            double visc = P.viscosity * (da.x + rhob) / (da.x * rhob);
+            vsig = 2.0 * P.spsound;
+            if (liqdem) visc = 2.0 * P.viscosity * P.dens / (da.x * rhob);
+            const double dd = dot3(dx, dy, dz, gb0, gb1, gb2);
+            const double den = r * r + 0.01 * (hav * hav);
+            fv0 = dv0 * mb * visc * dd / den;
+            fv1 = dv1 * mb * visc * dd / den;
+            fv2 = dv2 * mb * visc * dd / den;
           }
-          const double s = vb.w * avisc;
-          f.x += s * gb0;
-          f.y += s * gb1;
-          f.z += s * gb2;
-          vsig = dmax2(vsig, avsig);
+          if (artif) {
+            double avsig = 0, avisc = 0.0;
+            if (vdr < 0) {
+              double rr = 0.0;
+              if (r != 0.0) rr = 1 / r;
+              const double viss = vdr * rr;
+              avsig = 2.0 * P.spsound - 4.0 * viss;
+              avisc = viss * avsig * P.alpha_artificial / (da.x + rhob);
+            }
+            const double s = mb * avisc;
+            f.x += s * gb0;
+            f.y += s * gb1;
+            f.z += s * gb2;
+            vsig = dmax2(vsig, avsig);
+          }
+          newDt = dmin2(newDt, 0.6 * hav / vsig);
+          newDt = dmin2(newDt, 0.125 * (hav * hav) / P.viscosity);
+          fva.x += fv0;
+          fva.y += fv1;
+          fva.z += fv2;
+          f.x += fv0;
+          f.y += fv1;
+          f.z += fv2;
         }
-        newDt = dmin2(newDt, 0.6 * hav / vsig);
-        newDt = dmin2(newDt, 0.125 * (hav * hav) / P.viscosity);
-        fva.x += fv0;
-        fva.y += fv1;
-        fva.z += fv2;
-        f.x += fv0;
-        f.y += fv1;
-        f.z += fv2;
-      }
-      if (tb == T_SPH || tb == T_SPHB) {
-        const double antic = 0.01 * fabs(da.y + db.y) * 0.0;  // NO_ANTICLUMPING
-        double p0 = da.y * gWa[0] + db.y * gWb[0];
-        double p1 = da.y * gWa[1] + db.y * gWb[1];
-        double p2 = da.y * gWa[2] + db.y * gWb[2];
-        p0 = -vb.w * (p0 + antic);
-        p1 = -vb.w * (p1 + antic);
-        p2 = -vb.w * (p2 + antic);
+        // calcPressForce; antic = 0 under NO_ANTICLUMPING
+        const double antic = 0.01 * fabs(da.y + pdr2b) * 0.0;
+        double p0 = da.y * gWa[0] + pdr2b * gWb[0];
+        double p1 = da.y * gWa[1] + pdr2b * gWb[1];
+        double p2 = da.y * gWa[2] + pdr2b * gWb[2];
+        p0 = -mb * (p0 + antic);
+        p1 = -mb * (p1 + antic);
+        p2 = -mb * (p2 + antic);
         fpa.x += p0;
         fpa.y += p1;
         fpa.z += p2;
@@ -498,11 +562,13 @@ struct PairsOp {
     T.orig[l] = S.orig[j];
     T.code[l] = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
   }
-  __device__ void pair(Lane& L, const Tile& T, int k) const {
+  __device__ bool test(const Lane& L, const Tile& T, int k) const {
     const double dx = L.x - T.x[k], dy = L.y - T.y[k], dz = L.z - T.z[k];
     const double l2 = dot3(dx, dy, dz, dx, dy, dz);
     const double lim = 2.0 * dmax2(L.h, T.h[k]);
-    if (!(l2 <= lim * lim)) return;
+    return l2 <= lim * lim;
+  }
+  __device__ void accept(Lane& L, const Tile& T, int k) const {
     if (mode == 1) {
       oa[L.w] = L.orig;
       ob[L.w] = T.orig[k];
