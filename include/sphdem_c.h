@@ -230,7 +230,8 @@ int sphdem_num_deleted(const sphdem_ctx* ctx, int64_t* n);
  * wtTotal* wall-clock fields of CglobalVars, globalVars.h:163-188) ------------
  * While a timing window is open, every launch of the named kernel ("force",
  * "omega", "dddt", "coupling", "shepsum", "fpi", "contacts", "eos", "start",
- * "kick", "rest", "bin", "rank", "permute", "wrap") is bracketed by HIP events
+ * "kick", "rest", "bin", "rank", "permute", "wrap", "lists" = the SPH
+ * neighbour-list build) is bracketed by HIP events
  * on the context's stream; sphdem_timing_end synchronises and returns the
  * summed kernel time and the number of launches timed. */
 int sphdem_timing_begin(sphdem_ctx* ctx, const char* kernel);
