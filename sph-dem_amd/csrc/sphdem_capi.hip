@@ -35,7 +35,7 @@ struct sphdem_ctx {
   // cached SPH neighbour lists (valid from their first use until the next reset)
   int *nbl = nullptr, *nbcnt = nullptr;
   bool listsValid = false;
-  double* hsnap = nullptr;
+  double2* hinv = nullptr;  // {1/h, 1/h^5} per SPH-group particle
   double* partial = nullptr;
   int partialCap = 0;
   double* fieldbuf = nullptr;
@@ -258,11 +258,11 @@ int tileBlocks(const sphdem_ctx* ctx) {
   est = (est + 7) & ~7;
   return std::min(std::max(est, 8), 2048);
 }
-template <class Op>
-int launchTiles(sphdem_ctx* ctx, const Op& op) {
-  hipLaunchKernelGGL(k_tile<Op>, dim3(tileBlocks(ctx)), dim3(TPB), 0, ctx->stream, ctx->cur,
-                     ctx->gs, ctx->start, ctx->box, ctx->segs,
-                     ctx->segOff + ctx->gs.dim[0] * ctx->gs.dim[1], op);
+int launchLists(sphdem_ctx* ctx) {
+  hipLaunchKernelGGL(k_lists, dim3(tileBlocks(ctx)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                     ctx->cur.posh, ctx->cur.cell, ctx->gs, ctx->start, ctx->box, ctx->segs,
+                     ctx->segOff + ctx->gs.dim[0] * ctx->gs.dim[1], ctx->ns, ctx->nbl,
+                     ctx->nbcnt, ctx->dg);
   LAUNCHED();
   return 0;
 }
@@ -398,7 +398,7 @@ int doReset(sphdem_ctx* ctx) {
 int ensureLists(sphdem_ctx* ctx) {
   if (ctx->listsValid) return 0;
   Timed tm(ctx, KID_LISTS);
-  CHECK(launchTiles(ctx, ListOp{nullptr, ctx->ns, ctx->nbl, ctx->nbcnt, ctx->dg}));
+  CHECK(launchLists(ctx));
   ctx->listsValid = true;
   return 0;
 }
@@ -425,19 +425,53 @@ int passEos(sphdem_ctx* ctx) {
   LAUNCHED();
   return 0;
 }
+// {1/h, 1/h^5} of every SPH-group particle, for the passes that need them of
+// neighbours (recomputed per pass: h changes in start, driftRest and the reinit)
+int computeHinv(sphdem_ctx* ctx) {
+  hipLaunchKernelGGL(k_hinv, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur.posh,
+                     ctx->ns, ctx->hinv);
+  LAUNCHED();
+  return 0;
+}
+// calcForce instantiated per flag combination (bit 0 VAR_H_CORRECTION, 1 VISC_MORRIS,
+// 2 VISC_ARTIFICIAL, 3 LIQ_DEM)
+template <int F>
+void launchForce(sphdem_ctx* ctx) {
+  hipLaunchKernelGGL((k_force_l<(F & 1) != 0, (F & 2) != 0, (F & 4) != 0, (F & 8) != 0>),
+                     dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
+                     ctx->box, ctx->nbl, ctx->nbcnt, ctx->hinv, ctx->P, ctx->dg);
+}
 int passForce(sphdem_ctx* ctx) {
   CHECK(ensureLists(ctx));
+  CHECK(computeHinv(ctx));
+  using Launch = void (*)(sphdem_ctx*);
+  static const Launch table[16] = {
+      launchForce<0>, launchForce<1>, launchForce<2>,  launchForce<3>,
+      launchForce<4>, launchForce<5>, launchForce<6>,  launchForce<7>,
+      launchForce<8>, launchForce<9>, launchForce<10>, launchForce<11>,
+      launchForce<12>, launchForce<13>, launchForce<14>, launchForce<15>};
+  const uint32_t f = ctx->P.flags;
+  const int sel = ((f & SPHDEM_F_VAR_H_CORRECTION) ? 1 : 0) | ((f & SPHDEM_F_VISC_MORRIS) ? 2 : 0) |
+                  ((f & SPHDEM_F_VISC_ARTIFICIAL) ? 4 : 0) | ((f & SPHDEM_F_LIQ_DEM) ? 8 : 0);
   Timed tm(ctx, KID_FORCE);
-  hipLaunchKernelGGL(k_force_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
-                     ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->P, ctx->dg);
+  table[sel](ctx);
   LAUNCHED();
   return 0;
 }
 int passDddt(sphdem_ctx* ctx) {
   CHECK(ensureLists(ctx));
-  Timed tm(ctx, KID_DDDT);
-  hipLaunchKernelGGL(k_dddt_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
-                     ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->P.wcon, varH(ctx) ? 1 : 0);
+  if (varH(ctx)) {
+    Timed tm(ctx, KID_DDDT);
+    hipLaunchKernelGGL(k_dddt_l<true>, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
+                       ctx->cur, ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt,
+                       (const double2*)nullptr, ctx->P.wcon);
+  } else {
+    CHECK(computeHinv(ctx));
+    Timed tm(ctx, KID_DDDT);
+    hipLaunchKernelGGL(k_dddt_l<false>, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
+                       ctx->cur, ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->hinv,
+                       ctx->P.wcon);
+  }
   LAUNCHED();
   return 0;
 }
@@ -471,11 +505,10 @@ int passContacts(sphdem_ctx* ctx) {
   LAUNCHED();
   return 0;
 }
-// one iteration of the density re-initialisation (sphIncompress.cpp:29-34);
-// `first` snapshots h as the cached neighbour lists are built then
+// one iteration of the density re-initialisation (sphIncompress.cpp:29-34)
 int passDensity(sphdem_ctx* ctx) {
   hipLaunchKernelGGL(k_hfromdens, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
-                     ctx->ns, ctx->P.hfac, (double*)nullptr);
+                     ctx->ns, ctx->P.hfac);
   LAUNCHED();
   CHECK(ensureLists(ctx));  // the first iteration builds them with the new h
   hipLaunchKernelGGL(k_density_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
@@ -583,8 +616,9 @@ int hookSumDrag(sphdem_ctx* ctx) {
 void freeAll(sphdem_ctx* ctx) {
   void* ptrs[] = {ctx->storeMem[0], ctx->storeMem[1], ctx->key,     ctx->slot,   ctx->tmp,
                   ctx->perm,        ctx->counts,      ctx->start,   ctx->tiles,  ctx->tileOff,
-                  ctx->hsnap,       ctx->partial,     ctx->fieldbuf, ctx->errflag, ctx->dg,
-                  ctx->segCnt,      ctx->segOff,      ctx->segs,    ctx->nbl,    ctx->nbcnt};
+                  ctx->partial,     ctx->fieldbuf, ctx->errflag, ctx->dg,
+                  ctx->segCnt,      ctx->segOff,      ctx->segs,    ctx->nbl,    ctx->nbcnt,
+                  ctx->hinv};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : ctx->evPool) (void)hipEventDestroy(e);
@@ -625,9 +659,9 @@ int sphdem_create(const sphdem_params* params, int device, int64_t capacity, sph
     HIPC(hipMalloc(&ctx->slot, sizeof(int) * capacity));
     HIPC(hipMalloc(&ctx->tmp, sizeof(int) * capacity));
     HIPC(hipMalloc(&ctx->perm, sizeof(int) * capacity));
-    HIPC(hipMalloc(&ctx->hsnap, sizeof(double) * capacity));
     HIPC(hipMalloc(&ctx->nbl, sizeof(int) * (size_t)kMaxNb * capacity));
     HIPC(hipMalloc(&ctx->nbcnt, sizeof(int) * capacity));
+    HIPC(hipMalloc(&ctx->hinv, sizeof(double2) * capacity));
     HIPC(hipMalloc(&ctx->fieldbuf, sizeof(double) * 3 * capacity));
     HIPC(hipMalloc(&ctx->errflag, sizeof(int)));
     HIPC(hipMalloc(&ctx->dg, sizeof(DevGlobals)));
@@ -916,13 +950,24 @@ int sphdem_get_pairs(sphdem_ctx* ctx, int kind, int64_t* count, int32_t* a, int3
     *count = 0;
     return 0;
   }
+  // the SPH set is read from the cached lists the SPH passes use (built now
+  // if no pass has built them since the last reset)
+  if (kind == SPHDEM_PAIRS_SPH) CHECK(ensureLists(ctx));
   int* cnt = nullptr;
   long long* off = nullptr;
   HIPC(hipMalloc(&cnt, sizeof(int) * nq));
   HIPC(hipMemsetAsync(cnt, 0, sizeof(int) * nq, ctx->stream));
-  // the SPH set is enumerated by the same tile machinery as the SPH passes
   auto enumerate = [&](int mode, const long long* o, int* da, int* db, int* di) -> int {
-    if (kind == SPHDEM_PAIRS_SPH) return launchTiles(ctx, PairsOp{mode, cnt, o, da, db, di});
+    if (kind == SPHDEM_PAIRS_SPH) {
+      if (mode == 0)
+        HIPC(hipMemcpyAsync(cnt, ctx->nbcnt, sizeof(int) * nq, hipMemcpyDeviceToDevice,
+                            ctx->stream));
+      else
+        hipLaunchKernelGGL(k_list_pairs, dim3(blocks(nq)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                           ctx->ns, ctx->nbl, ctx->nbcnt, o, da, db, di);
+      LAUNCHED();
+      return 0;
+    }
     hipLaunchKernelGGL(k_pairs, dim3(blocks(nq)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
                        ctx->nd, kind, ctx->gs, ctx->gc, ctx->start, ctx->box, ctx->rule,
                        ctx->reachFull, mode, cnt, o, da, db, di);

@@ -68,6 +68,18 @@ struct DevGlobals {
   unsigned int pad_;
 };
 
+// Global thread index with the blocks of a 1-D grid regrouped by XCD.  The
+// dispatcher deals workgroups round robin to the 8 XCDs (block b to XCD b % 8);
+// here XCD x runs the x-th contiguous eighth of the blocks, in order, so the
+// neighbour gathers of a cell-sorted particle range stay within that XCD's L2
+// instead of every XCD streaming the whole domain.  A bijection on the grid.
+__device__ __forceinline__ int xcdThread() {
+  const int nb = gridDim.x, x = blockIdx.x & 7, r = blockIdx.x >> 3;
+  const int q = nb >> 3, rem = nb & 7;
+  const int b = x * q + min(x, rem) + r;
+  return b * blockDim.x + threadIdx.x;
+}
+
 // Uniform grid: cell_i = (int)((x_i - cmin_i)/edge + off); keys are
 // base + (cx*dim1 + cy)*dim2 + cz.
 struct Grid {

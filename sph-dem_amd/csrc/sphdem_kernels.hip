@@ -333,12 +333,11 @@ __global__ void k_accel(Store S, int n, DevGlobals* g) {
 }
 
 // Nsph::calcHFromDens sph.h:1823-1825
-__global__ void k_hfromdens(Store S, int ns, double hfac, double* hsnap) {
+__global__ void k_hfromdens(Store S, int ns, double hfac) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= ns || !isSphT(S.iam[a])) return;
   const double h = hfac * pow_third(S.vhm[a].w / S.dpo[a].x);
   S.posh[a].w = h;
-  if (hsnap) hsnap[a] = h;
 }
 
 // ============================================================ SPH <-> DEM coupling
@@ -434,7 +433,7 @@ __global__ void __launch_bounds__(256) k_shepsum(Store S, int ns, int nd, Grid g
 // (sph.h:1289-1296), one gather over the DEM partners.
 __global__ void k_coupling_sph(Store S, int ns, int nd, Grid gc, const int* __restrict__ start,
                                Box B, CouplingRule R, sphdem_params P) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = xcdThread();
   if (a >= ns || !isSphT(S.iam[a])) return;
   const double4 pa = S.posh[a];
   const double Rc = P.coupling_radius;

@@ -69,6 +69,39 @@ SD_HD double wendPdWh(double wcon, double q, double h) {
   return (1.0 / cube(h)) * wendPdKh(wcon, q, h) - 3.0 / pow4(h) * wendK(wcon, q);
 }
 
+// x / y correctly rounded, given ry = x-independent 1/y (itself correctly
+// rounded, i.e. computed once with a real division).  Markstein's theorem:
+// q0 = RN(x * ry) lies within one ulp of x/y, the remainder x - q0*y is exact
+// in one FMA, and RN(q0 + rem*ry) is then RN(x/y).  Three FMA-class operations
+// in place of a full division wherever one divisor serves many quotients.
+// Valid for finite operands without overflow or subnormal quotients (the only
+// difference otherwise: x = -0 gives +0).
+SD_HD double divr(double x, double y, double ry) {
+  const double q0 = x * ry;
+  const double rem = fma(-q0, y, x);
+  return fma(rem, ry, q0);
+}
+
+// The Wendland factors with the h-only parts computed once per particle:
+//   i5 = 1/pow5(h), rh = 1/h, i3 = 1/cube(h), c3 = 3/pow4(h).
+// Each equals the corresponding function above bit for bit.
+SD_HD double wendPoly(double q) {
+  const double t = 2.0 - q;
+  return -4.0 * cube(t) * (1.0 + 2.0 * q) + 2.0 * pow4(t);
+}
+SD_HD double wendFi(double wcon, double q, double i5) {
+  if (q <= 2.0) return i5 * wcon * wendPoly(q) / q;
+  return 0.0;
+}
+SD_HD double wendPdKhi(double wcon, double q, double h, double rh) {
+  if (q <= 2.0) return divr(-wcon * q * wendPoly(q), h, rh);
+  return 0.0;
+}
+SD_HD double wendWi(double wcon, double q, double i3) { return i3 * wendK(wcon, q); }
+SD_HD double wendPdWhi(double wcon, double q, double h, double rh, double i3, double c3) {
+  return i3 * wendPdKhi(wcon, q, h, rh) - c3 * wendK(wcon, q);
+}
+
 // Correctly rounded pow(x, y) for y = the double nearest 1/3, x > 0 finite.
 // y = 1/3 - d with d = 1/(3*2^54) exactly, so pow(x,y) = cbrt(x) * x^(-d) and
 // x^(-d) = 1 - d ln x to far below an ulp.  cbrt(x) is carried as a double-

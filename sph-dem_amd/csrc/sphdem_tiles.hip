@@ -99,7 +99,6 @@ __device__ __forceinline__ void axisRange(const Grid& g, const Box& B, int ax, i
 struct TileBase {
   double x[64], y[64], z[64];
   int cell[64];  // cell of the (image) position
-  int self[64];  // index in the store, or -1 for a periodic image
 };
 __device__ __forceinline__ void stageBase(TileBase& T, int l, const Store& S, const Grid& g,
                                           const Box& B, int j, int sx, int sy, int sz,
@@ -110,7 +109,6 @@ __device__ __forceinline__ void stageBase(TileBase& T, int l, const Store& S, co
     T.y[l] = p.y;
     T.z[l] = p.z;
     T.cell[l] = S.cell[j];
-    T.self[l] = j;
   } else {
     const Img im = imagePos(p, sx, sy, sz, B);
     T.x[l] = im.x;
@@ -119,7 +117,6 @@ __device__ __forceinline__ void stageBase(TileBase& T, int l, const Store& S, co
     T.cell[l] = packCell(cellOf(im.x, g.cmin[0], g.edge, g.off),
                          cellOf(im.y, g.cmin[1], g.edge, g.off),
                          cellOf(im.z, g.cmin[2], g.edge, g.off));
-    T.self[l] = -1;
   }
 }
 __device__ __forceinline__ bool inStencil(int a, int b) {
@@ -127,98 +124,33 @@ __device__ __forceinline__ bool inStencil(int a, int b) {
          abs(cellZ(a) - cellZ(b)) <= 1;
 }
 
-// The driver shared by every pass.  Consecutive workgroups land on different
-// XCDs (round robin over 8), so XCD x = blockIdx % 8 gets the x-th eighth of
-// the segments (a spatially coherent slab for its L2) and its wavefronts split
-// that slab into contiguous runs.
-template <class Op>
-__global__ void __launch_bounds__(256) k_tile(Store S, Grid g, const int* __restrict__ start,
-                                              Box B, const Segment* __restrict__ segs,
-                                              const int* __restrict__ nsegPtr, Op op) {
-  __shared__ typename Op::Tile tiles[kWavesPerBlock];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  typename Op::Tile& T = tiles[w];
-  const int nseg = *nsegPtr;
-  const int xcd = blockIdx.x & 7;                          // gridDim.x is a multiple of 8
+// Segments [s0, s1) of wavefront w of this workgroup.  Consecutive workgroups
+// land on different XCDs (round robin over 8), so XCD x = blockIdx % 8 gets the
+// x-th eighth of the segments (a spatially coherent slab for its L2) and its
+// wavefronts split that slab into contiguous runs.  gridDim.x is a multiple of 8.
+__device__ __forceinline__ void segRange(int nseg, int w, int& s0, int& s1) {
+  const int xcd = blockIdx.x & 7;
   const int slab0 = (int)((long long)nseg * xcd / 8);
   const int slab1 = (int)((long long)nseg * (xcd + 1) / 8);
-  const int wpx = (gridDim.x >> 3) * kWavesPerBlock;       // wavefronts per XCD
+  const int wpx = (gridDim.x >> 3) * kWavesPerBlock;  // wavefronts per XCD
   const int lw = (blockIdx.x >> 3) * kWavesPerBlock + w;
   const int chunk = (slab1 - slab0 + wpx - 1) / wpx;
-  const int s0 = slab0 + lw * chunk, s1 = min(s0 + chunk, slab1);
-  typename Op::Acc acc;
-  op.begin(acc);
-  for (int si = s0; si < s1; si++) {
-    const Segment sg = segs[si];
-    const int t = sg.p0 + lane;
-    const bool act = t < sg.p1;
-    const int cx = sg.col / g.dim[1], cy = sg.col - cx * g.dim[1];
-    const int z0 = sg.zz & 0xffff, z1 = sg.zz >> 16;
-    typename Op::Lane L;
-    int tcell = 0;
-    if (act) {
-      op.load(S, t, L);
-      tcell = S.cell[t];
-    }
-    const int nsx = B.periodic[0] ? 3 : 1, nsy = B.periodic[1] ? 3 : 1, nsz = B.periodic[2] ? 3 : 1;
-    for (int ix = 0; ix < nsx; ix++) {
-      const int sx = ix == 0 ? 0 : (ix == 1 ? -1 : 1);
-      int X0, X1;
-      axisRange(g, B, 0, sx, cx, cx, X0, X1);
-      if (X0 > X1) continue;
-      for (int iy = 0; iy < nsy; iy++) {
-        const int sy = iy == 0 ? 0 : (iy == 1 ? -1 : 1);
-        int Y0, Y1;
-        axisRange(g, B, 1, sy, cy, cy, Y0, Y1);
-        if (Y0 > Y1) continue;
-        for (int iz = 0; iz < nsz; iz++) {
-          const int sz = iz == 0 ? 0 : (iz == 1 ? -1 : 1);
-          int Z0, Z1;
-          axisRange(g, B, 2, sz, z0, z1, Z0, Z1);
-          if (Z0 > Z1) continue;
-          for (int X = X0; X <= X1; X++)
-            for (int Y = Y0; Y <= Y1; Y++) {
-              const int k0 = g.base + (X * g.dim[1] + Y) * g.dim[2];
-              const int j0 = start[k0 + Z0], j1 = start[k0 + Z1 + 1];
-              for (int b0 = j0; b0 < j1; b0 += 64) {
-                const int m = min(64, j1 - b0);
-                waveSync();
-                if (lane < m) op.stage(S, g, B, b0 + lane, sx, sy, sz, T, lane);
-                waveSync();
-                if (act) {
-                  // four staged candidates per step: the tests are evaluated
-                  // without branches so their LDS reads issue together, then
-                  // the admitted ones are taken in order
-                  for (int k = 0; k < m; k += 4) {
-                    bool ok[4];
-#pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                      const int kk = min(k + u, 63);
-                      ok[u] = (int)(k + u < m) & (int)inStencil(tcell, T.cell[kk]) &
-                              (int)(T.self[kk] != t) & (int)op.test(L, T, kk);
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; u++)
-                      if (ok[u]) op.accept(L, T, k + u);
-                  }
-                }
-              }
-            }
-        }
-      }
-    }
-    if (act) op.store(S, t, L, acc);
-  }
-  op.end(acc);
+  s0 = slab0 + lw * chunk;
+  s1 = min(s0 + chunk, slab1);
 }
 
 // ---- cached neighbour lists (CpInfo::neighbrs, built by calcNeighbours,
 // dataLL.cpp:1026-1075, and kept until the next reset, dataLL.impl.h:186-188) --
-// Entry-major layout: entry k of particle t at nbl[k * ns + t], so a wavefront
-// reading entry k of 64 consecutive particles reads 256 contiguous bytes.
-// An entry is the store index of the neighbour with its periodic image code in
-// the top bits.
-constexpr int kMaxNb = 256;
+// Entries are grouped in chunks of four per particle, chunk-major: chunk c of
+// particle t is the 16 bytes at nbl[(c * ns + t) * 4], so a wavefront reading
+// the next chunk of 64 consecutive particles reads 1 KiB of contiguous memory,
+// and the search writes whole chunks.  An entry is the store index of the
+// neighbour with its periodic image code in the top bits.
+constexpr int kMaxNb = 256;  // a multiple of the chunk size
+constexpr int kChunk = 4;
+__device__ __forceinline__ size_t nbChunk(int c, int t, int ns) {
+  return ((size_t)c * ns + t) * kChunk;
+}
 constexpr int kEntryShift = 26;  // store index < 2^26
 __device__ __forceinline__ int nbIndex(int e) { return e & ((1 << kEntryShift) - 1); }
 __device__ __forceinline__ int nbCode(int e) { return (int)((unsigned)e >> kEntryShift); }
@@ -228,57 +160,191 @@ __device__ __forceinline__ void codeShift(int code, int& sx, int& sy, int& sz) {
   sz = code / 9 - 1;
 }
 
-struct ListOp {
-  const double* hq;  // h to build with (null: current h)
-  int ns;
-  int* nbl;
-  int* nbcnt;
-  DevGlobals* g;
-  struct Tile : TileBase {
-    double hq[64];
-    int ent[64];
-  };
-  struct Lane {
-    double x, y, z, hq;
-    int t, c;
-  };
-  struct Acc {};
-  __device__ void begin(Acc&) const {}
-  __device__ void end(Acc&) const {}
-  __device__ void load(const Store& S, int t, Lane& L) const {
-    const double4 p = S.posh[t];
-    L.x = p.x;
-    L.y = p.y;
-    L.z = p.z;
-    L.hq = hq ? hq[t] : p.w;
-    L.t = t;
-    L.c = 0;
-  }
-  __device__ void stage(const Store& S, const Grid& gg, const Box& B, int j, int sx, int sy, int sz,
-                        Tile& T, int l) const {
-    double4 p;
-    stageBase(T, l, S, gg, B, j, sx, sy, sz, p);
-    T.hq[l] = hq ? hq[j] : p.w;
-    T.ent[l] = j | (((sx + 1) + 3 * (sy + 1) + 9 * (sz + 1)) << kEntryShift);
-  }
-  __device__ bool test(const Lane& L, const Tile& T, int k) const {
-    const double dx = L.x - T.x[k], dy = L.y - T.y[k], dz = L.z - T.z[k];
-    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
-    const double lim = 2.0 * dmax2(L.hq, T.hq[k]);
-    return l2 <= lim * lim;
-  }
-  __device__ void accept(Lane& L, const Tile& T, int k) const {
-    if (L.c < kMaxNb) nbl[(size_t)L.c * ns + L.t] = T.ent[k];
-    L.c++;
-  }
-  __device__ void store(const Store&, int t, Lane& L, Acc&) const {
-    if (L.c > kMaxNb) {
-      atomicOr(&g->errflags, 2u);
-      L.c = kMaxNb;
-    }
-    nbcnt[t] = L.c;
-  }
+// Pointer into the constant address space (scalar loads for uniform indices).
+template <class T>
+using CPtr = const __attribute__((address_space(4))) T*;
+
+// Staged periodic-image candidate of the list build.
+struct ImageTile : TileBase {
+  double h[64];
+  int ent[64];
 };
+
+// calcNeighbours for every SPH-group particle, one wavefront per Segment.
+//
+// Unshifted candidates: the wavefront streams the column ranges of the 3x3
+// columns around its segment (z extended by one cell) straight from the
+// cell-sorted store with wave-uniform loads, four candidates per step.  The
+// column is within +-1 of the lane's cell by construction, so the 27-cell
+// stencil reduces to "j lies in the store range of cells cz-1..cz+1 of this
+// column", an integer range test per lane; is_neighbr (particle.h:236-238) is
+// len2(ra - rb) <= (2 max(ha, hb))^2 with blitz's dot association.
+//
+// Periodic images (shift +-L per periodic axis) are staged in the wavefront's
+// LDS slice with the cell of the image position, which is tested against the
+// lane's cell on every axis.  They follow the unshifted candidates in the list.
+__global__ void __launch_bounds__(256) k_lists(Store S, const double4* __restrict__ pos,
+                                               const int* __restrict__ cellc, Grid g,
+                                               const int* __restrict__ start, Box B,
+                                               const Segment* __restrict__ segs,
+                                               const int* __restrict__ nsegPtr, int ns,
+                                               int* __restrict__ nbl, int* __restrict__ nbcnt,
+                                               DevGlobals* dg) {
+  __shared__ ImageTile tiles[kWavesPerBlock];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  ImageTile& T = tiles[w];
+  // Every address of the candidate stream is wave-uniform and nothing here
+  // writes the store, the cell starts or the segments, so they are read through
+  // the constant address space: scalar loads, one per candidate per wavefront.
+  const CPtr<double> cpos = (CPtr<double>)pos;
+  const CPtr<int> cstart = (CPtr<int>)start;
+  const CPtr<int> csegs = (CPtr<int>)segs;
+  int s0, s1;
+  segRange(*(CPtr<int>)nsegPtr, w, s0, s1);
+  const bool anyPeriodic = (B.periodic[0] | B.periodic[1] | B.periodic[2]) != 0;
+  for (int si = s0; si < s1; si++) {
+    const Segment sg{csegs[4 * si], csegs[4 * si + 1], csegs[4 * si + 2], csegs[4 * si + 3]};
+    const int t = sg.p0 + lane;
+    const bool act = t < sg.p1;
+    const int cx = sg.col / g.dim[1], cy = sg.col - cx * g.dim[1];
+    const int z0 = sg.zz & 0xffff, z1 = sg.zz >> 16;
+    const int Z0 = max(z0 - 1, 0), Z1 = min(z1 + 1, g.dim[2] - 1);
+    double px = 0.0, py = 0.0, pz = 0.0, ha = 0.0;
+    int tcell = 0;
+    if (act) {
+      const double4 p = pos[t];
+      px = p.x;
+      py = p.y;
+      pz = p.z;
+      ha = p.w;
+      tcell = cellc[t];
+    }
+    const int cz = cellZ(tcell);
+    int c = 0;      // entries found
+    int4 buf = make_int4(0, 0, 0, 0);  // the chunk being filled
+    auto push = [&](int e) {
+      if (c < kMaxNb) {
+        const int sl = c & (kChunk - 1);
+        buf.x = sl == 0 ? e : buf.x;
+        buf.y = sl == 1 ? e : buf.y;
+        buf.z = sl == 2 ? e : buf.z;
+        buf.w = sl == 3 ? e : buf.w;
+        if (sl == kChunk - 1) *reinterpret_cast<int4*>(nbl + nbChunk(c >> 2, t, ns)) = buf;
+      }
+      c++;
+    };
+    const int X0 = max(cx - 1, 0), X1 = min(cx + 1, g.dim[0] - 1);
+    const int Y0 = max(cy - 1, 0), Y1 = min(cy + 1, g.dim[1] - 1);
+    for (int X = X0; X <= X1; X++)
+      for (int Y = Y0; Y <= Y1; Y++) {
+        const int k0 = g.base + (X * g.dim[1] + Y) * g.dim[2];
+        const int J0 = cstart[k0 + Z0], J1 = cstart[k0 + Z1 + 1];
+        if (J0 >= J1) continue;
+        // store range of this column's cells cz-1 .. cz+1 (empty for idle lanes)
+        int lo = 0, wd = 0;
+        if (act) {
+          lo = start[k0 + max(cz - 1, 0)];
+          wd = start[k0 + min(cz + 2, g.dim[2])] - lo;
+        }
+        for (int j = J0; j < J1; j += 4) {
+          double4 q[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int jq = 4 * min(j + u, J1 - 1);
+            q[u] = make_double4(cpos[jq], cpos[jq + 1], cpos[jq + 2], cpos[jq + 3]);
+          }
+          bool ok[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int jj = j + u;
+            const double dx = px - q[u].x, dy = py - q[u].y, dz = pz - q[u].z;
+            const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+            // h > 0 and never NaN, so fmax (one instruction) equals std::max
+            const double lim = 2.0 * fmax(ha, q[u].w);
+            ok[u] = (int)(jj < J1) & (int)((unsigned)(jj - lo) < (unsigned)wd) & (int)(jj != t) &
+                    (int)(l2 <= lim * lim);
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++)
+            if (ok[u]) push((j + u) | (13 << kEntryShift));
+        }
+      }
+    if (anyPeriodic) {
+      const int nsx = B.periodic[0] ? 3 : 1, nsy = B.periodic[1] ? 3 : 1,
+                nsz = B.periodic[2] ? 3 : 1;
+      for (int ix = 0; ix < nsx; ix++) {
+        const int sx = ix == 0 ? 0 : (ix == 1 ? -1 : 1);
+        int XA, XB;
+        axisRange(g, B, 0, sx, cx, cx, XA, XB);
+        if (XA > XB) continue;
+        for (int iy = 0; iy < nsy; iy++) {
+          const int sy = iy == 0 ? 0 : (iy == 1 ? -1 : 1);
+          int YA, YB;
+          axisRange(g, B, 1, sy, cy, cy, YA, YB);
+          if (YA > YB) continue;
+          for (int iz = 0; iz < nsz; iz++) {
+            const int sz = iz == 0 ? 0 : (iz == 1 ? -1 : 1);
+            if ((sx | sy | sz) == 0) continue;  // unshifted: done above
+            int ZA, ZB;
+            axisRange(g, B, 2, sz, z0, z1, ZA, ZB);
+            if (ZA > ZB) continue;
+            const int code = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+            for (int X = XA; X <= XB; X++)
+              for (int Y = YA; Y <= YB; Y++) {
+                const int k0 = g.base + (X * g.dim[1] + Y) * g.dim[2];
+                const int j0 = start[k0 + ZA], j1 = start[k0 + ZB + 1];
+                for (int b0 = j0; b0 < j1; b0 += 64) {
+                  const int m = min(64, j1 - b0);
+                  waveSync();
+                  if (lane < m) {
+                    double4 p;
+                    stageBase(T, lane, S, g, B, b0 + lane, sx, sy, sz, p);
+                    T.h[lane] = p.w;
+                    T.ent[lane] = (b0 + lane) | (code << kEntryShift);
+                  }
+                  waveSync();
+                  if (act) {
+                    for (int k = 0; k < m; k++) {
+                      const double dx = px - T.x[k], dy = py - T.y[k], dz = pz - T.z[k];
+                      const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+                      const double lim = 2.0 * dmax2(ha, T.h[k]);
+                      if (inStencil(tcell, T.cell[k]) && l2 <= lim * lim) push(T.ent[k]);
+                    }
+                  }
+                }
+              }
+          }
+        }
+      }
+    }
+    if (act) {
+      if (c > kMaxNb) {
+        atomicOr(&dg->errflags, 2u);
+        c = kMaxNb;
+      }
+      if (c & (kChunk - 1)) *reinterpret_cast<int4*>(nbl + nbChunk(c >> 2, t, ns)) = buf;
+      nbcnt[t] = c;
+    }
+  }
+}
+
+// The cached lists as (a, b, image) triples of original indices, for the
+// parity enumeration of the SPH pair set.
+__global__ void k_list_pairs(Store S, int ns, const int* __restrict__ nbl,
+                             const int* __restrict__ nbcnt, const long long* __restrict__ off,
+                             int* __restrict__ oa, int* __restrict__ ob, int* __restrict__ oimg) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= ns) return;
+  const int n = nbcnt[a];
+  const int ia = S.orig[a];
+  long long w = off[a];
+  for (int k = 0; k < n; k++, w++) {
+    const int e = nbl[nbChunk(k >> 2, a, ns) + (k & 3)];
+    oa[w] = ia;
+    ob[w] = S.orig[nbIndex(e)];
+    oimg[w] = nbCode(e);
+  }
+}
 
 // Loads neighbour entry e: store index and (image) position.
 __device__ __forceinline__ void imageFix(const Box& B, int e, double4& p) {
@@ -296,20 +362,63 @@ __device__ __forceinline__ void imageFix(const Box& B, int e, double4& p) {
 // together (one memory round trip per batch instead of two per neighbour), and
 // the arithmetic runs over the batch in list order.  Every neighbour in the
 // SPH group is sph or sphBoundary, so no type needs to be read.
-#define SPHDEM_FETCH_ENTRIES(U)                                                   \
-  int e[U];                                                                       \
-  _Pragma("unroll") for (int u = 0; u < U; u++) e[u] =                             \
-      (k0 + u < n) ? nbl[(size_t)(k0 + u) * ns + a] : (a | (13 << kEntryShift));
+#define SPHDEM_FETCH_ENTRIES(U)                                                    \
+  int e[U];                                                                        \
+  _Pragma("unroll") for (int v = 0; v < U / kChunk; v++) {                         \
+    const int4 ch = *reinterpret_cast<const int4*>(nbl + nbChunk((k0 >> 2) + v, a, ns)); \
+    const int ce[4] = {ch.x, ch.y, ch.z, ch.w};                                      \
+    _Pragma("unroll") for (int i = 0; i < kChunk; i++) {                           \
+      const int u = v * kChunk + i;                                                \
+      e[u] = (k0 + u < n) ? ce[i] : (a | (13 << kEntryShift));                     \
+    }                                                                              \
+  }
+
+// ---- per-pass reciprocals --------------------------------------------------------
+// Quotients by a value that serves many pairs (the particle's own h and omega,
+// a pair's viscous denominator for its three components) are formed with divr
+// from one correctly rounded reciprocal: bit for bit the IEEE quotient the
+// reference computes, at a fraction of a division's cost.  Per neighbour, the
+// pair passes gather hinv = {1/h, 1/h^5} of the neighbour (computed once per
+// particle by k_hinv whenever h may have changed).
+__global__ void k_hinv(const double4* __restrict__ posh, int ns, double2* __restrict__ hinv) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= ns) return;
+  const double h = posh[a].w;
+  hinv[a] = make_double2(1.0 / h, 1.0 / pow5(h));
+}
+
+// gradW (sph.h:322-342) with 1/h^5 given and the division by pa.omega as divr
+__device__ __forceinline__ void gradWi(double wcon, bool varh, double omega_a, double romega,
+                                       double dx, double dy, double dz, double q, double i5,
+                                       double* g) {
+  double drF0 = 0.0, drF1 = 0.0, drF2 = 0.0;
+  if (q != 0) {
+    const double F = wendFi(wcon, q, i5);
+    drF0 = dx * F;
+    drF1 = dy * F;
+    drF2 = dz * F;
+  }
+  if (varh) {
+    g[0] = divr(drF0, omega_a, romega);
+    g[1] = divr(drF1, omega_a, romega);
+    g[2] = divr(drF2, omega_a, romega);
+  } else {
+    g[0] = drF0;
+    g[1] = drF1;
+    g[2] = drF2;
+  }
+}
 
 // ---- calcOmega (sph.h:437-452) over the cached list -----------------------------
 __global__ void __launch_bounds__(256) k_omega_l(Store S, int ns, Box B,
                                                  const int* __restrict__ nbl,
                                                  const int* __restrict__ nbcnt, double wcon) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = xcdThread();
   if (a >= ns) return;
   const double4 pa = S.posh[a];
   const double ha = pa.w;
-  double om = S.vhm[a].w * wendPdWh(wcon, 0, ha);
+  const double rha = 1.0 / ha, i3 = 1.0 / cube(ha), c3 = 3.0 / pow4(ha);
+  double om = S.vhm[a].w * wendPdWhi(wcon, 0, ha, rha, i3, c3);
   const int n = nbcnt[a];
   constexpr int U = 8;
   for (int k0 = 0; k0 < n; k0 += U) {
@@ -328,7 +437,7 @@ __global__ void __launch_bounds__(256) k_omega_l(Store S, int ns, Box B,
       imageFix(B, e[u], p[u]);
       const double dx = pa.x - p[u].x, dy = pa.y - p[u].y, dz = pa.z - p[u].z;
       const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
-      om += m[u] * wendPdWh(wcon, r / ha, ha);
+      om += m[u] * wendPdWhi(wcon, divr(r, ha, rha), ha, rha, i3, c3);
     }
   }
   S.dpo[a].z = 1 + ha * om / (S.dpo[a].x * 3);
@@ -338,11 +447,12 @@ __global__ void __launch_bounds__(256) k_omega_l(Store S, int ns, Box B,
 __global__ void __launch_bounds__(256) k_density_l(Store S, int ns, Box B,
                                                    const int* __restrict__ nbl,
                                                    const int* __restrict__ nbcnt, double wcon) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = xcdThread();
   if (a >= ns) return;
   const double4 pa = S.posh[a];
   const double ha = pa.w;
-  double rho = S.vhm[a].w * wendW(wcon, 0, ha);
+  const double rha = 1.0 / ha, i3 = 1.0 / cube(ha);
+  double rho = S.vhm[a].w * wendWi(wcon, 0, i3);
   const int n = nbcnt[a];
   constexpr int U = 8;
   for (int k0 = 0; k0 < n; k0 += U) {
@@ -361,34 +471,39 @@ __global__ void __launch_bounds__(256) k_density_l(Store S, int ns, Box B,
       imageFix(B, e[u], p[u]);
       const double dx = pa.x - p[u].x, dy = pa.y - p[u].y, dz = pa.z - p[u].z;
       const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
-      rho += m[u] * wendW(wcon, r / ha, ha);
+      rho += m[u] * wendWi(wcon, divr(r, ha, rha), i3);
     }
   }
   S.dpo[a].x = rho;
 }
 
 // ---- calcDddtDudt (sphIncompress.h:783-851) over the cached list ----------------
+template <bool VARH>
 __global__ void __launch_bounds__(256) k_dddt_l(Store S, int ns, Box B,
                                                 const int* __restrict__ nbl,
-                                                const int* __restrict__ nbcnt, double wcon,
-                                                int varh) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+                                                const int* __restrict__ nbcnt,
+                                                const double2* __restrict__ hinv, double wcon) {
+  const int a = xcdThread();
   if (a >= ns) return;
   const double4 pa = S.posh[a];
   const double4 va = S.vhm[a];
   const double ha = pa.w;
+  const double rha = 1.0 / ha, i5a = 1.0 / pow5(ha);
   const double om = S.dpo[a].z;
+  const double rom = 1.0 / om;
   double dddt = S.vh0d[a].w;
   const int n = nbcnt[a];
   constexpr int U = 8;
   for (int k0 = 0; k0 < n; k0 += U) {
     SPHDEM_FETCH_ENTRIES(U)
     double4 p[U], v[U];
+    double2 hi[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int j = nbIndex(e[u]);
       p[u] = S.posh[j];
       v[u] = S.vhm[j];
+      if (!VARH) hi[u] = hinv[j];
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -398,12 +513,12 @@ __global__ void __launch_bounds__(256) k_dddt_l(Store S, int ns, Box B,
       const double dv0 = va.x - v[u].x, dv1 = va.y - v[u].y, dv2 = va.z - v[u].z;
       const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
       double gW[3];
-      if (varh) {
-        gradW(wcon, true, om, dx, dy, dz, r / ha, ha, gW);
+      if (VARH) {
+        gradWi(wcon, true, om, rom, dx, dy, dz, divr(r, ha, rha), i5a, gW);
       } else {
         double g1[3], g2[3];
-        gradW(wcon, false, om, dx, dy, dz, r / ha, ha, g1);
-        gradW(wcon, false, om, dx, dy, dz, r / p[u].w, p[u].w, g2);
+        gradWi(wcon, false, om, rom, dx, dy, dz, divr(r, ha, rha), i5a, g1);
+        gradWi(wcon, false, om, rom, dx, dy, dz, divr(r, p[u].w, hi[u].x), hi[u].y, g2);
         for (int c = 0; c < 3; c++) gW[c] = 0.5 * (g1[c] + g2[c]);
       }
       dddt += v[u].w * dot3(dv0, dv1, dv2, gW[0], gW[1], gW[2]);
@@ -413,36 +528,46 @@ __global__ void __launch_bounds__(256) k_dddt_l(Store S, int ns, Box B,
 }
 
 // ---- calcForce (sphIncompress.h:899-972; sph.h:579-666, 692-734) over the list --
-// The per-pair dt conditions (sph.h:545-560) are min-reduced into newDt_bits.
+// One instantiation per combination of the flags that change the pair
+// arithmetic.  The per-pair dt conditions (sph.h:545-560) are min-reduced into
+// newDt_bits.  Without artificial viscosity the signal speed is the constant
+// 2*SPSOUND (or 0), and x -> RN(x/c) is monotone, so the minimum over pairs of
+// RN(0.6*hav / vsig) is RN(min(0.6*hav) / vsig): the numerators are min-reduced
+// and divided once (likewise 0.125*hav^2 / VISCOSITY).
+template <bool VARH, bool MORRIS, bool ARTIF, bool LIQDEM>
 __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
                                                  const int* __restrict__ nbl,
-                                                 const int* __restrict__ nbcnt, sphdem_params P,
-                                                 DevGlobals* g) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  double newDt = __longlong_as_double(0x7ff0000000000000ll);
+                                                 const int* __restrict__ nbcnt,
+                                                 const double2* __restrict__ hinv,
+                                                 sphdem_params P, DevGlobals* g) {
+  const int a = xcdThread();
+  const double inf = __longlong_as_double(0x7ff0000000000000ll);
+  double newDt = inf;
   if (a < ns) {
     const bool fluid = S.iam[a] == T_SPH;  // viscosity only acts on sph (sphIncompress.h:927)
     const double4 pa = S.posh[a];
     const double4 va = S.vhm[a];
     const double4 da = S.dpo[a];
     const double ha = pa.w;
-    const bool varh = (P.flags & SPHDEM_F_VAR_H_CORRECTION) != 0;
-    const bool morris = (P.flags & SPHDEM_F_VISC_MORRIS) != 0;
-    const bool artif = (P.flags & SPHDEM_F_VISC_ARTIFICIAL) != 0;
-    const bool liqdem = (P.flags & SPHDEM_F_LIQ_DEM) != 0;
+    const double rha = 1.0 / ha, i5a = 1.0 / pow5(ha);
+    const double rom = 1.0 / da.z;
+    const double viscLiq = 2.0 * P.viscosity * P.dens;  // numerator under LIQ_DEM
+    const double vsigC = MORRIS ? 2.0 * P.spsound : 0.0;
+    double minA = inf, minB = inf;  // min over pairs of 0.6*hav and 0.125*hav^2
     double4 f = S.frc[a], fpa = S.fp[a], fva = S.fv[a];
     const int n = nbcnt[a];
     constexpr int U = 4;
     for (int k0 = 0; k0 < n; k0 += U) {
       SPHDEM_FETCH_ENTRIES(U)
       double4 p[U], v[U];
-      double2 d[U];
+      double2 d[U], hi[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int j = nbIndex(e[u]);
         p[u] = S.posh[j];
         v[u] = S.vhm[j];
         d[u] = *reinterpret_cast<const double2*>(&S.dpo[j]);  // dens, pdr2
+        hi[u] = hinv[j];
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {
@@ -451,35 +576,36 @@ __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
         const double hb = p[u].w, mb = v[u].w, rhob = d[u].x, pdr2b = d[u].y;
         const double dx = pa.x - p[u].x, dy = pa.y - p[u].y, dz = pa.z - p[u].z;
         const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
+        const double qa = divr(r, ha, rha), qb = divr(r, hb, hi[u].x);
         double gWa[3], gWb[3];
-        if (varh) {
-          gradW(P.wcon, true, da.z, dx, dy, dz, r / ha, ha, gWa);
-          gradW(P.wcon, true, da.z, dx, dy, dz, r / hb, hb, gWb);
+        if (VARH) {
+          gradWi(P.wcon, true, da.z, rom, dx, dy, dz, qa, i5a, gWa);
+          gradWi(P.wcon, true, da.z, rom, dx, dy, dz, qb, hi[u].y, gWb);
         } else {
           double g1[3], g2[3];
-          gradW(P.wcon, false, da.z, dx, dy, dz, r / ha, ha, g1);
-          gradW(P.wcon, false, da.z, dx, dy, dz, r / hb, hb, g2);
+          gradWi(P.wcon, false, da.z, rom, dx, dy, dz, qa, i5a, g1);
+          gradWi(P.wcon, false, da.z, rom, dx, dy, dz, qb, hi[u].y, g2);
           for (int c = 0; c < 3; c++) gWa[c] = gWb[c] = 0.5 * (g1[c] + g2[c]);
         }
         if (fluid) {
           const double dv0 = va.x - v[u].x, dv1 = va.y - v[u].y, dv2 = va.z - v[u].z;
           const double gb0 = 0.5 * (gWa[0] + gWb[0]), gb1 = 0.5 * (gWa[1] + gWb[1]),
                        gb2 = 0.5 * (gWa[2] + gWb[2]);
-          const double vdr = dot3(dx, dy, dz, dv0, dv1, dv2);
           const double hav = 0.5 * (ha + hb);
           double fv0 = 0.0, fv1 = 0.0, fv2 = 0.0;
-          double vsig = 0.0;
-          if (morris) {
-            double visc = P.viscosity * (da.x + rhob) / (da.x * rhob);
-            vsig = 2.0 * P.spsound;
-            if (liqdem) visc = 2.0 * P.viscosity * P.dens / (da.x * rhob);
+          double vsig = vsigC;
+          if (MORRIS) {
+            const double visc = LIQDEM ? viscLiq / (da.x * rhob)
+                                       : P.viscosity * (da.x + rhob) / (da.x * rhob);
             const double dd = dot3(dx, dy, dz, gb0, gb1, gb2);
             const double den = r * r + 0.01 * (hav * hav);
-            fv0 = dv0 * mb * visc * dd / den;
-            fv1 = dv1 * mb * visc * dd / den;
-            fv2 = dv2 * mb * visc * dd / den;
+            const double rden = 1.0 / den;
+            fv0 = divr(dv0 * mb * visc * dd, den, rden);
+            fv1 = divr(dv1 * mb * visc * dd, den, rden);
+            fv2 = divr(dv2 * mb * visc * dd, den, rden);
           }
-          if (artif) {
+          if (ARTIF) {
+            const double vdr = dot3(dx, dy, dz, dv0, dv1, dv2);
             double avsig = 0, avisc = 0.0;
             if (vdr < 0) {
               double rr = 0.0;
@@ -493,9 +619,11 @@ __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
             f.y += s * gb1;
             f.z += s * gb2;
             vsig = dmax2(vsig, avsig);
+            newDt = dmin2(newDt, 0.6 * hav / vsig);
+          } else {
+            minA = dmin2(minA, 0.6 * hav);
           }
-          newDt = dmin2(newDt, 0.6 * hav / vsig);
-          newDt = dmin2(newDt, 0.125 * (hav * hav) / P.viscosity);
+          minB = dmin2(minB, 0.125 * (hav * hav));
           fva.x += fv0;
           fva.y += fv1;
           fva.z += fv2;
@@ -519,67 +647,13 @@ __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
         f.z += p2;
       }
     }
+    if (!ARTIF && minA != inf) newDt = dmin2(newDt, minA / vsigC);
+    if (minB != inf) newDt = dmin2(newDt, minB / P.viscosity);
     S.frc[a] = f;
     S.fp[a] = fpa;
     S.fv[a] = fva;
   }
   atomicMinPos(&g->newDt_bits, newDt);
 }
-
-// ---- the SPH pair set (parity): count, then write (a, b, image) triples -------------
-struct PairsOp {
-  int mode;                // 0 count, 1 write
-  int* cnt;                // per target particle
-  const long long* off;    // per target particle (mode 1)
-  int *oa, *ob, *oimg;
-  struct Tile : TileBase {
-    double h[64];
-    int orig[64], code[64];
-  };
-  struct Lane {
-    double x, y, z, h;
-    int orig, c;
-    long long w;
-  };
-  struct Acc {};
-  __device__ void begin(Acc&) const {}
-  __device__ void end(Acc&) const {}
-  __device__ void load(const Store& S, int t, Lane& L) const {
-    const double4 p = S.posh[t];
-    L.x = p.x;
-    L.y = p.y;
-    L.z = p.z;
-    L.h = p.w;
-    L.orig = S.orig[t];
-    L.c = 0;
-    L.w = mode == 1 ? off[t] : 0;
-  }
-  __device__ void stage(const Store& S, const Grid& g, const Box& B, int j, int sx, int sy, int sz,
-                        Tile& T, int l) const {
-    double4 p;
-    stageBase(T, l, S, g, B, j, sx, sy, sz, p);
-    T.h[l] = p.w;
-    T.orig[l] = S.orig[j];
-    T.code[l] = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
-  }
-  __device__ bool test(const Lane& L, const Tile& T, int k) const {
-    const double dx = L.x - T.x[k], dy = L.y - T.y[k], dz = L.z - T.z[k];
-    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
-    const double lim = 2.0 * dmax2(L.h, T.h[k]);
-    return l2 <= lim * lim;
-  }
-  __device__ void accept(Lane& L, const Tile& T, int k) const {
-    if (mode == 1) {
-      oa[L.w] = L.orig;
-      ob[L.w] = T.orig[k];
-      oimg[L.w] = T.code[k];
-      L.w++;
-    }
-    L.c++;
-  }
-  __device__ void store(const Store&, int t, Lane& L, Acc&) const {
-    if (mode == 0) cnt[t] = L.c;
-  }
-};
 
 }  // namespace sphdem

@@ -129,6 +129,36 @@ def test_contact_pairs_bit_exact():
     assert_same_pairs(g, o)
 
 
+def lists_by_particle(p):
+    """{a: [(b, image), ...]} in the order the pair array holds them."""
+    out = {}
+    for a, b, im in np.asarray(p, dtype=np.int64).tolist():
+        out.setdefault(a, []).append((b, im))
+    return out
+
+
+def test_sph_list_order(case):
+    """The cached SPH lists hold the neighbours in the reference's order (cells
+    x-outer, then y, then z; original index within a cell) for every particle
+    whose stencil has no periodic image; with images, the unshifted neighbours
+    come first in that order, then the images."""
+    sim, orc = both(case)
+    g = lists_by_particle(sim.pairs(sd.PAIRS_SPH))
+    o = lists_by_particle(orc.pairs(sd.PAIRS_SPH))
+    assert g.keys() == o.keys()
+    with_images = 0
+    for a, lo in o.items():
+        lg = g[a]
+        if all(im == 13 for _, im in lo):
+            assert lg == lo, f"particle {a}: list order differs"
+        else:
+            with_images += 1
+            assert [x for x in lg if x[1] == 13] == [x for x in lo if x[1] == 13]
+            assert sorted(lg) == sorted(lo)
+    if case.params.periodic[0]:
+        assert with_images > 0
+
+
 # ------------------------------------------------------------ per-pass parity
 def test_middle_passes(case):
     """middle()'s passes on identical state: coupling (porosity, Shepard sum,

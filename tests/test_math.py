@@ -114,3 +114,58 @@ def test_device_wendland_matches_oracle(hm):
             tol = 1e-14 if q > 0.05 else 1e-12
             assert a == pytest.approx(b, rel=tol, abs=1e-300)
     assert same > 0.97 * total
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def test_reciprocal_division_is_correctly_rounded(hm):
+    """divr(x, y, 1/y) (Markstein: one product, an exact FMA remainder, one FMA
+    correction) equals the IEEE quotient x/y bit for bit, including divisors
+    with all-ones significands and near powers of two, where a plain x*(1/y)
+    is off by an ulp in a sizeable fraction of cases."""
+    rng = np.random.default_rng(17)
+    n = 2_000_000
+    x = rng.uniform(1, 2, n) * np.exp2(rng.integers(-60, 60, n)) * rng.choice([-1, 1], n)
+    y = rng.uniform(1, 2, n) * np.exp2(rng.integers(-60, 60, n))
+    yb = y.view(np.uint64)
+    yb[::7] |= np.uint64((1 << 52) - 1)                      # all-ones significand
+    yb[1::11] &= ~np.uint64((1 << 52) - 1)                   # powers of two ...
+    yb[1::11] |= rng.integers(0, 256, yb[1::11].size).astype(np.uint64)  # ... and just above
+    x[::13] = 0.0
+    out = np.empty(n)
+    hm.sphdem_host_divr_arr(_ptr(x), _ptr(y), _ptr(out), ctypes.c_long(n))
+    exact = x / y
+    assert np.array_equal(out.view(np.uint64), exact.view(np.uint64))
+    xz = np.array([-0.0, 0.0])
+    yz = np.array([3.0, 3.0])
+    oz = np.empty(2)
+    hm.sphdem_host_divr_arr(_ptr(xz), _ptr(yz), _ptr(oz), ctypes.c_long(2))
+    assert np.array_equal(oz, [0.0, 0.0])  # -0/y comes out as +0
+    naive = x * (1.0 / y)
+    assert np.count_nonzero(naive != exact) > n // 100  # the correction step matters
+
+
+@pytest.mark.parametrize("name", ["wendF", "wendW", "wendPdWh"])
+def test_hoisted_wendland_forms_are_bitwise_equal(hm, name):
+    """The pair kernels use the Wendland functions with 1/h, 1/h^3, 3/h^4 and
+    1/h^5 computed once per particle (and divisions by h as divr): bit for bit
+    the functions they replace."""
+    rng = np.random.default_rng(23)
+    n = 400_000
+    q = rng.uniform(0, 2.05, n)
+    q[::50] = 2.0
+    q[1::50] = 0.0
+    h = rng.uniform(1e-3, 2.0, n)
+    wcon = 21.0 / (16.0 * 3.14159265)
+    a = np.empty(n)
+    b = np.empty(n)
+    getattr(hm, f"sphdem_host_{name}i_arr")(ctypes.c_double(wcon), _ptr(q), _ptr(h), _ptr(a),
+                                            ctypes.c_long(n))
+    getattr(hm, f"sphdem_host_{name}_arr")(ctypes.c_double(wcon), _ptr(q), _ptr(h), _ptr(b),
+                                           ctypes.c_long(n))
+    # bit for bit, except that a zero may come out as +0 where the plain form
+    # gives -0 (at q = 2 exactly); signed zeros vanish in the pair sums
+    same = (a.view(np.uint64) == b.view(np.uint64)) | ((a == 0) & (b == 0))
+    assert same.all(), f"{np.count_nonzero(~same)} differ, e.g. q={q[~same][:3]} h={h[~same][:3]}"
