@@ -11,7 +11,11 @@ timed steps (walls are processed but not counted).
 Alongside the metric the JSON line carries
   roofline      : the dominant kernel's algorithmic bytes per launch divided by
                   its mean duration, timed with HIP events on the library's own
-                  stream over the timed steps, against the 8 TB/s HBM peak;
+                  stream over the timed steps, against the 8 TB/s HBM peak
+                  (default kernel: the SPH neighbour-list build, two launches a
+                  step); `traffic` is the kernel's HBM bytes per launch from the
+                  rocprofv3 FETCH_SIZE / WRITE_SIZE summary committed under
+                  profiles/, scaled by the copy-kernel calibration it records;
   step_roofline : whole-step algorithmic bytes (1152 B per SPH/wall particle +
                   650 B per DEM particle per step) x steps/s against 8 TB/s;
   cpu_baseline  : the CPU FP64 restatement timed on this host on a bounded
@@ -36,6 +40,9 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per particle per launch (FP64 widths), from the pass table:
 # each input field read once, each output field written once per pass.
 B_ALG_KERNEL = {
+    # calcNeighbours: r h -> the list (4 B per entry, filled in below from the
+    # measured mean list length) + its count
+    "lists": 32 + 4,
     # calcForce: r vhat h mass dens pdr2 iam omega porosity -> f fp fv fb
     "force": 84 + 16 + 24 + 72,
     # calcOmega: r h mass dens iam -> omega
@@ -56,7 +63,10 @@ def parse():
     p.add_argument("--warmup", type=int, default=10)
     p.add_argument("--nf", type=int, default=64, help="fluid particles per axis")
     p.add_argument("--ndem", type=int, default=4096)
-    p.add_argument("--roofline-kernel", default="force")
+    p.add_argument("--roofline-kernel", default="lists")
+    p.add_argument("--traffic-file",
+                   default=os.path.join(ROOT, "profiles", "traffic_sediment_64cubed.txt"),
+                   help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary to take `traffic` from")
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="budget of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
@@ -88,6 +98,34 @@ def cpu_baseline(case, budget_s):
     }
 
 
+KERNEL_SYMBOL = {"lists": "k_lists", "force": "k_force_l", "omega": "k_omega_l",
+                 "dddt": "k_dddt_l", "coupling": "k_coupling_sph"}
+
+
+def traffic_from_summary(path, kernel):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 summary:
+    lines `calibration FETCH_SIZE <factor>` / `calibration WRITE_SIZE <factor>`
+    and `<symbol> <dispatches> <FETCH_SIZE KiB> <WRITE_SIZE KiB>` per kernel."""
+    if not os.path.exists(path):
+        return None
+    fac = {}
+    rows = {}
+    with open(path) as f:
+        for line in f:
+            w = line.split()
+            if not w or w[0].startswith("#"):
+                continue
+            if w[0] == "calibration" and len(w) >= 3:
+                fac[w[1]] = float(w[2])
+            elif len(w) >= 4 and w[1].isdigit():
+                rows[w[0]] = (float(w[2]), float(w[3]))
+    sym = KERNEL_SYMBOL.get(kernel)
+    if sym not in rows or "FETCH_SIZE" not in fac or "WRITE_SIZE" not in fac:
+        return None
+    fetch, write = rows[sym]
+    return 1024.0 * (fetch * fac["FETCH_SIZE"] + write * fac["WRITE_SIZE"])
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -112,12 +150,17 @@ def main():
         if not np.all(np.isfinite(sim.get(nm))):
             raise RuntimeError(f"non-finite {nm} after the run")
     g = sim.globals()
+    units = counts["fluid"] + counts["wall"]  # SPH-group particles a pair kernel processes
+    mean_nb = sim.pair_count(sd.PAIRS_SPH) / max(units, 1)
     sim.close()
     value = counted * a.steps / el
-    units = counts["fluid"] + counts["wall"]  # SPH-group particles a pair kernel processes
     if a.roofline_kernel in ("fpi", "contacts", "shepsum"):
         units = counts["dem"]
-    b_launch = B_ALG_KERNEL.get(a.roofline_kernel, 0) * units
+    per_unit = B_ALG_KERNEL.get(a.roofline_kernel, 0)
+    if a.roofline_kernel == "lists":
+        per_unit += 4 * mean_nb
+    b_launch = per_unit * units
+    traffic = traffic_from_summary(a.traffic_file, a.roofline_kernel)
     mean_s = (kms / 1e3) / max(launches, 1)
     achieved = b_launch / mean_s / 1e9 if launches else None
     step_bytes = B_ALG_STEP_SPH * (counts["fluid"] + counts["wall"]) + B_ALG_STEP_DEM * counts["dem"]
@@ -148,10 +191,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": None,
+            "traffic": traffic,
             "mean_launch_ms": mean_s * 1e3,
             "launches": launches,
             "alg_bytes_per_launch": b_launch,
+            "mean_sph_neighbours": mean_nb,
         },
         "step_roofline": {
             "alg_bytes_per_step": step_bytes,

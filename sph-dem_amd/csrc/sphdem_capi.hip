@@ -36,6 +36,15 @@ struct sphdem_ctx {
   int *nbl = nullptr, *nbcnt = nullptr;
   bool listsValid = false;
   double2* hinv = nullptr;  // {1/h, 1/h^5} per SPH-group particle
+  // periodic ghosts (store entries [n, n + ng) after a reset)
+  int ng = 0, gbase = 0;    // ghost count; first key of the ghost cells (0: none)
+  GhostBand band{};         // cells a ghost may occupy on the current grid
+  bool gridValid = false;   // gs/band describe the grid of the last reset
+  int64_t ghostCap = 0;     // store entries beyond cap available to ghosts
+  int *gcnt = nullptr, *goff = nullptr;                   // per real entry (n + 1)
+  int *gsrc = nullptr, *gord = nullptr, *gcell = nullptr;  // per ghost, before the permutation
+  int *gorig = nullptr, *gcode = nullptr;                  // per ghost, after it
+  int* inv = nullptr;                                      // old -> new position of real entries
   double* partial = nullptr;
   int partialCap = 0;
   double* fieldbuf = nullptr;
@@ -143,6 +152,62 @@ int allocStore(sphdem_ctx* ctx, Store& s, void*& mem, int64_t cap) {
   return 0;
 }
 
+double4* storeArr(Store& s, int k) {
+  double4* a[kNumVec] = {s.posh, s.vhm, s.dpo, s.vel, s.v0p, s.vh0d,
+                         s.frc,  s.fp,  s.fv,  s.fb,  s.fdrag, s.fvel};
+  return a[k];
+}
+
+// Per-entry buffers (the two stores, the sort scratch, hinv) for cap real
+// entries plus ghostCap ghosts; the ghost records sized ghostCap.  On growth
+// the first n entries of the current store and of key/slot are kept.
+int allocEntries(sphdem_ctx* ctx, int64_t ghostCap) {
+  const int64_t ent = ctx->cap + ghostCap;
+  Store cur{}, alt{};
+  void* mem[2] = {nullptr, nullptr};
+  CHECK(allocStore(ctx, cur, mem[0], ent));
+  CHECK(allocStore(ctx, alt, mem[1], ent));
+  int *key = nullptr, *slot = nullptr, *tmp = nullptr, *perm = nullptr;
+  double2* hinv = nullptr;
+  HIPC(hipMalloc(&key, sizeof(int) * ent));
+  HIPC(hipMalloc(&slot, sizeof(int) * ent));
+  HIPC(hipMalloc(&tmp, sizeof(int) * ent));
+  HIPC(hipMalloc(&perm, sizeof(int) * ent));
+  HIPC(hipMalloc(&hinv, sizeof(double2) * ent));
+  if (ctx->storeMem[0]) {
+    const size_t n = (size_t)ctx->n;
+    for (int k = 0; k < kNumVec; k++)
+      HIPC(hipMemcpyAsync(storeArr(cur, k), storeArr(ctx->cur, k), sizeof(double4) * n,
+                          hipMemcpyDeviceToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(cur.iam, ctx->cur.iam, sizeof(int) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(cur.orig, ctx->cur.orig, sizeof(int) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(cur.cell, ctx->cur.cell, sizeof(int) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(key, ctx->key, sizeof(int) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(slot, ctx->slot, sizeof(int) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+  }
+  void* old[] = {ctx->storeMem[0], ctx->storeMem[1], ctx->key, ctx->slot, ctx->tmp, ctx->perm,
+                 ctx->hinv, ctx->gsrc, ctx->gord, ctx->gcell, ctx->gorig, ctx->gcode};
+  for (void* q : old)
+    if (q) HIPC(hipFree(q));
+  ctx->cur = cur;
+  ctx->alt = alt;
+  ctx->storeMem[0] = mem[0];
+  ctx->storeMem[1] = mem[1];
+  ctx->key = key;
+  ctx->slot = slot;
+  ctx->tmp = tmp;
+  ctx->perm = perm;
+  ctx->hinv = hinv;
+  ctx->ghostCap = ghostCap;
+  HIPC(hipMalloc(&ctx->gsrc, sizeof(int) * ghostCap));
+  HIPC(hipMalloc(&ctx->gord, sizeof(int) * ghostCap));
+  HIPC(hipMalloc(&ctx->gcell, sizeof(int) * ghostCap));
+  HIPC(hipMalloc(&ctx->gorig, sizeof(int) * ghostCap));
+  HIPC(hipMalloc(&ctx->gcode, sizeof(int) * ghostCap));
+  return 0;
+}
+
 // where each field lives: array index in the Store, component offset, width
 struct FieldLoc {
   int arr, off, width;
@@ -174,11 +239,6 @@ bool fieldLoc(int field, FieldLoc* L) {
     default: return false;
   }
 }
-double4* storeArr(Store& s, int k) {
-  double4* a[kNumVec] = {s.posh, s.vhm, s.dpo, s.vel, s.v0p, s.vh0d,
-                         s.frc,  s.fp,  s.fv,  s.fb,  s.fdrag, s.fvel};
-  return a[k];
-}
 
 bool liqDem(const sphdem_ctx* c) { return (c->P.flags & SPHDEM_F_LIQ_DEM) != 0; }
 bool varH(const sphdem_ctx* c) { return (c->P.flags & SPHDEM_F_VAR_H_CORRECTION) != 0; }
@@ -191,23 +251,20 @@ int ensureKeys(sphdem_ctx* ctx, int K) {
     HIPC(hipMalloc(&ctx->counts, sizeof(int) * ctx->Kcap));
     HIPC(hipMalloc(&ctx->start, sizeof(int) * ctx->Kcap));
   }
-  const int tiles = (K + 1 + SCAN_TILE - 1) / SCAN_TILE;
-  if (tiles > SCAN_TILE)
-    return fail(ctx, SPHDEM_E_CAPACITY, "cell grid of %d keys exceeds the scan capacity", K);
-  if (tiles + 1 > ctx->tileCap) {
-    if (ctx->tiles) HIPC(hipFree(ctx->tiles));
-    if (ctx->tileOff) HIPC(hipFree(ctx->tileOff));
-    ctx->tileCap = tiles + 1;
-    HIPC(hipMalloc(&ctx->tiles, sizeof(int) * ctx->tileCap));
-    HIPC(hipMalloc(&ctx->tileOff, sizeof(int) * ctx->tileCap));
-  }
   return 0;
 }
 
-// exclusive scan of in[0..m) into out (m <= SCAN_TILE * (tileCap - 1))
+// exclusive scan of in[0..m) into out (m <= SCAN_TILE^2)
 int scanInts(sphdem_ctx* ctx, const int* in, int* out, int m) {
   const int tiles = (m + SCAN_TILE - 1) / SCAN_TILE;
-  if (tiles + 1 > ctx->tileCap) return fail(ctx, SPHDEM_E_CAPACITY, "scan of %d items", m);
+  if (tiles > SCAN_TILE) return fail(ctx, SPHDEM_E_CAPACITY, "scan of %d items", m);
+  if (tiles + 1 > ctx->tileCap) {
+    if (ctx->tiles) HIPC(hipFree(ctx->tiles));
+    if (ctx->tileOff) HIPC(hipFree(ctx->tileOff));
+    ctx->tileCap = tiles + 1 + (tiles + 1) / 4;
+    HIPC(hipMalloc(&ctx->tiles, sizeof(int) * ctx->tileCap));
+    HIPC(hipMalloc(&ctx->tileOff, sizeof(int) * ctx->tileCap));
+  }
   hipLaunchKernelGGL(k_scan_tile, dim3(tiles), dim3(SCAN_THREADS), 0, ctx->stream, in, out,
                      ctx->tiles, m);
   LAUNCHED();
@@ -258,9 +315,18 @@ int tileBlocks(const sphdem_ctx* ctx) {
   est = (est + 7) & ~7;
   return std::min(std::max(est, 8), 2048);
 }
+// ghost copies of the fields the pair passes read of neighbours
+int refreshGhosts(sphdem_ctx* ctx) {
+  if (ctx->ng == 0) return 0;
+  hipLaunchKernelGGL(k_ghost_refresh, dim3(blocks(ctx->ng)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                     ctx->n, ctx->ng, ctx->gorig, ctx->gcode, ctx->box);
+  LAUNCHED();
+  return 0;
+}
 int launchLists(sphdem_ctx* ctx) {
-  hipLaunchKernelGGL(k_lists, dim3(tileBlocks(ctx)), dim3(TPB), 0, ctx->stream, ctx->cur,
-                     ctx->cur.posh, ctx->cur.cell, ctx->gs, ctx->start, ctx->box, ctx->segs,
+  CHECK(refreshGhosts(ctx));
+  hipLaunchKernelGGL(k_lists, dim3(tileBlocks(ctx)), dim3(TPB), 0, ctx->stream, ctx->cur.posh,
+                     ctx->cur.cell, ctx->gs, ctx->gbase, ctx->start, ctx->segs,
                      ctx->segOff + ctx->gs.dim[0] * ctx->gs.dim[1], ctx->ns, ctx->nbl,
                      ctx->nbcnt, ctx->dg);
   LAUNCHED();
@@ -292,7 +358,11 @@ void setupGrids(sphdem_ctx* ctx) {
   }
   gc.base = Cs;
   const int Cc = gc.dim[0] * gc.dim[1] * gc.dim[2];
-  ctx->K = Cs + Cc + 1;  // last key: deleted particles
+  // keys: SPH cells, DEM cells, one for deleted particles, then (periodic
+  // boxes) the SPH cells again for the ghosts
+  const bool periodic = ctx->box.periodic[0] || ctx->box.periodic[1] || ctx->box.periodic[2];
+  ctx->gbase = periodic ? Cs + Cc + 1 : 0;
+  ctx->K = periodic ? Cs + Cc + 1 + Cs : Cs + Cc + 1;
   if (liqDem(ctx)) {
     const double Rc = P.coupling_radius;
     ctx->reachShep = std::max(1, (int)ceil(Rc * (1.0 + 1e-9) / gs.edge));
@@ -329,6 +399,19 @@ void setupStatic(sphdem_ctx* ctx) {
 }
 
 // CdataLL::reset (dataLL.cpp:140-236)
+// the ghost band of the current grid: the cells of the domain's extreme
+// coordinates (every wrapped particle lies between them), widened by one
+void setupBand(sphdem_ctx* ctx) {
+  const Grid& g = ctx->gs;
+  for (int d = 0; d < 3; d++) {
+    const double lo = ctx->P.proc_domain[2 * d], hi = ctx->P.proc_domain[2 * d + 1];
+    const int clo = (int)((lo - g.cmin[d]) / g.edge + g.off);
+    const int chi = (int)((hi - g.cmin[d]) / g.edge + g.off);
+    ctx->band.lo[d] = std::max(clo - 1, 0);
+    ctx->band.hi[d] = std::min(chi + 1, g.dim[d] - 1);
+  }
+}
+
 int doReset(sphdem_ctx* ctx) {
   const sphdem_params& P = ctx->P;
   const int n = ctx->n;
@@ -338,15 +421,28 @@ int doReset(sphdem_ctx* ctx) {
     dmin[d] = P.proc_domain[2 * d];
     dsp[d] = P.proc_domain[2 * d + 1] - dmin[d];
   }
+  // With a periodic box the ghosts are counted on the current grid while
+  // wrapping; the count stands if the grid survives this reset (one host
+  // round trip per reset), and is redone on the new grid otherwise.
+  const bool periodic = ctx->box.periodic[0] || ctx->box.periodic[1] || ctx->box.periodic[2];
+  const bool pre = periodic && ctx->gridValid && n > 0;
+  int ngPre = 0;
+  if (pre) HIPC(hipMemsetAsync(ctx->gcnt + n, 0, sizeof(int), ctx->stream));
   {
     Timed tm(ctx, KID_WRAP);
     hipLaunchKernelGGL(k_wrap, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, ctx->cur, n, dmin[0],
-                       dmin[1], dmin[2], dsp[0], dsp[1], dsp[2], ctx->box, ctx->dg);
+                       dmin[1], dmin[2], dsp[0], dsp[1], dsp[2], ctx->box, ctx->dg, ctx->gs,
+                       ctx->band, pre ? ctx->gcnt : (int*)nullptr);
     LAUNCHED();
+  }
+  if (pre) {
+    CHECK(scanInts(ctx, ctx->gcnt, ctx->goff, n + 1));
+    HIPC(hipMemcpyAsync(&ngPre, ctx->goff + n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   }
   DevGlobals hg;
   HIPC(hipMemcpyAsync(&hg, ctx->dg, sizeof(DevGlobals), hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
+  const double oldSsr = ctx->ssr;
   if (hg.errflags & 1u)
     return fail(ctx, SPHDEM_E_CAPACITY, "a particle fell outside the cell grid at the last reset");
   if (hg.errflags & 2u)
@@ -359,34 +455,69 @@ int doReset(sphdem_ctx* ctx) {
   if (ctx->ns + ctx->nd == 0) hmax = -100000;  // calcDomainLimits on an empty container
   if ((hmax > ctx->ssr) || (hmax < 0.9 * ctx->ssr)) ctx->ssr = hmax;
   setupGrids(ctx);
+  setupBand(ctx);
+  const bool sameGrid = ctx->gridValid && ctx->ssr == oldSsr;
+  ctx->gridValid = true;
   for (const Grid* gr : {&ctx->gs, &ctx->gc})
     if (gr->dim[0] > 2048 || gr->dim[1] > 2048 || gr->dim[2] > 1024)
       return fail(ctx, SPHDEM_E_CAPACITY, "cell grid %d x %d x %d exceeds 2048 x 2048 x 1024",
                   gr->dim[0], gr->dim[1], gr->dim[2]);
   CHECK(ensureKeys(ctx, ctx->K));
   HIPC(hipMemsetAsync(ctx->counts, 0, sizeof(int) * (ctx->K + 1), ctx->stream));
+  const int keyDeleted = ctx->gs.dim[0] * ctx->gs.dim[1] * ctx->gs.dim[2] +
+                         ctx->gc.dim[0] * ctx->gc.dim[1] * ctx->gc.dim[2];
   {
     Timed tm(ctx, KID_BIN);
     hipLaunchKernelGGL(k_bin, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, ctx->cur, n, ctx->gs,
-                       ctx->gc, ctx->K - 1, ctx->key, ctx->slot, ctx->counts, ctx->dg);
+                       ctx->gc, keyDeleted, ctx->key, ctx->slot, ctx->counts, ctx->dg);
     LAUNCHED();
   }
+  // ghosts (updateDomain's periodic images): count, size, then their keys
+  int ng = 0;
+  if (ctx->gbase > 0 && n > 0) {
+    if (pre && sameGrid) {
+      ng = ngPre;
+    } else {
+      hipLaunchKernelGGL(k_ghost_count, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, ctx->cur, n,
+                         ctx->gs, ctx->box, ctx->band, ctx->gcnt);
+      LAUNCHED();
+      HIPC(hipMemsetAsync(ctx->gcnt + n, 0, sizeof(int), ctx->stream));
+      CHECK(scanInts(ctx, ctx->gcnt, ctx->goff, n + 1));
+      HIPC(hipMemcpyAsync(&ng, ctx->goff + n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+      HIPC(hipStreamSynchronize(ctx->stream));
+    }
+    if (ng > ctx->ghostCap) CHECK(allocEntries(ctx, (int64_t)ng + ng / 4 + 1024));
+    if (ng > 0) {
+      hipLaunchKernelGGL(k_ghost_make, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, ctx->cur, n,
+                         ctx->gs, ctx->box, ctx->band, ctx->goff, ctx->gbase, ctx->key, ctx->slot,
+                         ctx->counts, ctx->gsrc, ctx->gord, ctx->gcell);
+      LAUNCHED();
+    }
+  }
+  const int ntot = n + ng;
   CHECK(scanCounts(ctx, ctx->K + 1));
-  hipLaunchKernelGGL(k_scatter, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, n, ctx->key,
+  hipLaunchKernelGGL(k_scatter, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ntot, ctx->key,
                      ctx->slot, ctx->start, ctx->tmp);
   LAUNCHED();
   {
     Timed tm(ctx, KID_RANK);
-    hipLaunchKernelGGL(k_rank, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, n, ctx->key,
-                       ctx->start, ctx->tmp, ctx->cur.orig, ctx->perm);
+    hipLaunchKernelGGL(k_rank, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ntot, n, ctx->key,
+                       ctx->start, ctx->tmp, ctx->cur.orig, ctx->gsrc, ctx->gord, ctx->perm);
     LAUNCHED();
   }
   {
     Timed tm(ctx, KID_PERMUTE);
-    hipLaunchKernelGGL(k_permute, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, n, ctx->perm,
-                       ctx->cur, ctx->alt);
+    hipLaunchKernelGGL(k_permute, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ntot, n,
+                       ctx->perm, ctx->cur, ctx->alt, ctx->gsrc, ctx->gord, ctx->gcell, ctx->box,
+                       ctx->inv);
     LAUNCHED();
   }
+  if (ng > 0) {
+    hipLaunchKernelGGL(k_ghost_link, dim3(blocks(ng)), dim3(TPB), 0, ctx->stream, n, ng,
+                       ctx->perm, ctx->inv, ctx->gsrc, ctx->gord, ctx->gorig, ctx->gcode);
+    LAUNCHED();
+  }
+  ctx->ng = ng;
   std::swap(ctx->cur, ctx->alt);
   std::swap(ctx->storeMem[0], ctx->storeMem[1]);
   ctx->listsValid = false;  // updateDomain clears every neighbour list (dataLL.cpp:374)
@@ -412,6 +543,7 @@ int checkLoaded(sphdem_ctx* ctx) {
 // ---- passes
 int passOmega(sphdem_ctx* ctx) {
   CHECK(ensureLists(ctx));
+  CHECK(refreshGhosts(ctx));
   Timed tm(ctx, KID_OMEGA);
   hipLaunchKernelGGL(k_omega_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
                      ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->P.wcon);
@@ -428,8 +560,9 @@ int passEos(sphdem_ctx* ctx) {
 // {1/h, 1/h^5} of every SPH-group particle, for the passes that need them of
 // neighbours (recomputed per pass: h changes in start, driftRest and the reinit)
 int computeHinv(sphdem_ctx* ctx) {
-  hipLaunchKernelGGL(k_hinv, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur.posh,
-                     ctx->ns, ctx->hinv);
+  const int ntot = ctx->n + ctx->ng;  // ghosts included (their h is their origin's)
+  hipLaunchKernelGGL(k_hinv, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ctx->cur.posh, ntot,
+                     ctx->hinv);
   LAUNCHED();
   return 0;
 }
@@ -443,6 +576,7 @@ void launchForce(sphdem_ctx* ctx) {
 }
 int passForce(sphdem_ctx* ctx) {
   CHECK(ensureLists(ctx));
+  CHECK(refreshGhosts(ctx));
   CHECK(computeHinv(ctx));
   using Launch = void (*)(sphdem_ctx*);
   static const Launch table[16] = {
@@ -460,6 +594,7 @@ int passForce(sphdem_ctx* ctx) {
 }
 int passDddt(sphdem_ctx* ctx) {
   CHECK(ensureLists(ctx));
+  CHECK(refreshGhosts(ctx));
   if (varH(ctx)) {
     Timed tm(ctx, KID_DDDT);
     hipLaunchKernelGGL(k_dddt_l<true>, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
@@ -511,6 +646,7 @@ int passDensity(sphdem_ctx* ctx) {
                      ctx->ns, ctx->P.hfac);
   LAUNCHED();
   CHECK(ensureLists(ctx));  // the first iteration builds them with the new h
+  CHECK(refreshGhosts(ctx));
   hipLaunchKernelGGL(k_density_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
                      ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->P.wcon);
   LAUNCHED();
@@ -618,7 +754,8 @@ void freeAll(sphdem_ctx* ctx) {
                   ctx->perm,        ctx->counts,      ctx->start,   ctx->tiles,  ctx->tileOff,
                   ctx->partial,     ctx->fieldbuf, ctx->errflag, ctx->dg,
                   ctx->segCnt,      ctx->segOff,      ctx->segs,    ctx->nbl,    ctx->nbcnt,
-                  ctx->hinv};
+                  ctx->hinv,        ctx->gcnt,        ctx->goff,    ctx->gsrc,   ctx->gord,
+                  ctx->gcell,       ctx->gorig,       ctx->gcode,   ctx->inv};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : ctx->evPool) (void)hipEventDestroy(e);
@@ -653,15 +790,12 @@ int sphdem_create(const sphdem_params* params, int device, int64_t capacity, sph
   auto setup = [&]() -> int {
     HIPC(hipSetDevice(device));
     HIPC(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    CHECK(allocStore(ctx, ctx->cur, ctx->storeMem[0], capacity));
-    CHECK(allocStore(ctx, ctx->alt, ctx->storeMem[1], capacity));
-    HIPC(hipMalloc(&ctx->key, sizeof(int) * capacity));
-    HIPC(hipMalloc(&ctx->slot, sizeof(int) * capacity));
-    HIPC(hipMalloc(&ctx->tmp, sizeof(int) * capacity));
-    HIPC(hipMalloc(&ctx->perm, sizeof(int) * capacity));
+    CHECK(allocEntries(ctx, capacity / 4 + 4096));
+    HIPC(hipMalloc(&ctx->gcnt, sizeof(int) * (capacity + 1)));
+    HIPC(hipMalloc(&ctx->goff, sizeof(int) * (capacity + 1)));
+    HIPC(hipMalloc(&ctx->inv, sizeof(int) * capacity));
     HIPC(hipMalloc(&ctx->nbl, sizeof(int) * (size_t)kMaxNb * capacity));
     HIPC(hipMalloc(&ctx->nbcnt, sizeof(int) * capacity));
-    HIPC(hipMalloc(&ctx->hinv, sizeof(double2) * capacity));
     HIPC(hipMalloc(&ctx->fieldbuf, sizeof(double) * 3 * capacity));
     HIPC(hipMalloc(&ctx->errflag, sizeof(int)));
     HIPC(hipMalloc(&ctx->dg, sizeof(DevGlobals)));
@@ -745,6 +879,7 @@ int sphdem_upload(sphdem_ctx* ctx, int64_t n, const double* r, const double* v, 
   ctx->sphStep = 0;
   // CdataLL::CdataLL (dataLL.cpp:8-40): hmax = sph_search_radius = H, then reset()
   ctx->ssr = ctx->P.h;
+  ctx->gridValid = false;
   ctx->loaded = true;
   CHECK(doReset(ctx));
   HIPC(hipStreamSynchronize(ctx->stream));
@@ -964,7 +1099,7 @@ int sphdem_get_pairs(sphdem_ctx* ctx, int kind, int64_t* count, int32_t* a, int3
                             ctx->stream));
       else
         hipLaunchKernelGGL(k_list_pairs, dim3(blocks(nq)), dim3(TPB), 0, ctx->stream, ctx->cur,
-                           ctx->ns, ctx->nbl, ctx->nbcnt, o, da, db, di);
+                           ctx->ns, ctx->n, ctx->nbl, ctx->nbcnt, ctx->gcode, o, da, db, di);
       LAUNCHED();
       return 0;
     }

@@ -20,7 +20,13 @@ enum : int {
   T_DELETED = 255,
 };
 
-// One particle = 12 double4 + 2 int (392 B).
+// One particle = 12 double4 + 3 int (396 B).
+//
+// Periodic images are stored as ghost entries after the real particles (store
+// positions [n, n + ng) after a reset, n = the uploaded count): copies of their
+// origin with the position shifted by whole periods, binned in a key range of
+// their own, exactly the ghosts CdataLL::updateDomain creates
+// (dataLL.cpp:503-583, 631-695).
 struct Store {
   double4* posh;   // r.xyz, h
   double4* vhm;    // vhat.xyz, mass
@@ -66,6 +72,12 @@ struct DevGlobals {
   unsigned long long ndel_sph, ndel_dem;
   unsigned int errflags;  // bit 0: a particle fell outside its cell grid
   unsigned int pad_;
+};
+
+// Cells (per axis) a ghost may occupy: one beyond the cells real particles of
+// a periodic axis can hold, [cellOf(min) - 1, cellOf(max) + 1] of the domain.
+struct GhostBand {
+  int lo[3], hi[3];
 };
 
 // Global thread index with the blocks of a 1-D grid regrouped by XCD.  The

@@ -138,14 +138,43 @@ __device__ __forceinline__ void gradW(double wcon, bool varh, double omega_a, do
   }
 }
 
+// Which periodic shifts of a particle along axis d give a ghost inside the band.
+__device__ __forceinline__ int ghostShiftsAxis(double x, int d, const Grid& g, const Box& B,
+                                               const GhostBand& gb, int* cell) {
+  // bit 0: shift -1 valid, bit 1: shift +1 valid; cell[0]/cell[1] their cells
+  int m = 0;
+  if (!B.periodic[d]) return 0;
+  const int lo = gb.lo[d], hi = gb.hi[d];
+  const int cm = cellOf(B.Lm[d] + x, g.cmin[d], g.edge, g.off);
+  const int cp = cellOf(B.Lp[d] + x, g.cmin[d], g.edge, g.off);
+  if (cm >= lo && cm <= hi) m |= 1;
+  if (cp >= lo && cp <= hi) m |= 2;
+  cell[0] = cm;
+  cell[1] = cp;
+  return m;
+}
+__device__ __forceinline__ int popc2(int m) { return (m & 1) + ((m >> 1) & 1); }
+
+__device__ __forceinline__ int ghostCount(const double4& p, const Grid& g, const Box& B,
+                                          const GhostBand& gb) {
+  int cc[2];
+  const int mx = ghostShiftsAxis(p.x, 0, g, B, gb, cc);
+  const int my = ghostShiftsAxis(p.y, 1, g, B, gb, cc);
+  const int mz = ghostShiftsAxis(p.z, 2, g, B, gb, cc);
+  return (1 + popc2(mx)) * (1 + popc2(my)) * (1 + popc2(mz)) - 1;
+}
 // ============================================================ cell build
 // CdataLL::updateDomain's migration for one rank whose neighbours are itself
 // through periodic faces (dataLL.cpp:370-403, 459-497): wrap or delete, then
 // hmax (calcDomainLimits, dataLL.cpp:954-975).
+// With gcnt set, also each particle's ghost count on the current grid (used
+// when the reset keeps that grid).
 __global__ void k_wrap(Store S, int n, double dmin0, double dmin1, double dmin2_,
-                       double dsp0, double dsp1, double dsp2, Box B, DevGlobals* g) {
+                       double dsp0, double dsp1, double dsp2, Box B, DevGlobals* g, Grid gs,
+                       GhostBand gb, int* __restrict__ gcnt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double hv = 0.0;
+  int ghosts = 0;
   if (i < n) {
     const int t = S.iam[i];
     if (t != T_DELETED) {
@@ -174,12 +203,16 @@ __global__ void k_wrap(Store S, int n, double dmin0, double dmin1, double dmin2_
           S.posh[i] = p;
         }
         hv = p.w;
+        if (gcnt && isSphT(t)) ghosts = ghostCount(p, gs, B, gb);
       }
     }
+    if (gcnt) gcnt[i] = ghosts;
   }
   atomicMaxPos(&g->hmax_bits, hv);
 }
 
+// Range of SPH-group cells, reduced per wavefront then by atomics.
+// Binning of the real entries [0, n) (CdataLL::insert, dataLL.cpp:978-997).
 __global__ void k_bin(Store S, int n, Grid gs, Grid gc, int keyDeleted, int* __restrict__ key,
                       int* __restrict__ slot, int* __restrict__ counts, DevGlobals* g) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -205,6 +238,61 @@ __global__ void k_bin(Store S, int n, Grid gs, Grid gc, int keyDeleted, int* __r
   }
   key[i] = k;
   slot[i] = atomicAdd(&counts[k], 1);
+}
+
+// ---- ghosts: the periodic images CdataLL::updateDomain receives
+// (dataLL.cpp:631-695).  A real SPH-group particle gets a ghost for every
+// combination of shifts (+-L on periodic axes) whose image cell, on each
+// shifted axis, lies within one cell of the SPH group's cell range: the only
+// ghosts a 27-cell stencil of a real particle can reach.  `ord` numbers the
+// combinations in the order the reference receives them (receive slot
+// c = shift + 1, x-major: ord = 9 (sx+1) + 3 (sy+1) + (sz+1)).
+__global__ void k_ghost_count(Store S, int n, Grid g, Box B, GhostBand gb,
+                              int* __restrict__ gcnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  gcnt[i] = isSphT(S.iam[i]) ? ghostCount(S.posh[i], g, B, gb) : 0;
+}
+
+// Writes ghost entry e = n + goff[i] + k for each ghost of real particle i:
+// its key in the ghost range (gbase + image cell), its slot, origin, `ord`
+// and packed image cell.
+__global__ void k_ghost_make(Store S, int n, Grid g, Box B, GhostBand gb,
+                             const int* __restrict__ goff, int gbase, int* __restrict__ key,
+                             int* __restrict__ slot, int* __restrict__ counts,
+                             int* __restrict__ gsrc, int* __restrict__ gord,
+                             int* __restrict__ gcell) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int cnt = goff[i + 1] - goff[i];
+  if (cnt == 0) return;
+  const double4 p = S.posh[i];
+  const int rc = S.cell[i];
+  int cx[2], cy[2], cz[2];
+  const int mx = ghostShiftsAxis(p.x, 0, g, B, gb, cx);
+  const int my = ghostShiftsAxis(p.y, 1, g, B, gb, cy);
+  const int mz = ghostShiftsAxis(p.z, 2, g, B, gb, cz);
+  int e = n + goff[i];
+  for (int sx = -1; sx <= 1; sx++) {
+    if (sx != 0 && !((mx >> (sx > 0)) & 1)) continue;
+    const int X = sx == 0 ? cellX(rc) : cx[sx > 0];
+    for (int sy = -1; sy <= 1; sy++) {
+      if (sy != 0 && !((my >> (sy > 0)) & 1)) continue;
+      const int Y = sy == 0 ? cellY(rc) : cy[sy > 0];
+      for (int sz = -1; sz <= 1; sz++) {
+        if (sz != 0 && !((mz >> (sz > 0)) & 1)) continue;
+        if ((sx | sy | sz) == 0) continue;
+        const int Z = sz == 0 ? cellZ(rc) : cz[sz > 0];
+        const int k = gbase + (X * g.dim[1] + Y) * g.dim[2] + Z;
+        key[e] = k;
+        slot[e] = atomicAdd(&counts[k], 1);
+        gsrc[e - n] = i;
+        gord[e - n] = 9 * (sx + 1) + 3 * (sy + 1) + (sz + 1);
+        gcell[e - n] = packCell(X, Y, Z);
+        e++;
+      }
+    }
+  }
 }
 
 // Exclusive scan, 4096 elements per 1024-thread block: per-thread serial scan
@@ -259,27 +347,63 @@ __global__ void k_scatter(int n, const int* __restrict__ key, const int* __restr
   if (i < n) tmp[start[key[i]] + slot[i]] = i;
 }
 
-// Order inside a cell = original index (the reference inserts particles in
-// container order, dataLL.cpp:226-231): each entry's rank is the number of
-// cell members with a smaller original index.
-__global__ void k_rank(int n, const int* __restrict__ key, const int* __restrict__ start,
+// Order inside a cell = the reference's insertion order: real particles in
+// container order (original index, dataLL.cpp:226-231); ghosts in the order
+// they are received (slot `ord`, then the origin's container order).  Each
+// entry's rank is the number of cell members with a smaller order key.
+__device__ __forceinline__ long long orderKey(int e, int n, const int* __restrict__ orig,
+                                              const int* __restrict__ gsrc,
+                                              const int* __restrict__ gord) {
+  if (e < n) return orig[e];
+  return ((long long)gord[e - n] << 32) | (unsigned)orig[gsrc[e - n]];
+}
+__global__ void k_rank(int ntot, int n, const int* __restrict__ key, const int* __restrict__ start,
                        const int* __restrict__ tmp, const int* __restrict__ orig,
+                       const int* __restrict__ gsrc, const int* __restrict__ gord,
                        int* __restrict__ perm) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
+  if (p >= ntot) return;
   const int i = tmp[p];
   const int k = key[i];
   const int s0 = start[k], s1 = start[k + 1];
-  const int o = orig[i];
+  const long long o = orderKey(i, n, orig, gsrc, gord);
   int rank = 0;
-  for (int q = s0; q < s1; q++) rank += (orig[tmp[q]] < o) ? 1 : 0;
+  for (int q = s0; q < s1; q++) rank += (orderKey(tmp[q], n, orig, gsrc, gord) < o) ? 1 : 0;
   perm[s0 + rank] = i;
 }
 
-__global__ void k_permute(int n, const int* __restrict__ perm, Store src, Store dst) {
+// Permutation into the second buffer.  A ghost entry copies its origin with
+// the position shifted by its periods (the ghost's r = r + (RMAX-RMIN) or
+// r + (RMIN-RMAX) per shifted axis, dataLL.cpp:651-666); inv maps the real
+// entries' old positions to their new ones.
+__device__ __forceinline__ void ordShift(int ord, int& sx, int& sy, int& sz) {
+  sx = ord / 9 - 1;
+  sy = (ord / 3) % 3 - 1;
+  sz = ord % 3 - 1;
+}
+__global__ void k_permute(int ntot, int n, const int* __restrict__ perm, Store src, Store dst,
+                          const int* __restrict__ gsrc, const int* __restrict__ gord,
+                          const int* __restrict__ gcell, Box B, int* __restrict__ inv) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const int i = perm[p];
+  if (p >= ntot) return;
+  const int e = perm[p];
+  int i = e;
+  if (e < n) {
+    inv[e] = p;
+  } else {
+    i = gsrc[e - n];
+    int sx, sy, sz;
+    ordShift(gord[e - n], sx, sy, sz);
+    const double4 q = src.posh[i];
+    const Img im = imagePos(q, sx, sy, sz, B);
+    dst.posh[p] = make_double4(im.x, im.y, im.z, q.w);
+    dst.vhm[p] = src.vhm[i];
+    dst.dpo[p] = src.dpo[i];
+    dst.iam[p] = src.iam[i];
+    dst.orig[p] = src.orig[i];
+    dst.cell[p] = gcell[e - n];
+    return;  // the other fields of a ghost are never read
+  }
   dst.posh[p] = src.posh[i];
   dst.vhm[p] = src.vhm[i];
   dst.dpo[p] = src.dpo[i];
@@ -295,6 +419,36 @@ __global__ void k_permute(int n, const int* __restrict__ perm, Store src, Store 
   dst.iam[p] = src.iam[i];
   dst.orig[p] = src.orig[i];
   dst.cell[p] = src.cell[i];
+}
+
+// For the ghost at store position n + g: its origin's new store position and
+// its image code (sx+1) + 3(sy+1) + 9(sz+1).
+__global__ void k_ghost_link(int n, int ng, const int* __restrict__ perm,
+                             const int* __restrict__ inv, const int* __restrict__ gsrc,
+                             const int* __restrict__ gord, int* __restrict__ gorig,
+                             int* __restrict__ gcode) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const int e = perm[n + g];
+  gorig[g] = inv[gsrc[e - n]];
+  int sx, sy, sz;
+  ordShift(gord[e - n], sx, sy, sz);
+  gcode[g] = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+}
+// Ghost fields the pair passes read (position and h, vhat and mass, dens,
+// pdr2, omega, porosity) refreshed from their origins: the values a
+// syncParticlesBetweenProcs would ship.
+__global__ void k_ghost_refresh(Store S, int n, int ng, const int* __restrict__ gorig,
+                                const int* __restrict__ gcode, Box B) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const int o = gorig[g];
+  const int code = gcode[g];
+  const double4 q = S.posh[o];
+  const Img im = imagePos(q, code % 3 - 1, (code / 3) % 3 - 1, code / 9 - 1, B);
+  S.posh[n + g] = make_double4(im.x, im.y, im.z, q.w);
+  S.vhm[n + g] = S.vhm[o];
+  S.dpo[n + g] = S.dpo[o];
 }
 
 // ============================================================ SPH pair passes

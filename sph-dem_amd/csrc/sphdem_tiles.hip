@@ -3,31 +3,22 @@
 //
 // Search: one 64-lane wavefront per Segment (whole z-adjacent cells of one
 // (x,y) column of the SPH grid, at most 64 particles; lane = target particle).
-// The wavefront walks the candidate columns of the segment (the 3x3 columns
-// around it, z range extended by one cell, plus periodic images), loads each
-// column range 64 particles at a time into its own LDS slice (coalesced reads
-// of the cell-sorted store) and every lane tests all staged candidates against
-// its particle.  Per lane a candidate is admitted only if the cell of its
-// (image) position lies within +-1 of the lane's cell on every axis -- the
-// reference's 27-cell stencil (dataLL.cpp:1049-1071) -- and passes
-// Cparticle::is_neighbr (particle.h:236-238).  Columns are walked x-outer,
-// then y, then z, particles in original-index order, so each list holds the
-// neighbours in the order of the reference's cached list.
+// The wavefront streams the candidates of the 3x3 columns around its segment
+// (z range extended by one cell) from the cell-sorted store -- real particles
+// and the periodic ghosts, which live in cells of their own key range -- with
+// wave-uniform scalar loads; every lane tests every candidate against its
+// particle.  Columns are walked x-outer, then y, then z, and within a cell
+// the real particles (original-index order) precede the ghosts (receive
+// order), so each list holds the neighbours in the order of the reference's
+// cached list (dataLL.cpp:1049-1071).
 //
 // Passes: one thread per particle walks its list (every lane busy with an
-// accepted pair, instead of the ~1/6 of lanes a divergent candidate test
-// leaves active), summing in list order.
+// accepted pair), summing in list order.
 #pragma once
 
 namespace sphdem {
 
 constexpr int kWavesPerBlock = 4;
-
-__device__ __forceinline__ void waveSync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // ---- segments -------------------------------------------------------------------
 // Greedy packing of one column's cells into segments of <= 64 particles.
@@ -77,53 +68,6 @@ __global__ void k_seg_write(const int* __restrict__ start, Grid g, const int* __
   });
 }
 
-// Real-cell range along one axis holding every particle whose image (shift s)
-// can fall into the target cells [lo-1, hi+1]; a superset, clipped to the grid.
-__device__ __forceinline__ void axisRange(const Grid& g, const Box& B, int ax, int s, int lo,
-                                          int hi, int& r0, int& r1) {
-  if (s == 0) {
-    r0 = lo - 1;
-    r1 = hi + 1;
-  } else {
-    const double L = (s == 1) ? B.Lp[ax] : B.Lm[ax];
-    const double xlo = g.cmin[ax] + ((lo - 1) - g.off) * g.edge - L;
-    const double xhi = g.cmin[ax] + ((hi + 2) - g.off) * g.edge - L;
-    r0 = cellOf(xlo, g.cmin[ax], g.edge, g.off) - 1;
-    r1 = cellOf(xhi, g.cmin[ax], g.edge, g.off) + 1;
-  }
-  r0 = max(r0, 0);
-  r1 = min(r1, g.dim[ax] - 1);
-}
-
-// Staged candidate common part.
-struct TileBase {
-  double x[64], y[64], z[64];
-  int cell[64];  // cell of the (image) position
-};
-__device__ __forceinline__ void stageBase(TileBase& T, int l, const Store& S, const Grid& g,
-                                          const Box& B, int j, int sx, int sy, int sz,
-                                          double4& p) {
-  p = S.posh[j];
-  if ((sx | sy | sz) == 0) {
-    T.x[l] = p.x;
-    T.y[l] = p.y;
-    T.z[l] = p.z;
-    T.cell[l] = S.cell[j];
-  } else {
-    const Img im = imagePos(p, sx, sy, sz, B);
-    T.x[l] = im.x;
-    T.y[l] = im.y;
-    T.z[l] = im.z;
-    T.cell[l] = packCell(cellOf(im.x, g.cmin[0], g.edge, g.off),
-                         cellOf(im.y, g.cmin[1], g.edge, g.off),
-                         cellOf(im.z, g.cmin[2], g.edge, g.off));
-  }
-}
-__device__ __forceinline__ bool inStencil(int a, int b) {
-  return abs(cellX(a) - cellX(b)) <= 1 && abs(cellY(a) - cellY(b)) <= 1 &&
-         abs(cellZ(a) - cellZ(b)) <= 1;
-}
-
 // Segments [s0, s1) of wavefront w of this workgroup.  Consecutive workgroups
 // land on different XCDs (round robin over 8), so XCD x = blockIdx % 8 gets the
 // x-th eighth of the segments (a spatially coherent slab for its L2) and its
@@ -145,54 +89,35 @@ __device__ __forceinline__ void segRange(int nseg, int w, int& s0, int& s1) {
 // particle t is the 16 bytes at nbl[(c * ns + t) * 4], so a wavefront reading
 // the next chunk of 64 consecutive particles reads 1 KiB of contiguous memory,
 // and the search writes whole chunks.  An entry is the store index of the
-// neighbour with its periodic image code in the top bits.
+// neighbour (a ghost's index for a periodic image).
 constexpr int kMaxNb = 256;  // a multiple of the chunk size
 constexpr int kChunk = 4;
 __device__ __forceinline__ size_t nbChunk(int c, int t, int ns) {
   return ((size_t)c * ns + t) * kChunk;
 }
-constexpr int kEntryShift = 26;  // store index < 2^26
-__device__ __forceinline__ int nbIndex(int e) { return e & ((1 << kEntryShift) - 1); }
-__device__ __forceinline__ int nbCode(int e) { return (int)((unsigned)e >> kEntryShift); }
-__device__ __forceinline__ void codeShift(int code, int& sx, int& sy, int& sz) {
-  sx = code % 3 - 1;
-  sy = (code / 3) % 3 - 1;
-  sz = code / 9 - 1;
-}
-
 // Pointer into the constant address space (scalar loads for uniform indices).
 template <class T>
 using CPtr = const __attribute__((address_space(4))) T*;
 
-// Staged periodic-image candidate of the list build.
-struct ImageTile : TileBase {
-  double h[64];
-  int ent[64];
-};
-
-// calcNeighbours for every SPH-group particle, one wavefront per Segment.
+// calcNeighbours for every real SPH-group particle, one wavefront per Segment.
 //
-// Unshifted candidates: the wavefront streams the column ranges of the 3x3
-// columns around its segment (z extended by one cell) straight from the
-// cell-sorted store with wave-uniform loads, four candidates per step.  The
-// column is within +-1 of the lane's cell by construction, so the 27-cell
-// stencil reduces to "j lies in the store range of cells cz-1..cz+1 of this
-// column", an integer range test per lane; is_neighbr (particle.h:236-238) is
-// len2(ra - rb) <= (2 max(ha, hb))^2 with blitz's dot association.
-//
-// Periodic images (shift +-L per periodic axis) are staged in the wavefront's
-// LDS slice with the cell of the image position, which is tested against the
-// lane's cell on every axis.  They follow the unshifted candidates in the list.
-__global__ void __launch_bounds__(256) k_lists(Store S, const double4* __restrict__ pos,
-                                               const int* __restrict__ cellc, Grid g,
-                                               const int* __restrict__ start, Box B,
+// Per candidate column (X, Y): the real cells Z0..Z1 hold store range
+// [start[k0+Z0], start[k0+Z1+1]), the ghost cells the same range of the ghost
+// keys.  Without ghosts the whole range is one stream and the 27-cell stencil
+// reduces to "j lies in the store range of cells cz-1..cz+1", an integer range
+// per lane; with ghosts the column is streamed cell by cell (the real part of
+// a cell, then its ghosts) with the lane's |z - cz| <= 1 test.  is_neighbr
+// (particle.h:236-238) is len2(ra - rb) <= (2 max(ha, hb))^2 with blitz's dot
+// association.  Candidates are loaded four at a time with scalar loads,
+// issued one batch ahead of the arithmetic.
+__global__ void __launch_bounds__(256) k_lists(const double4* __restrict__ pos,
+                                               const int* __restrict__ cellc, Grid g, int gbase,
+                                               const int* __restrict__ start,
                                                const Segment* __restrict__ segs,
                                                const int* __restrict__ nsegPtr, int ns,
                                                int* __restrict__ nbl, int* __restrict__ nbcnt,
                                                DevGlobals* dg) {
-  __shared__ ImageTile tiles[kWavesPerBlock];
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  ImageTile& T = tiles[w];
   // Every address of the candidate stream is wave-uniform and nothing here
   // writes the store, the cell starts or the segments, so they are read through
   // the constant address space: scalar loads, one per candidate per wavefront.
@@ -201,7 +126,6 @@ __global__ void __launch_bounds__(256) k_lists(Store S, const double4* __restric
   const CPtr<int> csegs = (CPtr<int>)segs;
   int s0, s1;
   segRange(*(CPtr<int>)nsegPtr, w, s0, s1);
-  const bool anyPeriodic = (B.periodic[0] | B.periodic[1] | B.periodic[2]) != 0;
   for (int si = s0; si < s1; si++) {
     const Segment sg{csegs[4 * si], csegs[4 * si + 1], csegs[4 * si + 2], csegs[4 * si + 3]};
     const int t = sg.p0 + lane;
@@ -210,17 +134,16 @@ __global__ void __launch_bounds__(256) k_lists(Store S, const double4* __restric
     const int z0 = sg.zz & 0xffff, z1 = sg.zz >> 16;
     const int Z0 = max(z0 - 1, 0), Z1 = min(z1 + 1, g.dim[2] - 1);
     double px = 0.0, py = 0.0, pz = 0.0, ha = 0.0;
-    int tcell = 0;
+    int cz = -4;  // idle lanes: no cell is within one of it
     if (act) {
       const double4 p = pos[t];
       px = p.x;
       py = p.y;
       pz = p.z;
       ha = p.w;
-      tcell = cellc[t];
+      cz = cellZ(cellc[t]);
     }
-    const int cz = cellZ(tcell);
-    int c = 0;      // entries found
+    int c = 0;                         // entries found
     int4 buf = make_int4(0, 0, 0, 0);  // the chunk being filled
     auto push = [&](int e) {
       if (c < kMaxNb) {
@@ -233,90 +156,63 @@ __global__ void __launch_bounds__(256) k_lists(Store S, const double4* __restric
       }
       c++;
     };
-    const int X0 = max(cx - 1, 0), X1 = min(cx + 1, g.dim[0] - 1);
-    const int Y0 = max(cy - 1, 0), Y1 = min(cy + 1, g.dim[1] - 1);
-    for (int X = X0; X <= X1; X++)
-      for (int Y = Y0; Y <= Y1; Y++) {
-        const int k0 = g.base + (X * g.dim[1] + Y) * g.dim[2];
+    // candidates [J0, J1), admitted for a lane when j lies in [lo, lo + wd)
+    auto stream = [&](int J0, int J1, int lo, int wd) {
+      double4 qn[4];
+      auto fetch = [&](int jb) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int jq = 4 * min(jb + u, J1 - 1);
+          qn[u] = make_double4(cpos[jq], cpos[jq + 1], cpos[jq + 2], cpos[jq + 3]);
+        }
+      };
+      fetch(J0);
+      for (int j = J0; j < J1; j += 4) {
+        double4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) q[u] = qn[u];
+        if (j + 4 < J1) fetch(j + 4);
+        bool ok[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int jj = j + u;
+          const double dx = px - q[u].x, dy = py - q[u].y, dz = pz - q[u].z;
+          const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+          // h > 0 and never NaN, so fmax (one instruction) equals std::max
+          const double lim = 2.0 * fmax(ha, q[u].w);
+          ok[u] = (int)(jj < J1) & (int)((unsigned)(jj - lo) < (unsigned)wd) & (int)(jj != t) &
+                  (int)(l2 <= lim * lim);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (ok[u]) push(j + u);
+      }
+    };
+    for (int X = max(cx - 1, 0); X <= min(cx + 1, g.dim[0] - 1); X++)
+      for (int Y = max(cy - 1, 0); Y <= min(cy + 1, g.dim[1] - 1); Y++) {
+        const int col = (X * g.dim[1] + Y) * g.dim[2];
+        const int k0 = g.base + col, q0 = gbase + col;
         const int J0 = cstart[k0 + Z0], J1 = cstart[k0 + Z1 + 1];
-        if (J0 >= J1) continue;
-        // store range of this column's cells cz-1 .. cz+1 (empty for idle lanes)
-        int lo = 0, wd = 0;
-        if (act) {
-          lo = start[k0 + max(cz - 1, 0)];
-          wd = start[k0 + min(cz + 2, g.dim[2])] - lo;
-        }
-        for (int j = J0; j < J1; j += 4) {
-          double4 q[4];
-#pragma unroll
-          for (int u = 0; u < 4; u++) {
-            const int jq = 4 * min(j + u, J1 - 1);
-            q[u] = make_double4(cpos[jq], cpos[jq + 1], cpos[jq + 2], cpos[jq + 3]);
+        const bool ghosts = gbase > 0 && cstart[q0 + Z0] != cstart[q0 + Z1 + 1];
+        if (!ghosts) {
+          if (J0 >= J1) continue;
+          // store range of this column's cells cz-1 .. cz+1 (empty for idle lanes)
+          int lo = 0, wd = 0;
+          if (act) {
+            lo = start[k0 + max(cz - 1, 0)];
+            wd = start[k0 + min(cz + 2, g.dim[2])] - lo;
           }
-          bool ok[4];
-#pragma unroll
-          for (int u = 0; u < 4; u++) {
-            const int jj = j + u;
-            const double dx = px - q[u].x, dy = py - q[u].y, dz = pz - q[u].z;
-            const double l2 = dot3(dx, dy, dz, dx, dy, dz);
-            // h > 0 and never NaN, so fmax (one instruction) equals std::max
-            const double lim = 2.0 * fmax(ha, q[u].w);
-            ok[u] = (int)(jj < J1) & (int)((unsigned)(jj - lo) < (unsigned)wd) & (int)(jj != t) &
-                    (int)(l2 <= lim * lim);
-          }
-#pragma unroll
-          for (int u = 0; u < 4; u++)
-            if (ok[u]) push((j + u) | (13 << kEntryShift));
-        }
-      }
-    if (anyPeriodic) {
-      const int nsx = B.periodic[0] ? 3 : 1, nsy = B.periodic[1] ? 3 : 1,
-                nsz = B.periodic[2] ? 3 : 1;
-      for (int ix = 0; ix < nsx; ix++) {
-        const int sx = ix == 0 ? 0 : (ix == 1 ? -1 : 1);
-        int XA, XB;
-        axisRange(g, B, 0, sx, cx, cx, XA, XB);
-        if (XA > XB) continue;
-        for (int iy = 0; iy < nsy; iy++) {
-          const int sy = iy == 0 ? 0 : (iy == 1 ? -1 : 1);
-          int YA, YB;
-          axisRange(g, B, 1, sy, cy, cy, YA, YB);
-          if (YA > YB) continue;
-          for (int iz = 0; iz < nsz; iz++) {
-            const int sz = iz == 0 ? 0 : (iz == 1 ? -1 : 1);
-            if ((sx | sy | sz) == 0) continue;  // unshifted: done above
-            int ZA, ZB;
-            axisRange(g, B, 2, sz, z0, z1, ZA, ZB);
-            if (ZA > ZB) continue;
-            const int code = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
-            for (int X = XA; X <= XB; X++)
-              for (int Y = YA; Y <= YB; Y++) {
-                const int k0 = g.base + (X * g.dim[1] + Y) * g.dim[2];
-                const int j0 = start[k0 + ZA], j1 = start[k0 + ZB + 1];
-                for (int b0 = j0; b0 < j1; b0 += 64) {
-                  const int m = min(64, j1 - b0);
-                  waveSync();
-                  if (lane < m) {
-                    double4 p;
-                    stageBase(T, lane, S, g, B, b0 + lane, sx, sy, sz, p);
-                    T.h[lane] = p.w;
-                    T.ent[lane] = (b0 + lane) | (code << kEntryShift);
-                  }
-                  waveSync();
-                  if (act) {
-                    for (int k = 0; k < m; k++) {
-                      const double dx = px - T.x[k], dy = py - T.y[k], dz = pz - T.z[k];
-                      const double l2 = dot3(dx, dy, dz, dx, dy, dz);
-                      const double lim = 2.0 * dmax2(ha, T.h[k]);
-                      if (inStencil(tcell, T.cell[k]) && l2 <= lim * lim) push(T.ent[k]);
-                    }
-                  }
-                }
-              }
+          stream(J0, J1, lo, wd);
+        } else {
+          for (int z = Z0; z <= Z1; z++) {
+            const bool near = abs(z - cz) <= 1;
+            const int r0 = cstart[k0 + z], r1 = cstart[k0 + z + 1];
+            if (r0 < r1) stream(r0, r1, r0, near ? r1 - r0 : 0);
+            const int h0 = cstart[q0 + z], h1 = cstart[q0 + z + 1];
+            if (h0 < h1) stream(h0, h1, h0, near ? h1 - h0 : 0);
           }
         }
       }
-    }
     if (act) {
       if (c > kMaxNb) {
         atomicOr(&dg->errflags, 2u);
@@ -329,39 +225,30 @@ __global__ void __launch_bounds__(256) k_lists(Store S, const double4* __restric
 }
 
 // The cached lists as (a, b, image) triples of original indices, for the
-// parity enumeration of the SPH pair set.
-__global__ void k_list_pairs(Store S, int ns, const int* __restrict__ nbl,
-                             const int* __restrict__ nbcnt, const long long* __restrict__ off,
-                             int* __restrict__ oa, int* __restrict__ ob, int* __restrict__ oimg) {
+// parity enumeration of the SPH pair set: a ghost neighbour reports its
+// origin and its image code.
+__global__ void k_list_pairs(Store S, int ns, int n, const int* __restrict__ nbl,
+                             const int* __restrict__ nbcnt, const int* __restrict__ gcode,
+                             const long long* __restrict__ off, int* __restrict__ oa,
+                             int* __restrict__ ob, int* __restrict__ oimg) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= ns) return;
-  const int n = nbcnt[a];
+  const int cnt = nbcnt[a];
   const int ia = S.orig[a];
   long long w = off[a];
-  for (int k = 0; k < n; k++, w++) {
+  for (int k = 0; k < cnt; k++, w++) {
     const int e = nbl[nbChunk(k >> 2, a, ns) + (k & 3)];
     oa[w] = ia;
-    ob[w] = S.orig[nbIndex(e)];
-    oimg[w] = nbCode(e);
+    ob[w] = S.orig[e];
+    oimg[w] = e >= n ? gcode[e - n] : 13;
   }
 }
 
-// Loads neighbour entry e: store index and (image) position.
-__device__ __forceinline__ void imageFix(const Box& B, int e, double4& p) {
-  const int code = nbCode(e);
-  if (code != 13) {
-    int sx, sy, sz;
-    codeShift(code, sx, sy, sz);
-    const Img im = imagePos(p, sx, sy, sz, B);
-    p.x = im.x;
-    p.y = im.y;
-    p.z = im.z;
-  }
-}
 // Batched list walk: U entries are fetched, then their gathers are issued
 // together (one memory round trip per batch instead of two per neighbour), and
 // the arithmetic runs over the batch in list order.  Every neighbour in the
-// SPH group is sph or sphBoundary, so no type needs to be read.
+// SPH group is sph or sphBoundary (or a ghost of one, already at its image
+// position), so no type needs to be read.
 #define SPHDEM_FETCH_ENTRIES(U)                                                    \
   int e[U];                                                                        \
   _Pragma("unroll") for (int v = 0; v < U / kChunk; v++) {                         \
@@ -369,7 +256,7 @@ __device__ __forceinline__ void imageFix(const Box& B, int e, double4& p) {
     const int ce[4] = {ch.x, ch.y, ch.z, ch.w};                                      \
     _Pragma("unroll") for (int i = 0; i < kChunk; i++) {                           \
       const int u = v * kChunk + i;                                                \
-      e[u] = (k0 + u < n) ? ce[i] : (a | (13 << kEntryShift));                     \
+      e[u] = (k0 + u < n) ? ce[i] : a;                                             \
     }                                                                              \
   }
 
@@ -427,14 +314,13 @@ __global__ void __launch_bounds__(256) k_omega_l(Store S, int ns, Box B,
     double m[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int j = nbIndex(e[u]);
+      const int j = e[u];
       p[u] = S.posh[j];
       m[u] = S.vhm[j].w;
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       if (k0 + u >= n) break;
-      imageFix(B, e[u], p[u]);
       const double dx = pa.x - p[u].x, dy = pa.y - p[u].y, dz = pa.z - p[u].z;
       const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
       om += m[u] * wendPdWhi(wcon, divr(r, ha, rha), ha, rha, i3, c3);
@@ -461,14 +347,13 @@ __global__ void __launch_bounds__(256) k_density_l(Store S, int ns, Box B,
     double m[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int j = nbIndex(e[u]);
+      const int j = e[u];
       p[u] = S.posh[j];
       m[u] = S.vhm[j].w;
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       if (k0 + u >= n) break;
-      imageFix(B, e[u], p[u]);
       const double dx = pa.x - p[u].x, dy = pa.y - p[u].y, dz = pa.z - p[u].z;
       const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
       rho += m[u] * wendWi(wcon, divr(r, ha, rha), i3);
@@ -500,7 +385,7 @@ __global__ void __launch_bounds__(256) k_dddt_l(Store S, int ns, Box B,
     double2 hi[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int j = nbIndex(e[u]);
+      const int j = e[u];
       p[u] = S.posh[j];
       v[u] = S.vhm[j];
       if (!VARH) hi[u] = hinv[j];
@@ -508,7 +393,6 @@ __global__ void __launch_bounds__(256) k_dddt_l(Store S, int ns, Box B,
 #pragma unroll
     for (int u = 0; u < U; u++) {
       if (k0 + u >= n) break;
-      imageFix(B, e[u], p[u]);
       const double dx = pa.x - p[u].x, dy = pa.y - p[u].y, dz = pa.z - p[u].z;
       const double dv0 = va.x - v[u].x, dv1 = va.y - v[u].y, dv2 = va.z - v[u].z;
       const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
@@ -563,7 +447,7 @@ __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
       double2 d[U], hi[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        const int j = nbIndex(e[u]);
+        const int j = e[u];
         p[u] = S.posh[j];
         v[u] = S.vhm[j];
         d[u] = *reinterpret_cast<const double2*>(&S.dpo[j]);  // dens, pdr2
@@ -572,8 +456,7 @@ __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
 #pragma unroll
       for (int u = 0; u < U; u++) {
         if (k0 + u >= n) break;
-        imageFix(B, e[u], p[u]);
-        const double hb = p[u].w, mb = v[u].w, rhob = d[u].x, pdr2b = d[u].y;
+          const double hb = p[u].w, mb = v[u].w, rhob = d[u].x, pdr2b = d[u].y;
         const double dx = pa.x - p[u].x, dy = pa.y - p[u].y, dz = pa.z - p[u].z;
         const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
         const double qa = divr(r, ha, rha), qb = divr(r, hb, hi[u].x);

@@ -579,6 +579,13 @@ class Sim:
             self.ctx, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
         return out
 
+    def pair_count(self, kind: int) -> int:
+        """Number of pairs of one neighbour search (the cached lists for SPH)."""
+        cnt = ctypes.c_int64()
+        self._check(self.lib.sphdem_get_pairs(self.ctx, kind, ctypes.byref(cnt), None, None,
+                                              None, 0))
+        return cnt.value
+
     def pairs(self, kind: int) -> np.ndarray:
         cnt = ctypes.c_int64()
         self._check(self.lib.sphdem_get_pairs(self.ctx, kind, ctypes.byref(cnt), None, None,
