@@ -70,7 +70,22 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="budget of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run a plain 512 MiB device-to-device copy (three times): the "
+                        "reference kernel that scales rocprofv3's FETCH_SIZE / WRITE_SIZE")
     return p.parse_args()
+
+
+def copy_calibration():
+    import torch
+
+    n = 64 << 20  # doubles: 512 MiB
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 3}), flush=True)
 
 
 def cpu_baseline(case, budget_s):
@@ -128,6 +143,9 @@ def traffic_from_summary(path, kernel):
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(the path shards by space, see the design notes)", file=sys.stderr)
