@@ -133,7 +133,11 @@ inline int blocks(int n) { return (n + TPB - 1) / TPB; }
 // kernels that give each item a whole 64-lane wavefront
 inline int waveBlocks(int n) { return (n + TPB / 64 - 1) / (TPB / 64); }
 
+// The store keeps kStoreSlack entries after its capacity: the list build reads
+// candidates in whole batches of four past the end of a range.
+constexpr int kStoreSlack = 4;
 int allocStore(sphdem_ctx* ctx, Store& s, void*& mem, int64_t cap) {
+  cap += kStoreSlack;
   const size_t vec = sizeof(double4) * (size_t)cap;
   const size_t total = vec * kNumVec + 3 * sizeof(int) * (size_t)cap;
   HIPC(hipMalloc(&mem, total));

@@ -133,38 +133,39 @@ __global__ void __launch_bounds__(256) k_lists(const double4* __restrict__ pos,
     const int cx = sg.col / g.dim[1], cy = sg.col - cx * g.dim[1];
     const int z0 = sg.zz & 0xffff, z1 = sg.zz >> 16;
     const int Z0 = max(z0 - 1, 0), Z1 = min(z1 + 1, g.dim[2] - 1);
-    double px = 0.0, py = 0.0, pz = 0.0, ha = 0.0;
+    double px = 0.0, py = 0.0, pz = 0.0, lima2 = 0.0;
     int cz = -4;  // idle lanes: no cell is within one of it
     if (act) {
       const double4 p = pos[t];
       px = p.x;
       py = p.y;
       pz = p.z;
-      ha = p.w;
+      lima2 = (2.0 * p.w) * (2.0 * p.w);
       cz = cellZ(cellc[t]);
     }
-    int c = 0;                         // entries found
-    int4 buf = make_int4(0, 0, 0, 0);  // the chunk being filled
+    int c = 0;  // entries found
+    // the chunk being filled, as a shift register: b0 the newest entry
+    int b0 = 0, b1 = 0, b2 = 0, b3 = 0;
     auto push = [&](int e) {
       if (c < kMaxNb) {
-        const int sl = c & (kChunk - 1);
-        buf.x = sl == 0 ? e : buf.x;
-        buf.y = sl == 1 ? e : buf.y;
-        buf.z = sl == 2 ? e : buf.z;
-        buf.w = sl == 3 ? e : buf.w;
-        if (sl == kChunk - 1) *reinterpret_cast<int4*>(nbl + nbChunk(c >> 2, t, ns)) = buf;
+        b3 = b2;
+        b2 = b1;
+        b1 = b0;
+        b0 = e;
+        if ((c & (kChunk - 1)) == kChunk - 1)
+          *reinterpret_cast<int4*>(nbl + nbChunk(c >> 2, t, ns)) = make_int4(b3, b2, b1, b0);
       }
       c++;
     };
     // candidates [J0, J1), admitted for a lane when j lies in [lo, lo + wd)
     auto stream = [&](int J0, int J1, int lo, int wd) {
+      // one contiguous 128-byte read per batch: entries past J1 (the store
+      // keeps slack after its last entry) are read and masked out
       double4 qn[4];
       auto fetch = [&](int jb) {
+        const CPtr<double> b = cpos + 4 * jb;
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int jq = 4 * min(jb + u, J1 - 1);
-          qn[u] = make_double4(cpos[jq], cpos[jq + 1], cpos[jq + 2], cpos[jq + 3]);
-        }
+        for (int u = 0; u < 4; u++) qn[u] = make_double4(b[4 * u], b[4 * u + 1], b[4 * u + 2], b[4 * u + 3]);
       };
       fetch(J0);
       for (int j = J0; j < J1; j += 4) {
@@ -178,10 +179,13 @@ __global__ void __launch_bounds__(256) k_lists(const double4* __restrict__ pos,
           const int jj = j + u;
           const double dx = px - q[u].x, dy = py - q[u].y, dz = pz - q[u].z;
           const double l2 = dot3(dx, dy, dz, dx, dy, dz);
-          // h > 0 and never NaN, so fmax (one instruction) equals std::max
-          const double lim = 2.0 * fmax(ha, q[u].w);
+          // l2 <= (2 max(ha,hb))^2  <=>  l2 <= (2ha)^2 || l2 <= (2hb)^2: squaring
+          // and rounding are monotone on h > 0.  2hb is formed exactly on the
+          // scalar unit by adding one to the exponent of hb (a normal number).
+          const double hb2 =
+              __longlong_as_double(__double_as_longlong(q[u].w) + (1ll << 52));
           ok[u] = (int)(jj < J1) & (int)((unsigned)(jj - lo) < (unsigned)wd) & (int)(jj != t) &
-                  (int)(l2 <= lim * lim);
+                  ((int)(l2 <= lima2) | (int)(l2 <= hb2 * hb2));
         }
 #pragma unroll
         for (int u = 0; u < 4; u++)
@@ -218,7 +222,11 @@ __global__ void __launch_bounds__(256) k_lists(const double4* __restrict__ pos,
         atomicOr(&dg->errflags, 2u);
         c = kMaxNb;
       }
-      if (c & (kChunk - 1)) *reinterpret_cast<int4*>(nbl + nbChunk(c >> 2, t, ns)) = buf;
+      const int rem = c & (kChunk - 1);  // entries of the last, partial chunk
+      if (rem)
+        *reinterpret_cast<int4*>(nbl + nbChunk(c >> 2, t, ns)) =
+            rem == 1 ? make_int4(b0, 0, 0, 0)
+                     : (rem == 2 ? make_int4(b1, b0, 0, 0) : make_int4(b2, b1, b0, 0));
       nbcnt[t] = c;
     }
   }
