@@ -338,6 +338,84 @@ def _finish(name, P, fluid, walls, dem, hookset, wall_dens, info):
     return Case(name, P, r, z.copy(), z.copy(), mass, h, dens, iam, hookset, info)
 
 
+# ---- files for the C++ host (sph-dem_amd/host): constants and initial conditions
+_FLAG_DEFINES = [
+    (F_VAR_H_CORRECTION, "VAR_H_CORRECTION"), (F_CONST_H, "CONST_H"), (F_LIQ_DEM, "LIQ_DEM"),
+    (F_VISC_MORRIS, "VISC_MORRIS"), (F_VISC_ARTIFICIAL, "VISC_ARTIFICIAL"),
+    (F_CUSTOM_WALL, "LIQ_DEM_CUSTOM_WALL_CONTACTS"), (F_NO_DEM_CONTACTS, "NO_DEM_CONTACTS"),
+    (F_FIXED_DEM, "FIXED_DEM"),
+]
+
+
+def custom_constants_header(P: Params, title: str) -> str:
+    """A customConstants.h with the names the reference's core headers read,
+    holding exactly the doubles of P (17 significant digits round-trip)."""
+    def d(x):
+        return repr(float(x))
+
+    out = [f"// customConstants.h -- {title}.", "// Generated by sphdem.custom_constants_header().",
+           "#pragma once", ""]
+    for bit, name in _FLAG_DEFINES:
+        if P.flags & bit:
+            out.append(f"#define {name}")
+    out.append("")
+    out.append("const int PERIODIC[3] = {%d, %d, %d};" % tuple(P.periodic))
+    out.append("const double RMIN[3] = {%s, %s, %s};" % tuple(d(x) for x in P.rmin))
+    out.append("const double RMAX[3] = {%s, %s, %s};" % tuple(d(x) for x in P.rmax))
+    ints = [("REINIT_DENS_EVERY_N_STEPS", P.reinit_every), ("REINIT_DENS_AT_N_STEPS", P.reinit_at),
+            ("DAMPTIME", P.damptime)]
+    for name, v in ints:
+        out.append(f"const int {name} = {int(v)};")
+    dbl = [("PI", P.pi), ("DENS", P.dens), ("REFD", P.refd), ("GAMMA", P.gamma),
+           ("HFAC", P.hfac), ("H", P.h), ("PSEP", P.psep), ("SPSOUND", P.spsound),
+           ("PRB", P.prb), ("VISCOSITY", P.viscosity), ("ALPHA_ARTIFICIAL", P.alpha_artificial),
+           ("WCON_WENDLAND", P.wcon), ("MAXDT", P.maxdt), ("DEM_RADIUS", P.dem_radius),
+           ("DEM_DENS", P.dem_dens), ("DEM_VOL", P.dem_vol), ("DEM_K", P.dem_k),
+           ("DEM_GAMMA", P.dem_gamma), ("DEM_SEARCH_RADIUS", P.dem_search_radius),
+           ("COUPLING_RADIUS", P.coupling_radius), ("TIME_DROP_PARTICLE", P.time_drop),
+           ("DEM_TIMESTEP", P.dem_timestep), ("LIQ_DEM_TIMESTEP", P.liq_dem_timestep),
+           ("POROSITY_INCR", P.porosity_incr), ("DRAG_A", P.drag_a), ("DRAG_B", P.drag_b)]
+    for name, v in dbl:
+        out.append(f"const double {name} = {d(v)};")
+    return "\n".join(out) + "\n"
+
+
+IC_MAGIC = b"SPHDEMIC"
+STATE_MAGIC = b"SPHDEMST"
+
+
+def write_ic(case: Case, path: str):
+    """Initial condition file the C++ host's run driver reads: magic, int64 n,
+    then r, v, vhat (3n doubles each), mass, h, dens (n doubles) and iam (n int32)."""
+    with open(path, "wb") as f:
+        f.write(IC_MAGIC)
+        f.write(np.int64(case.n).tobytes())
+        for a in (case.r, case.v, case.vhat):
+            f.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+        for a in (case.mass, case.h, case.dens):
+            f.write(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+        f.write(np.ascontiguousarray(case.iam, dtype=np.int32).tobytes())
+
+
+STATE_FIELDS = [("r", 3), ("v", 3), ("vhat", 3), ("f", 3), ("dens", 1), ("h", 1),
+                ("press", 1), ("omega", 1), ("porosity", 1)]
+
+
+def read_state(path: str) -> dict:
+    """State file written by the C++ run driver: magic, int64 n, time, dt, then
+    the STATE_FIELDS arrays per original index."""
+    with open(path, "rb") as f:
+        if f.read(8) != STATE_MAGIC:
+            raise ValueError(f"{path}: not a state file")
+        n = int(np.frombuffer(f.read(8), dtype=np.int64)[0])
+        time, dt = np.frombuffer(f.read(16), dtype=np.float64)
+        out = {"time": float(time), "dt": float(dt)}
+        for name, w in STATE_FIELDS:
+            a = np.frombuffer(f.read(8 * w * n), dtype=np.float64)
+            out[name] = a.reshape(n, w) if w == 3 else a.copy()
+    return out
+
+
 def dambreak_case(nf: int = 32, dx: float = 0.01, seed: int = 1000, with_dem: int = 0,
                   liq_dem_flags: bool = False) -> Case:
     """3D dam break: an nf^3 fluid block (jittered lattice) in the lower-left

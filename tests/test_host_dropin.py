@@ -1,0 +1,80 @@
+"""The C++ host drop-in (sph-dem_amd/host): the reference's CdataLL /
+CsphIncompress / CcustomSim surface over the C ABI, with the sediment3D case
+written against it (host functors through traverse<F, If>)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import sphdem as sd
+
+CASE_DIR = os.path.join(os.path.dirname(sd.LIB_PATH), "host", "cases", "sediment3D")
+RUN = os.path.join(CASE_DIR, "run")
+DUMP = os.path.join(CASE_DIR, "dump_params")
+
+
+def host_case():
+    """The case the committed customConstants.h was generated from."""
+    return sd.sediment_case(nf=12, n_dem=60)
+
+
+def built(path):
+    if not os.path.exists(path):
+        pytest.fail(f"{path} is not built; run __graft_entry__.build()")
+    return path
+
+
+def test_committed_constants_match_the_generator():
+    c = host_case()
+    want = sd.custom_constants_header(
+        c.params, "SPH-DEM sedimentation, 12^3 fluid + 60 DEM (the host test case)")
+    with open(os.path.join(CASE_DIR, "customConstants.h")) as f:
+        assert f.read() == want
+
+
+def test_constants_bridge_reproduces_the_parameters():
+    """paramsFromConstants() over the case's customConstants.h gives the same
+    sphdem_params, byte for byte, as the Python binding builds."""
+    out = subprocess.run([built(DUMP)], capture_output=True, text=True, timeout=60, check=True)
+    got = bytes.fromhex(out.stdout.strip())
+    assert got == bytes(host_case().params)
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK),
+                    reason="checks the behaviour without a GPU")
+def test_run_without_gpu_exits_like_the_reference(tmp_path):
+    """No gfx950 device: a message on stderr and exit(-1), as CdataLL's fatal
+    errors do in the reference (dataLL.cpp:200-211); no CPU fallback."""
+    ic = tmp_path / "ic.bin"
+    sd.write_ic(host_case(), str(ic))
+    r = subprocess.run([built(RUN), str(ic), "1", str(tmp_path / "out.bin")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 255
+    assert "gfx950" in r.stderr
+
+
+@pytest.mark.gpu
+def test_host_run_matches_the_library_run(tmp_path):
+    """Three steps of the sediment3D case through the C++ host (host functors
+    for the CcustomSim hooks, pulled from and pushed to the GPU around each
+    traverse) against the same steps through sphdem_run_steps with the device
+    hook set: the hooks do the same arithmetic, so the states agree to the
+    last bits (tolerance 1e-12 normwise)."""
+    c = host_case()
+    ic = tmp_path / "ic.bin"
+    out = tmp_path / "state.bin"
+    sd.write_ic(c, str(ic))
+    r = subprocess.run([built(RUN), str(ic), "3", str(out)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    st = sd.read_state(str(out))
+    sim = sd.Sim(c)
+    sim.run(3)
+    g = sim.globals()
+    assert st["time"] == g.time and st["dt"] == g.dt
+    for name, _ in sd.STATE_FIELDS:
+        a, b = st[name], sim.get(name)
+        scale = np.abs(b).max()
+        err = np.abs(a - b).max() / scale if scale else np.abs(a).max()
+        assert err <= 1e-12, f"{name}: {err}"
