@@ -70,6 +70,8 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="budget of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-mixed", action="store_true",
+                   help="skip the mixed-precision form (its timing and error)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a plain 512 MiB device-to-device copy (three times): the "
                         "reference kernel that scales rocprofv3's FETCH_SIZE / WRITE_SIZE")
@@ -141,6 +143,55 @@ def traffic_from_summary(path, kernel):
     return 1024.0 * (fetch * fac["FETCH_SIZE"] + write * fac["WRITE_SIZE"])
 
 
+def _errors(mix, ref, iam):
+    sph, dem = iam == sd.SPH, iam == sd.DEM
+    e = {"dens_rel_max": float(np.max(np.abs(mix["dens"][sph] - ref["dens"][sph])
+                                      / np.abs(ref["dens"][sph])))}
+    if "f" in ref:
+        fs = ref["f"][sph]
+        e["f_sph_normwise"] = float(np.abs(mix["f"][sph] - fs).max() / np.abs(fs).max())
+        if dem.any():
+            fd = ref["f"][dem]
+            e["f_dem_normwise"] = float(np.abs(mix["f"][dem] - fd).max() / np.abs(fd).max())
+    if dem.any():
+        e["dem_r_max_abs"] = float(np.abs(mix["r"][dem] - ref["r"][dem]).max())
+    return e
+
+
+def mixed_form(a, counts, counted, final64):
+    """The mixed-precision form (FP32 pair arithmetic over FP64 positions, lists,
+    sums, integration and dt) on the same workload: its throughput over the same
+    timed steps, and its error against the FP64 form after one step (arithmetic
+    error) and after warmup + timed steps (including the growth of differences
+    along the trajectory)."""
+    one = {}
+    for prec in (sd.FP64, sd.MIXED):
+        s = sd.Sim(sd.sediment_case(nf=a.nf, n_dem=a.ndem, seed=1001), device=0, precision=prec)
+        s.run(1)
+        one[prec] = {k: s.get(k) for k in ("r", "dens", "f")}
+        iam = s.get("iam")
+        s.close()
+    sim = sd.Sim(sd.sediment_case(nf=a.nf, n_dem=a.ndem, seed=1001), device=0,
+                 precision=sd.MIXED)
+    sim.run(a.warmup)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.run(a.steps)
+    sim.sync()
+    el = time.perf_counter() - t0
+    fin = {k: sim.get(k) for k in ("r", "dens")}
+    sim.close()
+    return {
+        "value": counted * a.steps / el,
+        "unit": "particle-steps/s",
+        "ms_per_step": 1e3 * el / a.steps,
+        "dtype": "f32 pair arithmetic, f64 positions/lists/sums/integration/dt",
+        "error_vs_fp64_1_step": _errors(one[sd.MIXED], one[sd.FP64], iam),
+        "error_vs_fp64_after_run": _errors(fin, final64, iam),
+        "steps_compared_after_run": a.warmup + a.steps,
+    }
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -170,7 +221,9 @@ def main():
     g = sim.globals()
     units = counts["fluid"] + counts["wall"]  # SPH-group particles a pair kernel processes
     mean_nb = sim.pair_count(sd.PAIRS_SPH) / max(units, 1)
+    final64 = {k: sim.get(k) for k in ("r", "dens")}
     sim.close()
+    mixed = None if a.no_mixed else mixed_form(a, counts, counted, final64)
     value = counted * a.steps / el
     if a.roofline_kernel in ("fpi", "contacts", "shepsum"):
         units = counts["dem"]
@@ -223,6 +276,8 @@ def main():
             "frac": step_gbs / HBM_PEAK_GBS,
         },
     }
+    if mixed is not None:
+        out["mixed"] = mixed
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, a.cpu_seconds)
     print(json.dumps(out), flush=True)

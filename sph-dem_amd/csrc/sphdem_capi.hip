@@ -14,6 +14,7 @@
 
 #include "sphdem_kernels.hip"
 #include "sphdem_tiles.hip"
+#include "sphdem_mixed.hip"
 
 using namespace sphdem;
 
@@ -545,12 +546,17 @@ int checkLoaded(sphdem_ctx* ctx) {
 }
 
 // ---- passes
+bool mixed(const sphdem_ctx* c) { return c->P.precision == SPHDEM_MIXED; }
 int passOmega(sphdem_ctx* ctx) {
   CHECK(ensureLists(ctx));
   CHECK(refreshGhosts(ctx));
   Timed tm(ctx, KID_OMEGA);
-  hipLaunchKernelGGL(k_omega_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
-                     ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->P.wcon);
+  if (mixed(ctx))
+    hipLaunchKernelGGL(k_omega_m, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                       ctx->ns, ctx->nbl, ctx->nbcnt, ctx->P.wcon);
+  else
+    hipLaunchKernelGGL(k_omega_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                       ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->P.wcon);
   LAUNCHED();
   return 0;
 }
@@ -574,9 +580,14 @@ int computeHinv(sphdem_ctx* ctx) {
 // 2 VISC_ARTIFICIAL, 3 LIQ_DEM)
 template <int F>
 void launchForce(sphdem_ctx* ctx) {
-  hipLaunchKernelGGL((k_force_l<(F & 1) != 0, (F & 2) != 0, (F & 4) != 0, (F & 8) != 0>),
-                     dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
-                     ctx->box, ctx->nbl, ctx->nbcnt, ctx->hinv, ctx->P, ctx->dg);
+  constexpr bool V = (F & 1) != 0, M = (F & 2) != 0, A = (F & 4) != 0, L = (F & 8) != 0;
+  if (mixed(ctx))
+    hipLaunchKernelGGL((k_force_m<V, M, A, L>), dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
+                       ctx->cur, ctx->ns, ctx->nbl, ctx->nbcnt, ctx->hinv, ctx->P, ctx->dg);
+  else
+    hipLaunchKernelGGL((k_force_l<V, M, A, L>), dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
+                       ctx->cur, ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->hinv, ctx->P,
+                       ctx->dg);
 }
 int passForce(sphdem_ctx* ctx) {
   CHECK(ensureLists(ctx));
@@ -601,15 +612,24 @@ int passDddt(sphdem_ctx* ctx) {
   CHECK(refreshGhosts(ctx));
   if (varH(ctx)) {
     Timed tm(ctx, KID_DDDT);
-    hipLaunchKernelGGL(k_dddt_l<true>, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
-                       ctx->cur, ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt,
-                       (const double2*)nullptr, ctx->P.wcon);
+    if (mixed(ctx))
+      hipLaunchKernelGGL(k_dddt_m<true>, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
+                         ctx->cur, ctx->ns, ctx->nbl, ctx->nbcnt, (const double2*)nullptr,
+                         ctx->P.wcon);
+    else
+      hipLaunchKernelGGL(k_dddt_l<true>, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
+                         ctx->cur, ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt,
+                         (const double2*)nullptr, ctx->P.wcon);
   } else {
     CHECK(computeHinv(ctx));
     Timed tm(ctx, KID_DDDT);
-    hipLaunchKernelGGL(k_dddt_l<false>, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
-                       ctx->cur, ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->hinv,
-                       ctx->P.wcon);
+    if (mixed(ctx))
+      hipLaunchKernelGGL(k_dddt_m<false>, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
+                         ctx->cur, ctx->ns, ctx->nbl, ctx->nbcnt, ctx->hinv, ctx->P.wcon);
+    else
+      hipLaunchKernelGGL(k_dddt_l<false>, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
+                         ctx->cur, ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->hinv,
+                         ctx->P.wcon);
   }
   LAUNCHED();
   return 0;
@@ -651,8 +671,12 @@ int passDensity(sphdem_ctx* ctx) {
   LAUNCHED();
   CHECK(ensureLists(ctx));  // the first iteration builds them with the new h
   CHECK(refreshGhosts(ctx));
-  hipLaunchKernelGGL(k_density_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
-                     ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->P.wcon);
+  if (mixed(ctx))
+    hipLaunchKernelGGL(k_density_m, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                       ctx->ns, ctx->nbl, ctx->nbcnt, ctx->P.wcon);
+  else
+    hipLaunchKernelGGL(k_density_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                       ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->P.wcon);
   LAUNCHED();
   return 0;
 }

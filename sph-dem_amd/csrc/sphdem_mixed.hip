@@ -1,0 +1,290 @@
+// sphdem_mixed.hip -- the mixed-precision form of the SPH pair passes.
+//
+// Everything that decides which pairs exist, and everything the integrator
+// carries, stays FP64: positions and their differences (computed in FP64 and
+// then rounded once to FP32), the cached neighbour lists (built by the FP64
+// search, so the pair sets are the reference's exactly), h, the sums (each
+// pair's contribution is accumulated in FP64) and the dt reduction.  The
+// per-pair kernel and force arithmetic runs in FP32 with hardware reciprocal
+// and square root, and uses the factored Wendland forms
+//   F(q)    = -10 WCON (2-q)^3 / h^5             (the reference divides
+//   dK/dh   =  10 WCON q^2 (2-q)^3 / h             -4(2-q)^3(1+2q)+2(2-q)^4 = -10q(2-q)^3
+//                                                  by q, cancelling near q = 0)
+// which are algebraically the reference's (sph.h:74-81, 285-292) without its
+// cancellation near q = 0.  Errors against the FP64 form are measured by the
+// tests and the benchmark.
+#pragma once
+
+namespace sphdem {
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float cube_f(float x) { return x * x * x; }
+
+// F(q)/h^5 factor: -10 WCON (2-q)^3 i5, zero beyond the support
+__device__ __forceinline__ float wendF_m(float wcon10, float q, float i5) {
+  const float t = 2.0f - q;
+  return q <= 2.0f ? -wcon10 * cube_f(t) * i5 : 0.0f;
+}
+// K(q) = WCON (2-q)^4 (1+2q)
+__device__ __forceinline__ float wendK_m(float wcon, float q) {
+  const float t = 2.0f - q;
+  return q <= 2.0f ? wcon * (t * t) * (t * t) * (1.0f + 2.0f * q) : 0.0f;
+}
+// dW/dh = (1/h^3) dK/dh - 3/h^4 K, dK/dh = 10 WCON q^2 (2-q)^3 / h
+__device__ __forceinline__ float wendPdWh_m(float wcon, float q, float rh, float i3, float c3) {
+  const float t = 2.0f - q;
+  const float pdKh = q <= 2.0f ? 10.0f * wcon * (q * q) * cube_f(t) * rh : 0.0f;
+  return i3 * pdKh - c3 * wendK_m(wcon, q);
+}
+
+// FP32 difference of two FP64 positions (rounded once)
+struct DxF {
+  float x, y, z, r2;
+};
+__device__ __forceinline__ DxF dxf(const double4& a, const double4& b) {
+  DxF d;
+  d.x = (float)(a.x - b.x);
+  d.y = (float)(a.y - b.y);
+  d.z = (float)(a.z - b.z);
+  d.r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+  return d;
+}
+
+// ---- calcOmega, mixed --------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_omega_m(Store S, int ns, const int* __restrict__ nbl,
+                                                 const int* __restrict__ nbcnt, double wcon) {
+  const int a = xcdThread();
+  if (a >= ns) return;
+  const double4 pa = S.posh[a];
+  const double ha = pa.w;
+  const float wc = (float)wcon;
+  const float rha = (float)(1.0 / ha), i3 = (float)(1.0 / cube(ha)), c3 = (float)(3.0 / pow4(ha));
+  double om = S.vhm[a].w * (double)wendPdWh_m(wc, 0.0f, rha, i3, c3);
+  const int n = nbcnt[a];
+  constexpr int U = 8;
+  for (int k0 = 0; k0 < n; k0 += U) {
+    SPHDEM_FETCH_ENTRIES(U)
+    double4 p[U];
+    double m[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      p[u] = S.posh[e[u]];
+      m[u] = S.vhm[e[u]].w;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (k0 + u >= n) break;
+      const DxF d = dxf(pa, p[u]);
+      const float q = fsqrt(d.r2) * rha;
+      om += m[u] * (double)wendPdWh_m(wc, q, rha, i3, c3);
+    }
+  }
+  S.dpo[a].z = 1 + ha * om / (S.dpo[a].x * 3);
+}
+
+// ---- calcDensity, mixed ------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_density_m(Store S, int ns, const int* __restrict__ nbl,
+                                                   const int* __restrict__ nbcnt, double wcon) {
+  const int a = xcdThread();
+  if (a >= ns) return;
+  const double4 pa = S.posh[a];
+  const double ha = pa.w;
+  const float wc = (float)wcon;
+  const float rha = (float)(1.0 / ha);
+  const double i3 = 1.0 / cube(ha);
+  double rho = S.vhm[a].w * (i3 * (double)wendK_m(wc, 0.0f));
+  const int n = nbcnt[a];
+  constexpr int U = 8;
+  for (int k0 = 0; k0 < n; k0 += U) {
+    SPHDEM_FETCH_ENTRIES(U)
+    double4 p[U];
+    double m[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      p[u] = S.posh[e[u]];
+      m[u] = S.vhm[e[u]].w;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (k0 + u >= n) break;
+      const DxF d = dxf(pa, p[u]);
+      const float q = fsqrt(d.r2) * rha;
+      rho += m[u] * (i3 * (double)wendK_m(wc, q));
+    }
+  }
+  S.dpo[a].x = rho;
+}
+
+// ---- calcDddtDudt, mixed -----------------------------------------------------------
+template <bool VARH>
+__global__ void __launch_bounds__(256) k_dddt_m(Store S, int ns, const int* __restrict__ nbl,
+                                                const int* __restrict__ nbcnt,
+                                                const double2* __restrict__ hinv, double wcon) {
+  const int a = xcdThread();
+  if (a >= ns) return;
+  const double4 pa = S.posh[a];
+  const double4 va = S.vhm[a];
+  const double ha = pa.w;
+  const float wc10 = (float)(10.0 * wcon);
+  const float rha = (float)(1.0 / ha), i5a = (float)(1.0 / pow5(ha));
+  const float rom = (float)(1.0 / S.dpo[a].z);
+  double dddt = S.vh0d[a].w;
+  const int n = nbcnt[a];
+  constexpr int U = 8;
+  for (int k0 = 0; k0 < n; k0 += U) {
+    SPHDEM_FETCH_ENTRIES(U)
+    double4 p[U], v[U];
+    double2 hi[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      p[u] = S.posh[e[u]];
+      v[u] = S.vhm[e[u]];
+      if (!VARH) hi[u] = hinv[e[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (k0 + u >= n) break;
+      const DxF d = dxf(pa, p[u]);
+      const float dv0 = (float)(va.x - v[u].x), dv1 = (float)(va.y - v[u].y),
+                  dv2 = (float)(va.z - v[u].z);
+      const float r = fsqrt(d.r2);
+      float gF;  // gradW = dx * gF
+      if (VARH) {
+        gF = wendF_m(wc10, r * rha, i5a) * rom;
+      } else {
+        const float Fa = wendF_m(wc10, r * rha, i5a);
+        const float Fb = wendF_m(wc10, r * (float)hi[u].x, (float)hi[u].y);
+        gF = 0.5f * (Fa + Fb);
+      }
+      dddt += v[u].w * (double)(gF * (dv0 * d.x + dv1 * d.y + dv2 * d.z));
+    }
+  }
+  S.vh0d[a].w = dddt;
+}
+
+// ---- calcForce, mixed --------------------------------------------------------------
+template <bool VARH, bool MORRIS, bool ARTIF, bool LIQDEM>
+__global__ void __launch_bounds__(256) k_force_m(Store S, int ns, const int* __restrict__ nbl,
+                                                 const int* __restrict__ nbcnt,
+                                                 const double2* __restrict__ hinv,
+                                                 sphdem_params P, DevGlobals* g) {
+  const int a = xcdThread();
+  const double inf = __longlong_as_double(0x7ff0000000000000ll);
+  double newDt = inf;
+  if (a < ns) {
+    const bool fluid = S.iam[a] == T_SPH;
+    const double4 pa = S.posh[a];
+    const double4 va = S.vhm[a];
+    const double4 da = S.dpo[a];
+    const double ha = pa.w;
+    const float wc10 = (float)(10.0 * P.wcon);
+    const float rha = (float)(1.0 / ha), i5a = (float)(1.0 / pow5(ha));
+    const float rom = VARH ? (float)(1.0 / da.z) : 1.0f;
+    const float rhoa = (float)da.x, pdr2a = (float)da.y;
+    const float nu = (float)P.viscosity, viscLiq = (float)(2.0 * P.viscosity * P.dens);
+    const float cs2 = (float)(2.0 * P.spsound), alpha = (float)P.alpha_artificial;
+    const double vsigC = MORRIS ? 2.0 * P.spsound : 0.0;
+    double minA = inf, minB = inf;
+    double fx = 0.0, fy = 0.0, fz = 0.0, px = 0.0, py = 0.0, pz = 0.0, vx = 0.0, vy = 0.0,
+           vz = 0.0;
+    const int n = nbcnt[a];
+    constexpr int U = 4;
+    for (int k0 = 0; k0 < n; k0 += U) {
+      SPHDEM_FETCH_ENTRIES(U)
+      double4 p[U], v[U];
+      double2 d[U], hi[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        p[u] = S.posh[e[u]];
+        v[u] = S.vhm[e[u]];
+        d[u] = *reinterpret_cast<const double2*>(&S.dpo[e[u]]);
+        hi[u] = hinv[e[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (k0 + u >= n) break;
+        const DxF dx = dxf(pa, p[u]);
+        const float mb = (float)v[u].w, rhob = (float)d[u].x, pdr2b = (float)d[u].y;
+        const float r = fsqrt(dx.r2);
+        const float Fa = wendF_m(wc10, r * rha, i5a);
+        const float Fb = wendF_m(wc10, r * (float)hi[u].x, (float)hi[u].y);
+        // gradients are dx * G; under VAR_H both divide by pa.omega (sph.h:331-332)
+        float Ga, Gb;
+        if (VARH) {
+          Ga = Fa * rom;
+          Gb = Fb * rom;
+        } else {
+          Ga = Gb = 0.5f * (Fa + Fb);
+        }
+        if (fluid) {
+          const float dv0 = (float)(va.x - v[u].x), dv1 = (float)(va.y - v[u].y),
+                      dv2 = (float)(va.z - v[u].z);
+          const float Gbar = 0.5f * (Ga + Gb);
+          const double hav = 0.5 * (ha + p[u].w);
+          const float havf = (float)hav;
+          float fvs = 0.0f;  // fv = dv * fvs
+          float vsig = (float)vsigC;
+          if (MORRIS) {
+            const float rr = frcp(rhoa * rhob);
+            const float visc = LIQDEM ? viscLiq * rr : nu * (rhoa + rhob) * rr;
+            const float dd = Gbar * dx.r2;  // dot(dx, dx * Gbar)
+            const float den = dx.r2 + 0.01f * (havf * havf);
+            fvs = mb * visc * dd * frcp(den);
+          }
+          if (ARTIF) {
+            const float vdr = dx.x * dv0 + dx.y * dv1 + dx.z * dv2;
+            float avsig = 0.0f, avisc = 0.0f;
+            if (vdr < 0.0f) {
+              const float viss = r != 0.0f ? vdr * frcp(r) : 0.0f;
+              avsig = cs2 - 4.0f * viss;
+              avisc = viss * avsig * alpha * frcp(rhoa + rhob);
+            }
+            const float s = mb * avisc * Gbar;
+            fx += (double)(s * dx.x);
+            fy += (double)(s * dx.y);
+            fz += (double)(s * dx.z);
+            vsig = fmaxf(vsig, avsig);
+            newDt = dmin2(newDt, 0.6 * hav / (double)vsig);
+          } else {
+            minA = dmin2(minA, 0.6 * hav);
+          }
+          minB = dmin2(minB, 0.125 * (hav * hav));
+          const double f0 = (double)(dv0 * fvs), f1 = (double)(dv1 * fvs), f2 = (double)(dv2 * fvs);
+          vx += f0;
+          vy += f1;
+          vz += f2;
+          fx += f0;
+          fy += f1;
+          fz += f2;
+        }
+        const float pp = -mb * (pdr2a * Ga + pdr2b * Gb);  // pressure force = dx * pp
+        const double q0 = (double)(pp * dx.x), q1 = (double)(pp * dx.y), q2 = (double)(pp * dx.z);
+        px += q0;
+        py += q1;
+        pz += q2;
+        fx += q0;
+        fy += q1;
+        fz += q2;
+      }
+    }
+    if (!ARTIF && minA != inf) newDt = dmin2(newDt, minA / vsigC);
+    if (minB != inf) newDt = dmin2(newDt, minB / P.viscosity);
+    double4 f = S.frc[a], fpa = S.fp[a], fva = S.fv[a];
+    f.x += fx;
+    f.y += fy;
+    f.z += fz;
+    fpa.x += px;
+    fpa.y += py;
+    fpa.z += pz;
+    fva.x += vx;
+    fva.y += vy;
+    fva.z += vz;
+    S.frc[a] = f;
+    S.fp[a] = fpa;
+    S.fv[a] = fva;
+  }
+  atomicMinPos(&g->newDt_bits, newDt);
+}
+
+}  // namespace sphdem
