@@ -61,12 +61,15 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=200)
     p.add_argument("--warmup", type=int, default=10)
-    p.add_argument("--nf", type=int, default=64, help="fluid particles per axis")
-    p.add_argument("--ndem", type=int, default=4096)
+    p.add_argument("--workload", choices=("sediment", "large"), default="sediment",
+                   help="sediment: 64^3 sedimentation (the headline); large: the 160^3 "
+                        "dam-break geometry with 65,536 DEM and the SPH-DEM flag set")
+    p.add_argument("--nf", type=int, default=None, help="fluid particles per axis")
+    p.add_argument("--ndem", type=int, default=None)
     p.add_argument("--roofline-kernel", default="lists")
-    p.add_argument("--traffic-file",
-                   default=os.path.join(ROOT, "profiles", "traffic_sediment_64cubed.txt"),
-                   help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary to take `traffic` from")
+    p.add_argument("--traffic-file", default=None,
+                   help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary to take `traffic` from "
+                        "(default: the committed summary of the default workload)")
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="budget of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
@@ -143,6 +146,24 @@ def traffic_from_summary(path, kernel):
     return 1024.0 * (fetch * fac["FETCH_SIZE"] + write * fac["WRITE_SIZE"])
 
 
+def make_case(a):
+    if a.workload == "large":
+        nf = a.nf or 160
+        return sd.dambreak_case(nf=nf, dx=0.32 / nf, seed=1002,
+                                with_dem=a.ndem if a.ndem is not None else 65536,
+                                liq_dem_flags=True)
+    nf = a.nf or 64
+    return sd.sediment_case(nf=nf, n_dem=a.ndem if a.ndem is not None else 4096, seed=1001)
+
+
+def workload_name(a, counts):
+    if a.workload == "large":
+        return (f"SPH-DEM dam-break geometry {round(counts['fluid'] ** (1 / 3))}^3 fluid + "
+                f"{counts['wall']} wall + {counts['dem']} DEM, closed box, SPH-DEM flags, FP64")
+    return (f"SPH-DEM sedimentation {round(counts['fluid'] ** (1 / 3))}^3 fluid + "
+            f"{counts['wall']} wall + {counts['dem']} DEM, periodic x/y, sediment3D flags, FP64")
+
+
 def _errors(mix, ref, iam):
     sph, dem = iam == sd.SPH, iam == sd.DEM
     e = {"dens_rel_max": float(np.max(np.abs(mix["dens"][sph] - ref["dens"][sph])
@@ -166,13 +187,12 @@ def mixed_form(a, counts, counted, final64):
     along the trajectory)."""
     one = {}
     for prec in (sd.FP64, sd.MIXED):
-        s = sd.Sim(sd.sediment_case(nf=a.nf, n_dem=a.ndem, seed=1001), device=0, precision=prec)
+        s = sd.Sim(make_case(a), device=0, precision=prec)
         s.run(1)
         one[prec] = {k: s.get(k) for k in ("r", "dens", "f")}
         iam = s.get("iam")
         s.close()
-    sim = sd.Sim(sd.sediment_case(nf=a.nf, n_dem=a.ndem, seed=1001), device=0,
-                 precision=sd.MIXED)
+    sim = sd.Sim(make_case(a), device=0, precision=sd.MIXED)
     sim.run(a.warmup)
     sim.sync()
     t0 = time.perf_counter()
@@ -201,7 +221,7 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(the path shards by space, see the design notes)", file=sys.stderr)
         sys.exit(2)
-    case = sd.sediment_case(nf=a.nf, n_dem=a.ndem, seed=1001)
+    case = make_case(a)
     counts = case.counts()
     counted = counts["fluid"] + counts["dem"]
     sim = sd.Sim(case, device=0)
@@ -231,7 +251,10 @@ def main():
     if a.roofline_kernel == "lists":
         per_unit += 4 * mean_nb
     b_launch = per_unit * units
-    traffic = traffic_from_summary(a.traffic_file, a.roofline_kernel)
+    tfile = a.traffic_file
+    if tfile is None and a.workload == "sediment" and a.nf is None and a.ndem is None:
+        tfile = os.path.join(ROOT, "profiles", "traffic_sediment_64cubed.txt")
+    traffic = traffic_from_summary(tfile, a.roofline_kernel) if tfile else None
     mean_s = (kms / 1e3) / max(launches, 1)
     achieved = b_launch / mean_s / 1e9 if launches else None
     step_bytes = B_ALG_STEP_SPH * (counts["fluid"] + counts["wall"]) + B_ALG_STEP_DEM * counts["dem"]
@@ -250,10 +273,10 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"SPH-DEM sedimentation {a.nf}^3 fluid + {counts['wall']} wall + "
-                        f"{counts['dem']} DEM, periodic x/y, sediment3D flags, FP64",
+            "workload": workload_name(a, counts),
             "fluid": counts["fluid"], "wall": counts["wall"], "dem": counts["dem"],
-            "seed": 1001, "sim_time_s": g.time, "dt_last": g.dt,
+            "seed": 1002 if a.workload == "large" else 1001, "sim_time_s": g.time,
+            "dt_last": g.dt,
         },
         "roofline": {
             "bound": "hbm",
