@@ -223,6 +223,20 @@ int sphdem_pass_dem_contacts(sphdem_ctx* ctx); /* neighboursGroup<calcDEMContact
 /* Global minimum of the accumulated dt candidates (setGlobalTimestep's min). */
 int sphdem_reduce_dt(sphdem_ctx* ctx, double* newdt);
 
+/* ---- full-state checkpoint (no counterpart: the reference's VTK restart
+ * keeps positions, velocities, densities and h but not the force sums, dddt,
+ * v0/vhat0, porosity, the Shepard sums, the DEM drag of the previous step or
+ * dt, io_data_vtk.cpp:20-115, so its resume is not exact) ---------------------
+ * Everything a continued run depends on: every per-particle field and type in
+ * original-index order, the globals (time, dt, newDt, sphStep, custom[]) and
+ * the container state (deletions, search radius).  Restoring into a context
+ * created with the same parameters continues the run bit for bit.  The buffer
+ * is plain bytes (sphdem_checkpoint_size of them) that the caller may write to
+ * a file. */
+int sphdem_checkpoint_size(const sphdem_ctx* ctx, int64_t* bytes);
+int sphdem_checkpoint_save(sphdem_ctx* ctx, void* host, int64_t capacity);
+int sphdem_checkpoint_restore(sphdem_ctx* ctx, const void* host, int64_t bytes);
+
 /* Number of particles deleted as out of range since the context was created. */
 int sphdem_num_deleted(const sphdem_ctx* ctx, int64_t* n);
 

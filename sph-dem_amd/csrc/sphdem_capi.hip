@@ -417,7 +417,9 @@ void setupBand(sphdem_ctx* ctx) {
   }
 }
 
-int doReset(sphdem_ctx* ctx) {
+// keepSsr: sort only, keeping the search radius (a checkpoint restore, which
+// must not move the hysteresis of calcDomainLimits)
+int doReset(sphdem_ctx* ctx, bool keepSsr = false) {
   const sphdem_params& P = ctx->P;
   const int n = ctx->n;
   HIPC(hipMemsetAsync(&ctx->dg->hmax_bits, 0, sizeof(unsigned long long), ctx->stream));
@@ -458,7 +460,7 @@ int doReset(sphdem_ctx* ctx) {
   unsigned long long hb = hg.hmax_bits;
   memcpy(&hmax, &hb, sizeof(hmax));
   if (ctx->ns + ctx->nd == 0) hmax = -100000;  // calcDomainLimits on an empty container
-  if ((hmax > ctx->ssr) || (hmax < 0.9 * ctx->ssr)) ctx->ssr = hmax;
+  if (!keepSsr && ((hmax > ctx->ssr) || (hmax < 0.9 * ctx->ssr))) ctx->ssr = hmax;
   setupGrids(ctx);
   setupBand(ctx);
   const bool sameGrid = ctx->gridValid && ctx->ssr == oldSsr;
@@ -918,6 +920,117 @@ int sphdem_num_particles(const sphdem_ctx* ctx, int64_t* n) {
   if (!ctx || !n) return SPHDEM_E_ARG;
   *n = ctx->n;
   return 0;
+}
+
+// ---- full-state checkpoint --------------------------------------------------------
+namespace {
+struct CheckpointHeader {
+  char magic[8];  // "SPHDEMCK"
+  int64_t version, n, ns0, nd0, ns, nd, sphStep;
+  double ssr;
+  int64_t paramsBytes, globalsBytes;
+};
+constexpr int64_t kCheckpointVersion = 1;
+int64_t checkpointBytes(int64_t n) {
+  return (int64_t)sizeof(CheckpointHeader) + (int64_t)sizeof(sphdem_params) +
+         (int64_t)sizeof(DevGlobals) + n * (int64_t)(kNumVec * sizeof(double4) + sizeof(int));
+}
+}  // namespace
+
+int sphdem_checkpoint_size(const sphdem_ctx* ctx, int64_t* bytes) {
+  if (!ctx || !bytes) return SPHDEM_E_ARG;
+  *bytes = checkpointBytes(ctx->n);
+  return 0;
+}
+
+int sphdem_checkpoint_save(sphdem_ctx* ctx, void* host, int64_t capacity) {
+  CHECK(checkLoaded(ctx));
+  if (!host) return SPHDEM_E_ARG;
+  const int64_t need = checkpointBytes(ctx->n);
+  if (capacity < need)
+    return fail(ctx, SPHDEM_E_CAPACITY, "checkpoint needs %lld bytes", (long long)need);
+  HIPC(hipSetDevice(ctx->device));
+  const int n = ctx->n;
+  char* p = (char*)host;
+  CheckpointHeader h;
+  memcpy(h.magic, "SPHDEMCK", 8);
+  h.version = kCheckpointVersion;
+  h.n = n;
+  h.ns0 = ctx->ns0;
+  h.nd0 = ctx->nd0;
+  h.ns = ctx->ns;
+  h.nd = ctx->nd;
+  h.sphStep = ctx->sphStep;
+  h.ssr = ctx->ssr;
+  h.paramsBytes = sizeof(sphdem_params);
+  h.globalsBytes = sizeof(DevGlobals);
+  memcpy(p, &h, sizeof(h));
+  p += sizeof(h);
+  memcpy(p, &ctx->P, sizeof(sphdem_params));
+  p += sizeof(sphdem_params);
+  HIPC(hipMemcpyAsync(p, ctx->dg, sizeof(DevGlobals), hipMemcpyDeviceToHost, ctx->stream));
+  p += sizeof(DevGlobals);
+  // each stream in original-index order, staged through the other store buffer
+  double4* stage = ctx->alt.posh;
+  for (int k = 0; k < kNumVec; k++) {
+    hipLaunchKernelGGL(k_gather_orig, dim3(blocks(n)), dim3(TPB), 0, ctx->stream,
+                       storeArr(ctx->cur, k), ctx->cur.orig, n, stage);
+    LAUNCHED();
+    HIPC(hipMemcpyAsync(p, stage, sizeof(double4) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    p += sizeof(double4) * n;
+  }
+  hipLaunchKernelGGL(k_gather_orig_int, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, ctx->cur.iam,
+                     ctx->cur.orig, n, ctx->alt.iam);
+  LAUNCHED();
+  HIPC(hipMemcpyAsync(p, ctx->alt.iam, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int sphdem_checkpoint_restore(sphdem_ctx* ctx, const void* host, int64_t bytes) {
+  if (!ctx || !host || bytes < (int64_t)sizeof(CheckpointHeader)) return SPHDEM_E_ARG;
+  CheckpointHeader h;
+  memcpy(&h, host, sizeof(h));
+  if (memcmp(h.magic, "SPHDEMCK", 8) != 0 || h.version != kCheckpointVersion ||
+      h.paramsBytes != (int64_t)sizeof(sphdem_params) ||
+      h.globalsBytes != (int64_t)sizeof(DevGlobals) || h.n < 0 || bytes != checkpointBytes(h.n))
+    return fail(ctx, SPHDEM_E_ARG, "not a checkpoint of this library version");
+  if (h.n > ctx->cap)
+    return fail(ctx, SPHDEM_E_CAPACITY, "%lld particles exceed capacity %lld", (long long)h.n,
+                (long long)ctx->cap);
+  const char* p = (const char*)host + sizeof(h);
+  if (memcmp(p, &ctx->P, sizeof(sphdem_params)) != 0)
+    return fail(ctx, SPHDEM_E_ARG, "the checkpoint was taken with other parameters");
+  p += sizeof(sphdem_params);
+  HIPC(hipSetDevice(ctx->device));
+  HIPC(hipMemcpyAsync(ctx->dg, p, sizeof(DevGlobals), hipMemcpyHostToDevice, ctx->stream));
+  p += sizeof(DevGlobals);
+  const int n = (int)h.n;
+  for (int k = 0; k < kNumVec; k++) {
+    HIPC(hipMemcpyAsync(storeArr(ctx->cur, k), p, sizeof(double4) * n, hipMemcpyHostToDevice,
+                        ctx->stream));
+    p += sizeof(double4) * n;
+  }
+  HIPC(hipMemcpyAsync(ctx->cur.iam, p, sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_identity, dim3(blocks(std::max(n, 1))), dim3(TPB), 0, ctx->stream,
+                     ctx->cur.orig, n);
+  LAUNCHED();
+  HIPC(hipStreamSynchronize(ctx->stream));
+  ctx->n = n;
+  ctx->ns0 = (int)h.ns0;
+  ctx->nd0 = (int)h.nd0;
+  ctx->ns = (int)h.ns;
+  ctx->nd = (int)h.nd;
+  ctx->sphStep = h.sphStep;
+  ctx->ssr = h.ssr;
+  ctx->ng = 0;
+  ctx->gridValid = false;
+  ctx->listsValid = false;
+  ctx->loaded = true;
+  // the store is in original-index order: sort it with the saved search
+  // radius (the grid of the saved state)
+  return doReset(ctx, true);
 }
 
 int sphdem_num_deleted(const sphdem_ctx* ctx, int64_t* n) {

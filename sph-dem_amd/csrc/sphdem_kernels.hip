@@ -1070,6 +1070,21 @@ __global__ void __launch_bounds__(256) k_sum_drag_final(const double* partial, i
 }
 
 // ============================================================ parity helpers
+// whole double4 streams between store order and original-index order
+__global__ void k_gather_orig(const double4* __restrict__ src, const int* __restrict__ orig, int n,
+                              double4* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[orig[i]] = src[i];
+}
+__global__ void k_gather_orig_int(const int* __restrict__ src, const int* __restrict__ orig, int n,
+                                  int* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[orig[i]] = src[i];
+}
+__global__ void k_identity(int* __restrict__ out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = i;
+}
 __global__ void k_get_field(const double* __restrict__ base, int off, int width,
                             const int* __restrict__ orig, int n, double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -513,6 +513,9 @@ def load_library(path: str = LIB_PATH):
     lib.sphdem_destroy.argtypes = [vp]
     lib.sphdem_last_error.argtypes = [vp]
     lib.sphdem_last_error.restype = ctypes.c_char_p
+    lib.sphdem_checkpoint_size.argtypes = [vp, ctypes.POINTER(i64)]
+    lib.sphdem_checkpoint_save.argtypes = [vp, ctypes.c_void_p, i64]
+    lib.sphdem_checkpoint_restore.argtypes = [vp, ctypes.c_char_p, i64]
     lib.sphdem_set_params.argtypes = [vp, ctypes.POINTER(Params)]
     lib.sphdem_upload.argtypes = [vp, i64, dp, dp, dp, dp, dp, dp, ip]
     lib.sphdem_num_particles.argtypes = [vp, ctypes.POINTER(i64)]
@@ -656,6 +659,18 @@ class Sim:
         self._check(self.lib.sphdem_get_cell_coords(
             self.ctx, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
         return out
+
+    def checkpoint(self) -> bytes:
+        """Full-state checkpoint (sphdem_checkpoint_save): continuing from it
+        with restore() reproduces the run bit for bit."""
+        size = ctypes.c_int64()
+        self._check(self.lib.sphdem_checkpoint_size(self.ctx, ctypes.byref(size)))
+        buf = (ctypes.c_char * size.value)()
+        self._check(self.lib.sphdem_checkpoint_save(self.ctx, buf, size.value))
+        return bytes(buf)
+
+    def restore(self, blob: bytes):
+        self._check(self.lib.sphdem_checkpoint_restore(self.ctx, blob, len(blob)))
 
     def pair_count(self, kind: int) -> int:
         """Number of pairs of one neighbour search (the cached lists for SPH)."""
