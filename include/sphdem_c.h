@@ -223,6 +223,21 @@ int sphdem_pass_dem_contacts(sphdem_ctx* ctx); /* neighboursGroup<calcDEMContact
 /* Global minimum of the accumulated dt candidates (setGlobalTimestep's min). */
 int sphdem_reduce_dt(sphdem_ctx* ctx, double* newdt);
 
+/* ---- diagnostics of an output step: CsphIncompress::calcOutputVars
+ * (sphIncompress.h:233-319) -- momentum and energies of the sph particles,
+ * maxima and density statistics over sph + sphBoundary, averages over the n
+ * particles of the container; aveDensFromMass compares each density with a
+ * fresh summation over the cached neighbour lists (calcAveDensFromMass).
+ * rmsFF, maxFF, eFF, edFFdt, eViscB and eBForce are zero under the supported
+ * flag sets (calcEnergies resets the last two). */
+typedef struct sphdem_output {
+  double linMom[3], angMom, eKE, eViscF, edViscFdt, eViscB, eBForce, eFF, edFFdt;
+  double eElast, edElastdt, eElastExact, eTotal;
+  double maxV, maxF, maxFF, rmsFF, aveDens, varDens, aveDensFromMass;
+  int64_t n, nSph;
+} sphdem_output;
+int sphdem_output_vars(sphdem_ctx* ctx, sphdem_output* out);
+
 /* ---- full-state checkpoint (no counterpart: the reference's VTK restart
  * keeps positions, velocities, densities and h but not the force sums, dddt,
  * v0/vhat0, porosity, the Shepard sums, the DEM drag of the previous step or

@@ -922,6 +922,64 @@ int sphdem_num_particles(const sphdem_ctx* ctx, int64_t* n) {
   return 0;
 }
 
+int sphdem_output_vars(sphdem_ctx* ctx, sphdem_output* out) {
+  CHECK(checkLoaded(ctx));
+  if (!out) return SPHDEM_E_ARG;
+  HIPC(hipSetDevice(ctx->device));
+  const int ns = ctx->ns;
+  CHECK(ensureLists(ctx));
+  CHECK(refreshGhosts(ctx));
+  double* densCalc = ctx->fieldbuf;
+  if (ns > 0) {
+    hipLaunchKernelGGL(k_density_l, dim3(blocks(ns)), dim3(TPB), 0, ctx->stream, ctx->cur, ns,
+                       ctx->box, ctx->nbl, ctx->nbcnt, ctx->P.wcon, densCalc);
+    LAUNCHED();
+  }
+  const int nb = std::max(blocks(ns), 1);
+  if (nb * OV_COUNT + OV_COUNT > ctx->partialCap) {
+    if (ctx->partial) HIPC(hipFree(ctx->partial));
+    ctx->partialCap = nb * OV_COUNT + OV_COUNT;
+    HIPC(hipMalloc(&ctx->partial, sizeof(double) * ctx->partialCap));
+  }
+  double* fin = ctx->partial + (size_t)nb * OV_COUNT;
+  double h0[OV_COUNT], h1[OV_COUNT];
+  auto run = [&](int pass, double ave, double* host) -> int {
+    hipLaunchKernelGGL(k_output_partials, dim3(nb), dim3(256), 0, ctx->stream, ctx->cur, ns,
+                       ctx->P, densCalc, ave, pass, ctx->partial);
+    LAUNCHED();
+    hipLaunchKernelGGL(k_output_final, dim3(1), dim3(64), 0, ctx->stream, ctx->partial, nb, fin);
+    LAUNCHED();
+    HIPC(hipMemcpyAsync(host, fin, sizeof(double) * OV_COUNT, hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    return 0;
+  };
+  CHECK(run(0, 0.0, h0));
+  const double n = (double)(ctx->ns + ctx->nd);  // the container's particles
+  const double aveDens = n > 0 ? h0[OV_AVEDENS] / n : 0.0;
+  CHECK(run(1, aveDens, h1));
+  sphdem_output o = {};
+  o.linMom[0] = h0[OV_LINX];
+  o.linMom[1] = h0[OV_LINY];
+  o.linMom[2] = h0[OV_LINZ];
+  o.angMom = h0[OV_ANG];
+  o.eKE = h0[OV_EKE];
+  o.eViscF = h0[OV_EVISCF];
+  o.edViscFdt = h0[OV_EDVISCFDT];
+  o.eElast = h0[OV_EELAST];
+  o.edElastdt = h0[OV_EDELASTDT];
+  o.eElastExact = h0[OV_EELASTEXACT];
+  o.maxV = h0[OV_MAXV];
+  o.maxF = h0[OV_MAXF];
+  o.aveDens = aveDens;
+  o.aveDensFromMass = n > 0 ? sqrt(h0[OV_DENSFROMMASS] / n) : 0.0;
+  o.varDens = n > 0 ? h1[OV_VARDENS] / n : 0.0;
+  o.n = (int64_t)n;
+  o.nSph = (int64_t)h0[OV_NSPH];
+  o.eTotal = o.eKE + o.eElast + o.eViscF + o.eViscB + o.eBForce + o.eFF;
+  *out = o;
+  return 0;
+}
+
 // ---- full-state checkpoint --------------------------------------------------------
 namespace {
 struct CheckpointHeader {

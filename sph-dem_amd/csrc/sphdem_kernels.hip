@@ -983,6 +983,10 @@ __global__ void k_end_rest(Store S, int ns, sphdem_params P, DevGlobals* g) {
   const double dudt = d.y * dddt;
   S.frc[a].w = dudt;
   S.vel[a].w += g->dt * dudt;
+  if (S.iam[a] == T_SPH) {  // the running viscous energy of calcEnergies
+    const double4 vh = S.vhm[a], fv = S.fv[a];
+    S.fvel[a].w += g->dt * -dot3(vh.x, vh.y, vh.z, fv.x, fv.y, fv.z);
+  }
 }
 
 // ============================================================ hooks
@@ -1070,6 +1074,79 @@ __global__ void __launch_bounds__(256) k_sum_drag_final(const double* partial, i
 }
 
 // ============================================================ parity helpers
+// ---- calcOutputVars (sphIncompress.h:233-319): the diagnostics of an output step.
+// Per-block partial sums in a fixed order, then one block adds them in block
+// order.  Sums over sph (momentum, energies, counts) and over sph+sphBoundary
+// (densities, maxima), as calcOutput's branches.
+enum : int {
+  OV_LINX = 0, OV_LINY, OV_LINZ, OV_ANG, OV_EKE, OV_EVISCF, OV_EDVISCFDT, OV_EELAST,
+  OV_EDELASTDT, OV_EELASTEXACT, OV_AVEDENS, OV_DENSFROMMASS, OV_NSPH, OV_MAXV, OV_MAXF,
+  OV_VARDENS, OV_COUNT
+};
+__global__ void __launch_bounds__(256) k_output_partials(Store S, int ns, sphdem_params P,
+                                                         const double* __restrict__ densCalc,
+                                                         double aveDens, int pass,
+                                                         double* __restrict__ partial) {
+  __shared__ double red[OV_COUNT][256];
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[OV_COUNT];
+  for (int k = 0; k < OV_COUNT; k++) v[k] = 0.0;
+  if (a < ns && isSphT(S.iam[a])) {
+    const double4 d = S.dpo[a];
+    if (pass == 0) {
+      const double4 p = S.posh[a], vh = S.vhm[a], vel = S.vel[a], f = S.frc[a];
+      const double m = vh.w;
+      if (S.iam[a] == T_SPH) {
+        v[OV_LINX] = vel.x * m;
+        v[OV_LINY] = vel.y * m;
+        v[OV_LINZ] = vel.z * m;
+        v[OV_ANG] = (p.x * vel.y - p.y * vel.x) * m;
+        v[OV_EKE] = 0.5 * m * dot3(vh.x, vh.y, vh.z, vh.x, vh.y, vh.z);
+        v[OV_EVISCF] = m * S.fvel[a].w;
+        const double4 fv = S.fv[a];
+        v[OV_EDVISCFDT] = m * -dot3(vh.x, vh.y, vh.z, fv.x, fv.y, fv.z);
+        v[OV_NSPH] = 1.0;
+      }
+      v[OV_MAXV] = sqrt(dot3(vel.x, vel.y, vel.z, vel.x, vel.y, vel.z));
+      v[OV_MAXF] = sqrt(dot3(f.x, f.y, f.z, f.x, f.y, f.z));
+      v[OV_AVEDENS] = d.x;
+      v[OV_EELAST] = m * vel.w;
+      v[OV_EDELASTDT] = m * f.w;
+      const double df = d.x / P.refd;
+      v[OV_EELASTEXACT] = m * (P.prb * ((pow(df, 6) - 1) / 6 + 1 / df - 1)) / P.refd;
+      const double dm = densCalc[a] - d.x;
+      v[OV_DENSFROMMASS] = dm * dm;
+    } else {
+      const double dv = d.x - aveDens;
+      v[OV_VARDENS] = dv * dv;
+    }
+  }
+  for (int k = 0; k < OV_COUNT; k++) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int k = 0; k < OV_COUNT; k++) {
+        if (k == OV_MAXV || k == OV_MAXF)
+          red[k][threadIdx.x] = dmax2(red[k][threadIdx.x], red[k][threadIdx.x + s]);
+        else
+          red[k][threadIdx.x] += red[k][threadIdx.x + s];
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < OV_COUNT) partial[(size_t)blockIdx.x * OV_COUNT + threadIdx.x] = red[threadIdx.x][0];
+}
+__global__ void k_output_final(const double* __restrict__ partial, int nblocks,
+                               double* __restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= OV_COUNT) return;
+  double acc = 0.0;
+  for (int b = 0; b < nblocks; b++) {
+    const double x = partial[(size_t)b * OV_COUNT + k];
+    acc = (k == OV_MAXV || k == OV_MAXF) ? dmax2(acc, x) : acc + x;
+  }
+  out[k] = acc;
+}
+
 // whole double4 streams between store order and original-index order
 __global__ void k_gather_orig(const double4* __restrict__ src, const int* __restrict__ orig, int n,
                               double4* __restrict__ out) {

@@ -338,9 +338,11 @@ __global__ void __launch_bounds__(256) k_omega_l(Store S, int ns, Box B,
 }
 
 // ---- calcDensity (sph.h:1149-1191) over the cached list -------------------------
+// out: where the sums go (null: the particles' dens)
 __global__ void __launch_bounds__(256) k_density_l(Store S, int ns, Box B,
                                                    const int* __restrict__ nbl,
-                                                   const int* __restrict__ nbcnt, double wcon) {
+                                                   const int* __restrict__ nbcnt, double wcon,
+                                                   double* __restrict__ out = nullptr) {
   const int a = xcdThread();
   if (a >= ns) return;
   const double4 pa = S.posh[a];
@@ -367,7 +369,10 @@ __global__ void __launch_bounds__(256) k_density_l(Store S, int ns, Box B,
       rho += m[u] * wendWi(wcon, divr(r, ha, rha), i3);
     }
   }
-  S.dpo[a].x = rho;
+  if (out)
+    out[a] = rho;
+  else
+    S.dpo[a].x = rho;
 }
 
 // ---- calcDddtDudt (sphIncompress.h:783-851) over the cached list ----------------

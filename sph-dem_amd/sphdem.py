@@ -99,6 +99,17 @@ class Params(ctypes.Structure):
         return p
 
 
+class Output(ctypes.Structure):
+    """Mirror of `sphdem_output` (calcOutputVars diagnostics)."""
+
+    _fields_ = [("linMom", ctypes.c_double * 3)] + [
+        (k, ctypes.c_double) for k in (
+            "angMom", "eKE", "eViscF", "edViscFdt", "eViscB", "eBForce", "eFF", "edFFdt",
+            "eElast", "edElastdt", "eElastExact", "eTotal", "maxV", "maxF", "maxFF", "rmsFF",
+            "aveDens", "varDens", "aveDensFromMass")] + [("n", ctypes.c_int64),
+                                                         ("nSph", ctypes.c_int64)]
+
+
 class Globals(ctypes.Structure):
     """Mirror of `sphdem_globals`."""
 
@@ -513,6 +524,7 @@ def load_library(path: str = LIB_PATH):
     lib.sphdem_destroy.argtypes = [vp]
     lib.sphdem_last_error.argtypes = [vp]
     lib.sphdem_last_error.restype = ctypes.c_char_p
+    lib.sphdem_output_vars.argtypes = [vp, ctypes.POINTER(Output)]
     lib.sphdem_checkpoint_size.argtypes = [vp, ctypes.POINTER(i64)]
     lib.sphdem_checkpoint_save.argtypes = [vp, ctypes.c_void_p, i64]
     lib.sphdem_checkpoint_restore.argtypes = [vp, ctypes.c_char_p, i64]
@@ -659,6 +671,14 @@ class Sim:
         self._check(self.lib.sphdem_get_cell_coords(
             self.ctx, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
         return out
+
+    def output_vars(self) -> dict:
+        """calcOutputVars diagnostics (sphdem_output_vars) as a dict."""
+        o = Output()
+        self._check(self.lib.sphdem_output_vars(self.ctx, ctypes.byref(o)))
+        d = {k: getattr(o, k) for k, _ in Output._fields_}
+        d["linMom"] = list(o.linMom)
+        return d
 
     def checkpoint(self) -> bytes:
         """Full-state checkpoint (sphdem_checkpoint_save): continuing from it
