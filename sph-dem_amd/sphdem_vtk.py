@@ -158,3 +158,38 @@ def case_from_restart(path: str, params, hookset: int, name: str = "restart"):
     return sd.Case(name, params, pts[keep].copy(), d["v"][keep].copy(), d["vhat"][keep].copy(),
                    d["mass"][keep].copy(), d["h"][keep].copy(), d["dens"][keep].copy(),
                    d["iam"][keep].astype(np.int32), hookset, {"restart": path})
+
+
+# ---- the globals file: header of ioGlobals.cpp:17, lines of myWriteLine (ioGlobals.cpp:42-50)
+GLOBALS_HEADER = ("# OutStep SPHStep Time linMomX linMomY angMom KE eBForce eElast eElastExact "
+                  "eViscF eViscB eFF eTotal wtTotalStep wtTotalForceCalc wtTotalDddtCalc "
+                  "wtTotalReset wtTotalUpdateDomain wtTotalMPI maxV aveDens varDens "
+                  "aveDensFromMass n nSph maxF maxFF rmsFF edElastdt edViscFdt edFFdt")
+
+
+def globals_line(outstep: int, g, o: dict, wall_times=(0.0,) * 6) -> str:
+    """The columns CioGlobals::myWriteLine writes (the wall-clock timers are
+    the caller's), followed by custom[]."""
+    cols = [outstep, g.sphStep, g.time, o["linMom"][0], o["linMom"][1], o["angMom"], o["eKE"],
+            o["eBForce"], o["eElast"], o["eElastExact"], o["eViscF"], o["eViscB"], o["eFF"],
+            o["eTotal"], *wall_times, o["maxV"], o["aveDens"], o["varDens"],
+            o["aveDensFromMass"], o["n"], o["nSph"], o["maxF"], o["maxFF"], o["rmsFF"],
+            o["edElastdt"], o["edViscFdt"], o["edFFdt"], *list(g.custom)]
+    return " ".join(repr(x) if isinstance(x, float) else str(x) for x in cols)
+
+
+def append_globals(path: str, outstep: int, sim, wall_times=(0.0,) * 6):
+    """Appends the output step's line to a Globals file (created with its
+    header line if missing), from the live simulation's diagnostics."""
+    import os
+
+    new = not os.path.exists(path)
+    with open(path, "a") as f:
+        if new:
+            f.write(GLOBALS_HEADER + "\n")
+        f.write(globals_line(outstep, sim.globals(), sim.output_vars(), wall_times) + "\n")
+
+
+def read_globals(path: str) -> np.ndarray:
+    """The numeric table of a Globals file (header skipped)."""
+    return np.loadtxt(path, comments="#", ndmin=2)

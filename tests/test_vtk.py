@@ -87,3 +87,30 @@ def test_output_and_restart_on_the_gpu(tmp_path):
     assert np.array_equal(s2.get("r"), rc.r)
     s2.run(1)
     assert np.isfinite(s2.get("dens")).all()
+
+
+def test_globals_line_columns():
+    class G:
+        sphStep, time = 7, 0.25
+        custom = [float(i) for i in range(9)]
+
+    o = {k: 0.0 for k in ("angMom", "eKE", "eBForce", "eElast", "eElastExact", "eViscF",
+                          "eViscB", "eFF", "eTotal", "maxV", "aveDens", "varDens",
+                          "aveDensFromMass", "maxF", "maxFF", "rmsFF", "edElastdt",
+                          "edViscFdt", "edFFdt")}
+    o.update(linMom=[1.5, -2.0, 0.0], n=10, nSph=8, eKE=3.0)
+    cols = vtk.globals_line(2, G(), o).split()
+    assert len(cols) == len(vtk.GLOBALS_HEADER.split()) - 1 + 9
+    assert cols[:4] == ["2", "7", "0.25", "1.5"] and cols[6] == "3.0"
+    assert cols[24:26] == ["10", "8"]
+
+
+@pytest.mark.gpu
+def test_globals_file_from_a_run(tmp_path):
+    sim = sd.Sim(sd.sediment_case(nf=12, n_dem=60))
+    p = tmp_path / "Globals0.dat"
+    for k in range(2):
+        sim.run(1)
+        vtk.append_globals(str(p), k, sim)
+    t = vtk.read_globals(str(p))
+    assert t.shape[0] == 2 and t[1, 1] == 2
