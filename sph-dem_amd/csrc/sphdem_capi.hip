@@ -27,7 +27,7 @@ struct sphdem_ctx {
   Store cur{}, alt{};
   void* storeMem[2] = {nullptr, nullptr};
   int *key = nullptr, *slot = nullptr, *tmp = nullptr, *perm = nullptr;
-  int *counts = nullptr, *start = nullptr, *tiles = nullptr, *tileOff = nullptr;
+  int *counts = nullptr, *start = nullptr, *tiles = nullptr;
   int Kcap = 0, tileCap = 0;
   // segments of the SPH grid (the wavefront work units of the pair passes)
   int *segCnt = nullptr, *segOff = nullptr;
@@ -39,6 +39,9 @@ struct sphdem_ctx {
   double2* hinv = nullptr;  // {1/h, 1/h^5} per SPH-group particle
   // periodic ghosts (store entries [n, n + ng) after a reset)
   int ng = 0, gbase = 0;    // ghost count; first key of the ghost cells (0: none)
+  // ghost field groups whose origins changed since the ghosts were copied
+  unsigned gstale = 7;
+  int gcntZeroAt = -1;  // the n for which gcnt[n] was cleared
   GhostBand band{};         // cells a ghost may occupy on the current grid
   bool gridValid = false;   // gs/band describe the grid of the last reset
   int64_t ghostCap = 0;     // store entries beyond cap available to ghosts
@@ -265,20 +268,15 @@ int scanInts(sphdem_ctx* ctx, const int* in, int* out, int m) {
   if (tiles > SCAN_TILE) return fail(ctx, SPHDEM_E_CAPACITY, "scan of %d items", m);
   if (tiles + 1 > ctx->tileCap) {
     if (ctx->tiles) HIPC(hipFree(ctx->tiles));
-    if (ctx->tileOff) HIPC(hipFree(ctx->tileOff));
     ctx->tileCap = tiles + 1 + (tiles + 1) / 4;
     HIPC(hipMalloc(&ctx->tiles, sizeof(int) * ctx->tileCap));
-    HIPC(hipMalloc(&ctx->tileOff, sizeof(int) * ctx->tileCap));
   }
   hipLaunchKernelGGL(k_scan_tile, dim3(tiles), dim3(SCAN_THREADS), 0, ctx->stream, in, out,
                      ctx->tiles, m);
   LAUNCHED();
   if (tiles > 1) {
-    hipLaunchKernelGGL(k_scan_tile, dim3(1), dim3(SCAN_THREADS), 0, ctx->stream, ctx->tiles,
-                       ctx->tileOff, (int*)nullptr, tiles);
-    LAUNCHED();
-    hipLaunchKernelGGL(k_scan_add, dim3(blocks(m)), dim3(TPB), 0, ctx->stream, out,
-                       ctx->tileOff, m);
+    hipLaunchKernelGGL(k_scan_add, dim3(tiles), dim3(SCAN_THREADS), 0, ctx->stream, out,
+                       ctx->tiles, m);
     LAUNCHED();
   }
   return 0;
@@ -304,13 +302,14 @@ int buildSegments(sphdem_ctx* ctx) {
     ctx->segCap = need + need / 4;
     HIPC(hipMalloc(&ctx->segs, sizeof(Segment) * ctx->segCap));
   }
-  HIPC(hipMemsetAsync(ctx->segCnt + ncol, 0, sizeof(int), ctx->stream));
-  hipLaunchKernelGGL(k_seg_count, dim3(blocks(ncol)), dim3(TPB), 0, ctx->stream, ctx->start, g,
-                     ctx->segCnt);
+  // one wavefront per column; the count kernel's waves cover ncol + 1 (the
+  // last one writes the scan's closing zero)
+  hipLaunchKernelGGL(k_seg_count, dim3((ncol + 1 + kSegWaves - 1) / kSegWaves),
+                     dim3(64 * kSegWaves), 0, ctx->stream, ctx->start, g, ctx->segCnt);
   LAUNCHED();
   CHECK(scanInts(ctx, ctx->segCnt, ctx->segOff, ncol + 1));
-  hipLaunchKernelGGL(k_seg_write, dim3(blocks(ncol)), dim3(TPB), 0, ctx->stream, ctx->start, g,
-                     ctx->segOff, ctx->segs);
+  hipLaunchKernelGGL(k_seg_write, dim3((ncol + kSegWaves - 1) / kSegWaves), dim3(64 * kSegWaves),
+                     0, ctx->stream, ctx->start, g, ctx->segOff, ctx->segs);
   LAUNCHED();
   return 0;
 }
@@ -320,16 +319,22 @@ int tileBlocks(const sphdem_ctx* ctx) {
   est = (est + 7) & ~7;
   return std::min(std::max(est, 8), 2048);
 }
-// ghost copies of the fields the pair passes read of neighbours
-int refreshGhosts(sphdem_ctx* ctx) {
-  if (ctx->ng == 0) return 0;
+// Ghost copies of the fields the pair passes read of neighbours, in three
+// groups: posh (r, h), vhm (vhat, mass), dpo (dens, pdr2, omega, porosity).
+// A launch that may write a group of real SPH-group particles marks it stale;
+// a pass refreshes the ghosts when a group it reads of neighbours is stale.
+constexpr unsigned GS_POSH = 1, GS_VHM = 2, GS_DPO = 4, GS_ALL = 7;
+void markStale(sphdem_ctx* ctx, unsigned groups) { ctx->gstale |= groups; }
+int refreshGhosts(sphdem_ctx* ctx, unsigned need) {
+  if (ctx->ng == 0 || !(ctx->gstale & need)) return 0;
   hipLaunchKernelGGL(k_ghost_refresh, dim3(blocks(ctx->ng)), dim3(TPB), 0, ctx->stream, ctx->cur,
                      ctx->n, ctx->ng, ctx->gorig, ctx->gcode, ctx->box);
   LAUNCHED();
+  ctx->gstale = 0;
   return 0;
 }
 int launchLists(sphdem_ctx* ctx) {
-  CHECK(refreshGhosts(ctx));
+  CHECK(refreshGhosts(ctx, GS_POSH));
   hipLaunchKernelGGL(k_lists, dim3(tileBlocks(ctx)), dim3(TPB), 0, ctx->stream, ctx->cur.posh,
                      ctx->cur.cell, ctx->gs, ctx->gbase, ctx->start, ctx->segs,
                      ctx->segOff + ctx->gs.dim[0] * ctx->gs.dim[1], ctx->ns, ctx->nbl,
@@ -422,7 +427,13 @@ void setupBand(sphdem_ctx* ctx) {
 int doReset(sphdem_ctx* ctx, bool keepSsr = false) {
   const sphdem_params& P = ctx->P;
   const int n = ctx->n;
-  HIPC(hipMemsetAsync(&ctx->dg->hmax_bits, 0, sizeof(unsigned long long), ctx->stream));
+  // hmax_bits is cleared by the previous reset's k_bin (or by the upload);
+  // gcnt[n], the scans' closing zero, is written by nothing else
+  if (n == 0) HIPC(hipMemsetAsync(&ctx->dg->hmax_bits, 0, sizeof(unsigned long long), ctx->stream));
+  if (ctx->gcntZeroAt != n) {
+    HIPC(hipMemsetAsync(ctx->gcnt + n, 0, sizeof(int), ctx->stream));
+    ctx->gcntZeroAt = n;
+  }
   double dmin[3], dsp[3];
   for (int d = 0; d < 3; d++) {
     dmin[d] = P.proc_domain[2 * d];
@@ -434,7 +445,6 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false) {
   const bool periodic = ctx->box.periodic[0] || ctx->box.periodic[1] || ctx->box.periodic[2];
   const bool pre = periodic && ctx->gridValid && n > 0;
   int ngPre = 0;
-  if (pre) HIPC(hipMemsetAsync(ctx->gcnt + n, 0, sizeof(int), ctx->stream));
   {
     Timed tm(ctx, KID_WRAP);
     hipLaunchKernelGGL(k_wrap, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, ctx->cur, n, dmin[0],
@@ -488,7 +498,6 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false) {
       hipLaunchKernelGGL(k_ghost_count, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, ctx->cur, n,
                          ctx->gs, ctx->box, ctx->band, ctx->gcnt);
       LAUNCHED();
-      HIPC(hipMemsetAsync(ctx->gcnt + n, 0, sizeof(int), ctx->stream));
       CHECK(scanInts(ctx, ctx->gcnt, ctx->goff, n + 1));
       HIPC(hipMemcpyAsync(&ng, ctx->goff + n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
       HIPC(hipStreamSynchronize(ctx->stream));
@@ -525,6 +534,7 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false) {
     LAUNCHED();
   }
   ctx->ng = ng;
+  ctx->gstale = 0;  // the permutation copied every ghost from its origin
   std::swap(ctx->cur, ctx->alt);
   std::swap(ctx->storeMem[0], ctx->storeMem[1]);
   ctx->listsValid = false;  // updateDomain clears every neighbour list (dataLL.cpp:374)
@@ -551,7 +561,7 @@ int checkLoaded(sphdem_ctx* ctx) {
 bool mixed(const sphdem_ctx* c) { return c->P.precision == SPHDEM_MIXED; }
 int passOmega(sphdem_ctx* ctx) {
   CHECK(ensureLists(ctx));
-  CHECK(refreshGhosts(ctx));
+  CHECK(refreshGhosts(ctx, GS_POSH | GS_VHM));
   Timed tm(ctx, KID_OMEGA);
   if (mixed(ctx))
     hipLaunchKernelGGL(k_omega_m, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
@@ -560,6 +570,7 @@ int passOmega(sphdem_ctx* ctx) {
     hipLaunchKernelGGL(k_omega_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
                        ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->P.wcon);
   LAUNCHED();
+  markStale(ctx, GS_DPO);
   return 0;
 }
 int passEos(sphdem_ctx* ctx) {
@@ -567,11 +578,13 @@ int passEos(sphdem_ctx* ctx) {
   hipLaunchKernelGGL(k_eos, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
                      ctx->P);
   LAUNCHED();
+  markStale(ctx, GS_DPO);
   return 0;
 }
 // {1/h, 1/h^5} of every SPH-group particle, for the passes that need them of
 // neighbours (recomputed per pass: h changes in start, driftRest and the reinit)
 int computeHinv(sphdem_ctx* ctx) {
+  CHECK(refreshGhosts(ctx, GS_POSH));
   const int ntot = ctx->n + ctx->ng;  // ghosts included (their h is their origin's)
   hipLaunchKernelGGL(k_hinv, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ctx->cur.posh, ntot,
                      ctx->hinv);
@@ -593,7 +606,7 @@ void launchForce(sphdem_ctx* ctx) {
 }
 int passForce(sphdem_ctx* ctx) {
   CHECK(ensureLists(ctx));
-  CHECK(refreshGhosts(ctx));
+  CHECK(refreshGhosts(ctx, GS_ALL));
   CHECK(computeHinv(ctx));
   using Launch = void (*)(sphdem_ctx*);
   static const Launch table[16] = {
@@ -611,7 +624,7 @@ int passForce(sphdem_ctx* ctx) {
 }
 int passDddt(sphdem_ctx* ctx) {
   CHECK(ensureLists(ctx));
-  CHECK(refreshGhosts(ctx));
+  CHECK(refreshGhosts(ctx, GS_POSH | GS_VHM));
   if (varH(ctx)) {
     Timed tm(ctx, KID_DDDT);
     if (mixed(ctx))
@@ -648,6 +661,7 @@ int passCoupling(sphdem_ctx* ctx) {
   hipLaunchKernelGGL(k_coupling_sph, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
                      ctx->ns, ctx->nd, ctx->gc, ctx->start, ctx->box, ctx->rule, ctx->P);
   LAUNCHED();
+  markStale(ctx, GS_DPO);
   return 0;
 }
 int passFpi(sphdem_ctx* ctx) {
@@ -656,6 +670,7 @@ int passFpi(sphdem_ctx* ctx) {
   hipLaunchKernelGGL(k_fpi, dim3(waveBlocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
                      ctx->nd, ctx->gs, ctx->start, ctx->box, ctx->P);
   LAUNCHED();
+  markStale(ctx, GS_ALL);
   return 0;
 }
 int passContacts(sphdem_ctx* ctx) {
@@ -664,6 +679,7 @@ int passContacts(sphdem_ctx* ctx) {
   hipLaunchKernelGGL(k_contacts, dim3(blocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur,
                      ctx->ns, ctx->nd, ctx->gc, ctx->start, ctx->box, ctx->P);
   LAUNCHED();
+  markStale(ctx, GS_ALL);
   return 0;
 }
 // one iteration of the density re-initialisation (sphIncompress.cpp:29-34)
@@ -671,8 +687,9 @@ int passDensity(sphdem_ctx* ctx) {
   hipLaunchKernelGGL(k_hfromdens, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
                      ctx->ns, ctx->P.hfac);
   LAUNCHED();
+  markStale(ctx, GS_POSH);
   CHECK(ensureLists(ctx));  // the first iteration builds them with the new h
-  CHECK(refreshGhosts(ctx));
+  CHECK(refreshGhosts(ctx, GS_POSH | GS_VHM));
   if (mixed(ctx))
     hipLaunchKernelGGL(k_density_m, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
                        ctx->ns, ctx->nbl, ctx->nbcnt, ctx->P.wcon);
@@ -680,6 +697,7 @@ int passDensity(sphdem_ctx* ctx) {
     hipLaunchKernelGGL(k_density_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
                        ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->P.wcon);
   LAUNCHED();
+  markStale(ctx, GS_DPO);
   return 0;
 }
 
@@ -691,6 +709,7 @@ int stepStart(sphdem_ctx* ctx) {
     hipLaunchKernelGGL(k_start, dim3(blocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream,
                        ctx->cur, na, ctx->P, ctx->dg);
     LAUNCHED();
+    markStale(ctx, GS_ALL);
   }
   if ((ctx->sphStep % ctx->P.reinit_every == 0) || (ctx->sphStep == ctx->P.reinit_at)) {
     CHECK(doReset(ctx));
@@ -716,12 +735,14 @@ int stepMiddle(sphdem_ctx* ctx) {
   hipLaunchKernelGGL(k_mid_kick, dim3(blocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream,
                      ctx->cur, na, ctx->P, ctx->dg);
   LAUNCHED();
+  markStale(ctx, GS_ALL);
   return 0;
 }
 int stepEnd(sphdem_ctx* ctx) {
   hipLaunchKernelGGL(k_end_drift, dim3(blocks(std::max(ctx->ns, 1))), dim3(TPB), 0, ctx->stream,
                      ctx->cur, ctx->ns, ctx->dg);
   LAUNCHED();
+  markStale(ctx, GS_ALL);
   CHECK(doReset(ctx));
   if (varH(ctx)) CHECK(passOmega(ctx));
   CHECK(passDddt(ctx));
@@ -732,12 +753,14 @@ int stepEnd(sphdem_ctx* ctx) {
       hipLaunchKernelGGL(k_dem_kick, dim3(blocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur,
                          ctx->ns, ctx->nd, ctx->P, ctx->dg);
       LAUNCHED();
+      markStale(ctx, GS_ALL);
     }
   }
   Timed tm(ctx, KID_REST);
   hipLaunchKernelGGL(k_end_rest, dim3(blocks(std::max(ctx->ns, 1))), dim3(TPB), 0, ctx->stream,
                      ctx->cur, ctx->ns, ctx->P, ctx->dg);
   LAUNCHED();
+  markStale(ctx, GS_ALL);
   return 0;
 }
 
@@ -746,6 +769,7 @@ int hookBody(sphdem_ctx* ctx, uint32_t mask, const double* a) {
   hipLaunchKernelGGL(k_body_force, dim3(blocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream,
                      ctx->cur, na, mask, a[0], a[1], a[2]);
   LAUNCHED();
+  markStale(ctx, GS_ALL);
   return 0;
 }
 int hookFloor(sphdem_ctx* ctx) {
@@ -753,6 +777,7 @@ int hookFloor(sphdem_ctx* ctx) {
   hipLaunchKernelGGL(k_dem_floor, dim3(blocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur,
                      ctx->ns, ctx->nd, ctx->P);
   LAUNCHED();
+  markStale(ctx, GS_ALL);
   return 0;
 }
 int hookFreeze(sphdem_ctx* ctx, uint32_t mask, double t) {
@@ -760,6 +785,7 @@ int hookFreeze(sphdem_ctx* ctx, uint32_t mask, double t) {
   hipLaunchKernelGGL(k_freeze, dim3(blocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream, ctx->cur,
                      na, mask, t, ctx->dg);
   LAUNCHED();
+  markStale(ctx, GS_ALL);
   return 0;
 }
 int hookSumDrag(sphdem_ctx* ctx) {
@@ -781,7 +807,7 @@ int hookSumDrag(sphdem_ctx* ctx) {
 
 void freeAll(sphdem_ctx* ctx) {
   void* ptrs[] = {ctx->storeMem[0], ctx->storeMem[1], ctx->key,     ctx->slot,   ctx->tmp,
-                  ctx->perm,        ctx->counts,      ctx->start,   ctx->tiles,  ctx->tileOff,
+                  ctx->perm,        ctx->counts,      ctx->start,   ctx->tiles,
                   ctx->partial,     ctx->fieldbuf, ctx->errflag, ctx->dg,
                   ctx->segCnt,      ctx->segOff,      ctx->segs,    ctx->nbl,    ctx->nbcnt,
                   ctx->hinv,        ctx->gcnt,        ctx->goff,    ctx->gsrc,   ctx->gord,
@@ -928,7 +954,7 @@ int sphdem_output_vars(sphdem_ctx* ctx, sphdem_output* out) {
   HIPC(hipSetDevice(ctx->device));
   const int ns = ctx->ns;
   CHECK(ensureLists(ctx));
-  CHECK(refreshGhosts(ctx));
+  CHECK(refreshGhosts(ctx, GS_POSH | GS_VHM));
   double* densCalc = ctx->fieldbuf;
   if (ns > 0) {
     hipLaunchKernelGGL(k_density_l, dim3(blocks(ns)), dim3(TPB), 0, ctx->stream, ctx->cur, ns,
@@ -1063,6 +1089,7 @@ int sphdem_checkpoint_restore(sphdem_ctx* ctx, const void* host, int64_t bytes) 
   p += sizeof(sphdem_params);
   HIPC(hipSetDevice(ctx->device));
   HIPC(hipMemcpyAsync(ctx->dg, p, sizeof(DevGlobals), hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemsetAsync(&ctx->dg->hmax_bits, 0, sizeof(unsigned long long), ctx->stream));
   p += sizeof(DevGlobals);
   const int n = (int)h.n;
   for (int k = 0; k < kNumVec; k++) {
@@ -1145,6 +1172,7 @@ int sphdem_set_field(sphdem_ctx* ctx, int field, const void* host) {
                      (double*)storeArr(ctx->cur, L.arr), L.off, L.width, ctx->cur.orig, n,
                      (const double*)ctx->fieldbuf);
   LAUNCHED();
+  markStale(ctx, GS_ALL);
   HIPC(hipStreamSynchronize(ctx->stream));
   return 0;
 }

@@ -211,33 +211,54 @@ __global__ void k_wrap(Store S, int n, double dmin0, double dmin1, double dmin2_
   atomicMaxPos(&g->hmax_bits, hv);
 }
 
-// Range of SPH-group cells, reduced per wavefront then by atomics.
-// Binning of the real entries [0, n) (CdataLL::insert, dataLL.cpp:978-997).
+// Binning of the real entries [0, n) (CdataLL::insert, dataLL.cpp:978-997):
+// key (cell) of every entry and a slot within its cell.  The store is sorted
+// by cell from the previous reset, so neighbouring lanes mostly share a key:
+// each run of equal keys in a wavefront takes its slots with one atomic
+// (slots only need to be distinct; the rank kernel orders each cell).  The
+// first thread also clears the hmax reduction target for the next reset, whose
+// value the host has read by now.
 __global__ void k_bin(Store S, int n, Grid gs, Grid gc, int keyDeleted, int* __restrict__ key,
                       int* __restrict__ slot, int* __restrict__ counts, DevGlobals* g) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int t = S.iam[i];
-  int k;
-  if (t == T_DELETED) {
-    k = keyDeleted;
-    S.cell[i] = 0;
-  } else {
-    const double4 p = S.posh[i];
-    const Grid& gr = (t == T_DEM) ? gc : gs;
-    int c[3];
-    for (int d = 0; d < 3; d++) {
-      c[d] = cellOf(comp(const_cast<double4&>(p), d), gr.cmin[d], gr.edge, gr.off);
-      if (c[d] < 0 || c[d] >= gr.dim[d]) {
-        atomicOr(&g->errflags, 1u);
-        c[d] = c[d] < 0 ? 0 : gr.dim[d] - 1;
+  const int lane = threadIdx.x & 63;
+  if (i == 0) g->hmax_bits = 0ull;
+  int k = -1;
+  if (i < n) {
+    const int t = S.iam[i];
+    if (t == T_DELETED) {
+      k = keyDeleted;
+      S.cell[i] = 0;
+    } else {
+      const double4 p = S.posh[i];
+      const Grid& gr = (t == T_DEM) ? gc : gs;
+      int c[3];
+      for (int d = 0; d < 3; d++) {
+        c[d] = cellOf(comp(const_cast<double4&>(p), d), gr.cmin[d], gr.edge, gr.off);
+        if (c[d] < 0 || c[d] >= gr.dim[d]) {
+          atomicOr(&g->errflags, 1u);
+          c[d] = c[d] < 0 ? 0 : gr.dim[d] - 1;
+        }
       }
+      k = gr.base + (c[0] * gr.dim[1] + c[1]) * gr.dim[2] + c[2];
+      S.cell[i] = packCell(c[0], c[1], c[2]);
     }
-    k = gr.base + (c[0] * gr.dim[1] + c[1]) * gr.dim[2] + c[2];
-    S.cell[i] = packCell(c[0], c[1], c[2]);
+    key[i] = k;
   }
-  key[i] = k;
-  slot[i] = atomicAdd(&counts[k], 1);
+  // runs of equal keys: a head where the key differs from the previous lane's
+  const int prev = __shfl_up(k, 1, 64);
+  const bool head = lane == 0 || prev != k;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const int hl = 63 - __builtin_clzll(heads & upto);  // this lane's run head
+  int base = 0;
+  if (head && k >= 0) {
+    const unsigned long long rest = lane == 63 ? 0ull : (heads >> (lane + 1));
+    const int len = rest ? __builtin_ctzll(rest) + 1 : 64 - lane;
+    base = atomicAdd(&counts[k], len);
+  }
+  base = __shfl(base, hl, 64);
+  if (i < n) slot[i] = base + (lane - hl);
 }
 
 // ---- ghosts: the periodic images CdataLL::updateDomain receives
@@ -336,9 +357,25 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tile(const int* __restric
   for (int k = 0; k < SCAN_ITEMS; k++)
     if (base + k < n) out[base + k] = excl + v[k];
 }
-__global__ void k_scan_add(int* __restrict__ out, const int* __restrict__ tileOff, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] += tileOff[i / SCAN_TILE];
+// Second pass: block b adds the sum of tiles [0, b) to tile b.  The first
+// wavefront reduces those tile sums (each block redoes this small prefix,
+// which saves a launch for the scan of the tile sums).
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_add(int* __restrict__ out,
+                                                           const int* __restrict__ tileSums,
+                                                           int n) {
+  __shared__ int pre;
+  const int b = blockIdx.x;
+  if (threadIdx.x < 64) {
+    int acc = 0;
+    for (int t = threadIdx.x; t < b; t += 64) acc += tileSums[t];
+    for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
+    if (threadIdx.x == 0) pre = acc;
+  }
+  __syncthreads();
+  if (b == 0) return;
+  const int base = b * SCAN_TILE + threadIdx.x;
+  for (int k = 0; k < SCAN_ITEMS; k++)
+    if (base + k * SCAN_THREADS < n) out[base + k * SCAN_THREADS] += pre;
 }
 
 __global__ void k_scatter(int n, const int* __restrict__ key, const int* __restrict__ slot,

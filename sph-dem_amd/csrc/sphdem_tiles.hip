@@ -21,14 +21,13 @@ namespace sphdem {
 constexpr int kWavesPerBlock = 4;
 
 // ---- segments -------------------------------------------------------------------
-// Greedy packing of one column's cells into segments of <= 64 particles.
+// Greedy packing of one column's cells into segments of <= 64 particles;
+// s[z] = store start of cell z of the column, z = 0..Z (s[Z] the column end).
 template <class E>
-__device__ __forceinline__ void walkColumn(const int* __restrict__ start, const Grid& g, int col,
-                                           E&& emit) {
-  const int base = g.base + col * g.dim[2];
+__device__ __forceinline__ void walkColumn(const int* s, int Z, E&& emit) {
   int z = 0;
-  while (z < g.dim[2]) {
-    const int c0 = start[base + z], c1 = start[base + z + 1];
+  while (z < Z) {
+    const int c0 = s[z], c1 = s[z + 1];
     if (c1 == c0) {
       z++;
       continue;
@@ -41,30 +40,58 @@ __device__ __forceinline__ void walkColumn(const int* __restrict__ start, const 
     const int z0 = z;
     int last = z, acc = c1 - c0;
     z++;
-    while (z < g.dim[2]) {
-      const int k = start[base + z + 1] - start[base + z];
+    while (z < Z) {
+      const int k = s[z + 1] - s[z];
       if (acc + k > 64) break;
       acc += k;
       if (k) last = z;
       z++;
     }
-    emit(c0, start[base + last + 1], z0, last);
+    emit(c0, s[last + 1], z0, last);
   }
 }
-__global__ void k_seg_count(const int* __restrict__ start, Grid g, int* __restrict__ cnt) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= g.dim[0] * g.dim[1]) return;
-  int c = 0;
-  walkColumn(start, g, col, [&](int, int, int, int) { c++; });
-  cnt[col] = c;
+// One wavefront per column: the column's Z + 1 cell starts are staged in LDS,
+// then every lane runs the (wave-uniform) packing and lane 0 writes.
+constexpr int kSegWaves = 4;
+constexpr int kMaxZ = 1024;  // the grid's z limit
+template <class E>
+__device__ __forceinline__ bool stageColumn(const int* __restrict__ start, const Grid& g,
+                                            int (*lds)[kMaxZ + 1], int& col, E&& body) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  col = blockIdx.x * kSegWaves + w;
+  if (col >= g.dim[0] * g.dim[1]) return false;
+  const int Z = g.dim[2];
+  const int* s = start + g.base + col * Z;
+  for (int z = lane; z <= Z; z += 64) lds[w][z] = s[z];
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  body(lds[w], Z, lane);
+  return true;
 }
-__global__ void k_seg_write(const int* __restrict__ start, Grid g, const int* __restrict__ off,
-                            Segment* __restrict__ segs) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= g.dim[0] * g.dim[1]) return;
-  int w = off[col];
-  walkColumn(start, g, col, [&](int p0, int p1, int z0, int z1) {
-    segs[w++] = Segment{p0, p1, col, z0 | (z1 << 16)};
+__global__ void __launch_bounds__(64 * kSegWaves) k_seg_count(const int* __restrict__ start,
+                                                               Grid g, int* __restrict__ cnt) {
+  __shared__ int lds[kSegWaves][kMaxZ + 1];
+  int col;
+  const int ncol = g.dim[0] * g.dim[1];
+  if (!stageColumn(start, g, lds, col, [&](const int* s, int Z, int lane) {
+        int c = 0;
+        walkColumn(s, Z, [&](int, int, int, int) { c++; });
+        if (lane == 0) cnt[col] = c;
+      }))
+    if (col == ncol && (threadIdx.x & 63) == 0) cnt[ncol] = 0;  // the scan's closing zero
+}
+__global__ void __launch_bounds__(64 * kSegWaves) k_seg_write(const int* __restrict__ start,
+                                                               Grid g,
+                                                               const int* __restrict__ off,
+                                                               Segment* __restrict__ segs) {
+  __shared__ int lds[kSegWaves][kMaxZ + 1];
+  int col;
+  stageColumn(start, g, lds, col, [&](const int* s, int Z, int lane) {
+    int w = off[col];
+    walkColumn(s, Z, [&](int p0, int p1, int z0, int z1) {
+      if (lane == 0) segs[w] = Segment{p0, p1, col, z0 | (z1 << 16)};
+      w++;
+    });
   });
 }
 
