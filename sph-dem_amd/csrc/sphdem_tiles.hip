@@ -126,6 +126,9 @@ __device__ __forceinline__ size_t nbChunk(int c, int t, int ns) {
 template <class T>
 using CPtr = const __attribute__((address_space(4))) T*;
 
+// bits [0, k) set, k in [0, 32]
+__device__ __forceinline__ unsigned lowBits(int k) { return k >= 32 ? ~0u : (1u << k) - 1u; }
+
 // calcNeighbours for every real SPH-group particle, one wavefront per Segment.
 //
 // Per candidate column (X, Y): the real cells Z0..Z1 hold store range
@@ -184,39 +187,58 @@ __global__ void __launch_bounds__(256) k_lists(const double4* __restrict__ pos,
       }
       c++;
     };
-    // candidates [J0, J1), admitted for a lane when j lies in [lo, lo + wd)
+    // candidates [J0, J1), admitted for a lane when j lies in [lo, lo + wd) and
+    // is not the lane's own particle.  Batches of 32: the distance tests fill a
+    // per-lane bit mask (candidate jb + k in bit 31 - k), the lane's index
+    // window is a second mask, and then each lane appends its accepted
+    // candidates in order.  Candidates come four at a time (one 128-byte
+    // scalar read; loads past J1 are clamped to J1 - 1, whose four entries the
+    // store's slack covers), each group loaded while the previous is tested.
     auto stream = [&](int J0, int J1, int lo, int wd) {
-      // one contiguous 128-byte read per batch: entries past J1 (the store
-      // keeps slack after its last entry) are read and masked out
-      double4 qn[4];
-      auto fetch = [&](int jb) {
-        const CPtr<double> b = cpos + 4 * jb;
+      for (int jb = J0; jb < J1; jb += 32) {
+        const int klo = min(max(lo - jb, 0), 32), khi = min(max(lo + wd - jb, 0), 32);
+        unsigned win = lowBits(khi) & ~lowBits(klo);
+        const int self = t - jb;
+        if ((unsigned)self < 32u) win &= ~(1u << self);
+        win = __builtin_bitreverse32(win);
+        unsigned acc = 0;
+        double4 qa[4], qb[4];
+        auto fetch = [&](double4* q, int j) {
+          const CPtr<double> b = cpos + 4 * min(j, J1 - 1);
 #pragma unroll
-        for (int u = 0; u < 4; u++) qn[u] = make_double4(b[4 * u], b[4 * u + 1], b[4 * u + 2], b[4 * u + 3]);
-      };
-      fetch(J0);
-      for (int j = J0; j < J1; j += 4) {
-        double4 q[4];
+          for (int u = 0; u < 4; u++)
+            q[u] = make_double4(b[4 * u], b[4 * u + 1], b[4 * u + 2], b[4 * u + 3]);
+        };
+        auto test = [&](const double4* q) {
 #pragma unroll
-        for (int u = 0; u < 4; u++) q[u] = qn[u];
-        if (j + 4 < J1) fetch(j + 4);
-        bool ok[4];
+          for (int u = 0; u < 4; u++) {
+            const double dx = px - q[u].x, dy = py - q[u].y, dz = pz - q[u].z;
+            const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+            // l2 <= (2 max(ha,hb))^2 = max((2ha)^2, (2hb)^2): squaring and
+            // rounding are monotone on h > 0.  2hb is formed exactly on the
+            // scalar unit by adding one to the exponent of hb (a normal number).
+            const double hb2 =
+                __longlong_as_double(__double_as_longlong(q[u].w) + (1ll << 52));
+            acc = (acc << 1) | (l2 <= fmax(lima2, hb2 * hb2) ? 1u : 0u);
+          }
+        };
+        fetch(qa, jb);
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int jj = j + u;
-          const double dx = px - q[u].x, dy = py - q[u].y, dz = pz - q[u].z;
-          const double l2 = dot3(dx, dy, dz, dx, dy, dz);
-          // l2 <= (2 max(ha,hb))^2  <=>  l2 <= (2ha)^2 || l2 <= (2hb)^2: squaring
-          // and rounding are monotone on h > 0.  2hb is formed exactly on the
-          // scalar unit by adding one to the exponent of hb (a normal number).
-          const double hb2 =
-              __longlong_as_double(__double_as_longlong(q[u].w) + (1ll << 52));
-          ok[u] = (int)(jj < J1) & (int)((unsigned)(jj - lo) < (unsigned)wd) & (int)(jj != t) &
-                  ((int)(l2 <= lima2) | (int)(l2 <= hb2 * hb2));
+        for (int gq = 0; gq < 8; gq += 2) {
+          fetch(qb, jb + 4 * gq + 4);
+          test(qa);
+          if (gq + 2 < 8) fetch(qa, jb + 4 * gq + 8);
+          test(qb);
         }
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-          if (ok[u]) push(j + u);
+        unsigned m = acc & win;
+        int j = jb;
+        while (m) {
+          const int z = __builtin_clz(m);
+          j += z;
+          push(j);
+          j++;
+          m = (m << z) << 1;
+        }
       }
     };
     for (int X = max(cx - 1, 0); X <= min(cx + 1, g.dim[0] - 1); X++)
