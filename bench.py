@@ -61,9 +61,17 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=200)
     p.add_argument("--warmup", type=int, default=10)
-    p.add_argument("--workload", choices=("sediment", "large"), default="sediment",
+    p.add_argument("--workload", choices=("sediment", "large", "ensemble"), default="sediment",
                    help="sediment: 64^3 sedimentation (the headline); large: the 160^3 "
-                        "dam-break geometry with 65,536 DEM and the SPH-DEM flag set")
+                        "dam-break geometry with 65,536 DEM and the SPH-DEM flag set; "
+                        "ensemble: 256 independent sedimentation boxes of 4,096-65,536 "
+                        "fluid particles advanced together, against the same boxes run "
+                        "one after another")
+    p.add_argument("--boxes", type=int, default=256, help="ensemble: number of boxes")
+    p.add_argument("--workers", type=int, default=8,
+                   help="ensemble: host threads stepping the boxes")
+    p.add_argument("--precision", choices=("fp64", "mixed"), default=None,
+                   help="ensemble: arithmetic form (default mixed)")
     p.add_argument("--nf", type=int, default=None, help="fluid particles per axis")
     p.add_argument("--ndem", type=int, default=None)
     p.add_argument("--roofline-kernel", default="lists")
@@ -212,6 +220,76 @@ def mixed_form(a, counts, counted, final64):
     }
 
 
+def ensemble(a):
+    """Independent boxes (sd.ensemble_cases) advanced together by
+    sphdem_run_steps_many, then the same boxes one after another (sphdem_run_steps
+    per box), both timed over `steps` steps after `warmup` steps together.
+    particle-steps/s counts fluid + DEM of every box."""
+    prec = sd.FP64 if a.precision == "fp64" else sd.MIXED
+    cases = sd.ensemble_cases(nbox=a.boxes, seed=1005)
+    sims = [sd.Sim(c, device=0, precision=prec) for c in cases]
+    counted = sum(int(np.sum((c.iam == sd.SPH) | (c.iam == sd.DEM))) for c in cases)
+    fluid = sum(int(np.sum(c.iam == sd.SPH)) for c in cases)
+    dem = sum(int(np.sum(c.iam == sd.DEM)) for c in cases)
+    wall = sum(int(np.sum(c.iam == sd.SPH_BOUNDARY)) for c in cases)
+    sd.run_many(sims, a.warmup, workers=a.workers)
+    for s in sims:
+        s.sync()
+    t0 = time.perf_counter()
+    sd.run_many(sims, a.steps, workers=a.workers)
+    for s in sims:
+        s.sync()
+    el = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    for s in sims:
+        s.run(a.steps)
+        s.sync()
+    seq = time.perf_counter() - t1
+    for s in sims:
+        if not np.all(np.isfinite(s.get("dens"))):
+            raise RuntimeError("non-finite density after the run")
+        s.close()
+    step_bytes = B_ALG_STEP_SPH * (fluid + wall) + B_ALG_STEP_DEM * dem
+    step_gbs = step_bytes * (a.steps / el) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": counted * a.steps / el,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * el / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == sd.FP64 else
+                 "f32 pair arithmetic, f64 positions/lists/sums/integration/dt",
+        "data": "synthetic",
+        "config": {
+            "workload": f"ensemble of {a.boxes} sedimentation boxes, fluid 4096-65536 per box "
+                        "(log-uniform), DEM = fluid/16, drag/stiffness/damping x U(0.8, 1.2)",
+            "boxes": a.boxes, "fluid": fluid, "wall": wall, "dem": dem, "seed": 1005,
+            "workers": a.workers,
+        },
+        "sequential": {
+            "value": counted * a.steps / seq,
+            "unit": "particle-steps/s",
+            "ms_per_step": 1e3 * seq / a.steps,
+            "what": "the same boxes one after another (sphdem_run_steps per box)",
+        },
+        "speedup_vs_sequential": seq / el,
+        "roofline": None,
+        "step_roofline": {
+            "alg_bytes_per_step": step_bytes,
+            "achieved": step_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": step_gbs / HBM_PEAK_GBS,
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -221,6 +299,9 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(the path shards by space, see the design notes)", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "ensemble":
+        ensemble(a)
+        return
     case = make_case(a)
     counts = case.counts()
     counted = counts["fluid"] + counts["dem"]

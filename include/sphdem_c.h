@@ -198,6 +198,20 @@ int sphdem_hook_sum_drag(sphdem_ctx* ctx);
 /* Runs nsteps of run.cpp:102-118: sphStep++, hooks, start/middle/end. */
 int sphdem_run_steps(sphdem_ctx* ctx, int nsteps, int hookset);
 
+/* Ensemble of independent boxes: advances each of the nbox contexts by nsteps
+ * (as sphdem_run_steps on each) with `workers` host threads (0: min(nbox, 8)),
+ * each thread stepping its share of the boxes round robin, every box on its
+ * own stream, so the small boxes' kernels and host round trips overlap on the
+ * device.  The reference runs such ensembles as separate processes, one case
+ * directory per run (the run.cpp loop once per box); this replaces that loop
+ * over boxes.  Every box ends exactly as sphdem_run_steps alone would leave it.
+ * The contexts must be distinct and must not be used by the caller during the
+ * call.  On an error the first failing box's index goes to *failed_box (if
+ * given; -1 otherwise) and its sphdem_last_error holds the message; boxes that
+ * had not failed may have advanced by fewer than nsteps steps. */
+int sphdem_run_steps_many(sphdem_ctx* const* ctxs, int nbox, int nsteps, int hookset,
+                          int workers, int* failed_box);
+
 /* ---- parity and per-pass entry points -------------------------------------- */
 /* CdataLL::reset() (dataLL.cpp:140-236): periodic wrap, out-of-range deletion,
  * hmax, cell grid hysteresis, binning.  Invalidates cached neighbour sets. */

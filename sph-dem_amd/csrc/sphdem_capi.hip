@@ -10,6 +10,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <vector>
 
 #include "sphdem_kernels.hip"
@@ -1277,6 +1279,40 @@ int sphdem_run_steps(sphdem_ctx* ctx, int nsteps, int hookset) {
     if (hookset == SPHDEM_HOOKS_SEDIMENT) CHECK(hookFreeze(ctx, demMask, ctx->P.time_drop));
   }
   return 0;
+}
+
+int sphdem_run_steps_many(sphdem_ctx* const* ctxs, int nbox, int nsteps, int hookset,
+                          int workers, int* failed_box) {
+  if (failed_box) *failed_box = -1;
+  if (nbox < 0 || nsteps < 0 || workers < 0 || (nbox > 0 && !ctxs)) return SPHDEM_E_ARG;
+  if (nbox == 0 || nsteps == 0) return 0;
+  std::vector<sphdem_ctx*> order(ctxs, ctxs + nbox);
+  std::sort(order.begin(), order.end());
+  if (!order[0] || std::adjacent_find(order.begin(), order.end()) != order.end())
+    return SPHDEM_E_ARG;  // a null or a repeated context
+  if (workers == 0) workers = 8;
+  workers = std::min(workers, nbox);
+  std::atomic<int> err{0};
+  std::atomic<int> errBox{-1};
+  auto work = [&](int w) {
+    for (int s = 0; s < nsteps; s++)
+      for (int b = w; b < nbox; b += workers) {
+        if (err.load(std::memory_order_relaxed)) return;
+        const int rc = sphdem_run_steps(ctxs[b], 1, hookset);
+        if (rc) {
+          int none = 0;
+          if (err.compare_exchange_strong(none, rc)) errBox.store(b);
+          return;
+        }
+      }
+  };
+  std::vector<std::thread> pool;
+  pool.reserve(workers - 1);
+  for (int w = 1; w < workers; w++) pool.emplace_back(work, w);
+  work(0);
+  for (std::thread& t : pool) t.join();
+  if (failed_box) *failed_box = errBox.load();
+  return err.load();
 }
 
 int sphdem_reset(sphdem_ctx* ctx) {

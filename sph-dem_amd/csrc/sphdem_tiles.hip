@@ -177,14 +177,12 @@ __global__ void __launch_bounds__(256) k_lists(const double4* __restrict__ pos,
     // the chunk being filled, as a shift register: b0 the newest entry
     int b0 = 0, b1 = 0, b2 = 0, b3 = 0;
     auto push = [&](int e) {
-      if (c < kMaxNb) {
-        b3 = b2;
-        b2 = b1;
-        b1 = b0;
-        b0 = e;
-        if ((c & (kChunk - 1)) == kChunk - 1)
-          *reinterpret_cast<int4*>(nbl + nbChunk(c >> 2, t, ns)) = make_int4(b3, b2, b1, b0);
-      }
+      b3 = b2;
+      b2 = b1;
+      b1 = b0;
+      b0 = e;
+      if ((c & (kChunk - 1)) == kChunk - 1 && c < kMaxNb)
+        *reinterpret_cast<int4*>(nbl + nbChunk(c >> 2, t, ns)) = make_int4(b3, b2, b1, b0);
       c++;
     };
     // candidates [J0, J1), admitted for a lane when j lies in [lo, lo + wd) and

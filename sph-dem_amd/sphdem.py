@@ -464,38 +464,70 @@ def dambreak_case(nf: int = 32, dx: float = 0.01, seed: int = 1000, with_dem: in
 
 
 def sediment_case(nf: int = 64, n_dem: int | None = None, seed: int = 1001,
-                  jitter: bool = True) -> Case:
+                  jitter: bool = True, nz: int | None = None) -> Case:
     """SPH-DEM sedimentation: an nf^3 fluid lattice (dx = 1/nf) filling the unit
     cube, periodic in x and y, 3 layers of sphBoundary walls below z=0 and above
     z=1, and n_dem (default nf^3/64) DEM spheres of diameter dx/2, density 2500,
-    placed without overlap in the upper half.  sediment3D flag set
+    placed without overlap in the upper half.  With `nz` the fluid has nz
+    layers and the box is nz*dx high (nf x nf x nz particles).  sediment3D flag set
     (sediment3D/customConstants.h:45-79): WENDLAND, VISC_MORRIS,
     VAR_H_CORRECTION, LIQ_DEM, LINEAR, NO_ANTICLUMPING,
     LIQ_DEM_CUSTOM_WALL_CONTACTS; coupling radius 2h, Di Felice drag,
     restitution 0.9."""
     dx = 1.0 / nf
+    lz = 1.0 if nz is None else nz * dx
+    nz = nf if nz is None else nz
     rmin = (0.0, 0.0, 0.0)
-    rmax = (1.0, 1.0, 1.0)
+    rmax = (1.0, 1.0, lz)
     vmax = 2.0 * math.sqrt(2 * 9.81 * (rmax[2] - rmin[2]))  # customConstants.h:97
     c0 = 10.0 * vmax
     flags = F_VISC_MORRIS | F_VAR_H_CORRECTION | F_LIQ_DEM | F_CUSTOM_WALL
     P = _base_params(flags, rmin, rmax, (1, 1, 0), dx, c0, 8.9e-7)
-    fluid = _lattice(nf, nf, nf, dx)
+    fluid = _lattice(nf, nf, nz, dx)
     if jitter:
         fluid = _jitter(fluid, dx, seed)
     lo = _lattice(nf, nf, 3, dx, origin=(0.0, 0.0, -3 * dx))
-    hi = _lattice(nf, nf, 3, dx, origin=(0.0, 0.0, 1.0))
+    hi = _lattice(nf, nf, 3, dx, origin=(0.0, 0.0, lz))
     walls = np.concatenate([lo, hi])
     wall_dens = P.refd * math.pow(9.81 * P.refd * (rmax[2] - rmin[2]) / P.prb + 1, 1.0 / 7.0)
     fluid_dt = 0.6 * P.h / (2.0 * P.spsound)
     _set_dem_constants(P, 0.25 * dx, 2500.0, fluid_dt)
     if n_dem is None:
-        n_dem = nf ** 3 // 64
+        n_dem = nf * nf * nz // 64
     R = P.dem_radius
-    dem = _place_dem(n_dem, (0.0, 0.0, 0.5), (1.0, 1.0, 1.0 - R), R, seed + 1,
+    dem = _place_dem(n_dem, (0.0, 0.0, 0.5 * lz), (1.0, 1.0, lz - R), R, seed + 1,
                      (1, 1, 0), rmax) if n_dem else np.zeros((0, 3))
     return _finish("sediment", P, fluid, walls, dem, HOOKS_SEDIMENT, wall_dens,
-                   {"nf": nf, "dx": dx, "seed": seed, "n_dem": n_dem})
+                   {"nf": nf, "nz": nz, "dx": dx, "seed": seed, "n_dem": n_dem})
+
+
+def ensemble_cases(nbox: int = 256, seed: int = 1005, nmin: int = 4096,
+                   nmax: int = 65536) -> list:
+    """Independent sedimentation boxes of uneven size: per box a fluid count N
+    drawn log-uniformly from [nmin, nmax] (an m x m x nz lattice with m the
+    nearest integer to N^(1/3), nz = round(N / m^2)), N/16 DEM spheres, and the
+    Di Felice chi-law coefficients, the contact stiffness k and the damping
+    gamma each scaled by a factor drawn uniformly from [0.8, 1.2] (DEM_TIMESTEP
+    recomputed from k and gamma as demConstants.h:6 does)."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for b in range(nbox):
+        N = int(math.floor(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+        m = max(4, int(round(N ** (1.0 / 3.0))))
+        nz = max(4, int(round(N / (m * m))))
+        c = sediment_case(nf=m, n_dem=(m * m * nz) // 16, seed=seed + 7 * (b + 1), nz=nz)
+        sd, sk, sg = rng.uniform(0.8, 1.2, size=3)
+        P = c.params
+        P.drag_a *= sd
+        P.drag_b *= sd
+        P.dem_k *= sk
+        P.dem_gamma *= sg
+        m_red = 0.5 * P.dem_vol * P.dem_dens
+        P.dem_timestep = (1.0 / 50.0) * PI_REF / math.sqrt(
+            P.dem_k / m_red - math.pow(0.5 * P.dem_gamma / m_red, 2))
+        c.info.update({"box": b, "scale_drag": sd, "scale_k": sk, "scale_gamma": sg})
+        cases.append(c)
+    return cases
 
 
 # ----------------------------------------------------------------------------
@@ -544,6 +576,9 @@ def load_library(path: str = LIB_PATH):
     lib.sphdem_hook_body_force.argtypes = [vp, ctypes.c_uint32, dp]
     lib.sphdem_hook_freeze.argtypes = [vp, ctypes.c_uint32, ctypes.c_double]
     lib.sphdem_run_steps.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    lib.sphdem_run_steps_many.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_int)]
     lib.sphdem_get_cell_coords.argtypes = [vp, ip]
     lib.sphdem_get_pairs.argtypes = [vp, ctypes.c_int, ctypes.POINTER(i64), ip, ip, ip, i64]
     lib.sphdem_reduce_dt.argtypes = [vp, dp]
@@ -552,6 +587,28 @@ def load_library(path: str = LIB_PATH):
     lib.sphdem_timing_end.argtypes = [vp, dp, ctypes.POINTER(i64)]
     _lib = lib
     return lib
+
+
+def run_many(sims: list, nsteps: int, hookset: int | None = None, workers: int = 0):
+    """Advances independent boxes together (sphdem_run_steps_many): each Sim by
+    nsteps steps, `workers` host threads stepping their share of the boxes."""
+    if not sims:
+        return
+    lib = load_library()
+    if hookset is None:
+        hookset = sims[0].case.hookset
+        if any(s.case.hookset != hookset for s in sims):
+            raise ValueError("boxes with different hook sets need an explicit hookset")
+    arr = (ctypes.c_void_p * len(sims))(*[s.ctx.value for s in sims])
+    failed = ctypes.c_int(-1)
+    rc = lib.sphdem_run_steps_many(arr, len(sims), nsteps, hookset, workers,
+                                   ctypes.byref(failed))
+    if rc != 0:
+        msg = b""
+        if failed.value >= 0:
+            msg = lib.sphdem_last_error(sims[failed.value].ctx) or b""
+        raise SphDemError(f"sphdem_run_steps_many failed ({rc}) in box {failed.value}: "
+                          f"{msg.decode(errors='replace')}")
 
 
 def _dptr(a):
