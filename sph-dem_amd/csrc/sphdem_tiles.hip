@@ -138,8 +138,8 @@ __device__ __forceinline__ unsigned lowBits(int k) { return k >= 32 ? ~0u : (1u 
 // per lane; with ghosts the column is streamed cell by cell (the real part of
 // a cell, then its ghosts) with the lane's |z - cz| <= 1 test.  is_neighbr
 // (particle.h:236-238) is len2(ra - rb) <= (2 max(ha, hb))^2 with blitz's dot
-// association.  Candidates are loaded four at a time with scalar loads,
-// issued one batch ahead of the arithmetic.
+// association.  Candidates arrive by scalar loads, two per load, three loads
+// ahead of the arithmetic.
 __global__ void __launch_bounds__(256) k_lists(const double4* __restrict__ pos,
                                                const int* __restrict__ cellc, Grid g, int gbase,
                                                const int* __restrict__ start,
@@ -189,9 +189,9 @@ __global__ void __launch_bounds__(256) k_lists(const double4* __restrict__ pos,
     // is not the lane's own particle.  Batches of 32: the distance tests fill a
     // per-lane bit mask (candidate jb + k in bit 31 - k), the lane's index
     // window is a second mask, and then each lane appends its accepted
-    // candidates in order.  Candidates come four at a time (one 128-byte
-    // scalar read; loads past J1 are clamped to J1 - 1, whose four entries the
-    // store's slack covers), each group loaded while the previous is tested.
+    // candidates in order.  Candidates come two at a time (one 64-byte scalar
+    // read; loads past J1 are clamped to J1 - 1, whose two entries the store's
+    // slack covers) into a ring of four groups, three groups ahead of the test.
     auto stream = [&](int J0, int J1, int lo, int wd) {
       for (int jb = J0; jb < J1; jb += 32) {
         const int klo = min(max(lo - jb, 0), 32), khi = min(max(lo + wd - jb, 0), 32);
@@ -200,33 +200,30 @@ __global__ void __launch_bounds__(256) k_lists(const double4* __restrict__ pos,
         if ((unsigned)self < 32u) win &= ~(1u << self);
         win = __builtin_bitreverse32(win);
         unsigned acc = 0;
-        double4 qa[4], qb[4];
-        auto fetch = [&](double4* q, int j) {
+        double4 q[4][2];  // ring of four groups of two candidates
+        auto fetch = [&](double4* d, int j) {
           const CPtr<double> b = cpos + 4 * min(j, J1 - 1);
 #pragma unroll
-          for (int u = 0; u < 4; u++)
-            q[u] = make_double4(b[4 * u], b[4 * u + 1], b[4 * u + 2], b[4 * u + 3]);
+          for (int u = 0; u < 2; u++)
+            d[u] = make_double4(b[4 * u], b[4 * u + 1], b[4 * u + 2], b[4 * u + 3]);
         };
-        auto test = [&](const double4* q) {
+        auto test = [&](const double4* d) {
 #pragma unroll
-          for (int u = 0; u < 4; u++) {
-            const double dx = px - q[u].x, dy = py - q[u].y, dz = pz - q[u].z;
+          for (int u = 0; u < 2; u++) {
+            const double dx = px - d[u].x, dy = py - d[u].y, dz = pz - d[u].z;
             const double l2 = dot3(dx, dy, dz, dx, dy, dz);
-            // l2 <= (2 max(ha,hb))^2 = max((2ha)^2, (2hb)^2): squaring and
-            // rounding are monotone on h > 0.  2hb is formed exactly on the
-            // scalar unit by adding one to the exponent of hb (a normal number).
             const double hb2 =
-                __longlong_as_double(__double_as_longlong(q[u].w) + (1ll << 52));
+                __longlong_as_double(__double_as_longlong(d[u].w) + (1ll << 52));
             acc = (acc << 1) | (l2 <= fmax(lima2, hb2 * hb2) ? 1u : 0u);
           }
         };
-        fetch(qa, jb);
+        fetch(q[0], jb);
+        fetch(q[1], jb + 2);
+        fetch(q[2], jb + 4);
 #pragma unroll
-        for (int gq = 0; gq < 8; gq += 2) {
-          fetch(qb, jb + 4 * gq + 4);
-          test(qa);
-          if (gq + 2 < 8) fetch(qa, jb + 4 * gq + 8);
-          test(qb);
+        for (int gq = 0; gq < 16; gq++) {
+          if (gq + 3 < 16) fetch(q[(gq + 3) & 3], jb + 2 * (gq + 3));
+          test(q[gq & 3]);
         }
         unsigned m = acc & win;
         int j = jb;
