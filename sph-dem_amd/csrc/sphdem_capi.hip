@@ -44,6 +44,7 @@ struct sphdem_ctx {
   // ghost field groups whose origins changed since the ghosts were copied
   unsigned gstale = 7;
   int gcntZeroAt = -1;  // the n for which gcnt[n] was cleared
+  int countsClean = 0;  // counts[0, countsClean) are zero (cleared by k_scatter)
   GhostBand band{};         // cells a ghost may occupy on the current grid
   bool gridValid = false;   // gs/band describe the grid of the last reset
   int64_t ghostCap = 0;     // store entries beyond cap available to ghosts
@@ -258,14 +259,16 @@ int ensureKeys(sphdem_ctx* ctx, int K) {
     if (ctx->counts) HIPC(hipFree(ctx->counts));
     if (ctx->start) HIPC(hipFree(ctx->start));
     ctx->Kcap = (K + 1) + (K + 1) / 4;
+    ctx->countsClean = 0;
     HIPC(hipMalloc(&ctx->counts, sizeof(int) * ctx->Kcap));
     HIPC(hipMalloc(&ctx->start, sizeof(int) * ctx->Kcap));
   }
   return 0;
 }
 
-// exclusive scan of in[0..m) into out (m <= SCAN_TILE^2)
-int scanInts(sphdem_ctx* ctx, const int* in, int* out, int m) {
+// exclusive scan of in[0..m) into out (m <= SCAN_TILE^2); copyLast (device,
+// optional) also receives out[m - 1]
+int scanInts(sphdem_ctx* ctx, const int* in, int* out, int m, int* copyLast = nullptr) {
   const int tiles = (m + SCAN_TILE - 1) / SCAN_TILE;
   if (tiles > SCAN_TILE) return fail(ctx, SPHDEM_E_CAPACITY, "scan of %d items", m);
   if (tiles + 1 > ctx->tileCap) {
@@ -274,11 +277,11 @@ int scanInts(sphdem_ctx* ctx, const int* in, int* out, int m) {
     HIPC(hipMalloc(&ctx->tiles, sizeof(int) * ctx->tileCap));
   }
   hipLaunchKernelGGL(k_scan_tile, dim3(tiles), dim3(SCAN_THREADS), 0, ctx->stream, in, out,
-                     ctx->tiles, m);
+                     ctx->tiles, m, tiles == 1 ? copyLast : (int*)nullptr);
   LAUNCHED();
   if (tiles > 1) {
     hipLaunchKernelGGL(k_scan_add, dim3(tiles), dim3(SCAN_THREADS), 0, ctx->stream, out,
-                       ctx->tiles, m);
+                       ctx->tiles, m, copyLast);
     LAUNCHED();
   }
   return 0;
@@ -426,7 +429,8 @@ void setupBand(sphdem_ctx* ctx) {
 
 // keepSsr: sort only, keeping the search radius (a checkpoint restore, which
 // must not move the hysteresis of calcDomainLimits)
-int doReset(sphdem_ctx* ctx, bool keepSsr = false) {
+// midInit: middle()'s reset, which also starts the dt minimum at maxdt
+int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false) {
   const sphdem_params& P = ctx->P;
   const int n = ctx->n;
   // hmax_bits is cleared by the previous reset's k_bin (or by the upload);
@@ -451,16 +455,14 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false) {
     Timed tm(ctx, KID_WRAP);
     hipLaunchKernelGGL(k_wrap, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, ctx->cur, n, dmin[0],
                        dmin[1], dmin[2], dsp[0], dsp[1], dsp[2], ctx->box, ctx->dg, ctx->gs,
-                       ctx->band, pre ? ctx->gcnt : (int*)nullptr);
+                       ctx->band, pre ? ctx->gcnt : (int*)nullptr, midInit ? 1 : 0);
     LAUNCHED();
   }
-  if (pre) {
-    CHECK(scanInts(ctx, ctx->gcnt, ctx->goff, n + 1));
-    HIPC(hipMemcpyAsync(&ngPre, ctx->goff + n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  }
+  if (pre) CHECK(scanInts(ctx, ctx->gcnt, ctx->goff, n + 1, &ctx->dg->ghostTotal));
   DevGlobals hg;
   HIPC(hipMemcpyAsync(&hg, ctx->dg, sizeof(DevGlobals), hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
+  if (pre) ngPre = hg.ghostTotal;
   const double oldSsr = ctx->ssr;
   if (hg.errflags & 1u)
     return fail(ctx, SPHDEM_E_CAPACITY, "a particle fell outside the cell grid at the last reset");
@@ -482,7 +484,9 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false) {
       return fail(ctx, SPHDEM_E_CAPACITY, "cell grid %d x %d x %d exceeds 2048 x 2048 x 1024",
                   gr->dim[0], gr->dim[1], gr->dim[2]);
   CHECK(ensureKeys(ctx, ctx->K));
-  HIPC(hipMemsetAsync(ctx->counts, 0, sizeof(int) * (ctx->K + 1), ctx->stream));
+  if (ctx->countsClean < ctx->K + 1)
+    HIPC(hipMemsetAsync(ctx->counts, 0, sizeof(int) * (ctx->K + 1), ctx->stream));
+  ctx->countsClean = 0;  // in use until k_scatter clears them
   const int keyDeleted = ctx->gs.dim[0] * ctx->gs.dim[1] * ctx->gs.dim[2] +
                          ctx->gc.dim[0] * ctx->gc.dim[1] * ctx->gc.dim[2];
   {
@@ -515,8 +519,9 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false) {
   const int ntot = n + ng;
   CHECK(scanCounts(ctx, ctx->K + 1));
   hipLaunchKernelGGL(k_scatter, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ntot, ctx->key,
-                     ctx->slot, ctx->start, ctx->tmp);
+                     ctx->slot, ctx->start, ctx->tmp, ctx->counts, ctx->K + 1);
   LAUNCHED();
+  ctx->countsClean = ctx->K + 1;
   {
     Timed tm(ctx, KID_RANK);
     hipLaunchKernelGGL(k_rank, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ntot, n, ctx->key,
@@ -704,12 +709,16 @@ int passDensity(sphdem_ctx* ctx) {
 }
 
 // ---- half-steps
-int stepStart(sphdem_ctx* ctx) {
+// bodyMask/accel: the beforeMiddle body-force hook, applied by k_start right
+// after initSumsMiddle zeroes f (nothing in between touches f)
+int stepStart(sphdem_ctx* ctx, uint32_t bodyMask = 0, const double* accel = nullptr) {
   const int na = ctx->ns + ctx->nd;
+  const double a0 = accel ? accel[0] : 0.0, a1 = accel ? accel[1] : 0.0,
+               a2 = accel ? accel[2] : 0.0;
   {
     Timed tm(ctx, KID_START);
     hipLaunchKernelGGL(k_start, dim3(blocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream,
-                       ctx->cur, na, ctx->P, ctx->dg);
+                       ctx->cur, na, ctx->P, ctx->dg, accel ? bodyMask : 0u, a0, a1, a2);
     LAUNCHED();
     markStale(ctx, GS_ALL);
   }
@@ -720,18 +729,14 @@ int stepStart(sphdem_ctx* ctx) {
   return 0;
 }
 int stepMiddle(sphdem_ctx* ctx) {
-  hipLaunchKernelGGL(k_mid_init, dim3(1), dim3(1), 0, ctx->stream, ctx->dg);
-  LAUNCHED();
-  CHECK(doReset(ctx));
+  CHECK(doReset(ctx, false, true));
   if (liqDem(ctx)) CHECK(passCoupling(ctx));
   if (varH(ctx)) CHECK(passOmega(ctx));
   CHECK(passEos(ctx));
   CHECK(passForce(ctx));
   const int na = ctx->ns + ctx->nd;
   hipLaunchKernelGGL(k_accel, dim3(blocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream, ctx->cur,
-                     na, ctx->dg);
-  LAUNCHED();
-  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1), 0, ctx->stream, ctx->P, ctx->dg);
+                     na, ctx->P, ctx->dg);
   LAUNCHED();
   Timed tm(ctx, KID_KICK);
   hipLaunchKernelGGL(k_mid_kick, dim3(blocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream,
@@ -799,10 +804,7 @@ int hookSumDrag(sphdem_ctx* ctx) {
     HIPC(hipMalloc(&ctx->partial, sizeof(double) * ctx->partialCap));
   }
   hipLaunchKernelGGL(k_sum_drag, dim3(nb), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns + ctx->nd,
-                     ctx->partial);
-  LAUNCHED();
-  hipLaunchKernelGGL(k_sum_drag_final, dim3(1), dim3(256), 0, ctx->stream, ctx->partial, nb,
-                     ctx->dg);
+                     ctx->partial, ctx->dg);
   LAUNCHED();
   return 0;
 }
@@ -1090,8 +1092,13 @@ int sphdem_checkpoint_restore(sphdem_ctx* ctx, const void* host, int64_t bytes) 
     return fail(ctx, SPHDEM_E_ARG, "the checkpoint was taken with other parameters");
   p += sizeof(sphdem_params);
   HIPC(hipSetDevice(ctx->device));
-  HIPC(hipMemcpyAsync(ctx->dg, p, sizeof(DevGlobals), hipMemcpyHostToDevice, ctx->stream));
-  HIPC(hipMemsetAsync(&ctx->dg->hmax_bits, 0, sizeof(unsigned long long), ctx->stream));
+  // the reduction targets and counters start clear, whatever the saved values
+  DevGlobals dgh;
+  memcpy(&dgh, p, sizeof(DevGlobals));
+  dgh.hmax_bits = 0;
+  dgh.ticket = 0;
+  HIPC(hipMemcpyAsync(ctx->dg, &dgh, sizeof(DevGlobals), hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));  // dgh is a local
   p += sizeof(DevGlobals);
   const int n = (int)h.n;
   for (int k = 0; k < kNumVec; k++) {
@@ -1266,14 +1273,22 @@ int sphdem_run_steps(sphdem_ctx* ctx, int nsteps, int hookset) {
   const uint32_t sphMask = 1u << SPHDEM_SPH, demMask = 1u << SPHDEM_DEM;
   for (int s = 0; s < nsteps; s++) {
     ctx->sphStep++;
-    CHECK(stepStart(ctx));
-    if (hookset != SPHDEM_HOOKS_NONE) CHECK(hookBody(ctx, sphMask, g));  // beforeMiddle
+    // start(), then the beforeMiddle gravity on sph (fused into k_start)
+    if (hookset != SPHDEM_HOOKS_NONE)
+      CHECK(stepStart(ctx, sphMask, g));
+    else
+      CHECK(stepStart(ctx));
     CHECK(stepMiddle(ctx));
     if (hookset == SPHDEM_HOOKS_SEDIMENT) {  // beforeEnd, sediment3D/customSim.cpp:97-107
       CHECK(hookSumDrag(ctx));
-      CHECK(hookBody(ctx, demMask, g));
-      CHECK(hookFloor(ctx));
-      CHECK(hookFreeze(ctx, demMask, ctx->P.time_drop));
+      // gravity, the floor contact and the freeze on dem, fused per particle
+      if (ctx->nd > 0) {
+        hipLaunchKernelGGL(k_hooks_dem_end, dim3(blocks(ctx->nd)), dim3(TPB), 0, ctx->stream,
+                           ctx->cur, ctx->ns, ctx->nd, g[0], g[1], g[2], ctx->P,
+                           ctx->P.time_drop, ctx->dg);
+        LAUNCHED();
+        markStale(ctx, GS_ALL);
+      }
     }
     CHECK(stepEnd(ctx));
     if (hookset == SPHDEM_HOOKS_SEDIMENT) CHECK(hookFreeze(ctx, demMask, ctx->P.time_drop));

@@ -171,8 +171,10 @@ __device__ __forceinline__ int ghostCount(const double4& p, const Grid& g, const
 // when the reset keeps that grid).
 __global__ void k_wrap(Store S, int n, double dmin0, double dmin1, double dmin2_,
                        double dsp0, double dsp1, double dsp2, Box B, DevGlobals* g, Grid gs,
-                       GhostBand gb, int* __restrict__ gcnt) {
+                       GhostBand gb, int* __restrict__ gcnt, int initNewDt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // middle() starts its dt minimum at maxdt (nothing reads it during the reset)
+  if (initNewDt && i == 0) g->newDt_bits = (unsigned long long)__double_as_longlong(g->maxdt);
   double hv = 0.0;
   int ghosts = 0;
   if (i < n) {
@@ -322,9 +324,11 @@ constexpr int SCAN_THREADS = 1024;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 
+// copyLast (optional): also receives out[n - 1]
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tile(const int* __restrict__ in,
                                                             int* __restrict__ out,
-                                                            int* __restrict__ tileSums, int n) {
+                                                            int* __restrict__ tileSums, int n,
+                                                            int* copyLast) {
   __shared__ int waveTot[SCAN_THREADS / 64];
   const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
   int v[SCAN_ITEMS];
@@ -355,14 +359,17 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_tile(const int* __restric
   __syncthreads();
   const int excl = waveTot[wave] + incl - run;
   for (int k = 0; k < SCAN_ITEMS; k++)
-    if (base + k < n) out[base + k] = excl + v[k];
+    if (base + k < n) {
+      out[base + k] = excl + v[k];
+      if (copyLast && base + k == n - 1) *copyLast = excl + v[k];
+    }
 }
 // Second pass: block b adds the sum of tiles [0, b) to tile b.  The first
 // wavefront reduces those tile sums (each block redoes this small prefix,
 // which saves a launch for the scan of the tile sums).
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_add(int* __restrict__ out,
                                                            const int* __restrict__ tileSums,
-                                                           int n) {
+                                                           int n, int* copyLast) {
   __shared__ int pre;
   const int b = blockIdx.x;
   if (threadIdx.x < 64) {
@@ -374,14 +381,24 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_add(int* __restrict__ out
   __syncthreads();
   if (b == 0) return;
   const int base = b * SCAN_TILE + threadIdx.x;
-  for (int k = 0; k < SCAN_ITEMS; k++)
-    if (base + k * SCAN_THREADS < n) out[base + k * SCAN_THREADS] += pre;
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    const int i = base + k * SCAN_THREADS;
+    if (i < n) {
+      const int v = out[i] + pre;
+      out[i] = v;
+      if (copyLast && i == n - 1) *copyLast = v;
+    }
+  }
 }
 
+// Also clears the cell counts [0, k1) for the next reset (the scan has
+// consumed them).
 __global__ void k_scatter(int n, const int* __restrict__ key, const int* __restrict__ slot,
-                          const int* __restrict__ start, int* __restrict__ tmp) {
+                          const int* __restrict__ start, int* __restrict__ tmp,
+                          int* __restrict__ counts, int k1) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) tmp[start[key[i]] + slot[i]] = i;
+  for (int k = i; k < k1; k += gridDim.x * blockDim.x) counts[k] = 0;
 }
 
 // Order inside a cell = the reference's insertion order: real particles in
@@ -510,7 +527,39 @@ __global__ void k_eos(Store S, int ns, sphdem_params P) {
 }
 
 // Nsph::accelCondition sph.h:566-568 over sph, sphBoundary and dem
-__global__ void k_accel(Store S, int n, DevGlobals* g) {
+// Last-block completion of a grid-wide reduction: every block's thread 0 calls
+// this after publishing its part; it returns true in exactly one block, the
+// last to arrive, once every other block's part is visible (the counter is
+// left at zero for the next kernel that uses it).
+__device__ __forceinline__ bool lastBlock(DevGlobals* g) {
+  __threadfence();
+  const unsigned done = atomicAdd(&g->ticket, 1u);
+  if (done != gridDim.x - 1) return false;
+  atomicExch(&g->ticket, 0u);
+  __threadfence();
+  return true;
+}
+__device__ __forceinline__ double loadAgent(const unsigned long long* p) {
+  return __longlong_as_double(
+      (long long)__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// dt rule (sphIncompress.cpp:126-154): min with LIQ_DEM_TIMESTEP and, after
+// TIME_DROP_PARTICLE, DEM_TIMESTEP; both branches set dt = newDt.
+__device__ void dtRule(const sphdem_params& P, DevGlobals* g) {
+  double nd = loadAgent(&g->newDt_bits);
+  if (P.flags & SPHDEM_F_LIQ_DEM) {
+    nd = dmin2(nd, P.liq_dem_timestep);
+    if (!(P.flags & (SPHDEM_F_FIXED_DEM | SPHDEM_F_NO_DEM_CONTACTS)))
+      if (g->time > P.time_drop) nd = dmin2(nd, P.dem_timestep);
+  }
+  g->newDt = nd;
+  g->oldDt = g->dt;
+  g->dt = nd;
+}
+
+// accelCondition over sph, sphBoundary and dem, then (last block) the dt rule
+__global__ void k_accel(Store S, int n, sphdem_params P, DevGlobals* g) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   double v = __longlong_as_double(0x7ff0000000000000ll);
   if (a < n) {
@@ -521,6 +570,7 @@ __global__ void k_accel(Store S, int n, DevGlobals* g) {
     }
   }
   atomicMinPos(&g->newDt_bits, v);
+  if (threadIdx.x == 0 && lastBlock(g)) dtRule(P, g);
 }
 
 // Nsph::calcHFromDens sph.h:1823-1825
@@ -820,7 +870,10 @@ __global__ void k_contacts(Store S, int ns, int nd, Grid gc, const int* __restri
 // start(): driftRAndKick + driftRest + initSumsMiddle on sph/sphBoundary,
 // driftR on dem (sphIncompress.cpp:4-16,37; sphIncompress.h:620-704, 744-763);
 // time += dt/2.
-__global__ void k_start(Store S, int n, sphdem_params P, DevGlobals* g) {
+// bodyMask/ax..az: a body force added right after initSumsMiddle (the
+// beforeMiddle hook of run.cpp:112-113, addGravity) for the types in bodyMask.
+__global__ void k_start(Store S, int n, sphdem_params P, DevGlobals* g, unsigned bodyMask,
+                        double ax, double ay, double az) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   const double dt = g->dt / 2;
   if (a == 0) g->time += g->dt / 2;
@@ -852,9 +905,15 @@ __global__ void k_start(Store S, int n, sphdem_params P, DevGlobals* g) {
     S.dpo[a] = d;
     if (!(P.flags & SPHDEM_F_CONST_H)) p.w = P.hfac * pow_third(vh.w / d.x);
     S.posh[a] = p;
-    // initSumsMiddle
+    // initSumsMiddle, then the body force of the hook
     const double4 z = make_double4(0.0, 0.0, 0.0, 0.0);
-    S.frc[a] = z;
+    double4 f0 = z;
+    if ((bodyMask >> t) & 1u) {
+      if (ax != 0.0) f0.x += ax;
+      if (ay != 0.0) f0.y += ay;
+      if (az != 0.0) f0.z += az;
+    }
+    S.frc[a] = f0;
     S.fp[a] = make_double4(0.0, 0.0, 0.0, S.fp[a].w);
     S.fv[a] = z;
     S.fb[a] = z;
@@ -868,21 +927,6 @@ __global__ void k_start(Store S, int n, sphdem_params P, DevGlobals* g) {
   }
 }
 
-__global__ void k_mid_init(DevGlobals* g) { g->newDt_bits = (unsigned long long)__double_as_longlong(g->maxdt); }
-
-// dt rule (sphIncompress.cpp:126-154): min with LIQ_DEM_TIMESTEP and, after
-// TIME_DROP_PARTICLE, DEM_TIMESTEP; both branches set dt = newDt.
-__global__ void k_dt_final(sphdem_params P, DevGlobals* g) {
-  double nd = __longlong_as_double((long long)g->newDt_bits);
-  if (P.flags & SPHDEM_F_LIQ_DEM) {
-    nd = dmin2(nd, P.liq_dem_timestep);
-    if (!(P.flags & (SPHDEM_F_FIXED_DEM | SPHDEM_F_NO_DEM_CONTACTS)))
-      if (g->time > P.time_drop) nd = dmin2(nd, P.dem_timestep);
-  }
-  g->newDt = nd;
-  g->oldDt = g->dt;
-  g->dt = nd;
-}
 
 // kick (sphIncompress.h:706-742) for sph with (oldDt/2, dt/2), initSumsEnd on
 // sph/sphBoundary, then driftRAndKick + initSumsMiddle on dem
@@ -1063,9 +1107,12 @@ __global__ void k_freeze(Store S, int n, unsigned mask, double tfreeze, const De
   S.vhm[a] = make_double4(0.0, 0.0, 0.0, S.vhm[a].w);
 }
 // sumDragForces sediment3D/customSim.cpp:27-42: per-block partial sums of
-// m*fdrag for sph (0..2) and dem (3..5); k_sum_drag_final adds them in block order.
-__global__ void k_sum_drag(Store S, int n, double* partial) {
+// m*fdrag for sph (0..2) and dem (3..5); the last block to finish adds them in
+// a fixed order (strided per thread, then a tree) into custom[0..5].
+__global__ void __launch_bounds__(256) k_sum_drag(Store S, int n, double* partial,
+                                                  DevGlobals* g) {
   __shared__ double red[6][256];
+  __shared__ bool last;
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   double v[6] = {0, 0, 0, 0, 0, 0};
   if (a < n) {
@@ -1089,17 +1136,19 @@ __global__ void k_sum_drag(Store S, int n, double* partial) {
       for (int k = 0; k < 6; k++) red[k][threadIdx.x] += red[k][threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0)
+  if (threadIdx.x == 0) {
     for (int k = 0; k < 6; k++) partial[6 * blockIdx.x + k] = red[k][0];
-}
-// one 256-thread block: strided partial sums in a fixed order, then a tree
-__global__ void __launch_bounds__(256) k_sum_drag_final(const double* partial, int nb,
-                                                        DevGlobals* g) {
-  __shared__ double red[6][256];
-  double s[6] = {0, 0, 0, 0, 0, 0};
+    last = lastBlock(g);
+  }
+  __syncthreads();
+  if (!last) return;
+  const int nb = gridDim.x;
+  double sum[6] = {0, 0, 0, 0, 0, 0};
   for (int b = threadIdx.x; b < nb; b += blockDim.x)
-    for (int k = 0; k < 6; k++) s[k] += partial[6 * b + k];
-  for (int k = 0; k < 6; k++) red[k][threadIdx.x] = s[k];
+    for (int k = 0; k < 6; k++)
+      sum[k] += loadAgent(reinterpret_cast<const unsigned long long*>(&partial[6 * b + k]));
+  __syncthreads();
+  for (int k = 0; k < 6; k++) red[k][threadIdx.x] = sum[k];
   __syncthreads();
   for (int st = blockDim.x / 2; st > 0; st >>= 1) {
     if (threadIdx.x < st)
@@ -1108,6 +1157,35 @@ __global__ void __launch_bounds__(256) k_sum_drag_final(const double* partial, i
   }
   if (threadIdx.x == 0)
     for (int k = 0; k < 6; k++) g->custom[k] = red[k][0];
+}
+
+// The sediment3D beforeEnd hooks on the DEM range [ns, ns + nd), fused per
+// particle in their order (customSim.cpp:97-107): addGravity (p.f += a),
+// addCustomBoundaries (the floor contact, k_dem_floor's body) and
+// freezeParticles (when time <= t_freeze).
+__global__ void k_hooks_dem_end(Store S, int ns, int nd, double ax, double ay, double az,
+                                sphdem_params P, double tfreeze, const DevGlobals* g) {
+  const int a = ns + blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= ns + nd || S.iam[a] != T_DEM) return;
+  double4 f = S.frc[a];
+  if (ax != 0.0) f.x += ax;
+  if (ay != 0.0) f.y += ay;
+  if (az != 0.0) f.z += az;
+  const double4 p = S.posh[a];
+  const double overlap = P.rmin[2] + P.dem_radius - p.z;
+  if (overlap > 0.0) {
+    const double od = -S.vhm[a].z;
+    const double s = P.dem_k * overlap + P.dem_gamma * od;
+    const double fz = s * 1.0 / S.vhm[a].w;
+    S.fb[a].z += fz;
+    f.z += fz;
+  }
+  if (g->time <= tfreeze) {
+    f = make_double4(0.0, 0.0, 0.0, f.w);
+    S.vel[a] = make_double4(0.0, 0.0, 0.0, S.vel[a].w);
+    S.vhm[a] = make_double4(0.0, 0.0, 0.0, S.vhm[a].w);
+  }
+  S.frc[a] = f;
 }
 
 // ============================================================ parity helpers
