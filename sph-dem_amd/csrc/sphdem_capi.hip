@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <thread>
 #include <vector>
 
@@ -39,6 +40,11 @@ struct sphdem_ctx {
   int *nbl = nullptr, *nbcnt = nullptr;
   bool listsValid = false;
   double2* hinv = nullptr;  // {1/h, 1/h^5} per SPH-group particle
+  // mixed form: packed neighbour records per entry (allocated on first use)
+  uint4* mixP = nullptr;
+  float4 *mixV = nullptr, *mixD = nullptr;
+  int64_t mixCap = 0;
+  float mixRes = 0.0f;  // fixed-point unit of the last packing
   // periodic ghosts (store entries [n, n + ng) after a reset)
   int ng = 0, gbase = 0;    // ghost count; first key of the ghost cells (0: none)
   // ghost field groups whose origins changed since the ghosts were copied
@@ -566,14 +572,50 @@ int checkLoaded(sphdem_ctx* ctx) {
 
 // ---- passes
 bool mixed(const sphdem_ctx* c) { return c->P.precision == SPHDEM_MIXED; }
+// Packs the mixed form's neighbour records (groups MIX_P/V/D) of every
+// SPH-group entry and ghost, positions in fixed point over the current grid.
+int packMix(sphdem_ctx* ctx, unsigned groups) {
+  const int64_t need = ctx->cap + ctx->ghostCap;
+  if (ctx->mixCap < need) {
+    for (void* q : {(void*)ctx->mixP, (void*)ctx->mixV, (void*)ctx->mixD})
+      if (q) HIPC(hipFree(q));
+    ctx->mixP = nullptr;
+    ctx->mixV = ctx->mixD = nullptr;
+    ctx->mixCap = 0;
+    HIPC(hipMalloc(&ctx->mixP, sizeof(uint4) * need));
+    HIPC(hipMalloc(&ctx->mixV, sizeof(float4) * need));
+    HIPC(hipMalloc(&ctx->mixD, sizeof(float4) * need));
+    ctx->mixCap = need;
+  }
+  const Grid& g = ctx->gs;
+  Quant Q;
+  double extent = 0.0;
+  for (int d = 0; d < 3; d++) {
+    Q.o[d] = g.cmin[d] - g.off * g.edge;  // lower corner of cell 0
+    extent = std::max(extent, g.dim[d] * g.edge);
+  }
+  // unit: the power of two that spreads the grid's extent over 2^32 steps
+  const int k = (int)std::ceil(std::log2(extent)) - 32;
+  Q.inv = std::ldexp(1.0, -k);
+  Q.res = std::ldexp(1.0f, k);
+  ctx->mixRes = Q.res;
+  const int cnt = ctx->ns + ctx->ng;
+  if (cnt > 0) {
+    hipLaunchKernelGGL(k_pack_mix, dim3(blocks(cnt)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                       ctx->ns, ctx->n, ctx->ng, Q, groups, ctx->mixP, ctx->mixV, ctx->mixD);
+    LAUNCHED();
+  }
+  return 0;
+}
 int passOmega(sphdem_ctx* ctx) {
   CHECK(ensureLists(ctx));
   CHECK(refreshGhosts(ctx, GS_POSH | GS_VHM));
   Timed tm(ctx, KID_OMEGA);
-  if (mixed(ctx))
+  if (mixed(ctx)) {
+    CHECK(packMix(ctx, MIX_P));
     hipLaunchKernelGGL(k_omega_m, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
-                       ctx->ns, ctx->nbl, ctx->nbcnt, ctx->P.wcon);
-  else
+                       ctx->ns, ctx->nbl, ctx->nbcnt, ctx->mixP, ctx->mixRes, ctx->P.wcon);
+  } else
     hipLaunchKernelGGL(k_omega_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
                        ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->P.wcon);
   LAUNCHED();
@@ -605,7 +647,8 @@ void launchForce(sphdem_ctx* ctx) {
   constexpr bool V = (F & 1) != 0, M = (F & 2) != 0, A = (F & 4) != 0, L = (F & 8) != 0;
   if (mixed(ctx))
     hipLaunchKernelGGL((k_force_m<V, M, A, L>), dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
-                       ctx->cur, ctx->ns, ctx->nbl, ctx->nbcnt, ctx->hinv, ctx->P, ctx->dg);
+                       ctx->cur, ctx->ns, ctx->nbl, ctx->nbcnt, ctx->mixP, ctx->mixV, ctx->mixD,
+                       ctx->mixRes, ctx->P, ctx->dg);
   else
     hipLaunchKernelGGL((k_force_l<V, M, A, L>), dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
                        ctx->cur, ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->hinv, ctx->P,
@@ -614,7 +657,7 @@ void launchForce(sphdem_ctx* ctx) {
 int passForce(sphdem_ctx* ctx) {
   CHECK(ensureLists(ctx));
   CHECK(refreshGhosts(ctx, GS_ALL));
-  CHECK(computeHinv(ctx));
+  if (!mixed(ctx)) CHECK(computeHinv(ctx));
   using Launch = void (*)(sphdem_ctx*);
   static const Launch table[16] = {
       launchForce<0>, launchForce<1>, launchForce<2>,  launchForce<3>,
@@ -625,6 +668,7 @@ int passForce(sphdem_ctx* ctx) {
   const int sel = ((f & SPHDEM_F_VAR_H_CORRECTION) ? 1 : 0) | ((f & SPHDEM_F_VISC_MORRIS) ? 2 : 0) |
                   ((f & SPHDEM_F_VISC_ARTIFICIAL) ? 4 : 0) | ((f & SPHDEM_F_LIQ_DEM) ? 8 : 0);
   Timed tm(ctx, KID_FORCE);
+  if (mixed(ctx)) CHECK(packMix(ctx, MIX_P | MIX_V | MIX_D));
   table[sel](ctx);
   LAUNCHED();
   return 0;
@@ -632,26 +676,27 @@ int passForce(sphdem_ctx* ctx) {
 int passDddt(sphdem_ctx* ctx) {
   CHECK(ensureLists(ctx));
   CHECK(refreshGhosts(ctx, GS_POSH | GS_VHM));
-  if (varH(ctx)) {
+  if (mixed(ctx)) {
     Timed tm(ctx, KID_DDDT);
-    if (mixed(ctx))
+    CHECK(packMix(ctx, MIX_P | MIX_V));
+    if (varH(ctx))
       hipLaunchKernelGGL(k_dddt_m<true>, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
-                         ctx->cur, ctx->ns, ctx->nbl, ctx->nbcnt, (const double2*)nullptr,
-                         ctx->P.wcon);
+                         ctx->cur, ctx->ns, ctx->nbl, ctx->nbcnt, ctx->mixP, ctx->mixV,
+                         ctx->mixRes, ctx->P.wcon);
     else
-      hipLaunchKernelGGL(k_dddt_l<true>, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
-                         ctx->cur, ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt,
-                         (const double2*)nullptr, ctx->P.wcon);
+      hipLaunchKernelGGL(k_dddt_m<false>, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
+                         ctx->cur, ctx->ns, ctx->nbl, ctx->nbcnt, ctx->mixP, ctx->mixV,
+                         ctx->mixRes, ctx->P.wcon);
+  } else if (varH(ctx)) {
+    Timed tm(ctx, KID_DDDT);
+    hipLaunchKernelGGL(k_dddt_l<true>, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                       ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, (const double2*)nullptr,
+                       ctx->P.wcon);
   } else {
     CHECK(computeHinv(ctx));
     Timed tm(ctx, KID_DDDT);
-    if (mixed(ctx))
-      hipLaunchKernelGGL(k_dddt_m<false>, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
-                         ctx->cur, ctx->ns, ctx->nbl, ctx->nbcnt, ctx->hinv, ctx->P.wcon);
-    else
-      hipLaunchKernelGGL(k_dddt_l<false>, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
-                         ctx->cur, ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->hinv,
-                         ctx->P.wcon);
+    hipLaunchKernelGGL(k_dddt_l<false>, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                       ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->hinv, ctx->P.wcon);
   }
   LAUNCHED();
   return 0;
@@ -697,10 +742,11 @@ int passDensity(sphdem_ctx* ctx) {
   markStale(ctx, GS_POSH);
   CHECK(ensureLists(ctx));  // the first iteration builds them with the new h
   CHECK(refreshGhosts(ctx, GS_POSH | GS_VHM));
-  if (mixed(ctx))
+  if (mixed(ctx)) {
+    CHECK(packMix(ctx, MIX_P));
     hipLaunchKernelGGL(k_density_m, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
-                       ctx->ns, ctx->nbl, ctx->nbcnt, ctx->P.wcon);
-  else
+                       ctx->ns, ctx->nbl, ctx->nbcnt, ctx->mixP, ctx->mixRes, ctx->P.wcon);
+  } else
     hipLaunchKernelGGL(k_density_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
                        ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->P.wcon);
   LAUNCHED();
@@ -815,7 +861,8 @@ void freeAll(sphdem_ctx* ctx) {
                   ctx->partial,     ctx->fieldbuf, ctx->errflag, ctx->dg,
                   ctx->segCnt,      ctx->segOff,      ctx->segs,    ctx->nbl,    ctx->nbcnt,
                   ctx->hinv,        ctx->gcnt,        ctx->goff,    ctx->gsrc,   ctx->gord,
-                  ctx->gcell,       ctx->gorig,       ctx->gcode,   ctx->inv};
+                  ctx->gcell,       ctx->gorig,       ctx->gcode,   ctx->inv,
+                  ctx->mixP,        ctx->mixV,        ctx->mixD};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : ctx->evPool) (void)hipEventDestroy(e);

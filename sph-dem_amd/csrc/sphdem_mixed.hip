@@ -1,12 +1,19 @@
 // sphdem_mixed.hip -- the mixed-precision form of the SPH pair passes.
 //
 // Everything that decides which pairs exist, and everything the integrator
-// carries, stays FP64: positions and their differences (computed in FP64 and
-// then rounded once to FP32), the cached neighbour lists (built by the FP64
-// search, so the pair sets are the reference's exactly), h, the sums (each
-// pair's contribution is accumulated in FP64) and the dt reduction.  The
-// per-pair kernel and force arithmetic runs in FP32 with hardware reciprocal
-// and square root, and uses the factored Wendland forms
+// carries, stays FP64: the cached neighbour lists (built by the FP64 search,
+// so the pair sets are the reference's exactly), positions, h, the per-particle
+// sums (each pair's contribution is accumulated in FP64) and the dt
+// reduction.  What the pair passes gather of a neighbour is a compact 16-byte
+// record per field group, packed from the FP64 store right before the pass:
+//   P: position as 32-bit fixed point over the cell grid (unit a power of two,
+//      grid extent / 2^32 or finer), mass as FP32
+//   V: vhat and h as FP32
+//   D: dens, pdr2, 1/h, 1/h^5 as FP32
+// so a neighbour costs one to three 16-byte loads instead of three to six
+// 32-byte ones.  Position differences are exact in the fixed-point units and
+// rounded once to FP32; the per-pair kernel and force arithmetic runs in FP32
+// with hardware reciprocal and square root and the factored Wendland forms
 //   F(q)    = -10 WCON (2-q)^3 / h^5             (the reference divides
 //   dK/dh   =  10 WCON q^2 (2-q)^3 / h             -4(2-q)^3(1+2q)+2(2-q)^4 = -10q(2-q)^3
 //                                                  by q, cancelling near q = 0)
@@ -38,26 +45,61 @@ __device__ __forceinline__ float wendPdWh_m(float wcon, float q, float rh, float
   return i3 * pdKh - c3 * wendK_m(wcon, q);
 }
 
-// FP32 difference of two FP64 positions (rounded once)
+// Fixed-point frame of the packed positions: grid lower corner and the unit.
+struct Quant {
+  double o[3];
+  double inv;  // 1 / unit (a power of two)
+  float res;   // the unit
+};
+__device__ __forceinline__ unsigned quant1(double x, double o, double inv) {
+  const double u = rint((x - o) * inv);
+  return (unsigned)fmin(fmax(u, 0.0), 4294967295.0);
+}
 struct DxF {
   float x, y, z, r2;
 };
-__device__ __forceinline__ DxF dxf(const double4& a, const double4& b) {
+// a - b from packed positions: exact in fixed point, then one rounding
+__device__ __forceinline__ DxF dxq(const uint4& a, const uint4& b, float res) {
   DxF d;
-  d.x = (float)(a.x - b.x);
-  d.y = (float)(a.y - b.y);
-  d.z = (float)(a.z - b.z);
+  d.x = (float)(int)(a.x - b.x) * res;
+  d.y = (float)(int)(a.y - b.y) * res;
+  d.z = (float)(int)(a.z - b.z) * res;
   d.r2 = d.x * d.x + d.y * d.y + d.z * d.z;
   return d;
 }
 
+// Packs the records of the SPH-group entries [0, ns) and the ghosts
+// [n, n + ng); groups: bit 0 P, bit 1 V, bit 2 D.
+constexpr unsigned MIX_P = 1, MIX_V = 2, MIX_D = 4;
+__global__ void k_pack_mix(Store S, int ns, int n, int ng, Quant Q, unsigned groups,
+                           uint4* __restrict__ P, float4* __restrict__ V, float4* __restrict__ D) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns + ng) return;
+  const int e = i < ns ? i : n + (i - ns);
+  const double4 p = S.posh[e];
+  if (groups & MIX_P)
+    P[e] = make_uint4(quant1(p.x, Q.o[0], Q.inv), quant1(p.y, Q.o[1], Q.inv),
+                      quant1(p.z, Q.o[2], Q.inv), __float_as_uint((float)S.vhm[e].w));
+  if (groups & MIX_V) {
+    const double4 v = S.vhm[e];
+    V[e] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)p.w);
+  }
+  if (groups & MIX_D) {
+    const double4 d = S.dpo[e];
+    const double rh = 1.0 / p.w;
+    D[e] = make_float4((float)d.x, (float)d.y, (float)rh, (float)(rh * rh * rh * rh * rh));
+  }
+}
+
 // ---- calcOmega, mixed --------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_omega_m(Store S, int ns, const int* __restrict__ nbl,
-                                                 const int* __restrict__ nbcnt, double wcon) {
+                                                 const int* __restrict__ nbcnt,
+                                                 const uint4* __restrict__ P, float res,
+                                                 double wcon) {
   const int a = xcdThread();
   if (a >= ns) return;
-  const double4 pa = S.posh[a];
-  const double ha = pa.w;
+  const uint4 qa = P[a];
+  const double ha = S.posh[a].w;
   const float wc = (float)wcon;
   const float rha = (float)(1.0 / ha), i3 = (float)(1.0 / cube(ha)), c3 = (float)(3.0 / pow4(ha));
   double om = S.vhm[a].w * (double)wendPdWh_m(wc, 0.0f, rha, i3, c3);
@@ -65,19 +107,15 @@ __global__ void __launch_bounds__(256) k_omega_m(Store S, int ns, const int* __r
   constexpr int U = 8;
   for (int k0 = 0; k0 < n; k0 += U) {
     SPHDEM_FETCH_ENTRIES(U)
-    double4 p[U];
-    double m[U];
+    uint4 q[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      p[u] = S.posh[e[u]];
-      m[u] = S.vhm[e[u]].w;
-    }
+    for (int u = 0; u < U; u++) q[u] = P[e[u]];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       if (k0 + u >= n) break;
-      const DxF d = dxf(pa, p[u]);
-      const float q = fsqrt(d.r2) * rha;
-      om += m[u] * (double)wendPdWh_m(wc, q, rha, i3, c3);
+      const DxF d = dxq(qa, q[u], res);
+      const float qq = fsqrt(d.r2) * rha;
+      om += (double)__uint_as_float(q[u].w) * (double)wendPdWh_m(wc, qq, rha, i3, c3);
     }
   }
   S.dpo[a].z = 1 + ha * om / (S.dpo[a].x * 3);
@@ -85,11 +123,13 @@ __global__ void __launch_bounds__(256) k_omega_m(Store S, int ns, const int* __r
 
 // ---- calcDensity, mixed ------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_density_m(Store S, int ns, const int* __restrict__ nbl,
-                                                   const int* __restrict__ nbcnt, double wcon) {
+                                                   const int* __restrict__ nbcnt,
+                                                   const uint4* __restrict__ P, float res,
+                                                   double wcon) {
   const int a = xcdThread();
   if (a >= ns) return;
-  const double4 pa = S.posh[a];
-  const double ha = pa.w;
+  const uint4 qa = P[a];
+  const double ha = S.posh[a].w;
   const float wc = (float)wcon;
   const float rha = (float)(1.0 / ha);
   const double i3 = 1.0 / cube(ha);
@@ -98,19 +138,15 @@ __global__ void __launch_bounds__(256) k_density_m(Store S, int ns, const int* _
   constexpr int U = 8;
   for (int k0 = 0; k0 < n; k0 += U) {
     SPHDEM_FETCH_ENTRIES(U)
-    double4 p[U];
-    double m[U];
+    uint4 q[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      p[u] = S.posh[e[u]];
-      m[u] = S.vhm[e[u]].w;
-    }
+    for (int u = 0; u < U; u++) q[u] = P[e[u]];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       if (k0 + u >= n) break;
-      const DxF d = dxf(pa, p[u]);
-      const float q = fsqrt(d.r2) * rha;
-      rho += m[u] * (i3 * (double)wendK_m(wc, q));
+      const DxF d = dxq(qa, q[u], res);
+      const float qq = fsqrt(d.r2) * rha;
+      rho += (double)__uint_as_float(q[u].w) * (i3 * (double)wendK_m(wc, qq));
     }
   }
   S.dpo[a].x = rho;
@@ -120,12 +156,14 @@ __global__ void __launch_bounds__(256) k_density_m(Store S, int ns, const int* _
 template <bool VARH>
 __global__ void __launch_bounds__(256) k_dddt_m(Store S, int ns, const int* __restrict__ nbl,
                                                 const int* __restrict__ nbcnt,
-                                                const double2* __restrict__ hinv, double wcon) {
+                                                const uint4* __restrict__ P,
+                                                const float4* __restrict__ V, float res,
+                                                double wcon) {
   const int a = xcdThread();
   if (a >= ns) return;
-  const double4 pa = S.posh[a];
-  const double4 va = S.vhm[a];
-  const double ha = pa.w;
+  const uint4 qa = P[a];
+  const float4 va = V[a];
+  const double ha = S.posh[a].w;
   const float wc10 = (float)(10.0 * wcon);
   const float rha = (float)(1.0 / ha), i5a = (float)(1.0 / pow5(ha));
   const float rom = (float)(1.0 / S.dpo[a].z);
@@ -134,30 +172,30 @@ __global__ void __launch_bounds__(256) k_dddt_m(Store S, int ns, const int* __re
   constexpr int U = 8;
   for (int k0 = 0; k0 < n; k0 += U) {
     SPHDEM_FETCH_ENTRIES(U)
-    double4 p[U], v[U];
-    double2 hi[U];
+    uint4 q[U];
+    float4 v[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      p[u] = S.posh[e[u]];
-      v[u] = S.vhm[e[u]];
-      if (!VARH) hi[u] = hinv[e[u]];
+      q[u] = P[e[u]];
+      v[u] = V[e[u]];
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       if (k0 + u >= n) break;
-      const DxF d = dxf(pa, p[u]);
-      const float dv0 = (float)(va.x - v[u].x), dv1 = (float)(va.y - v[u].y),
-                  dv2 = (float)(va.z - v[u].z);
+      const DxF d = dxq(qa, q[u], res);
+      const float dv0 = va.x - v[u].x, dv1 = va.y - v[u].y, dv2 = va.z - v[u].z;
       const float r = fsqrt(d.r2);
       float gF;  // gradW = dx * gF
       if (VARH) {
         gF = wendF_m(wc10, r * rha, i5a) * rom;
       } else {
+        const float rhb = frcp(v[u].w);
+        const float i5b = (rhb * rhb) * (rhb * rhb) * rhb;
         const float Fa = wendF_m(wc10, r * rha, i5a);
-        const float Fb = wendF_m(wc10, r * (float)hi[u].x, (float)hi[u].y);
+        const float Fb = wendF_m(wc10, r * rhb, i5b);
         gF = 0.5f * (Fa + Fb);
       }
-      dddt += v[u].w * (double)(gF * (dv0 * d.x + dv1 * d.y + dv2 * d.z));
+      dddt += (double)__uint_as_float(q[u].w) * (double)(gF * (dv0 * d.x + dv1 * d.y + dv2 * d.z));
     }
   }
   S.vh0d[a].w = dddt;
@@ -167,17 +205,19 @@ __global__ void __launch_bounds__(256) k_dddt_m(Store S, int ns, const int* __re
 template <bool VARH, bool MORRIS, bool ARTIF, bool LIQDEM>
 __global__ void __launch_bounds__(256) k_force_m(Store S, int ns, const int* __restrict__ nbl,
                                                  const int* __restrict__ nbcnt,
-                                                 const double2* __restrict__ hinv,
+                                                 const uint4* __restrict__ Pq,
+                                                 const float4* __restrict__ V,
+                                                 const float4* __restrict__ D, float res,
                                                  sphdem_params P, DevGlobals* g) {
   const int a = xcdThread();
   const double inf = __longlong_as_double(0x7ff0000000000000ll);
   double newDt = inf;
   if (a < ns) {
     const bool fluid = S.iam[a] == T_SPH;
-    const double4 pa = S.posh[a];
-    const double4 va = S.vhm[a];
+    const uint4 qa = Pq[a];
+    const float4 va = V[a];
     const double4 da = S.dpo[a];
-    const double ha = pa.w;
+    const double ha = S.posh[a].w;
     const float wc10 = (float)(10.0 * P.wcon);
     const float rha = (float)(1.0 / ha), i5a = (float)(1.0 / pow5(ha));
     const float rom = VARH ? (float)(1.0 / da.z) : 1.0f;
@@ -192,23 +232,22 @@ __global__ void __launch_bounds__(256) k_force_m(Store S, int ns, const int* __r
     constexpr int U = 4;
     for (int k0 = 0; k0 < n; k0 += U) {
       SPHDEM_FETCH_ENTRIES(U)
-      double4 p[U], v[U];
-      double2 d[U], hi[U];
+      uint4 q[U];
+      float4 v[U], d[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        p[u] = S.posh[e[u]];
-        v[u] = S.vhm[e[u]];
-        d[u] = *reinterpret_cast<const double2*>(&S.dpo[e[u]]);
-        hi[u] = hinv[e[u]];
+        q[u] = Pq[e[u]];
+        v[u] = V[e[u]];
+        d[u] = D[e[u]];
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {
         if (k0 + u >= n) break;
-        const DxF dx = dxf(pa, p[u]);
-        const float mb = (float)v[u].w, rhob = (float)d[u].x, pdr2b = (float)d[u].y;
+        const DxF dx = dxq(qa, q[u], res);
+        const float mb = __uint_as_float(q[u].w), rhob = d[u].x, pdr2b = d[u].y;
         const float r = fsqrt(dx.r2);
         const float Fa = wendF_m(wc10, r * rha, i5a);
-        const float Fb = wendF_m(wc10, r * (float)hi[u].x, (float)hi[u].y);
+        const float Fb = wendF_m(wc10, r * d[u].z, d[u].w);
         // gradients are dx * G; under VAR_H both divide by pa.omega (sph.h:331-332)
         float Ga, Gb;
         if (VARH) {
@@ -218,10 +257,9 @@ __global__ void __launch_bounds__(256) k_force_m(Store S, int ns, const int* __r
           Ga = Gb = 0.5f * (Fa + Fb);
         }
         if (fluid) {
-          const float dv0 = (float)(va.x - v[u].x), dv1 = (float)(va.y - v[u].y),
-                      dv2 = (float)(va.z - v[u].z);
+          const float dv0 = va.x - v[u].x, dv1 = va.y - v[u].y, dv2 = va.z - v[u].z;
           const float Gbar = 0.5f * (Ga + Gb);
-          const double hav = 0.5 * (ha + p[u].w);
+          const double hav = 0.5 * (ha + (double)v[u].w);
           const float havf = (float)hav;
           float fvs = 0.0f;  // fv = dv * fvs
           float vsig = (float)vsigC;
