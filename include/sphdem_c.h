@@ -202,7 +202,8 @@ int sphdem_run_steps(sphdem_ctx* ctx, int nsteps, int hookset);
  * (as sphdem_run_steps on each) with `workers` host threads (0: min(nbox, 8)),
  * each thread stepping its share of the boxes round robin, every box on its
  * own stream, so the small boxes' kernels and host round trips overlap on the
- * device.  The reference runs such ensembles as separate processes, one case
+ * device (during the call each box runs all its passes on its main stream:
+ * the boxes already overlap one another).  The reference runs such ensembles as separate processes, one case
  * directory per run (the run.cpp loop once per box); this replaces that loop
  * over boxes.  Every box ends exactly as sphdem_run_steps alone would leave it.
  * The contexts must be distinct and must not be used by the caller during the

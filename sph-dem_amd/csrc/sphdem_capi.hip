@@ -25,6 +25,10 @@ struct sphdem_ctx {
   sphdem_params P;
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream for passes that run beside the main chain, and its events
+  hipStream_t side = nullptr;
+  hipEvent_t evFork = nullptr, evJoin = nullptr;
+  bool overlap = true;  // run the independent passes on the side stream
   int64_t cap = 0;
   int n = 0, ns0 = 0, nd0 = 0, ns = 0, nd = 0;
   Store cur{}, alt{};
@@ -722,8 +726,7 @@ int passFpi(sphdem_ctx* ctx) {
   hipLaunchKernelGGL(k_fpi, dim3(waveBlocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
                      ctx->nd, ctx->gs, ctx->start, ctx->box, ctx->P);
   LAUNCHED();
-  markStale(ctx, GS_ALL);
-  return 0;
+  return 0;  // writes DEM fields only (no ghost copies them)
 }
 int passContacts(sphdem_ctx* ctx) {
   if (ctx->nd == 0) return 0;
@@ -731,8 +734,7 @@ int passContacts(sphdem_ctx* ctx) {
   hipLaunchKernelGGL(k_contacts, dim3(blocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur,
                      ctx->ns, ctx->nd, ctx->gc, ctx->start, ctx->box, ctx->P);
   LAUNCHED();
-  markStale(ctx, GS_ALL);
-  return 0;
+  return 0;  // writes DEM fields only
 }
 // one iteration of the density re-initialisation (sphIncompress.cpp:29-34)
 int passDensity(sphdem_ctx* ctx) {
@@ -753,6 +755,50 @@ int passDensity(sphdem_ctx* ctx) {
   markStale(ctx, GS_DPO);
   return 0;
 }
+
+// ---- work beside the main chain
+// Side(ctx) makes the side stream wait for everything issued so far on the
+// main stream and routes the context's launches to the side stream until
+// back() (or the destructor); the main stream then waits, at join() (or the
+// destructor), for everything the side stream was given.  Every entry point
+// therefore returns with the side work ordered before the main stream's
+// next operation.
+struct Side {
+  sphdem_ctx* c;
+  hipStream_t mainStream;
+  bool onSide = false, joined = false;
+  explicit Side(sphdem_ctx* c_) : c(c_), mainStream(c_->stream) {
+    // the side stream is made on first use (a context that never overlaps,
+    // as in an ensemble, keeps one stream: HIP deals streams to its hardware
+    // queues in creation order)
+    if (c->overlap && !c->side &&
+        (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+         hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming) != hipSuccess ||
+         hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming) != hipSuccess))
+      c->overlap = false;
+    if (c->overlap && hipEventRecord(c->evFork, mainStream) == hipSuccess &&
+        hipStreamWaitEvent(c->side, c->evFork, 0) == hipSuccess) {
+      c->stream = c->side;
+      onSide = true;
+    } else {
+      joined = true;  // no overlap: everything stays on the main stream
+    }
+  }
+  void back() {
+    if (onSide) c->stream = mainStream;
+    onSide = false;
+  }
+  int join() {
+    back();
+    if (joined) return 0;
+    joined = true;
+    sphdem_ctx* ctx = c;
+    HIPC(hipEventRecord(ctx->evJoin, ctx->side));
+    HIPC(hipStreamWaitEvent(mainStream, ctx->evJoin, 0));
+    return 0;
+  }
+  ~Side() { (void)join(); }
+};
 
 // ---- half-steps
 // bodyMask/accel: the beforeMiddle body-force hook, applied by k_start right
@@ -776,13 +822,24 @@ int stepStart(sphdem_ctx* ctx, uint32_t bodyMask = 0, const double* accel = null
 }
 int stepMiddle(sphdem_ctx* ctx) {
   CHECK(doReset(ctx, false, true));
-  if (liqDem(ctx)) CHECK(passCoupling(ctx));
+  if (liqDem(ctx)) {
+    // porosity, Shepard sums and DEM->SPH drag on the side stream, beside the
+    // list build: the build reads positions, h, cells and segments, which
+    // the coupling pass does not write
+    Side side(ctx);
+    CHECK(passCoupling(ctx));
+    side.back();
+    CHECK(ensureLists(ctx));
+    CHECK(side.join());
+  }
   if (varH(ctx)) CHECK(passOmega(ctx));
   CHECK(passEos(ctx));
   CHECK(passForce(ctx));
   const int na = ctx->ns + ctx->nd;
   hipLaunchKernelGGL(k_accel, dim3(blocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream, ctx->cur,
-                     na, ctx->P, ctx->dg);
+                     na, ctx->dg);
+  LAUNCHED();
+  hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1), 0, ctx->stream, ctx->P, ctx->dg);
   LAUNCHED();
   Timed tm(ctx, KID_KICK);
   hipLaunchKernelGGL(k_mid_kick, dim3(blocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream,
@@ -798,16 +855,24 @@ int stepEnd(sphdem_ctx* ctx) {
   markStale(ctx, GS_ALL);
   CHECK(doReset(ctx));
   if (varH(ctx)) CHECK(passOmega(ctx));
-  CHECK(passDddt(ctx));
   if (liqDem(ctx)) {
+    // calcFPI, DEM contacts and the DEM kick on the side stream, beside dddt:
+    // they read SPH positions, h, dens, vhat, porosity and forces and write DEM
+    // fields only; dddt writes the SPH dddt only.  driftRest (k_end_rest)
+    // changes SPH dens and h, so it waits for both.
+    Side side(ctx);
     CHECK(passFpi(ctx));
     if (!(ctx->P.flags & (SPHDEM_F_FIXED_DEM | SPHDEM_F_NO_DEM_CONTACTS))) CHECK(passContacts(ctx));
     if (ctx->nd > 0) {
       hipLaunchKernelGGL(k_dem_kick, dim3(blocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur,
                          ctx->ns, ctx->nd, ctx->P, ctx->dg);
       LAUNCHED();
-      markStale(ctx, GS_ALL);
     }
+    side.back();
+    CHECK(passDddt(ctx));
+    CHECK(side.join());
+  } else {
+    CHECK(passDddt(ctx));
   }
   Timed tm(ctx, KID_REST);
   hipLaunchKernelGGL(k_end_rest, dim3(blocks(std::max(ctx->ns, 1))), dim3(TPB), 0, ctx->stream,
@@ -850,7 +915,10 @@ int hookSumDrag(sphdem_ctx* ctx) {
     HIPC(hipMalloc(&ctx->partial, sizeof(double) * ctx->partialCap));
   }
   hipLaunchKernelGGL(k_sum_drag, dim3(nb), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns + ctx->nd,
-                     ctx->partial, ctx->dg);
+                     ctx->partial);
+  LAUNCHED();
+  hipLaunchKernelGGL(k_sum_drag_final, dim3(1), dim3(256), 0, ctx->stream, ctx->partial, nb,
+                     ctx->dg);
   LAUNCHED();
   return 0;
 }
@@ -867,6 +935,9 @@ void freeAll(sphdem_ctx* ctx) {
     if (p) (void)hipFree(p);
   for (hipEvent_t e : ctx->evPool) (void)hipEventDestroy(e);
   ctx->evPool.clear();
+  if (ctx->evFork) (void)hipEventDestroy(ctx->evFork);
+  if (ctx->evJoin) (void)hipEventDestroy(ctx->evJoin);
+  if (ctx->side) (void)hipStreamDestroy(ctx->side);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
 }
 
@@ -924,6 +995,7 @@ int sphdem_destroy(sphdem_ctx* ctx) {
   if (!ctx) return SPHDEM_E_ARG;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->side) (void)hipStreamSynchronize(ctx->side);
   freeAll(ctx);
   delete ctx;
   return 0;
@@ -1139,11 +1211,10 @@ int sphdem_checkpoint_restore(sphdem_ctx* ctx, const void* host, int64_t bytes) 
     return fail(ctx, SPHDEM_E_ARG, "the checkpoint was taken with other parameters");
   p += sizeof(sphdem_params);
   HIPC(hipSetDevice(ctx->device));
-  // the reduction targets and counters start clear, whatever the saved values
+  // the hmax reduction target starts clear, whatever the saved value
   DevGlobals dgh;
   memcpy(&dgh, p, sizeof(DevGlobals));
   dgh.hmax_bits = 0;
-  dgh.ticket = 0;
   HIPC(hipMemcpyAsync(ctx->dg, &dgh, sizeof(DevGlobals), hipMemcpyHostToDevice, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));  // dgh is a local
   p += sizeof(DevGlobals);
@@ -1354,6 +1425,13 @@ int sphdem_run_steps_many(sphdem_ctx* const* ctxs, int nbox, int nsteps, int hoo
     return SPHDEM_E_ARG;  // a null or a repeated context
   if (workers == 0) workers = 8;
   workers = std::min(workers, nbox);
+  // the boxes overlap one another, so each box keeps its passes on one stream
+  // (the extra fork/join of a second stream per box only costs here)
+  std::vector<bool> overlap(nbox);
+  for (int b = 0; b < nbox; b++) {
+    overlap[b] = ctxs[b]->overlap;
+    ctxs[b]->overlap = false;
+  }
   std::atomic<int> err{0};
   std::atomic<int> errBox{-1};
   auto work = [&](int w) {
@@ -1373,6 +1451,7 @@ int sphdem_run_steps_many(sphdem_ctx* const* ctxs, int nbox, int nsteps, int hoo
   for (int w = 1; w < workers; w++) pool.emplace_back(work, w);
   work(0);
   for (std::thread& t : pool) t.join();
+  for (int b = 0; b < nbox; b++) ctxs[b]->overlap = overlap[b];
   if (failed_box) *failed_box = errBox.load();
   return err.load();
 }

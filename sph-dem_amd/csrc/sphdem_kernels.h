@@ -71,9 +71,7 @@ struct DevGlobals {
   unsigned long long hmax_bits;   // atomicMax target
   unsigned long long ndel_sph, ndel_dem;
   unsigned int errflags;  // bit 0: a particle fell outside its cell grid
-  unsigned int ticket;    // blocks done, for a kernel whose last block finishes a reduction
   int ghostTotal;         // ghost entries counted by the last reset's scan
-  int pad_;
 };
 
 // Cells (per axis) a ghost may occupy: one beyond the cells real particles of

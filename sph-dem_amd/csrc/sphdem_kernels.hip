@@ -527,27 +527,10 @@ __global__ void k_eos(Store S, int ns, sphdem_params P) {
 }
 
 // Nsph::accelCondition sph.h:566-568 over sph, sphBoundary and dem
-// Last-block completion of a grid-wide reduction: every block's thread 0 calls
-// this after publishing its part; it returns true in exactly one block, the
-// last to arrive, once every other block's part is visible (the counter is
-// left at zero for the next kernel that uses it).
-__device__ __forceinline__ bool lastBlock(DevGlobals* g) {
-  __threadfence();
-  const unsigned done = atomicAdd(&g->ticket, 1u);
-  if (done != gridDim.x - 1) return false;
-  atomicExch(&g->ticket, 0u);
-  __threadfence();
-  return true;
-}
-__device__ __forceinline__ double loadAgent(const unsigned long long* p) {
-  return __longlong_as_double(
-      (long long)__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-}
-
 // dt rule (sphIncompress.cpp:126-154): min with LIQ_DEM_TIMESTEP and, after
 // TIME_DROP_PARTICLE, DEM_TIMESTEP; both branches set dt = newDt.
-__device__ void dtRule(const sphdem_params& P, DevGlobals* g) {
-  double nd = loadAgent(&g->newDt_bits);
+__global__ void k_dt_final(sphdem_params P, DevGlobals* g) {
+  double nd = __longlong_as_double((long long)g->newDt_bits);
   if (P.flags & SPHDEM_F_LIQ_DEM) {
     nd = dmin2(nd, P.liq_dem_timestep);
     if (!(P.flags & (SPHDEM_F_FIXED_DEM | SPHDEM_F_NO_DEM_CONTACTS)))
@@ -558,8 +541,8 @@ __device__ void dtRule(const sphdem_params& P, DevGlobals* g) {
   g->dt = nd;
 }
 
-// accelCondition over sph, sphBoundary and dem, then (last block) the dt rule
-__global__ void k_accel(Store S, int n, sphdem_params P, DevGlobals* g) {
+// accelCondition over sph, sphBoundary and dem
+__global__ void k_accel(Store S, int n, DevGlobals* g) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   double v = __longlong_as_double(0x7ff0000000000000ll);
   if (a < n) {
@@ -570,7 +553,6 @@ __global__ void k_accel(Store S, int n, sphdem_params P, DevGlobals* g) {
     }
   }
   atomicMinPos(&g->newDt_bits, v);
-  if (threadIdx.x == 0 && lastBlock(g)) dtRule(P, g);
 }
 
 // Nsph::calcHFromDens sph.h:1823-1825
@@ -1107,12 +1089,9 @@ __global__ void k_freeze(Store S, int n, unsigned mask, double tfreeze, const De
   S.vhm[a] = make_double4(0.0, 0.0, 0.0, S.vhm[a].w);
 }
 // sumDragForces sediment3D/customSim.cpp:27-42: per-block partial sums of
-// m*fdrag for sph (0..2) and dem (3..5); the last block to finish adds them in
-// a fixed order (strided per thread, then a tree) into custom[0..5].
-__global__ void __launch_bounds__(256) k_sum_drag(Store S, int n, double* partial,
-                                                  DevGlobals* g) {
+// m*fdrag for sph (0..2) and dem (3..5); k_sum_drag_final adds them in a fixed order.
+__global__ void k_sum_drag(Store S, int n, double* partial) {
   __shared__ double red[6][256];
-  __shared__ bool last;
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   double v[6] = {0, 0, 0, 0, 0, 0};
   if (a < n) {
@@ -1136,19 +1115,17 @@ __global__ void __launch_bounds__(256) k_sum_drag(Store S, int n, double* partia
       for (int k = 0; k < 6; k++) red[k][threadIdx.x] += red[k][threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
     for (int k = 0; k < 6; k++) partial[6 * blockIdx.x + k] = red[k][0];
-    last = lastBlock(g);
-  }
-  __syncthreads();
-  if (!last) return;
-  const int nb = gridDim.x;
-  double sum[6] = {0, 0, 0, 0, 0, 0};
+}
+// one 256-thread block: strided partial sums in a fixed order, then a tree
+__global__ void __launch_bounds__(256) k_sum_drag_final(const double* partial, int nb,
+                                                        DevGlobals* g) {
+  __shared__ double red[6][256];
+  double s[6] = {0, 0, 0, 0, 0, 0};
   for (int b = threadIdx.x; b < nb; b += blockDim.x)
-    for (int k = 0; k < 6; k++)
-      sum[k] += loadAgent(reinterpret_cast<const unsigned long long*>(&partial[6 * b + k]));
-  __syncthreads();
-  for (int k = 0; k < 6; k++) red[k][threadIdx.x] = sum[k];
+    for (int k = 0; k < 6; k++) s[k] += partial[6 * b + k];
+  for (int k = 0; k < 6; k++) red[k][threadIdx.x] = s[k];
   __syncthreads();
   for (int st = blockDim.x / 2; st > 0; st >>= 1) {
     if (threadIdx.x < st)
