@@ -490,8 +490,12 @@ __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
     const double4 va = S.vhm[a];
     const double4 da = S.dpo[a];
     const double ha = pa.w;
-    const double rha = 1.0 / ha, i5a = 1.0 / pow5(ha);
-    const double rom = 1.0 / da.z;
+    const double rha = 1.0 / ha;
+    const double rom = VARH ? 1.0 / da.z : 1.0;
+    // gradW = dr * F(q, h) (/ omega_a under VAR_H), F = WCON poly(q) / (q h^5)
+    // = dr * poly(q) * K(h) / r with K(h) = WCON h^-4 (/ omega_a)
+    const double Ka = P.wcon * (rha * rha) * (rha * rha) * rom;
+    const double wrom = P.wcon * rom;
     const double viscLiq = 2.0 * P.viscosity * P.dens;  // numerator under LIQ_DEM
     const double vsigC = MORRIS ? 2.0 * P.spsound : 0.0;
     double minA = inf, minB = inf;  // min over pairs of 0.6*hav and 0.125*hav^2
@@ -513,57 +517,47 @@ __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
 #pragma unroll
       for (int u = 0; u < U; u++) {
         if (k0 + u >= n) break;
-          const double hb = p[u].w, mb = v[u].w, rhob = d[u].x, pdr2b = d[u].y;
+        const double hb = p[u].w, mb = v[u].w, rhob = d[u].x, pdr2b = d[u].y;
         const double dx = pa.x - p[u].x, dy = pa.y - p[u].y, dz = pa.z - p[u].z;
-        const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
+        const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+        const double r = sqrt(l2);
+        const double rinv = r > 0.0 ? 1.0 / r : 0.0;  // r = 0: dr = 0, gradW = 0
         const double qa = divr(r, ha, rha), qb = divr(r, hb, hi[u].x);
-        double gWa[3], gWb[3];
-        if (VARH) {
-          gradWi(P.wcon, true, da.z, rom, dx, dy, dz, qa, i5a, gWa);
-          gradWi(P.wcon, true, da.z, rom, dx, dy, dz, qb, hi[u].y, gWb);
-        } else {
-          double g1[3], g2[3];
-          gradWi(P.wcon, false, da.z, rom, dx, dy, dz, qa, i5a, g1);
-          gradWi(P.wcon, false, da.z, rom, dx, dy, dz, qb, hi[u].y, g2);
-          for (int c = 0; c < 3; c++) gWa[c] = gWb[c] = 0.5 * (g1[c] + g2[c]);
-        }
+        const double Kb = wrom * (hi[u].y * hb);  // WCON hb^-4 (/ omega_a)
+        double Ga = qa <= 2.0 ? wendPoly(qa) * Ka * rinv : 0.0;
+        double Gb = qb <= 2.0 ? wendPoly(qb) * Kb * rinv : 0.0;
+        if (!VARH) Ga = Gb = 0.5 * (Ga + Gb);  // the average gradient
         if (fluid) {
           const double dv0 = va.x - v[u].x, dv1 = va.y - v[u].y, dv2 = va.z - v[u].z;
-          const double gb0 = 0.5 * (gWa[0] + gWb[0]), gb1 = 0.5 * (gWa[1] + gWb[1]),
-                       gb2 = 0.5 * (gWa[2] + gWb[2]);
+          const double Gbar = 0.5 * (Ga + Gb);  // gradW-bar = dr * Gbar
           const double hav = 0.5 * (ha + hb);
-          double fv0 = 0.0, fv1 = 0.0, fv2 = 0.0;
+          double fvs = 0.0;  // fv = dv * fvs
           double vsig = vsigC;
           if (MORRIS) {
-            const double visc = LIQDEM ? viscLiq / (da.x * rhob)
-                                       : P.viscosity * (da.x + rhob) / (da.x * rhob);
-            const double dd = dot3(dx, dy, dz, gb0, gb1, gb2);
-            const double den = r * r + 0.01 * (hav * hav);
-            const double rden = 1.0 / den;
-            fv0 = divr(dv0 * mb * visc * dd, den, rden);
-            fv1 = divr(dv1 * mb * visc * dd, den, rden);
-            fv2 = divr(dv2 * mb * visc * dd, den, rden);
+            const double rab = 1.0 / (da.x * rhob);
+            const double visc = LIQDEM ? viscLiq * rab : P.viscosity * (da.x + rhob) * rab;
+            const double den = l2 + 0.01 * (hav * hav);
+            fvs = mb * visc * (Gbar * l2) / den;  // dr . gradW-bar = Gbar |dr|^2
           }
           if (ARTIF) {
             const double vdr = dot3(dx, dy, dz, dv0, dv1, dv2);
             double avsig = 0, avisc = 0.0;
             if (vdr < 0) {
-              double rr = 0.0;
-              if (r != 0.0) rr = 1 / r;
-              const double viss = vdr * rr;
+              const double viss = vdr * rinv;
               avsig = 2.0 * P.spsound - 4.0 * viss;
               avisc = viss * avsig * P.alpha_artificial / (da.x + rhob);
             }
-            const double s = mb * avisc;
-            f.x += s * gb0;
-            f.y += s * gb1;
-            f.z += s * gb2;
+            const double sg = mb * avisc * Gbar;
+            f.x += sg * dx;
+            f.y += sg * dy;
+            f.z += sg * dz;
             vsig = dmax2(vsig, avsig);
             newDt = dmin2(newDt, 0.6 * hav / vsig);
           } else {
             minA = dmin2(minA, 0.6 * hav);
           }
           minB = dmin2(minB, 0.125 * (hav * hav));
+          const double fv0 = dv0 * fvs, fv1 = dv1 * fvs, fv2 = dv2 * fvs;
           fva.x += fv0;
           fva.y += fv1;
           fva.z += fv2;
@@ -571,14 +565,10 @@ __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
           f.y += fv1;
           f.z += fv2;
         }
-        // calcPressForce; antic = 0 under NO_ANTICLUMPING
-        const double antic = 0.01 * fabs(da.y + pdr2b) * 0.0;
-        double p0 = da.y * gWa[0] + pdr2b * gWb[0];
-        double p1 = da.y * gWa[1] + pdr2b * gWb[1];
-        double p2 = da.y * gWa[2] + pdr2b * gWb[2];
-        p0 = -mb * (p0 + antic);
-        p1 = -mb * (p1 + antic);
-        p2 = -mb * (p2 + antic);
+        // calcPressForce (sph.h:698-710), antic = 0 under NO_ANTICLUMPING:
+        // -mb (pdr2a gradWa + pdr2b gradWb) = dr * pp
+        const double pp = -mb * (da.y * Ga + pdr2b * Gb);
+        const double p0 = pp * dx, p1 = pp * dy, p2 = pp * dz;
         fpa.x += p0;
         fpa.y += p1;
         fpa.z += p2;
