@@ -439,6 +439,53 @@ void setupBand(sphdem_ctx* ctx) {
 
 // keepSsr: sort only, keeping the search radius (a checkpoint restore, which
 // must not move the hysteresis of calcDomainLimits)
+// ---- work beside the main chain
+// Side(ctx) makes the side stream wait for everything issued so far on the
+// main stream and routes the context's launches to the side stream until
+// back() (or the destructor); the main stream then waits, at join() (or the
+// destructor), for everything the side stream was given.  Every entry point
+// therefore returns with the side work ordered before the main stream's
+// next operation.
+struct Side {
+  sphdem_ctx* c;
+  hipStream_t mainStream;
+  bool onSide = false, joined = false;
+  // create: make the side stream if there is none yet; otherwise only an
+  // existing one is used.  The side stream is made by the step passes on
+  // first use, so a context that never overlaps (the boxes of an ensemble)
+  // keeps one stream: HIP deals streams to its hardware queues in creation
+  // order, and a second stream per box would halve the queues the boxes'
+  // main streams get.
+  explicit Side(sphdem_ctx* c_, bool create = true) : c(c_), mainStream(c_->stream) {
+    if (c->overlap && !c->side && create &&
+        (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+         hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming) != hipSuccess ||
+         hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming) != hipSuccess))
+      c->overlap = false;
+    if (c->overlap && c->side && hipEventRecord(c->evFork, mainStream) == hipSuccess &&
+        hipStreamWaitEvent(c->side, c->evFork, 0) == hipSuccess) {
+      c->stream = c->side;
+      onSide = true;
+    } else {
+      joined = true;  // no overlap: everything stays on the main stream
+    }
+  }
+  void back() {
+    if (onSide) c->stream = mainStream;
+    onSide = false;
+  }
+  int join() {
+    back();
+    if (joined) return 0;
+    joined = true;
+    sphdem_ctx* ctx = c;
+    HIPC(hipEventRecord(ctx->evJoin, ctx->side));
+    HIPC(hipStreamWaitEvent(mainStream, ctx->evJoin, 0));
+    return 0;
+  }
+  ~Side() { (void)join(); }
+};
+
 // midInit: middle()'s reset, which also starts the dt minimum at maxdt
 int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false) {
   const sphdem_params& P = ctx->P;
@@ -528,6 +575,11 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false) {
   }
   const int ntot = n + ng;
   CHECK(scanCounts(ctx, ctx->K + 1));
+  // the segments need only the cell starts: packed on the side stream (if
+  // the steps have made one) while the main stream scatters, ranks and permutes
+  Side side(ctx, false);
+  CHECK(buildSegments(ctx));
+  side.back();
   hipLaunchKernelGGL(k_scatter, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ntot, ctx->key,
                      ctx->slot, ctx->start, ctx->tmp, ctx->counts, ctx->K + 1);
   LAUNCHED();
@@ -555,7 +607,7 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false) {
   std::swap(ctx->cur, ctx->alt);
   std::swap(ctx->storeMem[0], ctx->storeMem[1]);
   ctx->listsValid = false;  // updateDomain clears every neighbour list (dataLL.cpp:374)
-  return buildSegments(ctx);
+  return side.join();
 }
 
 // The cached lists are built by the first pair pass after a reset, with the h
@@ -755,50 +807,6 @@ int passDensity(sphdem_ctx* ctx) {
   markStale(ctx, GS_DPO);
   return 0;
 }
-
-// ---- work beside the main chain
-// Side(ctx) makes the side stream wait for everything issued so far on the
-// main stream and routes the context's launches to the side stream until
-// back() (or the destructor); the main stream then waits, at join() (or the
-// destructor), for everything the side stream was given.  Every entry point
-// therefore returns with the side work ordered before the main stream's
-// next operation.
-struct Side {
-  sphdem_ctx* c;
-  hipStream_t mainStream;
-  bool onSide = false, joined = false;
-  explicit Side(sphdem_ctx* c_) : c(c_), mainStream(c_->stream) {
-    // the side stream is made on first use (a context that never overlaps,
-    // as in an ensemble, keeps one stream: HIP deals streams to its hardware
-    // queues in creation order)
-    if (c->overlap && !c->side &&
-        (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
-         hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming) != hipSuccess ||
-         hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming) != hipSuccess))
-      c->overlap = false;
-    if (c->overlap && hipEventRecord(c->evFork, mainStream) == hipSuccess &&
-        hipStreamWaitEvent(c->side, c->evFork, 0) == hipSuccess) {
-      c->stream = c->side;
-      onSide = true;
-    } else {
-      joined = true;  // no overlap: everything stays on the main stream
-    }
-  }
-  void back() {
-    if (onSide) c->stream = mainStream;
-    onSide = false;
-  }
-  int join() {
-    back();
-    if (joined) return 0;
-    joined = true;
-    sphdem_ctx* ctx = c;
-    HIPC(hipEventRecord(ctx->evJoin, ctx->side));
-    HIPC(hipStreamWaitEvent(mainStream, ctx->evJoin, 0));
-    return 0;
-  }
-  ~Side() { (void)join(); }
-};
 
 // ---- half-steps
 // bodyMask/accel: the beforeMiddle body-force hook, applied by k_start right
