@@ -147,6 +147,9 @@ int fail(sphdem_ctx* c, int code, const char* fmt, ...) {
 
 constexpr int TPB = 256;
 inline int blocks(int n) { return (n + TPB - 1) / TPB; }
+// grid of a grid-stride kernel that ends in one atomic per block on a single
+// address: those atomics serialise, so at most two blocks per CU
+inline int reduceBlocks(int n) { return std::min(blocks(n), 512); }
 // kernels that give each item a whole 64-lane wavefront
 inline int waveBlocks(int n) { return (n + TPB / 64 - 1) / (TPB / 64); }
 
@@ -510,7 +513,7 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false) {
   int ngPre = 0;
   {
     Timed tm(ctx, KID_WRAP);
-    hipLaunchKernelGGL(k_wrap, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, ctx->cur, n, dmin[0],
+    hipLaunchKernelGGL(k_wrap, dim3(reduceBlocks(n)), dim3(TPB), 0, ctx->stream, ctx->cur, n, dmin[0],
                        dmin[1], dmin[2], dsp[0], dsp[1], dsp[2], ctx->box, ctx->dg, ctx->gs,
                        ctx->band, pre ? ctx->gcnt : (int*)nullptr, midInit ? 1 : 0);
     LAUNCHED();
@@ -844,8 +847,8 @@ int stepMiddle(sphdem_ctx* ctx) {
   CHECK(passEos(ctx));
   CHECK(passForce(ctx));
   const int na = ctx->ns + ctx->nd;
-  hipLaunchKernelGGL(k_accel, dim3(blocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream, ctx->cur,
-                     na, ctx->dg);
+  hipLaunchKernelGGL(k_accel, dim3(reduceBlocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream,
+                     ctx->cur, na, ctx->dg);
   LAUNCHED();
   hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1), 0, ctx->stream, ctx->P, ctx->dg);
   LAUNCHED();

@@ -172,12 +172,13 @@ __device__ __forceinline__ int ghostCount(const double4& p, const Grid& g, const
 __global__ void k_wrap(Store S, int n, double dmin0, double dmin1, double dmin2_,
                        double dsp0, double dsp1, double dsp2, Box B, DevGlobals* g, Grid gs,
                        GhostBand gb, int* __restrict__ gcnt, int initNewDt) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   // middle() starts its dt minimum at maxdt (nothing reads it during the reset)
-  if (initNewDt && i == 0) g->newDt_bits = (unsigned long long)__double_as_longlong(g->maxdt);
+  if (initNewDt && blockIdx.x == 0 && threadIdx.x == 0)
+    g->newDt_bits = (unsigned long long)__double_as_longlong(g->maxdt);
+  // grid-stride over the particles: a few hundred blocks, so few atomics on hmax
   double hv = 0.0;
-  int ghosts = 0;
-  if (i < n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int ghosts = 0;
     const int t = S.iam[i];
     if (t != T_DELETED) {
       double4 p = S.posh[i];
@@ -204,7 +205,7 @@ __global__ void k_wrap(Store S, int n, double dmin0, double dmin1, double dmin2_
           }
           S.posh[i] = p;
         }
-        hv = p.w;
+        hv = dmax2(hv, p.w);
         if (gcnt && isSphT(t)) ghosts = ghostCount(p, gs, B, gb);
       }
     }
@@ -542,14 +543,14 @@ __global__ void k_dt_final(sphdem_params P, DevGlobals* g) {
 }
 
 // accelCondition over sph, sphBoundary and dem
+// (grid-stride over a few hundred blocks: few atomics on newDt)
 __global__ void k_accel(Store S, int n, DevGlobals* g) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
   double v = __longlong_as_double(0x7ff0000000000000ll);
-  if (a < n) {
+  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < n; a += gridDim.x * blockDim.x) {
     const int t = S.iam[a];
     if (isSphT(t) || t == T_DEM) {
       const double4 f = S.frc[a];
-      v = 0.25 * sqrt(S.posh[a].w / sqrt(dot3(f.x, f.y, f.z, f.x, f.y, f.z)));
+      v = dmin2(v, 0.25 * sqrt(S.posh[a].w / sqrt(dot3(f.x, f.y, f.z, f.x, f.y, f.z))));
     }
   }
   atomicMinPos(&g->newDt_bits, v);
