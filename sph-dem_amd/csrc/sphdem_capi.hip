@@ -44,6 +44,7 @@ struct sphdem_ctx {
   int *nbl = nullptr, *nbcnt = nullptr;
   bool listsValid = false;
   double2* hinv = nullptr;  // {1/h, 1/h^5} per SPH-group particle
+  long long* okey = nullptr;  // order key of each entry, in scatter order
   // mixed form: packed neighbour records per entry (allocated on first use)
   uint4* mixP = nullptr;
   float4 *mixV = nullptr, *mixD = nullptr;
@@ -193,6 +194,8 @@ int allocEntries(sphdem_ctx* ctx, int64_t ghostCap) {
   CHECK(allocStore(ctx, alt, mem[1], ent));
   int *key = nullptr, *slot = nullptr, *tmp = nullptr, *perm = nullptr;
   double2* hinv = nullptr;
+  long long* okey = nullptr;
+  HIPC(hipMalloc(&okey, sizeof(long long) * ent));
   HIPC(hipMalloc(&key, sizeof(int) * ent));
   HIPC(hipMalloc(&slot, sizeof(int) * ent));
   HIPC(hipMalloc(&tmp, sizeof(int) * ent));
@@ -211,7 +214,8 @@ int allocEntries(sphdem_ctx* ctx, int64_t ghostCap) {
     HIPC(hipStreamSynchronize(ctx->stream));
   }
   void* old[] = {ctx->storeMem[0], ctx->storeMem[1], ctx->key, ctx->slot, ctx->tmp, ctx->perm,
-                 ctx->hinv, ctx->gsrc, ctx->gord, ctx->gcell, ctx->gorig, ctx->gcode};
+                 ctx->hinv, ctx->okey, ctx->gsrc, ctx->gord, ctx->gcell, ctx->gorig,
+                 ctx->gcode};
   for (void* q : old)
     if (q) HIPC(hipFree(q));
   ctx->cur = cur;
@@ -223,6 +227,7 @@ int allocEntries(sphdem_ctx* ctx, int64_t ghostCap) {
   ctx->tmp = tmp;
   ctx->perm = perm;
   ctx->hinv = hinv;
+  ctx->okey = okey;
   ctx->ghostCap = ghostCap;
   HIPC(hipMalloc(&ctx->gsrc, sizeof(int) * ghostCap));
   HIPC(hipMalloc(&ctx->gord, sizeof(int) * ghostCap));
@@ -583,14 +588,15 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false) {
   Side side(ctx, false);
   CHECK(buildSegments(ctx));
   side.back();
-  hipLaunchKernelGGL(k_scatter, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ntot, ctx->key,
-                     ctx->slot, ctx->start, ctx->tmp, ctx->counts, ctx->K + 1);
+  hipLaunchKernelGGL(k_scatter, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ntot, n, ctx->key,
+                     ctx->slot, ctx->start, ctx->tmp, ctx->okey, ctx->cur.orig, ctx->gsrc,
+                     ctx->gord, ctx->counts, ctx->K + 1);
   LAUNCHED();
   ctx->countsClean = ctx->K + 1;
   {
     Timed tm(ctx, KID_RANK);
-    hipLaunchKernelGGL(k_rank, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ntot, n, ctx->key,
-                       ctx->start, ctx->tmp, ctx->cur.orig, ctx->gsrc, ctx->gord, ctx->perm);
+    hipLaunchKernelGGL(k_rank, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ntot, ctx->key,
+                       ctx->start, ctx->tmp, ctx->okey, ctx->perm);
     LAUNCHED();
   }
   {
@@ -941,7 +947,7 @@ void freeAll(sphdem_ctx* ctx) {
                   ctx->segCnt,      ctx->segOff,      ctx->segs,    ctx->nbl,    ctx->nbcnt,
                   ctx->hinv,        ctx->gcnt,        ctx->goff,    ctx->gsrc,   ctx->gord,
                   ctx->gcell,       ctx->gorig,       ctx->gcode,   ctx->inv,
-                  ctx->mixP,        ctx->mixV,        ctx->mixD};
+                  ctx->mixP,        ctx->mixV,        ctx->mixD,    ctx->okey};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : ctx->evPool) (void)hipEventDestroy(e);

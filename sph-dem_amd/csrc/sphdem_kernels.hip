@@ -392,38 +392,44 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_add(int* __restrict__ out
   }
 }
 
-// Also clears the cell counts [0, k1) for the next reset (the scan has
-// consumed them).
-__global__ void k_scatter(int n, const int* __restrict__ key, const int* __restrict__ slot,
-                          const int* __restrict__ start, int* __restrict__ tmp,
-                          int* __restrict__ counts, int k1) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) tmp[start[key[i]] + slot[i]] = i;
-  for (int k = i; k < k1; k += gridDim.x * blockDim.x) counts[k] = 0;
-}
-
 // Order inside a cell = the reference's insertion order: real particles in
 // container order (original index, dataLL.cpp:226-231); ghosts in the order
-// they are received (slot `ord`, then the origin's container order).  Each
-// entry's rank is the number of cell members with a smaller order key.
+// they are received (slot `ord`, then the origin's container order).
 __device__ __forceinline__ long long orderKey(int e, int n, const int* __restrict__ orig,
                                               const int* __restrict__ gsrc,
                                               const int* __restrict__ gord) {
   if (e < n) return orig[e];
   return ((long long)gord[e - n] << 32) | (unsigned)orig[gsrc[e - n]];
 }
-__global__ void k_rank(int ntot, int n, const int* __restrict__ key, const int* __restrict__ start,
-                       const int* __restrict__ tmp, const int* __restrict__ orig,
-                       const int* __restrict__ gsrc, const int* __restrict__ gord,
+// Scatter of the entries into their cells' ranges (any order within a cell),
+// with each entry's order key beside it.  Also clears the cell counts
+// [0, k1) for the next reset (the scan has consumed them).
+__global__ void k_scatter(int n, int nreal, const int* __restrict__ key,
+                          const int* __restrict__ slot, const int* __restrict__ start,
+                          int* __restrict__ tmp, long long* __restrict__ okey,
+                          const int* __restrict__ orig, const int* __restrict__ gsrc,
+                          const int* __restrict__ gord, int* __restrict__ counts, int k1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const int q = start[key[i]] + slot[i];
+    tmp[q] = i;
+    okey[q] = orderKey(i, nreal, orig, gsrc, gord);
+  }
+  for (int k = i; k < k1; k += gridDim.x * blockDim.x) counts[k] = 0;
+}
+
+// Each entry's rank is the number of cell members with a smaller order key.
+__global__ void k_rank(int ntot, const int* __restrict__ key, const int* __restrict__ start,
+                       const int* __restrict__ tmp, const long long* __restrict__ okey,
                        int* __restrict__ perm) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= ntot) return;
   const int i = tmp[p];
   const int k = key[i];
   const int s0 = start[k], s1 = start[k + 1];
-  const long long o = orderKey(i, n, orig, gsrc, gord);
+  const long long o = okey[p];
   int rank = 0;
-  for (int q = s0; q < s1; q++) rank += (orderKey(tmp[q], n, orig, gsrc, gord) < o) ? 1 : 0;
+  for (int q = s0; q < s1; q++) rank += (okey[q] < o) ? 1 : 0;
   perm[s0 + rank] = i;
 }
 
