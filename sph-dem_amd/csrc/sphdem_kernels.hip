@@ -214,6 +214,26 @@ __global__ void k_wrap(Store S, int n, double dmin0, double dmin1, double dmin2_
   atomicMaxPos(&g->hmax_bits, hv);
 }
 
+// Slots within a cell for a wavefront's lanes: each run of equal keys
+// (k >= 0) in lane order takes its slots with one atomic; lanes with k < 0
+// take none.  Returns the lane's slot (undefined for k < 0).
+__device__ __forceinline__ int runSlots(int k, int* __restrict__ counts) {
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(k, 1, 64);
+  const bool head = lane == 0 || prev != k;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const int hl = 63 - __builtin_clzll(heads & upto);
+  int base = 0;
+  if (head && k >= 0) {
+    const unsigned long long rest = lane == 63 ? 0ull : (heads >> (lane + 1));
+    const int len = rest ? __builtin_ctzll(rest) + 1 : 64 - lane;
+    base = atomicAdd(&counts[k], len);
+  }
+  base = __shfl(base, hl, 64);
+  return base + (lane - hl);
+}
+
 // Binning of the real entries [0, n) (CdataLL::insert, dataLL.cpp:978-997):
 // key (cell) of every entry and a slot within its cell.  The store is sorted
 // by cell from the previous reset, so neighbouring lanes mostly share a key:
@@ -224,7 +244,6 @@ __global__ void k_wrap(Store S, int n, double dmin0, double dmin1, double dmin2_
 __global__ void k_bin(Store S, int n, Grid gs, Grid gc, int keyDeleted, int* __restrict__ key,
                       int* __restrict__ slot, int* __restrict__ counts, DevGlobals* g) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
   if (i == 0) g->hmax_bits = 0ull;
   int k = -1;
   if (i < n) {
@@ -248,20 +267,8 @@ __global__ void k_bin(Store S, int n, Grid gs, Grid gc, int keyDeleted, int* __r
     }
     key[i] = k;
   }
-  // runs of equal keys: a head where the key differs from the previous lane's
-  const int prev = __shfl_up(k, 1, 64);
-  const bool head = lane == 0 || prev != k;
-  const unsigned long long heads = __ballot(head);
-  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-  const int hl = 63 - __builtin_clzll(heads & upto);  // this lane's run head
-  int base = 0;
-  if (head && k >= 0) {
-    const unsigned long long rest = lane == 63 ? 0ull : (heads >> (lane + 1));
-    const int len = rest ? __builtin_ctzll(rest) + 1 : 64 - lane;
-    base = atomicAdd(&counts[k], len);
-  }
-  base = __shfl(base, hl, 64);
-  if (i < n) slot[i] = base + (lane - hl);
+  const int sl = runSlots(k, counts);
+  if (i < n) slot[i] = sl;
 }
 
 // ---- ghosts: the periodic images CdataLL::updateDomain receives
@@ -281,40 +288,49 @@ __global__ void k_ghost_count(Store S, int n, Grid g, Box B, GhostBand gb,
 // Writes ghost entry e = n + goff[i] + k for each ghost of real particle i:
 // its key in the ghost range (gbase + image cell), its slot, origin, `ord`
 // and packed image cell.
+// Ghost entries n + goff[i] ... of each real particle, one per valid
+// combination of shifts in receive order.  The 26 combinations are walked
+// wave-uniformly, so neighbouring lanes with the same combination (mostly the
+// same ghost cell: the store is sorted by cell) share one atomic per run.
 __global__ void k_ghost_make(Store S, int n, Grid g, Box B, GhostBand gb,
                              const int* __restrict__ goff, int gbase, int* __restrict__ key,
                              int* __restrict__ slot, int* __restrict__ counts,
                              int* __restrict__ gsrc, int* __restrict__ gord,
                              int* __restrict__ gcell) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int cnt = goff[i + 1] - goff[i];
-  if (cnt == 0) return;
-  const double4 p = S.posh[i];
-  const int rc = S.cell[i];
-  int cx[2], cy[2], cz[2];
-  const int mx = ghostShiftsAxis(p.x, 0, g, B, gb, cx);
-  const int my = ghostShiftsAxis(p.y, 1, g, B, gb, cy);
-  const int mz = ghostShiftsAxis(p.z, 2, g, B, gb, cz);
-  int e = n + goff[i];
-  for (int sx = -1; sx <= 1; sx++) {
-    if (sx != 0 && !((mx >> (sx > 0)) & 1)) continue;
-    const int X = sx == 0 ? cellX(rc) : cx[sx > 0];
-    for (int sy = -1; sy <= 1; sy++) {
-      if (sy != 0 && !((my >> (sy > 0)) & 1)) continue;
-      const int Y = sy == 0 ? cellY(rc) : cy[sy > 0];
-      for (int sz = -1; sz <= 1; sz++) {
-        if (sz != 0 && !((mz >> (sz > 0)) & 1)) continue;
-        if ((sx | sy | sz) == 0) continue;
-        const int Z = sz == 0 ? cellZ(rc) : cz[sz > 0];
-        const int k = gbase + (X * g.dim[1] + Y) * g.dim[2] + Z;
-        key[e] = k;
-        slot[e] = atomicAdd(&counts[k], 1);
-        gsrc[e - n] = i;
-        gord[e - n] = 9 * (sx + 1) + 3 * (sy + 1) + (sz + 1);
-        gcell[e - n] = packCell(X, Y, Z);
-        e++;
-      }
+  const int cnt = i < n ? goff[i + 1] - goff[i] : 0;
+  if (__ballot(cnt > 0) == 0ull) return;  // wave-uniform: no lane has ghosts
+  int cx[2] = {0, 0}, cy[2] = {0, 0}, cz[2] = {0, 0};
+  int mx = 0, my = 0, mz = 0, rc = 0, e = 0;
+  if (cnt > 0) {
+    const double4 p = S.posh[i];
+    rc = S.cell[i];
+    mx = ghostShiftsAxis(p.x, 0, g, B, gb, cx);
+    my = ghostShiftsAxis(p.y, 1, g, B, gb, cy);
+    mz = ghostShiftsAxis(p.z, 2, g, B, gb, cz);
+    e = n + goff[i];
+  }
+  for (int ord = 0; ord < 27; ord++) {
+    if (ord == 13) continue;  // no shift: the particle itself
+    const int sx = ord / 9 - 1, sy = (ord / 3) % 3 - 1, sz = ord % 3 - 1;
+    const bool on = cnt > 0 && (sx == 0 || ((mx >> (sx > 0)) & 1)) &&
+                    (sy == 0 || ((my >> (sy > 0)) & 1)) && (sz == 0 || ((mz >> (sz > 0)) & 1));
+    int k = -1, X = 0, Y = 0, Z = 0;
+    if (on) {
+      X = sx == 0 ? cellX(rc) : cx[sx > 0];
+      Y = sy == 0 ? cellY(rc) : cy[sy > 0];
+      Z = sz == 0 ? cellZ(rc) : cz[sz > 0];
+      k = gbase + (X * g.dim[1] + Y) * g.dim[2] + Z;
+    }
+    if (__ballot(on) == 0ull) continue;
+    const int sl = runSlots(k, counts);
+    if (on) {
+      key[e] = k;
+      slot[e] = sl;
+      gsrc[e - n] = i;
+      gord[e - n] = ord;
+      gcell[e - n] = packCell(X, Y, Z);
+      e++;
     }
   }
 }
