@@ -50,6 +50,8 @@ struct sphdem_ctx {
   float4 *mixV = nullptr, *mixD = nullptr;
   int64_t mixCap = 0;
   float mixRes = 0.0f;  // fixed-point unit of the last packing
+  double4* posm = nullptr;  // FP64 {x, y, z, mass} per entry for omega/density
+  int64_t posmCap = 0;
   // periodic ghosts (store entries [n, n + ng) after a reset)
   int ng = 0, gbase = 0;    // ghost count; first key of the ghost cells (0: none)
   // ghost field groups whose origins changed since the ghosts were copied
@@ -637,6 +639,25 @@ int checkLoaded(sphdem_ctx* ctx) {
 
 // ---- passes
 bool mixed(const sphdem_ctx* c) { return c->P.precision == SPHDEM_MIXED; }
+// Packs {x, y, z, mass} of every SPH-group entry and ghost for the FP64
+// omega and density sums.
+int packPosm(sphdem_ctx* ctx) {
+  const int64_t need = ctx->cap + ctx->ghostCap;
+  if (ctx->posmCap < need) {
+    if (ctx->posm) HIPC(hipFree(ctx->posm));
+    ctx->posm = nullptr;
+    ctx->posmCap = 0;
+    HIPC(hipMalloc(&ctx->posm, sizeof(double4) * need));
+    ctx->posmCap = need;
+  }
+  const int cnt = ctx->ns + ctx->ng;
+  if (cnt > 0) {
+    hipLaunchKernelGGL(k_pack_posm, dim3(blocks(cnt)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                       ctx->ns, ctx->n, ctx->ng, ctx->posm);
+    LAUNCHED();
+  }
+  return 0;
+}
 // Packs the mixed form's neighbour records (groups MIX_P/V/D) of every
 // SPH-group entry and ghost, positions in fixed point over the current grid.
 int packMix(sphdem_ctx* ctx, unsigned groups) {
@@ -680,9 +701,11 @@ int passOmega(sphdem_ctx* ctx) {
     CHECK(packMix(ctx, MIX_P));
     hipLaunchKernelGGL(k_omega_m, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
                        ctx->ns, ctx->nbl, ctx->nbcnt, ctx->mixP, ctx->mixRes, ctx->P.wcon);
-  } else
+  } else {
+    CHECK(packPosm(ctx));
     hipLaunchKernelGGL(k_omega_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
-                       ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->P.wcon);
+                       ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->posm, ctx->P.wcon);
+  }
   LAUNCHED();
   markStale(ctx, GS_DPO);
   return 0;
@@ -809,9 +832,11 @@ int passDensity(sphdem_ctx* ctx) {
     CHECK(packMix(ctx, MIX_P));
     hipLaunchKernelGGL(k_density_m, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
                        ctx->ns, ctx->nbl, ctx->nbcnt, ctx->mixP, ctx->mixRes, ctx->P.wcon);
-  } else
+  } else {
+    CHECK(packPosm(ctx));
     hipLaunchKernelGGL(k_density_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
-                       ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->P.wcon);
+                       ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->posm, ctx->P.wcon);
+  }
   LAUNCHED();
   markStale(ctx, GS_DPO);
   return 0;
@@ -947,7 +972,8 @@ void freeAll(sphdem_ctx* ctx) {
                   ctx->segCnt,      ctx->segOff,      ctx->segs,    ctx->nbl,    ctx->nbcnt,
                   ctx->hinv,        ctx->gcnt,        ctx->goff,    ctx->gsrc,   ctx->gord,
                   ctx->gcell,       ctx->gorig,       ctx->gcode,   ctx->inv,
-                  ctx->mixP,        ctx->mixV,        ctx->mixD,    ctx->okey};
+                  ctx->mixP,        ctx->mixV,        ctx->mixD,    ctx->okey,
+                  ctx->posm};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : ctx->evPool) (void)hipEventDestroy(e);
@@ -1097,8 +1123,9 @@ int sphdem_output_vars(sphdem_ctx* ctx, sphdem_output* out) {
   CHECK(refreshGhosts(ctx, GS_POSH | GS_VHM));
   double* densCalc = ctx->fieldbuf;
   if (ns > 0) {
+    CHECK(packPosm(ctx));
     hipLaunchKernelGGL(k_density_l, dim3(blocks(ns)), dim3(TPB), 0, ctx->stream, ctx->cur, ns,
-                       ctx->box, ctx->nbl, ctx->nbcnt, ctx->P.wcon, densCalc);
+                       ctx->box, ctx->nbl, ctx->nbcnt, ctx->posm, ctx->P.wcon, densCalc);
     LAUNCHED();
   }
   const int nb = std::max(blocks(ns), 1);

@@ -348,10 +348,22 @@ __device__ __forceinline__ void gradWi(double wcon, bool varh, double omega_a, d
   }
 }
 
+// {x, y, z, mass} of every SPH-group entry [0, ns) and ghost [n, n + ng):
+// what the omega and density sums read of a neighbour, in one 32-byte record
+// (two loads instead of three).
+__global__ void k_pack_posm(Store S, int ns, int n, int ng, double4* __restrict__ posm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns + ng) return;
+  const int e = i < ns ? i : n + (i - ns);
+  const double4 p = S.posh[e];
+  posm[e] = make_double4(p.x, p.y, p.z, S.vhm[e].w);
+}
+
 // ---- calcOmega (sph.h:437-452) over the cached list -----------------------------
 __global__ void __launch_bounds__(256) k_omega_l(Store S, int ns, Box B,
                                                  const int* __restrict__ nbl,
-                                                 const int* __restrict__ nbcnt, double wcon) {
+                                                 const int* __restrict__ nbcnt,
+                                                 const double4* __restrict__ posm, double wcon) {
   const int a = xcdThread();
   if (a >= ns) return;
   const double4 pa = S.posh[a];
@@ -362,20 +374,15 @@ __global__ void __launch_bounds__(256) k_omega_l(Store S, int ns, Box B,
   constexpr int U = 8;
   for (int k0 = 0; k0 < n; k0 += U) {
     SPHDEM_FETCH_ENTRIES(U)
-    double4 p[U];
-    double m[U];
+    double4 p[U];  // x, y, z, mass
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int j = e[u];
-      p[u] = S.posh[j];
-      m[u] = S.vhm[j].w;
-    }
+    for (int u = 0; u < U; u++) p[u] = posm[e[u]];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       if (k0 + u >= n) break;
       const double dx = pa.x - p[u].x, dy = pa.y - p[u].y, dz = pa.z - p[u].z;
       const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
-      om += m[u] * wendPdWhi(wcon, divr(r, ha, rha), ha, rha, i3, c3);
+      om += p[u].w * wendPdWhi(wcon, divr(r, ha, rha), ha, rha, i3, c3);
     }
   }
   S.dpo[a].z = 1 + ha * om / (S.dpo[a].x * 3);
@@ -385,7 +392,8 @@ __global__ void __launch_bounds__(256) k_omega_l(Store S, int ns, Box B,
 // out: where the sums go (null: the particles' dens)
 __global__ void __launch_bounds__(256) k_density_l(Store S, int ns, Box B,
                                                    const int* __restrict__ nbl,
-                                                   const int* __restrict__ nbcnt, double wcon,
+                                                   const int* __restrict__ nbcnt,
+                                                   const double4* __restrict__ posm, double wcon,
                                                    double* __restrict__ out = nullptr) {
   const int a = xcdThread();
   if (a >= ns) return;
@@ -397,20 +405,15 @@ __global__ void __launch_bounds__(256) k_density_l(Store S, int ns, Box B,
   constexpr int U = 8;
   for (int k0 = 0; k0 < n; k0 += U) {
     SPHDEM_FETCH_ENTRIES(U)
-    double4 p[U];
-    double m[U];
+    double4 p[U];  // x, y, z, mass
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int j = e[u];
-      p[u] = S.posh[j];
-      m[u] = S.vhm[j].w;
-    }
+    for (int u = 0; u < U; u++) p[u] = posm[e[u]];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       if (k0 + u >= n) break;
       const double dx = pa.x - p[u].x, dy = pa.y - p[u].y, dz = pa.z - p[u].z;
       const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
-      rho += m[u] * wendWi(wcon, divr(r, ha, rha), i3);
+      rho += p[u].w * wendWi(wcon, divr(r, ha, rha), i3);
     }
   }
   if (out)
