@@ -326,27 +326,6 @@ __global__ void k_hinv(const double4* __restrict__ posh, int ns, double2* __rest
   hinv[a] = make_double2(1.0 / h, 1.0 / pow5(h));
 }
 
-// gradW (sph.h:322-342) with 1/h^5 given and the division by pa.omega as divr
-__device__ __forceinline__ void gradWi(double wcon, bool varh, double omega_a, double romega,
-                                       double dx, double dy, double dz, double q, double i5,
-                                       double* g) {
-  double drF0 = 0.0, drF1 = 0.0, drF2 = 0.0;
-  if (q != 0) {
-    const double F = wendFi(wcon, q, i5);
-    drF0 = dx * F;
-    drF1 = dy * F;
-    drF2 = dz * F;
-  }
-  if (varh) {
-    g[0] = divr(drF0, omega_a, romega);
-    g[1] = divr(drF1, omega_a, romega);
-    g[2] = divr(drF2, omega_a, romega);
-  } else {
-    g[0] = drF0;
-    g[1] = drF1;
-    g[2] = drF2;
-  }
-}
 
 // {x, y, z, mass} of every SPH-group entry [0, ns) and ghost [n, n + ng):
 // what the omega and density sums read of a neighbour, in one 32-byte record
@@ -433,9 +412,11 @@ __global__ void __launch_bounds__(256) k_dddt_l(Store S, int ns, Box B,
   const double4 pa = S.posh[a];
   const double4 va = S.vhm[a];
   const double ha = pa.w;
-  const double rha = 1.0 / ha, i5a = 1.0 / pow5(ha);
-  const double om = S.dpo[a].z;
-  const double rom = 1.0 / om;
+  const double rha = 1.0 / ha;
+  const double rom = VARH ? 1.0 / S.dpo[a].z : 1.0;
+  // gradW = dr * G with G = poly(q) K(h) / r, K(h) = WCON h^-4 (/ omega_a
+  // under VAR_H, sph.h:331-332): F(q, h) = WCON poly(q) / (q h^5) and 1/q = h/r
+  const double Ka = wcon * (rha * rha) * (rha * rha) * rom;
   double dddt = S.vh0d[a].w;
   const int n = nbcnt[a];
   constexpr int U = 8;
@@ -456,16 +437,16 @@ __global__ void __launch_bounds__(256) k_dddt_l(Store S, int ns, Box B,
       const double dx = pa.x - p[u].x, dy = pa.y - p[u].y, dz = pa.z - p[u].z;
       const double dv0 = va.x - v[u].x, dv1 = va.y - v[u].y, dv2 = va.z - v[u].z;
       const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
-      double gW[3];
-      if (VARH) {
-        gradWi(wcon, true, om, rom, dx, dy, dz, divr(r, ha, rha), i5a, gW);
-      } else {
-        double g1[3], g2[3];
-        gradWi(wcon, false, om, rom, dx, dy, dz, divr(r, ha, rha), i5a, g1);
-        gradWi(wcon, false, om, rom, dx, dy, dz, divr(r, p[u].w, hi[u].x), hi[u].y, g2);
-        for (int c = 0; c < 3; c++) gW[c] = 0.5 * (g1[c] + g2[c]);
+      const double rinv = r > 0.0 ? 1.0 / r : 0.0;  // r = 0: dr = 0, gradW = 0
+      const double qa = divr(r, ha, rha);
+      double G = qa <= 2.0 ? wendPoly(qa) * Ka * rinv : 0.0;
+      if (!VARH) {
+        const double qb = divr(r, p[u].w, hi[u].x);
+        const double Kb = wcon * (hi[u].y * p[u].w);  // WCON hb^-4
+        const double Gb = qb <= 2.0 ? wendPoly(qb) * Kb * rinv : 0.0;
+        G = 0.5 * (G + Gb);  // the average gradient
       }
-      dddt += v[u].w * dot3(dv0, dv1, dv2, gW[0], gW[1], gW[2]);
+      dddt += v[u].w * (dot3(dv0, dv1, dv2, dx, dy, dz) * G);
     }
   }
   S.vh0d[a].w = dddt;
