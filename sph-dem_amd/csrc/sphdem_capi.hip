@@ -742,10 +742,11 @@ void launchForce(sphdem_ctx* ctx) {
                        ctx->cur, ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->hinv, ctx->P,
                        ctx->dg);
 }
-int passForce(sphdem_ctx* ctx) {
+// hinvReady: the 1/h table is already current (made beside the list build)
+int passForce(sphdem_ctx* ctx, bool hinvReady = false) {
   CHECK(ensureLists(ctx));
   CHECK(refreshGhosts(ctx, GS_ALL));
-  if (!mixed(ctx)) CHECK(computeHinv(ctx));
+  if (!mixed(ctx) && !hinvReady) CHECK(computeHinv(ctx));
   using Launch = void (*)(sphdem_ctx*);
   static const Launch table[16] = {
       launchForce<0>, launchForce<1>, launchForce<2>,  launchForce<3>,
@@ -864,19 +865,28 @@ int stepStart(sphdem_ctx* ctx, uint32_t bodyMask = 0, const double* accel = null
 }
 int stepMiddle(sphdem_ctx* ctx) {
   CHECK(doReset(ctx, false, true));
-  if (liqDem(ctx)) {
-    // porosity, Shepard sums and DEM->SPH drag on the side stream, beside the
-    // list build: the build reads positions, h, cells and segments, which
-    // the coupling pass does not write
+  bool hinvReady = false;
+  {
+    // On the side stream, beside the list build and omega: the coupling pass
+    // (porosity, Shepard sums, DEM->SPH drag), then the EOS (reads dens and
+    // porosity, stores pdr2 and press) and the FP64 force pass's 1/h table.
+    // None of them writes what the list build or omega read, and omega
+    // stores only omega.  (The ghosts are fresh after the reset, so nothing
+    // on the side stream refreshes them.)
+    CHECK(refreshGhosts(ctx, GS_POSH));
     Side side(ctx);
-    CHECK(passCoupling(ctx));
+    if (liqDem(ctx)) CHECK(passCoupling(ctx));
+    CHECK(passEos(ctx));
+    if (!mixed(ctx)) {
+      CHECK(computeHinv(ctx));
+      hinvReady = true;
+    }
     side.back();
     CHECK(ensureLists(ctx));
+    if (varH(ctx)) CHECK(passOmega(ctx));
     CHECK(side.join());
   }
-  if (varH(ctx)) CHECK(passOmega(ctx));
-  CHECK(passEos(ctx));
-  CHECK(passForce(ctx));
+  CHECK(passForce(ctx, hinvReady));
   const int na = ctx->ns + ctx->nd;
   hipLaunchKernelGGL(k_accel, dim3(reduceBlocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream,
                      ctx->cur, na, ctx->dg);

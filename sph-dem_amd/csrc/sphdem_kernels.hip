@@ -544,8 +544,8 @@ __global__ void k_eos(Store S, int ns, sphdem_params P) {
     press = P.prb * (pow(d.x / (P.refd * d.w), P.gamma) - 1.0);
   else
     press = P.prb * (pow(d.x / P.refd, 7.0) - 1.0);
-  d.y = press / (d.x * d.x);
-  S.dpo[a] = d;
+  // only pdr2 and press are stored (omega may be written beside this pass)
+  S.dpo[a].y = press / (d.x * d.x);
   S.v0p[a].w = press;
 }
 
