@@ -292,3 +292,21 @@ def test_periodic_wrap_matches():
         x.reset()
     assert np.array_equal(sim.get("r"), orc.get("r"))
     assert np.array_equal(sim.cells(), orc.cells())
+
+
+def test_larger_sediment_steps():
+    """16^3 fluid, 200 DEM: more cells per column, ghost columns on all four
+    periodic faces and several segments per column; two whole steps, then the
+    pair set and the list order of the evolved state."""
+    c = sd.sediment_case(nf=16, n_dem=200, seed=5)
+    sim, orc = both(c)
+    sim.run(2)
+    orc.run(2)
+    alive = orc.get("iam") != -1
+    check_fields(sim, orc, ["r", "v", "vhat", "dens", "h", "f", "porosity", "fdrag", "shepSum"],
+                 alive, 1e-10)
+    sim.reset()
+    orc.reset()
+    g = lists_by_particle(sim.pairs(sd.PAIRS_SPH))
+    o = lists_by_particle(orc.pairs(sd.PAIRS_SPH))
+    assert g == o
