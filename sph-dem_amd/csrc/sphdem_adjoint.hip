@@ -1,0 +1,569 @@
+// sphdem_adjoint.hip -- forward and backward (vector-Jacobian product)
+// kernels of the pair passes of the differentiable SPH-DEM step: SPH force,
+// fluid-particle interaction with Di Felice drag, DEM contacts.
+//
+// The forward kernels restate the reference operators over explicit pair sets
+// in original particle order (sphIncompress.h:899-972 with sph.h:579-666,
+// 692-734; sph.h:1708-1792 with 1575-1631; sph.h:1676-1706 with 1495-1516).
+// Each backward kernel recomputes its pair's forward terms and propagates the
+// output adjoints to every differentiable input by the chain rule written out
+// by hand (derivations in the comments).  Sums over pairs go through hardware
+// FP64 atomics on global memory; the parameter adjoints are reduced per block
+// first, so one atomic per block lands on each.
+//
+// Wendland C2 (3D, support 2h) in the closed forms used below, with q = r/h,
+// t = 2 - q and c = WCON:
+//   K(q) = c t^4 (1 + 2q),   K'(q) = -10 c q t^3
+//   W    = K / h^3           dW/dr = K'/h^4      dW/dh = -(q K' + 3 K)/h^4
+//   F    = K'/(q h^5) = -10 c t^3 / h^5  (the gradW factor, sph.h:74-81)
+//   dF/dr = 30 c t^2 / h^6   dF/dh = -10 c t^2 (3q - 5t) / h^6
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "../../include/sphdem_adjoint.h"
+#include "../../include/sphdem_c.h"
+
+namespace sphdem_adj {
+
+constexpr int kTPB = 256;
+constexpr int SPH_T = 0;
+
+__device__ __forceinline__ double3 ld3(const double* p, int i) {
+  return make_double3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+}
+__device__ __forceinline__ void add3(double* p, int i, double x, double y, double z) {
+  unsafeAtomicAdd(p + 3 * i, x);
+  unsafeAtomicAdd(p + 3 * i + 1, y);
+  unsafeAtomicAdd(p + 3 * i + 2, z);
+}
+__device__ __forceinline__ void add1(double* p, int i, double x) { unsafeAtomicAdd(p + i, x); }
+
+// b's periodic image: r_b + shift
+__device__ __forceinline__ double3 imageOf(const double* r, int b, int img, const double* L) {
+  const double3 rb = ld3(r, b);
+  return make_double3(rb.x + (double)(img % 3 - 1) * L[0], rb.y + (double)((img / 3) % 3 - 1) * L[1],
+                      rb.z + (double)(img / 9 - 1) * L[2]);
+}
+
+__device__ __forceinline__ void wendF(double c, double r, double h, double& F, double& dFdr,
+                                      double& dFdh) {
+  const double q = r / h;
+  if (q <= 2.0) {
+    const double t = 2.0 - q, ih = 1.0 / h, ih2 = ih * ih, ih5 = ih2 * ih2 * ih, ih6 = ih5 * ih;
+    F = -10.0 * c * t * t * t * ih5;
+    dFdr = 30.0 * c * t * t * ih6;
+    dFdh = -10.0 * c * t * t * (3.0 * q - 5.0 * t) * ih6;
+  } else {
+    F = dFdr = dFdh = 0.0;
+  }
+}
+__device__ __forceinline__ void wendW(double c, double r, double h, double& W, double& dWdr,
+                                      double& dWdh) {
+  const double q = r / h;
+  if (q <= 2.0) {
+    const double t = 2.0 - q, ih = 1.0 / h, ih3 = ih * ih * ih, ih4 = ih3 * ih;
+    const double K = c * t * t * t * t * (1.0 + 2.0 * q);
+    const double Kp = -10.0 * c * q * t * t * t;
+    W = K * ih3;
+    dWdr = Kp * ih4;
+    dWdh = -(q * Kp + 3.0 * K) * ih4;
+  } else {
+    W = dWdr = dWdh = 0.0;
+  }
+}
+
+// Sums NV per-thread values over the block and adds each total to dst[k]
+// with one atomic.  Every thread of the block must call it.
+template <int NV>
+__device__ void blockAdd(double (&v)[NV], double* dst) {
+  __shared__ double part[NV][kTPB / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double x = v[k];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+    if (lane == 0) part[k][w] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0.0;
+    for (int j = 0; j < kTPB / 64; j++) s += part[threadIdx.x][j];
+    if (s != 0.0) unsafeAtomicAdd(dst + threadIdx.x, s);
+  }
+}
+
+// ---------------------------------------------------------------- SPH force
+// Pair (a, b), dx = r_a - r_b', r = |dx|, Fa = F(r, h_a), Fb = F(r, h_b), ia = 1/omega_a:
+//   viscous (a SPH):  A  = m_b * visc * d / den,  fv_a += A (vhat_a - vhat_b)
+//     visc = 2 nu rho0 / (rho_a rho_b),  d = r^2 (Fa + Fb) ia / 2,
+//     den = r^2 + 0.01 hav^2,  hav = (h_a + h_b)/2
+//   pressure:         B  = -m_b (pdr2_a Fa + pdr2_b Fb) ia,  fp_a += B dx
+__global__ void k_force_fwd(int64_t np, const int32_t* __restrict__ pa,
+                            const int32_t* __restrict__ pb, const int32_t* __restrict__ pimg,
+                            sphdem_adj_fields in, sphdem_adj_consts C, double* __restrict__ fv,
+                            double* __restrict__ fp) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  const int a = pa[k], b = pb[k];
+  const double3 ra = ld3(in.r, a), rb = imageOf(in.r, b, pimg[k], C.shift_len);
+  const double dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
+  const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
+  const double ha = in.h[a], hb = in.h[b];
+  double Fa, Fb, t0, t1;
+  wendF(C.wcon, r, ha, Fa, t0, t1);
+  wendF(C.wcon, r, hb, Fb, t0, t1);
+  const double ia = 1.0 / in.omega[a];
+  const double mb = in.mass[b];
+  if (in.iam[a] == SPH_T) {
+    const double visc = 2.0 * C.viscosity * C.dens / (in.dens[a] * in.dens[b]);
+    const double d = r2 * (Fa + Fb) * 0.5 * ia;
+    const double hav = 0.5 * (ha + hb);
+    const double den = r2 + 0.01 * hav * hav;
+    const double A = mb * visc * d / den;
+    const double3 va = ld3(in.vhat, a), vb = ld3(in.vhat, b);
+    add3(fv, a, A * (va.x - vb.x), A * (va.y - vb.y), A * (va.z - vb.z));
+  }
+  const double B = -mb * (in.pdr2[a] * Fa + in.pdr2[b] * Fb) * ia;
+  add3(fp, a, B * dx, B * dy, B * dz);
+}
+
+__global__ void k_force_bwd(int64_t np, const int32_t* __restrict__ pa,
+                            const int32_t* __restrict__ pb, const int32_t* __restrict__ pimg,
+                            sphdem_adj_fields in, sphdem_adj_consts C,
+                            const double* __restrict__ gfv, const double* __restrict__ gfp,
+                            sphdem_adj_grads g) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double gnu[1] = {0.0};
+  if (k < np) {
+    const int a = pa[k], b = pb[k];
+    const double3 ra = ld3(in.r, a), rb = imageOf(in.r, b, pimg[k], C.shift_len);
+    const double dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
+    const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
+    const double ha = in.h[a], hb = in.h[b];
+    double Fa, dFa_r, dFa_h, Fb, dFb_r, dFb_h;
+    wendF(C.wcon, r, ha, Fa, dFa_r, dFa_h);
+    wendF(C.wcon, r, hb, Fb, dFb_r, dFb_h);
+    const double ia = 1.0 / in.omega[a];
+    const double mb = in.mass[b];
+    double r2bar = 0.0, Fabar = 0.0, Fbbar = 0.0, iabar = 0.0, habar = 0.0, hbbar = 0.0;
+    if (in.iam[a] == SPH_T) {
+      const double rhoa = in.dens[a], rhob = in.dens[b];
+      const double visc = 2.0 * C.viscosity * C.dens / (rhoa * rhob);
+      const double Fs = Fa + Fb;
+      const double d = r2 * Fs * 0.5 * ia;
+      const double hav = 0.5 * (ha + hb);
+      const double den = r2 + 0.01 * hav * hav;
+      const double A = mb * visc * d / den;
+      const double3 va = ld3(in.vhat, a), vb = ld3(in.vhat, b);
+      const double3 gv = ld3(gfv, a);
+      const double dvx = va.x - vb.x, dvy = va.y - vb.y, dvz = va.z - vb.z;
+      // fv = A dv
+      const double Abar = gv.x * dvx + gv.y * dvy + gv.z * dvz;
+      add3(g.vhat, a, A * gv.x, A * gv.y, A * gv.z);
+      add3(g.vhat, b, -A * gv.x, -A * gv.y, -A * gv.z);
+      // A = m_b visc d / den
+      const double viscbar = Abar * mb * d / den;
+      const double dbar = Abar * mb * visc / den;
+      const double denbar = -Abar * A / den;
+      // visc = 2 nu rho0 / (rho_a rho_b)
+      gnu[0] = viscbar * visc / C.viscosity;
+      add1(g.dens, a, -viscbar * visc / rhoa);
+      add1(g.dens, b, -viscbar * visc / rhob);
+      // d = r^2 Fs ia / 2
+      r2bar += dbar * Fs * 0.5 * ia;
+      const double Fsbar = dbar * r2 * 0.5 * ia;
+      iabar += dbar * r2 * Fs * 0.5;
+      Fabar += Fsbar;
+      Fbbar += Fsbar;
+      // den = r^2 + 0.01 hav^2
+      r2bar += denbar;
+      const double havbar = denbar * 0.02 * hav;
+      habar += 0.5 * havbar;
+      hbbar += 0.5 * havbar;
+    }
+    // fp = B dx, B = -m_b (pdr2_a Fa + pdr2_b Fb) ia
+    const double pda = in.pdr2[a], pdb = in.pdr2[b];
+    const double B = -mb * (pda * Fa + pdb * Fb) * ia;
+    const double3 gp = ld3(gfp, a);
+    const double Bbar = gp.x * dx + gp.y * dy + gp.z * dz;
+    add1(g.pdr2, a, -mb * Fa * ia * Bbar);
+    add1(g.pdr2, b, -mb * Fb * ia * Bbar);
+    Fabar += -mb * pda * ia * Bbar;
+    Fbbar += -mb * pdb * ia * Bbar;
+    iabar += -mb * (pda * Fa + pdb * Fb) * Bbar;
+    // F(r, h)
+    const double rbar = Fabar * dFa_r + Fbbar * dFb_r;
+    habar += Fabar * dFa_h;
+    hbbar += Fbbar * dFb_h;
+    add1(g.h, a, habar);
+    add1(g.h, b, hbbar);
+    // ia = 1/omega_a
+    add1(g.omega, a, -iabar * ia * ia);
+    // dx: direct term B gp, through r^2 (2 dx) and through r (dx / r)
+    const double s = 2.0 * r2bar + (r > 0.0 ? rbar / r : 0.0);
+    const double gx = B * gp.x + s * dx, gy = B * gp.y + s * dy, gz = B * gp.z + s * dz;
+    add3(g.r, a, gx, gy, gz);
+    add3(g.r, b, -gx, -gy, -gz);
+  }
+  blockAdd<1>(gnu, g.params);
+}
+
+// ------------------------------------------------ fluid-particle interaction
+// For DEM particle p over fluid partners n (W = W(|r_p - r_n'|, h_n),
+// dvol = m_n / rho_n):
+//   S  = sum dvol W,  U = sum dvol W vhat_n,  Pn = sum dvol W por_n,
+//   Gp = sum m_n W fpfb_n,  Gv = sum (m_n W + dvol W) fv_n
+// and, when S > 0.5, with fvel = U/S, por = Pn/S, fpfb = Gp/S, fvv = Gv/S:
+//   dv = por (fvel - vhat_p),  vdiff = |dv|,  Re = (2R/nu) vdiff,
+//   chi = drag_a - drag_b exp(-(1.5 - log10 Re)^2 / 2),
+//   C = (0.63 + 4.8 / sqrt(Re))^2,  s = A C vdiff / 2  (A = pi R^2),
+//   fd = s por^(-chi) dv  (0 when vdiff = 0)
+//   fdrag_p = (V/m_p)(fvv + fpfb) + (rho0/m_p) fd
+struct Fpi {
+  double S, Pn;
+  double3 U, Gp, Gv;
+};
+
+__device__ __forceinline__ void fpiSums(const sphdem_adj_fields& in, const sphdem_adj_consts& C,
+                                        double3 rp, int64_t j0, int64_t j1,
+                                        const int32_t* __restrict__ pb,
+                                        const int32_t* __restrict__ pimg, Fpi& o) {
+  o.S = o.Pn = 0.0;
+  o.U = o.Gp = o.Gv = make_double3(0.0, 0.0, 0.0);
+  for (int64_t j = j0; j < j1; j++) {
+    const int n = pb[j];
+    const double3 rn = imageOf(in.r, n, pimg[j], C.shift_len);
+    const double dx = rp.x - rn.x, dy = rp.y - rn.y, dz = rp.z - rn.z;
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    double W, t0, t1;
+    wendW(C.wcon, r, in.h[n], W, t0, t1);
+    const double mn = in.mass[n];
+    const double dvW = (mn / in.dens[n]) * W, mW = mn * W;
+    const double3 vn = ld3(in.vhat, n), fpn = ld3(in.fpfb, n), fvn = ld3(in.fv, n);
+    o.S += dvW;
+    o.Pn += dvW * in.por[n];
+    o.U.x += dvW * vn.x;
+    o.U.y += dvW * vn.y;
+    o.U.z += dvW * vn.z;
+    o.Gp.x += mW * fpn.x;
+    o.Gp.y += mW * fpn.y;
+    o.Gp.z += mW * fpn.z;
+    o.Gv.x += (mW + dvW) * fvn.x;
+    o.Gv.y += (mW + dvW) * fvn.y;
+    o.Gv.z += (mW + dvW) * fvn.z;
+  }
+}
+
+__global__ void k_fpi_fwd(int32_t nq, const int32_t* __restrict__ query,
+                          const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
+                          const int32_t* __restrict__ pimg, sphdem_adj_fields in,
+                          sphdem_adj_consts C, double* __restrict__ fdrag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const int p = query[i];
+  const double3 rp = ld3(in.r, p);
+  Fpi o;
+  fpiSums(in, C, rp, off[i], off[i + 1], pb, pimg, o);
+  if (!(o.S > 0.5)) return;
+  const double iS = 1.0 / o.S;
+  const double por = o.Pn * iS;
+  const double3 vp = ld3(in.vhat, p);
+  const double dvx = por * (o.U.x * iS - vp.x), dvy = por * (o.U.y * iS - vp.y),
+               dvz = por * (o.U.z * iS - vp.z);
+  const double vdiff = sqrt(dvx * dvx + dvy * dvy + dvz * dvz);
+  double sp = 0.0;
+  if (vdiff != 0.0) {
+    const double R = C.dem_radius;
+    const double Re = (2.0 * R / C.viscosity) * vdiff;
+    const double L10 = log10(Re);
+    const double chi = C.drag_a - C.drag_b * exp(-(1.5 - L10) * (1.5 - L10) / 2.0);
+    const double cc = 0.63 + 4.8 / sqrt(Re);
+    const double s = 0.5 * (C.pi * R * R) * (cc * cc) * vdiff;
+    sp = s * pow(por, -chi);
+  }
+  const double mp = in.mass[p];
+  const double kv = C.dem_vol / mp, kd = C.dens / mp;
+  fdrag[3 * p] += kv * (o.Gv.x + o.Gp.x) * iS + kd * sp * dvx;
+  fdrag[3 * p + 1] += kv * (o.Gv.y + o.Gp.y) * iS + kd * sp * dvy;
+  fdrag[3 * p + 2] += kv * (o.Gv.z + o.Gp.z) * iS + kd * sp * dvz;
+}
+
+__global__ void k_fpi_bwd(int32_t nq, const int32_t* __restrict__ query,
+                          const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
+                          const int32_t* __restrict__ pimg, sphdem_adj_fields in,
+                          sphdem_adj_consts C, const double* __restrict__ gfd,
+                          sphdem_adj_grads g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double gpar[3] = {0.0, 0.0, 0.0};  // nu, drag_a, drag_b
+  const int p = i < nq ? query[i] : 0;
+  Fpi o;
+  const double3 rp = i < nq ? ld3(in.r, p) : make_double3(0.0, 0.0, 0.0);
+  if (i < nq) fpiSums(in, C, rp, off[i], off[i + 1], pb, pimg, o);
+  if (i < nq && o.S > 0.5) {
+    const double iS = 1.0 / o.S;
+    const double por = o.Pn * iS;
+    const double3 fvel = make_double3(o.U.x * iS, o.U.y * iS, o.U.z * iS);
+    const double3 vp = ld3(in.vhat, p);
+    const double ex = fvel.x - vp.x, ey = fvel.y - vp.y, ez = fvel.z - vp.z;
+    const double dvx = por * ex, dvy = por * ey, dvz = por * ez;
+    const double vdiff = sqrt(dvx * dvx + dvy * dvy + dvz * dvz);
+    const double mp = in.mass[p];
+    const double kv = C.dem_vol / mp, kd = C.dens / mp;
+    const double3 gf = ld3(gfd, p);
+    // fdrag = kv (Gv + Gp)/S + kd fd
+    const double3 Gbar = make_double3(kv * gf.x * iS, kv * gf.y * iS, kv * gf.z * iS);
+    double Sbar = -(Gbar.x * (o.Gv.x + o.Gp.x) + Gbar.y * (o.Gv.y + o.Gp.y) +
+                    Gbar.z * (o.Gv.z + o.Gp.z)) * iS;
+    double porbar = 0.0;
+    double3 fvelbar = make_double3(0.0, 0.0, 0.0);
+    if (vdiff != 0.0) {
+      const double R = C.dem_radius, A = C.pi * R * R;
+      const double Re = (2.0 * R / C.viscosity) * vdiff;
+      const double L10 = log10(Re);
+      const double E = exp(-(1.5 - L10) * (1.5 - L10) / 2.0);
+      const double chi = C.drag_a - C.drag_b * E;
+      const double cc = 0.63 + 4.8 / sqrt(Re);
+      const double Cd = cc * cc;
+      const double s = 0.5 * A * Cd * vdiff;
+      const double pw = pow(por, -chi);
+      // fd = (s pw) dv
+      const double fdx = kd * gf.x, fdy = kd * gf.y, fdz = kd * gf.z;
+      const double tau = fdx * dvx + fdy * dvy + fdz * dvz;
+      double dvbx = s * pw * fdx, dvby = s * pw * fdy, dvbz = s * pw * fdz;
+      const double sbar = tau * pw, pwbar = tau * s;
+      // pw = por^(-chi)
+      const double chibar = -log(por) * pw * pwbar;
+      porbar += -chi * pw / por * pwbar;
+      gpar[1] = chibar;
+      gpar[2] = -E * chibar;
+      const double Ebar = -C.drag_b * chibar;
+      // s = A Cd vdiff / 2;  Cd = (0.63 + 4.8 Re^-1/2)^2;  E = exp(-(1.5 - L10)^2/2)
+      double vdbar = sbar * 0.5 * A * Cd;
+      const double Cdbar = sbar * 0.5 * A * vdiff;
+      const double dCd = -4.8 * cc * pow(Re, -1.5);
+      const double dE = E * (1.5 - L10) / (Re * log(10.0));
+      const double Rebar = Cdbar * dCd + Ebar * dE;
+      // Re = (2R/nu) vdiff
+      vdbar += Rebar * (2.0 * R / C.viscosity);
+      gpar[0] = -Rebar * Re / C.viscosity;
+      // vdiff = |dv|
+      dvbx += vdbar * dvx / vdiff;
+      dvby += vdbar * dvy / vdiff;
+      dvbz += vdbar * dvz / vdiff;
+      // dv = por (fvel - vhat_p)
+      porbar += dvbx * ex + dvby * ey + dvbz * ez;
+      fvelbar = make_double3(por * dvbx, por * dvby, por * dvbz);
+      add3(g.vhat, p, -por * dvbx, -por * dvby, -por * dvbz);
+    }
+    // fvel = U/S, por = Pn/S
+    const double3 Ubar = make_double3(fvelbar.x * iS, fvelbar.y * iS, fvelbar.z * iS);
+    Sbar -= (fvelbar.x * fvel.x + fvelbar.y * fvel.y + fvelbar.z * fvel.z) * iS;
+    const double Pnbar = porbar * iS;
+    Sbar -= porbar * por * iS;
+    // back through the sums, pair by pair
+    double rpx = 0.0, rpy = 0.0, rpz = 0.0;
+    for (int64_t j = off[i]; j < off[i + 1]; j++) {
+      const int n = pb[j];
+      const double3 rn = imageOf(in.r, n, pimg[j], C.shift_len);
+      const double dx = rp.x - rn.x, dy = rp.y - rn.y, dz = rp.z - rn.z;
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      double W, dWdr, dWdh;
+      wendW(C.wcon, r, in.h[n], W, dWdr, dWdh);
+      if (W == 0.0 && dWdr == 0.0 && dWdh == 0.0) continue;
+      const double mn = in.mass[n], rhon = in.dens[n];
+      const double dvol = mn / rhon;
+      const double dvW = dvol * W, mW = mn * W;
+      const double3 vn = ld3(in.vhat, n), fpn = ld3(in.fpfb, n), fvn = ld3(in.fv, n);
+      const double porn = in.por[n];
+      const double dvWbar = Ubar.x * vn.x + Ubar.y * vn.y + Ubar.z * vn.z + Pnbar * porn + Sbar +
+                            Gbar.x * fvn.x + Gbar.y * fvn.y + Gbar.z * fvn.z;
+      const double mWbar = Gbar.x * (fpn.x + fvn.x) + Gbar.y * (fpn.y + fvn.y) +
+                           Gbar.z * (fpn.z + fvn.z);
+      add3(g.vhat, n, dvW * Ubar.x, dvW * Ubar.y, dvW * Ubar.z);
+      add1(g.por, n, dvW * Pnbar);
+      add3(g.fpfb, n, mW * Gbar.x, mW * Gbar.y, mW * Gbar.z);
+      add3(g.fv, n, (mW + dvW) * Gbar.x, (mW + dvW) * Gbar.y, (mW + dvW) * Gbar.z);
+      const double Wbar = dvWbar * dvol + mWbar * mn;
+      add1(g.dens, n, -dvWbar * W * mn / (rhon * rhon));
+      add1(g.h, n, Wbar * dWdh);
+      if (r > 0.0) {
+        const double c = Wbar * dWdr / r;
+        rpx += c * dx;
+        rpy += c * dy;
+        rpz += c * dz;
+        add3(g.r, n, -c * dx, -c * dy, -c * dz);
+      }
+    }
+    add3(g.r, p, rpx, rpy, rpz);
+  }
+  blockAdd<3>(gpar, g.params);
+}
+
+// -------------------------------------------------------------- DEM contacts
+// Pair (a, b): dx = r_a - r_b', r = |dx|, n = dx/r, overlap = 2R - r; when
+// overlap > 0: od = -(vhat_a - vhat_b).n, sN = k overlap + gamma od,
+// fc_a += sN n / m_a.
+__global__ void k_contacts_fwd(int64_t np, const int32_t* __restrict__ pa,
+                               const int32_t* __restrict__ pb, const int32_t* __restrict__ pimg,
+                               sphdem_adj_fields in, sphdem_adj_consts C,
+                               double* __restrict__ fc) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  const int a = pa[k], b = pb[k];
+  const double3 ra = ld3(in.r, a), rb = imageOf(in.r, b, pimg[k], C.shift_len);
+  const double dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
+  const double r = sqrt(dx * dx + dy * dy + dz * dz);
+  const double overlap = 2.0 * C.dem_radius - r;
+  if (!(overlap > 0.0) || !(r > 0.0)) return;
+  const double nx = dx / r, ny = dy / r, nz = dz / r;
+  const double3 va = ld3(in.vhat, a), vb = ld3(in.vhat, b);
+  const double od = -((va.x - vb.x) * nx + (va.y - vb.y) * ny + (va.z - vb.z) * nz);
+  const double s = (C.dem_k * overlap + C.dem_gamma * od) / in.mass[a];
+  add3(fc, a, s * nx, s * ny, s * nz);
+}
+
+__global__ void k_contacts_bwd(int64_t np, const int32_t* __restrict__ pa,
+                               const int32_t* __restrict__ pb, const int32_t* __restrict__ pimg,
+                               sphdem_adj_fields in, sphdem_adj_consts C,
+                               const double* __restrict__ gfc, sphdem_adj_grads g) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double gpar[2] = {0.0, 0.0};  // dem_k, dem_gamma
+  if (k < np) {
+    const int a = pa[k], b = pb[k];
+    const double3 ra = ld3(in.r, a), rb = imageOf(in.r, b, pimg[k], C.shift_len);
+    const double dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    const double overlap = 2.0 * C.dem_radius - r;
+    if (overlap > 0.0 && r > 0.0) {
+      const double nx = dx / r, ny = dy / r, nz = dz / r;
+      const double3 va = ld3(in.vhat, a), vb = ld3(in.vhat, b);
+      const double dvx = va.x - vb.x, dvy = va.y - vb.y, dvz = va.z - vb.z;
+      const double od = -(dvx * nx + dvy * ny + dvz * nz);
+      const double im = 1.0 / in.mass[a];
+      const double sN = C.dem_k * overlap + C.dem_gamma * od;
+      const double3 gf = ld3(gfc, a);
+      // fc = sN n / m
+      const double sNbar = (gf.x * nx + gf.y * ny + gf.z * nz) * im;
+      double nbx = sN * im * gf.x, nby = sN * im * gf.y, nbz = sN * im * gf.z;
+      gpar[0] = sNbar * overlap;
+      gpar[1] = sNbar * od;
+      const double odbar = C.dem_gamma * sNbar;
+      double rbar = -C.dem_k * sNbar;  // overlap = 2R - r
+      // od = -(dv . n)
+      add3(g.vhat, a, -odbar * nx, -odbar * ny, -odbar * nz);
+      add3(g.vhat, b, odbar * nx, odbar * ny, odbar * nz);
+      nbx -= odbar * dvx;
+      nby -= odbar * dvy;
+      nbz -= odbar * dvz;
+      // n = dx / r: d n / d dx = (I - n n^T) / r;  r = |dx|
+      const double nn = nbx * nx + nby * ny + nbz * nz;
+      const double gx = (nbx - nn * nx) / r + rbar * nx;
+      const double gy = (nby - nn * ny) / r + rbar * ny;
+      const double gz = (nbz - nn * nz) / r + rbar * nz;
+      add3(g.r, a, gx, gy, gz);
+      add3(g.r, b, -gx, -gy, -gz);
+    }
+  }
+  blockAdd<2>(gpar, g.params + 3);
+}
+
+inline unsigned gridFor(int64_t n) { return (unsigned)((n + kTPB - 1) / kTPB); }
+
+}  // namespace sphdem_adj
+
+// ---------------------------------------------------------------- C ABI
+namespace {
+int adjLaunched() { return hipGetLastError() == hipSuccess ? 0 : SPHDEM_E_HIP; }
+}  // namespace
+
+extern "C" {
+
+int sphdem_adj_force(int64_t npairs, const int32_t* a, const int32_t* b, const int32_t* img,
+                     const sphdem_adj_fields* in, const sphdem_adj_consts* c, double* fv,
+                     double* fp, void* stream) {
+  if (npairs < 0 || !in || !c || !fv || !fp) return SPHDEM_E_ARG;
+  if (npairs == 0) return 0;
+  if (!a || !b || !img || !in->r || !in->vhat || !in->h || !in->dens || !in->mass ||
+      !in->omega || !in->pdr2 || !in->iam)
+    return SPHDEM_E_ARG;
+  hipLaunchKernelGGL(sphdem_adj::k_force_fwd, dim3(sphdem_adj::gridFor(npairs)),
+                     dim3(sphdem_adj::kTPB), 0, (hipStream_t)stream, npairs, a, b, img, *in, *c,
+                     fv, fp);
+  return adjLaunched();
+}
+
+int sphdem_adj_force_bwd(int64_t npairs, const int32_t* a, const int32_t* b, const int32_t* img,
+                         const sphdem_adj_fields* in, const sphdem_adj_consts* c,
+                         const double* gfv, const double* gfp, const sphdem_adj_grads* g,
+                         void* stream) {
+  if (npairs < 0 || !in || !c || !gfv || !gfp || !g) return SPHDEM_E_ARG;
+  if (npairs == 0) return 0;
+  if (!a || !b || !img || !in->r || !in->vhat || !in->h || !in->dens || !in->mass ||
+      !in->omega || !in->pdr2 || !in->iam || !g->r || !g->vhat || !g->h || !g->dens ||
+      !g->omega || !g->pdr2 || !g->params)
+    return SPHDEM_E_ARG;
+  hipLaunchKernelGGL(sphdem_adj::k_force_bwd, dim3(sphdem_adj::gridFor(npairs)),
+                     dim3(sphdem_adj::kTPB), 0, (hipStream_t)stream, npairs, a, b, img, *in, *c,
+                     gfv, gfp, *g);
+  return adjLaunched();
+}
+
+int sphdem_adj_fpi(int32_t nquery, const int32_t* query, const int64_t* off, const int32_t* b,
+                   const int32_t* img, const sphdem_adj_fields* in, const sphdem_adj_consts* c,
+                   double* fdrag, void* stream) {
+  if (nquery < 0 || !in || !c || !fdrag) return SPHDEM_E_ARG;
+  if (nquery == 0) return 0;
+  if (!query || !off || !b || !img || !in->r || !in->vhat || !in->h || !in->dens ||
+      !in->mass || !in->fpfb || !in->fv || !in->por)
+    return SPHDEM_E_ARG;
+  hipLaunchKernelGGL(sphdem_adj::k_fpi_fwd, dim3(sphdem_adj::gridFor(nquery)),
+                     dim3(sphdem_adj::kTPB), 0, (hipStream_t)stream, nquery, query, off, b, img,
+                     *in, *c, fdrag);
+  return adjLaunched();
+}
+
+int sphdem_adj_fpi_bwd(int32_t nquery, const int32_t* query, const int64_t* off,
+                       const int32_t* b, const int32_t* img, const sphdem_adj_fields* in,
+                       const sphdem_adj_consts* c, const double* gfdrag,
+                       const sphdem_adj_grads* g, void* stream) {
+  if (nquery < 0 || !in || !c || !gfdrag || !g) return SPHDEM_E_ARG;
+  if (nquery == 0) return 0;
+  if (!query || !off || !b || !img || !in->r || !in->vhat || !in->h || !in->dens ||
+      !in->mass || !in->fpfb || !in->fv || !in->por || !g->r || !g->vhat || !g->h ||
+      !g->dens || !g->fpfb || !g->fv || !g->por || !g->params)
+    return SPHDEM_E_ARG;
+  hipLaunchKernelGGL(sphdem_adj::k_fpi_bwd, dim3(sphdem_adj::gridFor(nquery)),
+                     dim3(sphdem_adj::kTPB), 0, (hipStream_t)stream, nquery, query, off, b, img,
+                     *in, *c, gfdrag, *g);
+  return adjLaunched();
+}
+
+int sphdem_adj_contacts(int64_t npairs, const int32_t* a, const int32_t* b, const int32_t* img,
+                        const sphdem_adj_fields* in, const sphdem_adj_consts* c, double* fc,
+                        void* stream) {
+  if (npairs < 0 || !in || !c || !fc) return SPHDEM_E_ARG;
+  if (npairs == 0) return 0;
+  if (!a || !b || !img || !in->r || !in->vhat || !in->mass) return SPHDEM_E_ARG;
+  hipLaunchKernelGGL(sphdem_adj::k_contacts_fwd, dim3(sphdem_adj::gridFor(npairs)),
+                     dim3(sphdem_adj::kTPB), 0, (hipStream_t)stream, npairs, a, b, img, *in, *c,
+                     fc);
+  return adjLaunched();
+}
+
+int sphdem_adj_contacts_bwd(int64_t npairs, const int32_t* a, const int32_t* b,
+                            const int32_t* img, const sphdem_adj_fields* in,
+                            const sphdem_adj_consts* c, const double* gfc,
+                            const sphdem_adj_grads* g, void* stream) {
+  if (npairs < 0 || !in || !c || !gfc || !g) return SPHDEM_E_ARG;
+  if (npairs == 0) return 0;
+  if (!a || !b || !img || !in->r || !in->vhat || !in->mass || !g->r || !g->vhat || !g->params)
+    return SPHDEM_E_ARG;
+  hipLaunchKernelGGL(sphdem_adj::k_contacts_bwd, dim3(sphdem_adj::gridFor(npairs)),
+                     dim3(sphdem_adj::kTPB), 0, (hipStream_t)stream, npairs, a, b, img, *in, *c,
+                     gfc, *g);
+  return adjLaunched();
+}
+
+}  // extern "C"

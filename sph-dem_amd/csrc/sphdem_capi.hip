@@ -18,6 +18,7 @@
 #include "sphdem_kernels.hip"
 #include "sphdem_tiles.hip"
 #include "sphdem_mixed.hip"
+#include "sphdem_adjoint.hip"
 
 using namespace sphdem;
 

@@ -501,6 +501,42 @@ def sediment_case(nf: int = 64, n_dem: int | None = None, seed: int = 1001,
                    {"nf": nf, "nz": nz, "dx": dx, "seed": seed, "n_dem": n_dem})
 
 
+def adjoint_case(nf: int = 64, nz: int = 16, n_dem: int = 8192, seed: int = 1004,
+                 vscale: float = 0.05, touching: int = 0, floor: int = 0) -> Case:
+    """The adjoint workload: the sedimentation geometry (sediment_case: nf x nf x
+    nz fluid, dx = 1/nf, periodic in x and y) with n_dem DEM spheres, seeded
+    initial velocities uniform in [-vscale, vscale) per component on fluid and
+    DEM particles (walls at rest), the drop time moved before t = 0 so that
+    no particle is frozen, `touching` DEM pairs moved into contact (centre
+    distance 1.9 R) and `floor` DEM spheres moved onto the z = 0 floor
+    (centre at 0.8 R), so that the contact terms take part."""
+    c = sediment_case(nf=nf, n_dem=n_dem, seed=seed, nz=nz)
+    P = c.params
+    P.time_drop = -1.0
+    R = P.dem_radius
+    dem = np.where(c.iam == DEM)[0]
+    for k in range(touching):
+        a, b = dem[2 * k], dem[2 * k + 1]
+        c.r[b] = c.r[a] + np.array([1.9 * R * math.cos(0.3 * k), 1.9 * R * math.sin(0.3 * k), 0.0])
+    for k in range(floor):
+        c.r[dem[2 * touching + k], 2] = 0.8 * R
+    u = splitmix64(seed + 3, 3 * c.n).reshape(c.n, 3)
+    v = vscale * (2.0 * u - 1.0)
+    v[c.iam == SPH_BOUNDARY] = 0.0
+    c.v[:] = v
+    c.vhat[:] = v
+    c.name = "adjoint"
+    c.info.update({"vscale": vscale, "touching": touching, "floor": floor})
+    return c
+
+
+def adjoint_target(case: Case, seed: int = 1004, depth: float = 1e-4) -> np.ndarray:
+    """Seeded target profile of the DEM heights: each DEM's initial z lowered by
+    depth * U[0, 1)."""
+    z = case.r[case.iam == DEM, 2]
+    return z - depth * splitmix64(seed + 5, len(z))
+
+
 def ensemble_cases(nbox: int = 256, seed: int = 1005, nmin: int = 4096,
                    nmax: int = 65536) -> list:
     """Independent sedimentation boxes of uneven size: per box a fluid count N

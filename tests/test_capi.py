@@ -1,5 +1,5 @@
-"""The C-ABI library loads and exports every entry point include/sphdem_c.h
-declares (no GPU calls).  CPU only."""
+"""The C-ABI library loads and exports every entry point the headers under
+include/ declare (no GPU calls).  CPU only."""
 import ctypes
 import os
 import re
@@ -12,7 +12,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_symbols():
-    text = open(os.path.join(ROOT, "include", "sphdem_c.h")).read()
+    """Entry points of every header under include/."""
+    inc = os.path.join(ROOT, "include")
+    text = "".join(open(os.path.join(inc, f)).read() for f in sorted(os.listdir(inc))
+                   if f.endswith(".h"))
     return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(sphdem_\w+)\s*\(", text, re.M)))
 
 
