@@ -61,7 +61,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=200)
     p.add_argument("--warmup", type=int, default=10)
-    p.add_argument("--workload", choices=("sediment", "large", "ensemble"), default="sediment",
+    p.add_argument("--workload", choices=("sediment", "large", "ensemble", "adjoint"),
+                   default="sediment",
                    help="sediment: 64^3 sedimentation (the headline); large: the 160^3 "
                         "dam-break geometry with 65,536 DEM and the SPH-DEM flag set; "
                         "ensemble: 256 independent sedimentation boxes of 4,096-65,536 "
@@ -290,6 +291,164 @@ def ensemble(a):
     print(json.dumps(out), flush=True)
 
 
+# Algorithmic bytes of the backward SPH force kernel: per SPH-group particle
+# its inputs r vhat h dens mass omega pdr2 iam and output adjoints gfv gfp read
+# once (140 B) and its input adjoints r vhat h dens omega pdr2 written once
+# (80 B); per pair the (a, b, image) triple (12 B).
+B_ADJ_FORCE_BWD_PARTICLE = 220
+B_ADJ_PAIR = 12
+
+
+def adjoint(a):
+    """Gradient of a 32-step rollout (sd.adjoint_case: 64 x 64 x 16 fluid,
+    8,192 DEM) by the checkpointed reverse mode of sphdem_adjoint: the loss is
+    the mean squared DEM z against sd.adjoint_target; gradients with respect
+    to the drag coefficients, contact stiffness and damping, viscosity and the
+    initial velocities.  value = (fluid + DEM) * rollout steps / wall time of
+    one gradient (forward rollout + recomputation + backward)."""
+    import torch
+    import sphdem_adjoint as sa
+
+    steps = a.steps if a.steps != 200 else 32
+    nf = a.nf or 64
+    ndem = a.ndem or 8192
+    case = sd.adjoint_case(nf=nf, nz=16, n_dem=ndem)
+    counts = case.counts()
+    counted = counts["fluid"] + counts["dem"]
+    target_np = sd.adjoint_target(case)
+    target = torch.as_tensor(target_np, device="cuda")
+    pairs = sa.LibraryPairs(case)
+    model = sa.StepModel(case, device="cuda", pairs=pairs)
+    model.gradient(min(2, steps), target, every=8)                   # warm-up
+    torch.cuda.synchronize()
+    pairs.seconds = 0.0
+    sa.KERNEL_TIMES = {}
+    t0 = time.perf_counter()
+    loss, g = model.gradient(steps, target, every=8)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    kt = sa.kernel_ms()
+    sa.KERNEL_TIMES = None
+    search_s = pairs.seconds
+    peak_mem = torch.cuda.max_memory_allocated() / 2 ** 30
+    # the library's own forward over the same rollout, and finite differences of it
+    def prod_loss(P=None, v=None):
+        import dataclasses
+        c = dataclasses.replace(case, v=case.v.copy(), vhat=case.vhat.copy(),
+                                params=(P or case.params).copy())
+        if v is not None:
+            c.v[:] = v
+            c.vhat[:] = v
+        sim = sd.Sim(c)
+        sim.sync()
+        t = time.perf_counter()
+        sim.run(steps)
+        sim.sync()
+        dt = time.perf_counter() - t
+        z = sim.get("r")[case.iam == sd.DEM, 2]
+        sim.close()
+        return float(np.mean((z - target_np) ** 2)), dt
+    base, fwd_s = prod_loss()
+    fd = {}
+    for k in ("drag_a", "dem_k", "viscosity"):
+        b0 = getattr(case.params, k)
+        row = {"adjoint": g[k]}
+        for rel in (1e-6, 1e-3):
+            Pp, Pm = case.params.copy(), case.params.copy()
+            setattr(Pp, k, b0 * (1 + rel))
+            setattr(Pm, k, b0 * (1 - rel))
+            d = (prod_loss(Pp)[0] - prod_loss(Pm)[0]) / (2 * rel * abs(b0))
+            row[f"fd_rel_step_{rel:g}"] = d
+            row[f"rel_err_{rel:g}"] = abs(g[k] - d) / abs(d) if d else None
+        fd[k] = row
+    dirn = sd.splitmix64(7, case.n * 3).reshape(case.n, 3) - 0.5
+    dirn[case.iam == sd.SPH_BOUNDARY] = 0.0
+    adj_dir = float((g["v_init"].cpu().numpy() * dirn).sum())
+    eps = 1e-6
+    d = (prod_loss(v=case.v + eps * dirn)[0] - prod_loss(v=case.v - eps * dirn)[0]) / (2 * eps)
+    fd["v_init_direction"] = {"adjoint": adj_dir, "fd_step_1e-06": d,
+                              "rel_err_1e-06": abs(adj_dir - d) / abs(d) if d else None,
+                              "what": "all fluid and DEM velocities"}
+    dd = np.where((case.iam == sd.DEM)[:, None], dirn, 0.0)
+    adj_dd = float((g["v_init"].cpu().numpy() * dd).sum())
+    d = (prod_loss(v=case.v + eps * dd)[0] - prod_loss(v=case.v - eps * dd)[0]) / (2 * eps)
+    fd["v_init_dem_direction"] = {"adjoint": adj_dd, "fd_step_1e-06": d,
+                                  "rel_err_1e-06": abs(adj_dd - d) / abs(d) if d else None,
+                                  "what": "DEM velocities only"}
+    kernels = {k: {"ms_per_launch": ms / n, "launches": n, "items_per_launch": it / n}
+               for k, (ms, n, it) in kt.items()}
+    roof = None
+    if "force_bwd" in kt:
+        ms, n, it = kt["force_bwd"]
+        units = counts["fluid"] + counts["wall"]
+        b_launch = B_ADJ_FORCE_BWD_PARTICLE * units + B_ADJ_PAIR * it / n
+        ach = b_launch / (ms / n / 1e3) / 1e9
+        roof = {"bound": "hbm", "kernel": "adj_force_bwd", "achieved": ach, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                "mean_launch_ms": ms / n, "launches": n, "alg_bytes_per_launch": b_launch}
+    out = {
+        "metric": METRIC,
+        "value": counted * steps / el,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 1,
+        "ms_per_step": 1e3 * el / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"adjoint: sedimentation {nf}x{nf}x16 fluid + {ndem} DEM, {steps}-step "
+                        "rollout, loss = mean squared DEM z vs a seeded target, gradient wrt "
+                        "drag coefficients, contact stiffness/damping, viscosity and initial "
+                        "velocities, state checkpoint every 8 steps",
+            "fluid": counts["fluid"], "wall": counts["wall"], "dem": counts["dem"], "seed": 1004,
+        },
+        "loss": loss,
+        "loss_library_forward": base,
+        "library_forward_particle_steps_per_s": counted * steps / fwd_s,
+        "gradient_over_forward_time": el / fwd_s,
+        "gradient_seconds": el,
+        "pair_search_seconds": search_s,
+        "peak_device_memory_gib": peak_mem,
+        "grads": {k: g[k] for k in sa.PARAMS},
+        "fd_check": fd,
+        "kernels": kernels,
+        "roofline": roof,
+    }
+    pairs.close()
+    if not a.no_cpu:
+        out["cpu_baseline"] = adjoint_cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+def adjoint_cpu_baseline():
+    """The same differentiable step on the host (torch on the CPU, the torch
+    expressions of the pair passes, the oracle's pair searches): the gradient
+    of a 1-step rollout of a 32 x 32 x 16 box of the same geometry (the same
+    neighbour counts and DEM fraction per particle), timed."""
+    import torch
+    import sphdem_adjoint as sa
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from adjoint_ref import OraclePairs, TorchPasses
+
+    case = sd.adjoint_case(nf=32, nz=16, n_dem=2048)
+    target_np = sd.adjoint_target(case)
+    threads = torch.get_num_threads()
+    pairs = OraclePairs(case)
+    model = sa.StepModel(case, device="cpu", passes=TorchPasses(), pairs=pairs)
+    t0 = time.perf_counter()
+    model.gradient(1, torch.as_tensor(target_np), every=8)
+    el = time.perf_counter() - t0
+    pairs.close()
+    counted = int(np.sum((case.iam == sd.SPH) | (case.iam == sd.DEM)))
+    return {"value": counted / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": "gradient of a 1-step rollout of a 32x32x16 sedimentation box with 2048 "
+                      "DEM on the host (torch CPU model, oracle pair searches)"}
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -301,6 +460,9 @@ def main():
         sys.exit(2)
     if a.workload == "ensemble":
         ensemble(a)
+        return
+    if a.workload == "adjoint":
+        adjoint(a)
         return
     case = make_case(a)
     counts = case.counts()

@@ -225,6 +225,13 @@ int sphdem_get_cell_coords(sphdem_ctx* ctx, int32_t* ijk);
  * (13 = b itself).  With a == NULL only *count is returned. */
 int sphdem_get_pairs(sphdem_ctx* ctx, int kind, int64_t* count, int32_t* a,
                      int32_t* b, int32_t* image, int64_t capacity);
+/* sphdem_get_pairs into device memory of this context's device (a, b, image
+ * device pointers of capacity entries each), for callers that keep the pair
+ * set on the GPU (the adjoint's differentiable step); no counterpart in the
+ * reference, whose neighbour lists never leave the host.  Returns once the
+ * pairs are written. */
+int sphdem_get_pairs_device(sphdem_ctx* ctx, int kind, int64_t* count, int32_t* a,
+                            int32_t* b, int32_t* image, int64_t capacity);
 /* One operator pass over the current state (neighbour sets of the last reset). */
 int sphdem_pass_omega(sphdem_ctx* ctx);      /* neighboursGroup<calcOmega>        */
 int sphdem_pass_eos(sphdem_ctx* ctx);        /* traverse<calcPressSpsoundPdr2>     */

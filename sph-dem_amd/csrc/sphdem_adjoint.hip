@@ -1,6 +1,7 @@
 // sphdem_adjoint.hip -- forward and backward (vector-Jacobian product)
 // kernels of the pair passes of the differentiable SPH-DEM step: SPH force,
-// fluid-particle interaction with Di Felice drag, DEM contacts.
+// omega, dddt, the fluid-side coupling sums (porosity, Shepard sums,
+// DEM->SPH drag), fluid-particle interaction with Di Felice drag, DEM contacts.
 //
 // The forward kernels restate the reference operators over explicit pair sets
 // in original particle order (sphIncompress.h:899-972 with sph.h:579-666,
@@ -95,119 +96,457 @@ __device__ void blockAdd(double (&v)[NV], double* dst) {
 }
 
 // ---------------------------------------------------------------- SPH force
-// Pair (a, b), dx = r_a - r_b', r = |dx|, Fa = F(r, h_a), Fb = F(r, h_b), ia = 1/omega_a:
+// Query a over its pairs (a, b), dx = r_a - r_b', r = |dx|, Fa = F(r, h_a),
+// Fb = F(r, h_b), ia = 1/omega_a:
 //   viscous (a SPH):  A  = m_b * visc * d / den,  fv_a += A (vhat_a - vhat_b)
 //     visc = 2 nu rho0 / (rho_a rho_b),  d = r^2 (Fa + Fb) ia / 2,
 //     den = r^2 + 0.01 hav^2,  hav = (h_a + h_b)/2
 //   pressure:         B  = -m_b (pdr2_a Fa + pdr2_b Fb) ia,  fp_a += B dx
-__global__ void k_force_fwd(int64_t np, const int32_t* __restrict__ pa,
-                            const int32_t* __restrict__ pb, const int32_t* __restrict__ pimg,
-                            sphdem_adj_fields in, sphdem_adj_consts C, double* __restrict__ fv,
+// One thread per query particle: its sums stay in registers.
+__global__ void k_force_fwd(int32_t nq, const int32_t* __restrict__ query,
+                            const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
+                            const int32_t* __restrict__ pimg, sphdem_adj_fields in,
+                            sphdem_adj_consts C, double* __restrict__ fv,
                             double* __restrict__ fp) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= np) return;
-  const int a = pa[k], b = pb[k];
-  const double3 ra = ld3(in.r, a), rb = imageOf(in.r, b, pimg[k], C.shift_len);
-  const double dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
-  const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
-  const double ha = in.h[a], hb = in.h[b];
-  double Fa, Fb, t0, t1;
-  wendF(C.wcon, r, ha, Fa, t0, t1);
-  wendF(C.wcon, r, hb, Fb, t0, t1);
-  const double ia = 1.0 / in.omega[a];
-  const double mb = in.mass[b];
-  if (in.iam[a] == SPH_T) {
-    const double visc = 2.0 * C.viscosity * C.dens / (in.dens[a] * in.dens[b]);
-    const double d = r2 * (Fa + Fb) * 0.5 * ia;
-    const double hav = 0.5 * (ha + hb);
-    const double den = r2 + 0.01 * hav * hav;
-    const double A = mb * visc * d / den;
-    const double3 va = ld3(in.vhat, a), vb = ld3(in.vhat, b);
-    add3(fv, a, A * (va.x - vb.x), A * (va.y - vb.y), A * (va.z - vb.z));
-  }
-  const double B = -mb * (in.pdr2[a] * Fa + in.pdr2[b] * Fb) * ia;
-  add3(fp, a, B * dx, B * dy, B * dz);
-}
-
-__global__ void k_force_bwd(int64_t np, const int32_t* __restrict__ pa,
-                            const int32_t* __restrict__ pb, const int32_t* __restrict__ pimg,
-                            sphdem_adj_fields in, sphdem_adj_consts C,
-                            const double* __restrict__ gfv, const double* __restrict__ gfp,
-                            sphdem_adj_grads g) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double gnu[1] = {0.0};
-  if (k < np) {
-    const int a = pa[k], b = pb[k];
-    const double3 ra = ld3(in.r, a), rb = imageOf(in.r, b, pimg[k], C.shift_len);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const int a = query[i];
+  const double3 ra = ld3(in.r, a), va = ld3(in.vhat, a);
+  const double ha = in.h[a], ia = 1.0 / in.omega[a], rhoa = in.dens[a], pda = in.pdr2[a];
+  const bool sphA = in.iam[a] == SPH_T;
+  double vx = 0.0, vy = 0.0, vz = 0.0, px = 0.0, py = 0.0, pz = 0.0;
+  for (int64_t j = off[i]; j < off[i + 1]; j++) {
+    const int b = pb[j];
+    const double3 rb = imageOf(in.r, b, pimg[j], C.shift_len);
     const double dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
     const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
-    const double ha = in.h[a], hb = in.h[b];
-    double Fa, dFa_r, dFa_h, Fb, dFb_r, dFb_h;
-    wendF(C.wcon, r, ha, Fa, dFa_r, dFa_h);
-    wendF(C.wcon, r, hb, Fb, dFb_r, dFb_h);
-    const double ia = 1.0 / in.omega[a];
+    const double hb = in.h[b];
+    double Fa, Fb, t0, t1;
+    wendF(C.wcon, r, ha, Fa, t0, t1);
+    wendF(C.wcon, r, hb, Fb, t0, t1);
     const double mb = in.mass[b];
-    double r2bar = 0.0, Fabar = 0.0, Fbbar = 0.0, iabar = 0.0, habar = 0.0, hbbar = 0.0;
-    if (in.iam[a] == SPH_T) {
-      const double rhoa = in.dens[a], rhob = in.dens[b];
-      const double visc = 2.0 * C.viscosity * C.dens / (rhoa * rhob);
-      const double Fs = Fa + Fb;
-      const double d = r2 * Fs * 0.5 * ia;
+    if (sphA) {
+      const double visc = 2.0 * C.viscosity * C.dens / (rhoa * in.dens[b]);
+      const double d = r2 * (Fa + Fb) * 0.5 * ia;
       const double hav = 0.5 * (ha + hb);
       const double den = r2 + 0.01 * hav * hav;
       const double A = mb * visc * d / den;
-      const double3 va = ld3(in.vhat, a), vb = ld3(in.vhat, b);
-      const double3 gv = ld3(gfv, a);
-      const double dvx = va.x - vb.x, dvy = va.y - vb.y, dvz = va.z - vb.z;
-      // fv = A dv
-      const double Abar = gv.x * dvx + gv.y * dvy + gv.z * dvz;
-      add3(g.vhat, a, A * gv.x, A * gv.y, A * gv.z);
-      add3(g.vhat, b, -A * gv.x, -A * gv.y, -A * gv.z);
-      // A = m_b visc d / den
-      const double viscbar = Abar * mb * d / den;
-      const double dbar = Abar * mb * visc / den;
-      const double denbar = -Abar * A / den;
-      // visc = 2 nu rho0 / (rho_a rho_b)
-      gnu[0] = viscbar * visc / C.viscosity;
-      add1(g.dens, a, -viscbar * visc / rhoa);
-      add1(g.dens, b, -viscbar * visc / rhob);
-      // d = r^2 Fs ia / 2
-      r2bar += dbar * Fs * 0.5 * ia;
-      const double Fsbar = dbar * r2 * 0.5 * ia;
-      iabar += dbar * r2 * Fs * 0.5;
-      Fabar += Fsbar;
-      Fbbar += Fsbar;
-      // den = r^2 + 0.01 hav^2
-      r2bar += denbar;
-      const double havbar = denbar * 0.02 * hav;
-      habar += 0.5 * havbar;
-      hbbar += 0.5 * havbar;
+      const double3 vb = ld3(in.vhat, b);
+      vx += A * (va.x - vb.x);
+      vy += A * (va.y - vb.y);
+      vz += A * (va.z - vb.z);
     }
-    // fp = B dx, B = -m_b (pdr2_a Fa + pdr2_b Fb) ia
-    const double pda = in.pdr2[a], pdb = in.pdr2[b];
-    const double B = -mb * (pda * Fa + pdb * Fb) * ia;
-    const double3 gp = ld3(gfp, a);
-    const double Bbar = gp.x * dx + gp.y * dy + gp.z * dz;
-    add1(g.pdr2, a, -mb * Fa * ia * Bbar);
-    add1(g.pdr2, b, -mb * Fb * ia * Bbar);
-    Fabar += -mb * pda * ia * Bbar;
-    Fbbar += -mb * pdb * ia * Bbar;
-    iabar += -mb * (pda * Fa + pdb * Fb) * Bbar;
-    // F(r, h)
-    const double rbar = Fabar * dFa_r + Fbbar * dFb_r;
-    habar += Fabar * dFa_h;
-    hbbar += Fbbar * dFb_h;
-    add1(g.h, a, habar);
-    add1(g.h, b, hbbar);
-    // ia = 1/omega_a
-    add1(g.omega, a, -iabar * ia * ia);
-    // dx: direct term B gp, through r^2 (2 dx) and through r (dx / r)
-    const double s = 2.0 * r2bar + (r > 0.0 ? rbar / r : 0.0);
-    const double gx = B * gp.x + s * dx, gy = B * gp.y + s * dy, gz = B * gp.z + s * dz;
-    add3(g.r, a, gx, gy, gz);
-    add3(g.r, b, -gx, -gy, -gz);
+    const double B = -mb * (pda * Fa + in.pdr2[b] * Fb) * ia;
+    px += B * dx;
+    py += B * dy;
+    pz += B * dz;
+  }
+  fv[3 * a] += vx;
+  fv[3 * a + 1] += vy;
+  fv[3 * a + 2] += vz;
+  fp[3 * a] += px;
+  fp[3 * a + 1] += py;
+  fp[3 * a + 2] += pz;
+}
+
+__global__ void k_force_bwd(int32_t nq, const int32_t* __restrict__ query,
+                            const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
+                            const int32_t* __restrict__ pimg, sphdem_adj_fields in,
+                            sphdem_adj_consts C, const double* __restrict__ gfv,
+                            const double* __restrict__ gfp, sphdem_adj_grads g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double gnu[1] = {0.0};
+  if (i < nq) {
+    const int a = query[i];
+    const double3 ra = ld3(in.r, a), va = ld3(in.vhat, a);
+    const double ha = in.h[a], ia = 1.0 / in.omega[a], rhoa = in.dens[a], pda = in.pdr2[a];
+    const bool sphA = in.iam[a] == SPH_T;
+    const double3 gv = ld3(gfv, a), gp = ld3(gfp, a);
+    // adjoints of a's inputs, summed over its pairs
+    double grx = 0.0, gry = 0.0, grz = 0.0, gvx = 0.0, gvy = 0.0, gvz = 0.0;
+    double gha = 0.0, gda = 0.0, gia = 0.0, gpa = 0.0;
+    for (int64_t j = off[i]; j < off[i + 1]; j++) {
+      const int b = pb[j];
+      const double3 rb = imageOf(in.r, b, pimg[j], C.shift_len);
+      const double dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
+      const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
+      const double hb = in.h[b];
+      double Fa, dFa_r, dFa_h, Fb, dFb_r, dFb_h;
+      wendF(C.wcon, r, ha, Fa, dFa_r, dFa_h);
+      wendF(C.wcon, r, hb, Fb, dFb_r, dFb_h);
+      const double mb = in.mass[b];
+      double r2bar = 0.0, Fabar = 0.0, Fbbar = 0.0, hbbar = 0.0;
+      if (sphA) {
+        const double rhob = in.dens[b];
+        const double visc = 2.0 * C.viscosity * C.dens / (rhoa * rhob);
+        const double Fs = Fa + Fb;
+        const double d = r2 * Fs * 0.5 * ia;
+        const double hav = 0.5 * (ha + hb);
+        const double den = r2 + 0.01 * hav * hav;
+        const double A = mb * visc * d / den;
+        const double3 vb = ld3(in.vhat, b);
+        const double dvx = va.x - vb.x, dvy = va.y - vb.y, dvz = va.z - vb.z;
+        // fv = A dv
+        const double Abar = gv.x * dvx + gv.y * dvy + gv.z * dvz;
+        gvx += A * gv.x;
+        gvy += A * gv.y;
+        gvz += A * gv.z;
+        add3(g.vhat, b, -A * gv.x, -A * gv.y, -A * gv.z);
+        // A = m_b visc d / den
+        const double viscbar = Abar * mb * d / den;
+        const double dbar = Abar * mb * visc / den;
+        const double denbar = -Abar * A / den;
+        // visc = 2 nu rho0 / (rho_a rho_b)
+        gnu[0] += viscbar * visc / C.viscosity;
+        gda += -viscbar * visc / rhoa;
+        add1(g.dens, b, -viscbar * visc / rhob);
+        // d = r^2 Fs ia / 2
+        r2bar += dbar * Fs * 0.5 * ia;
+        const double Fsbar = dbar * r2 * 0.5 * ia;
+        gia += dbar * r2 * Fs * 0.5;
+        Fabar += Fsbar;
+        Fbbar += Fsbar;
+        // den = r^2 + 0.01 hav^2
+        r2bar += denbar;
+        const double havbar = denbar * 0.02 * hav;
+        gha += 0.5 * havbar;
+        hbbar += 0.5 * havbar;
+      }
+      // fp = B dx, B = -m_b (pdr2_a Fa + pdr2_b Fb) ia
+      const double pdb = in.pdr2[b];
+      const double B = -mb * (pda * Fa + pdb * Fb) * ia;
+      const double Bbar = gp.x * dx + gp.y * dy + gp.z * dz;
+      gpa += -mb * Fa * ia * Bbar;
+      add1(g.pdr2, b, -mb * Fb * ia * Bbar);
+      Fabar += -mb * pda * ia * Bbar;
+      Fbbar += -mb * pdb * ia * Bbar;
+      gia += -mb * (pda * Fa + pdb * Fb) * Bbar;
+      // F(r, h)
+      const double rbar = Fabar * dFa_r + Fbbar * dFb_r;
+      gha += Fabar * dFa_h;
+      hbbar += Fbbar * dFb_h;
+      add1(g.h, b, hbbar);
+      // dx: direct term B gp, through r^2 (2 dx) and through r (dx / r)
+      const double s = 2.0 * r2bar + (r > 0.0 ? rbar / r : 0.0);
+      const double gx = B * gp.x + s * dx, gy = B * gp.y + s * dy, gz = B * gp.z + s * dz;
+      grx += gx;
+      gry += gy;
+      grz += gz;
+      add3(g.r, b, -gx, -gy, -gz);
+    }
+    add3(g.r, a, grx, gry, grz);
+    add3(g.vhat, a, gvx, gvy, gvz);
+    add1(g.h, a, gha);
+    add1(g.dens, a, gda);
+    add1(g.omega, a, -gia * ia * ia);  // ia = 1/omega_a
+    add1(g.pdr2, a, gpa);
   }
   blockAdd<1>(gnu, g.params);
+}
+
+// ---------------------------------------------------------------- omega
+// Nsph::calcOmega (sph.h:437-452): omega_a = 1 + h_a S_a / (3 rho_a),
+// S_a = m_a P(0, h_a) + sum_b m_b P(r, h_a), P = dW/dh = G(q)/h^4 with
+//   G = 10 c q^2 t^3 - 3 K,  G' = 50 c q t^3 - 30 c q^2 t^2,
+//   dP/dr = G'/h^5,  dP/dh = -(q G' + 4 G)/h^5.
+__device__ __forceinline__ void wendP(double c, double r, double h, double& P, double& dPdr,
+                                      double& dPdh) {
+  const double q = r / h;
+  if (q <= 2.0) {
+    const double t = 2.0 - q, ih = 1.0 / h, ih4 = (ih * ih) * (ih * ih), ih5 = ih4 * ih;
+    const double t2 = t * t, t3 = t2 * t;
+    const double G = 10.0 * c * q * q * t3 - 3.0 * c * t3 * t * (1.0 + 2.0 * q);
+    const double Gp = 50.0 * c * q * t3 - 30.0 * c * q * q * t2;
+    P = G * ih4;
+    dPdr = Gp * ih5;
+    dPdh = -(q * Gp + 4.0 * G) * ih5;
+  } else {
+    P = dPdr = dPdh = 0.0;
+  }
+}
+
+__global__ void k_omega_fwd(int32_t nq, const int32_t* __restrict__ query,
+                            const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
+                            const int32_t* __restrict__ pimg, sphdem_adj_fields in,
+                            sphdem_adj_consts C, double* __restrict__ omega) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const int a = query[i];
+  const double3 ra = ld3(in.r, a);
+  const double ha = in.h[a];
+  double P, t0, t1;
+  wendP(C.wcon, 0.0, ha, P, t0, t1);
+  double S = in.mass[a] * P;
+  for (int64_t j = off[i]; j < off[i + 1]; j++) {
+    const int b = pb[j];
+    const double3 rb = imageOf(in.r, b, pimg[j], C.shift_len);
+    const double dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
+    wendP(C.wcon, sqrt(dx * dx + dy * dy + dz * dz), ha, P, t0, t1);
+    S += in.mass[b] * P;
+  }
+  omega[a] += ha * S / (in.dens[a] * 3.0);
+}
+
+__global__ void k_omega_bwd(int32_t nq, const int32_t* __restrict__ query,
+                            const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
+                            const int32_t* __restrict__ pimg, sphdem_adj_fields in,
+                            sphdem_adj_consts C, const double* __restrict__ gomega,
+                            sphdem_adj_grads g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const int a = query[i];
+  const double go = gomega[a];
+  if (go == 0.0) return;
+  const double3 ra = ld3(in.r, a);
+  const double ha = in.h[a], rhoa = in.dens[a];
+  double P0, dP0r, dP0h, P, dPr, dPh;
+  wendP(C.wcon, 0.0, ha, P0, dP0r, dP0h);
+  double S = in.mass[a] * P0;
+  for (int64_t j = off[i]; j < off[i + 1]; j++) {
+    const int b = pb[j];
+    const double3 rb = imageOf(in.r, b, pimg[j], C.shift_len);
+    const double dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
+    wendP(C.wcon, sqrt(dx * dx + dy * dy + dz * dz), ha, P, dPr, dPh);
+    S += in.mass[b] * P;
+  }
+  const double Sbar = go * ha / (3.0 * rhoa);
+  double gha = go * S / (3.0 * rhoa) + Sbar * in.mass[a] * dP0h;
+  add1(g.dens, a, -go * ha * S / (3.0 * rhoa * rhoa));
+  double grx = 0.0, gry = 0.0, grz = 0.0;
+  for (int64_t j = off[i]; j < off[i + 1]; j++) {
+    const int b = pb[j];
+    const double3 rb = imageOf(in.r, b, pimg[j], C.shift_len);
+    const double dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    wendP(C.wcon, r, ha, P, dPr, dPh);
+    const double c = Sbar * in.mass[b];
+    gha += c * dPh;
+    if (r > 0.0) {
+      const double k = c * dPr / r;
+      grx += k * dx;
+      gry += k * dy;
+      grz += k * dz;
+      add3(g.r, b, -k * dx, -k * dy, -k * dz);
+    }
+  }
+  add3(g.r, a, grx, gry, grz);
+  add1(g.h, a, gha);
+}
+
+// ---------------------------------------------------------------- dddt
+// CsphIncompress::calcDddtDudt (sphIncompress.h:783-851) with gradW on h_a
+// and omega_a: dddt_a = sum_b m_b (vhat_a - vhat_b).dx F(r, h_a) / omega_a.
+__global__ void k_dddt_fwd(int32_t nq, const int32_t* __restrict__ query,
+                           const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
+                           const int32_t* __restrict__ pimg, sphdem_adj_fields in,
+                           sphdem_adj_consts C, double* __restrict__ dddt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const int a = query[i];
+  const double3 ra = ld3(in.r, a), va = ld3(in.vhat, a);
+  const double ha = in.h[a], ia = 1.0 / in.omega[a];
+  double sum = 0.0;
+  for (int64_t j = off[i]; j < off[i + 1]; j++) {
+    const int b = pb[j];
+    const double3 rb = imageOf(in.r, b, pimg[j], C.shift_len);
+    const double dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    double F, t0, t1;
+    wendF(C.wcon, r, ha, F, t0, t1);
+    const double3 vb = ld3(in.vhat, b);
+    const double u = (va.x - vb.x) * dx + (va.y - vb.y) * dy + (va.z - vb.z) * dz;
+    sum += in.mass[b] * u * F * ia;
+  }
+  dddt[a] += sum;
+}
+
+__global__ void k_dddt_bwd(int32_t nq, const int32_t* __restrict__ query,
+                           const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
+                           const int32_t* __restrict__ pimg, sphdem_adj_fields in,
+                           sphdem_adj_consts C, const double* __restrict__ gdddt,
+                           sphdem_adj_grads g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const int a = query[i];
+  const double gd = gdddt[a];
+  if (gd == 0.0) return;
+  const double3 ra = ld3(in.r, a), va = ld3(in.vhat, a);
+  const double ha = in.h[a], ia = 1.0 / in.omega[a];
+  double grx = 0.0, gry = 0.0, grz = 0.0, gvx = 0.0, gvy = 0.0, gvz = 0.0, gha = 0.0, gia = 0.0;
+  for (int64_t j = off[i]; j < off[i + 1]; j++) {
+    const int b = pb[j];
+    const double3 rb = imageOf(in.r, b, pimg[j], C.shift_len);
+    const double dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    double F, dFr, dFh;
+    wendF(C.wcon, r, ha, F, dFr, dFh);
+    const double3 vb = ld3(in.vhat, b);
+    const double dvx = va.x - vb.x, dvy = va.y - vb.y, dvz = va.z - vb.z;
+    const double u = dvx * dx + dvy * dy + dvz * dz;
+    const double mb = in.mass[b];
+    // term = m_b u F ia
+    const double ubar = gd * mb * F * ia;
+    const double Fbar = gd * mb * u * ia;
+    gia += gd * mb * u * F;
+    gha += Fbar * dFh;
+    // u = dv . dx
+    gvx += ubar * dx;
+    gvy += ubar * dy;
+    gvz += ubar * dz;
+    add3(g.vhat, b, -ubar * dx, -ubar * dy, -ubar * dz);
+    const double k = r > 0.0 ? Fbar * dFr / r : 0.0;
+    const double gx = ubar * dvx + k * dx, gy = ubar * dvy + k * dy, gz = ubar * dvz + k * dz;
+    grx += gx;
+    gry += gy;
+    grz += gz;
+    add3(g.r, b, -gx, -gy, -gz);
+  }
+  add3(g.r, a, grx, gry, grz);
+  add3(g.vhat, a, gvx, gvy, gvz);
+  add1(g.h, a, gha);
+  add1(g.omega, a, -gia * ia * ia);
+}
+
+// ------------------------------------------------------ fluid-side coupling
+// For fluid query a over its DEM partners d (the SPH->DEM coupling set),
+// W = W(r, Rc/2) (calcPorosityIncr sph.h:1246-1256):
+//   por_sum_a = sum_d POROSITY_INCR W                   (calcPorosity)
+//   shep_d   += (m_a / rho_a) W                         (calcShepSum)
+//   drag_a    = -sum_d m_d W fdrag_d / (rho_a shep_d)   (calcDEMtoSPHDrag)
+// sph.h:1303-1362.  The Shepard sums need every fluid partner of d before
+// the drag, so the forward is two launches, and so is the backward (the
+// adjoint of shep_d is complete only after the first).
+__global__ void k_cpl_fwd1(int32_t nq, const int32_t* __restrict__ query,
+                           const int64_t* __restrict__ off, const int32_t* __restrict__ pd,
+                           const int32_t* __restrict__ pimg, sphdem_adj_fields in,
+                           sphdem_adj_consts C, double* __restrict__ por_sum,
+                           double* __restrict__ shep) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const int a = query[i];
+  const double3 ra = ld3(in.r, a);
+  const double va = in.mass[a] / in.dens[a], hc = 0.5 * C.coupling_radius;
+  double por = 0.0;
+  for (int64_t j = off[i]; j < off[i + 1]; j++) {
+    const int d = pd[j];
+    const double3 rd = imageOf(in.r, d, pimg[j], C.shift_len);
+    const double dx = ra.x - rd.x, dy = ra.y - rd.y, dz = ra.z - rd.z;
+    double W, t0, t1;
+    wendW(C.wcon, sqrt(dx * dx + dy * dy + dz * dz), hc, W, t0, t1);
+    por += C.porosity_incr * W;
+    if (W != 0.0) add1(shep, d, va * W);
+  }
+  por_sum[a] += por;
+}
+
+__global__ void k_cpl_fwd2(int32_t nq, const int32_t* __restrict__ query,
+                           const int64_t* __restrict__ off, const int32_t* __restrict__ pd,
+                           const int32_t* __restrict__ pimg, sphdem_adj_fields in,
+                           sphdem_adj_consts C, double* __restrict__ drag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const int a = query[i];
+  const double3 ra = ld3(in.r, a);
+  const double rhoa = in.dens[a], hc = 0.5 * C.coupling_radius;
+  double x = 0.0, y = 0.0, z = 0.0;
+  for (int64_t j = off[i]; j < off[i + 1]; j++) {
+    const int d = pd[j];
+    const double sd = in.shep[d];
+    if (!(sd > 0.0)) continue;
+    const double3 rd = imageOf(in.r, d, pimg[j], C.shift_len);
+    const double dx = ra.x - rd.x, dy = ra.y - rd.y, dz = ra.z - rd.z;
+    double W, t0, t1;
+    wendW(C.wcon, sqrt(dx * dx + dy * dy + dz * dz), hc, W, t0, t1);
+    const double c = in.mass[d] * W / (rhoa * sd);
+    const double3 fd = ld3(in.fdrag, d);
+    x -= c * fd.x;
+    y -= c * fd.y;
+    z -= c * fd.z;
+  }
+  drag[3 * a] += x;
+  drag[3 * a + 1] += y;
+  drag[3 * a + 2] += z;
+}
+
+// first backward launch: the drag and porosity terms; shepbar[d] collects the
+// adjoint of the Shepard sums
+__global__ void k_cpl_bwd1(int32_t nq, const int32_t* __restrict__ query,
+                           const int64_t* __restrict__ off, const int32_t* __restrict__ pd,
+                           const int32_t* __restrict__ pimg, sphdem_adj_fields in,
+                           sphdem_adj_consts C, const double* __restrict__ gpor,
+                           const double* __restrict__ gdrag, double* __restrict__ shepbar,
+                           sphdem_adj_grads g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const int a = query[i];
+  const double3 ra = ld3(in.r, a), gdr = ld3(gdrag, a);
+  const double gp = gpor[a];
+  const double rhoa = in.dens[a], hc = 0.5 * C.coupling_radius;
+  double grx = 0.0, gry = 0.0, grz = 0.0, gda = 0.0;
+  for (int64_t j = off[i]; j < off[i + 1]; j++) {
+    const int d = pd[j];
+    const double3 rd = imageOf(in.r, d, pimg[j], C.shift_len);
+    const double dx = ra.x - rd.x, dy = ra.y - rd.y, dz = ra.z - rd.z;
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    double W, dWr, t1;
+    wendW(C.wcon, r, hc, W, dWr, t1);
+    double Wbar = gp * C.porosity_incr;
+    const double sd = in.shep[d];
+    if (sd > 0.0) {
+      const double md = in.mass[d];
+      const double c = md * W / (rhoa * sd);
+      const double3 fd = ld3(in.fdrag, d);
+      const double cbar = -(gdr.x * fd.x + gdr.y * fd.y + gdr.z * fd.z);
+      add3(g.fdrag, d, -c * gdr.x, -c * gdr.y, -c * gdr.z);
+      if (cbar != 0.0) add1(shepbar, d, -cbar * c / sd);
+      gda += -cbar * c / rhoa;
+      Wbar += cbar * md / (rhoa * sd);
+    }
+    if (r > 0.0 && Wbar != 0.0) {
+      const double k = Wbar * dWr / r;
+      grx += k * dx;
+      gry += k * dy;
+      grz += k * dz;
+      add3(g.r, d, -k * dx, -k * dy, -k * dz);
+    }
+  }
+  add3(g.r, a, grx, gry, grz);
+  add1(g.dens, a, gda);
+}
+
+// second backward launch: the Shepard sums
+__global__ void k_cpl_bwd2(int32_t nq, const int32_t* __restrict__ query,
+                           const int64_t* __restrict__ off, const int32_t* __restrict__ pd,
+                           const int32_t* __restrict__ pimg, sphdem_adj_fields in,
+                           sphdem_adj_consts C, const double* __restrict__ shepbar,
+                           sphdem_adj_grads g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const int a = query[i];
+  const double3 ra = ld3(in.r, a);
+  const double rhoa = in.dens[a], va = in.mass[a] / rhoa, hc = 0.5 * C.coupling_radius;
+  double grx = 0.0, gry = 0.0, grz = 0.0, gda = 0.0;
+  for (int64_t j = off[i]; j < off[i + 1]; j++) {
+    const int d = pd[j];
+    const double sb = shepbar[d];
+    if (sb == 0.0) continue;
+    const double3 rd = imageOf(in.r, d, pimg[j], C.shift_len);
+    const double dx = ra.x - rd.x, dy = ra.y - rd.y, dz = ra.z - rd.z;
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    double W, dWr, t1;
+    wendW(C.wcon, r, hc, W, dWr, t1);
+    // shep_d += (m_a / rho_a) W
+    gda += -sb * va * W / rhoa;
+    if (r > 0.0) {
+      const double k = sb * va * dWr / r;
+      grx += k * dx;
+      gry += k * dy;
+      grz += k * dz;
+      add3(g.r, d, -k * dx, -k * dy, -k * dz);
+    }
+  }
+  add3(g.r, a, grx, gry, grz);
+  add1(g.dens, a, gda);
 }
 
 // ------------------------------------------------ fluid-particle interaction
@@ -480,33 +819,126 @@ int adjLaunched() { return hipGetLastError() == hipSuccess ? 0 : SPHDEM_E_HIP; }
 
 extern "C" {
 
-int sphdem_adj_force(int64_t npairs, const int32_t* a, const int32_t* b, const int32_t* img,
-                     const sphdem_adj_fields* in, const sphdem_adj_consts* c, double* fv,
-                     double* fp, void* stream) {
-  if (npairs < 0 || !in || !c || !fv || !fp) return SPHDEM_E_ARG;
-  if (npairs == 0) return 0;
-  if (!a || !b || !img || !in->r || !in->vhat || !in->h || !in->dens || !in->mass ||
-      !in->omega || !in->pdr2 || !in->iam)
+#define SPHDEM_ADJ_CSR_OK(in)                                                       \
+  (query && off && b && img && (in) && (in)->r && (in)->mass)
+
+int sphdem_adj_force(int32_t nquery, const int32_t* query, const int64_t* off, const int32_t* b,
+                     const int32_t* img, const sphdem_adj_fields* in, const sphdem_adj_consts* c,
+                     double* fv, double* fp, void* stream) {
+  if (nquery < 0 || !in || !c || !fv || !fp) return SPHDEM_E_ARG;
+  if (nquery == 0) return 0;
+  if (!SPHDEM_ADJ_CSR_OK(in) || !in->h || !in->vhat || !in->dens || !in->omega || !in->pdr2 ||
+      !in->iam)
     return SPHDEM_E_ARG;
-  hipLaunchKernelGGL(sphdem_adj::k_force_fwd, dim3(sphdem_adj::gridFor(npairs)),
-                     dim3(sphdem_adj::kTPB), 0, (hipStream_t)stream, npairs, a, b, img, *in, *c,
-                     fv, fp);
+  hipLaunchKernelGGL(sphdem_adj::k_force_fwd, dim3(sphdem_adj::gridFor(nquery)),
+                     dim3(sphdem_adj::kTPB), 0, (hipStream_t)stream, nquery, query, off, b, img,
+                     *in, *c, fv, fp);
   return adjLaunched();
 }
 
-int sphdem_adj_force_bwd(int64_t npairs, const int32_t* a, const int32_t* b, const int32_t* img,
-                         const sphdem_adj_fields* in, const sphdem_adj_consts* c,
-                         const double* gfv, const double* gfp, const sphdem_adj_grads* g,
-                         void* stream) {
-  if (npairs < 0 || !in || !c || !gfv || !gfp || !g) return SPHDEM_E_ARG;
-  if (npairs == 0) return 0;
-  if (!a || !b || !img || !in->r || !in->vhat || !in->h || !in->dens || !in->mass ||
-      !in->omega || !in->pdr2 || !in->iam || !g->r || !g->vhat || !g->h || !g->dens ||
-      !g->omega || !g->pdr2 || !g->params)
+int sphdem_adj_force_bwd(int32_t nquery, const int32_t* query, const int64_t* off,
+                         const int32_t* b, const int32_t* img, const sphdem_adj_fields* in,
+                         const sphdem_adj_consts* c, const double* gfv, const double* gfp,
+                         const sphdem_adj_grads* g, void* stream) {
+  if (nquery < 0 || !in || !c || !gfv || !gfp || !g) return SPHDEM_E_ARG;
+  if (nquery == 0) return 0;
+  if (!SPHDEM_ADJ_CSR_OK(in) || !in->h || !in->vhat || !in->dens || !in->omega || !in->pdr2 ||
+      !in->iam || !g->r || !g->vhat || !g->h || !g->dens || !g->omega || !g->pdr2 || !g->params)
     return SPHDEM_E_ARG;
-  hipLaunchKernelGGL(sphdem_adj::k_force_bwd, dim3(sphdem_adj::gridFor(npairs)),
-                     dim3(sphdem_adj::kTPB), 0, (hipStream_t)stream, npairs, a, b, img, *in, *c,
-                     gfv, gfp, *g);
+  hipLaunchKernelGGL(sphdem_adj::k_force_bwd, dim3(sphdem_adj::gridFor(nquery)),
+                     dim3(sphdem_adj::kTPB), 0, (hipStream_t)stream, nquery, query, off, b, img,
+                     *in, *c, gfv, gfp, *g);
+  return adjLaunched();
+}
+
+int sphdem_adj_omega(int32_t nquery, const int32_t* query, const int64_t* off, const int32_t* b,
+                     const int32_t* img, const sphdem_adj_fields* in, const sphdem_adj_consts* c,
+                     double* omega, void* stream) {
+  if (nquery < 0 || !in || !c || !omega) return SPHDEM_E_ARG;
+  if (nquery == 0) return 0;
+  if (!SPHDEM_ADJ_CSR_OK(in) || !in->h || !in->dens) return SPHDEM_E_ARG;
+  hipLaunchKernelGGL(sphdem_adj::k_omega_fwd, dim3(sphdem_adj::gridFor(nquery)),
+                     dim3(sphdem_adj::kTPB), 0, (hipStream_t)stream, nquery, query, off, b, img,
+                     *in, *c, omega);
+  return adjLaunched();
+}
+
+int sphdem_adj_omega_bwd(int32_t nquery, const int32_t* query, const int64_t* off,
+                         const int32_t* b, const int32_t* img, const sphdem_adj_fields* in,
+                         const sphdem_adj_consts* c, const double* gomega,
+                         const sphdem_adj_grads* g, void* stream) {
+  if (nquery < 0 || !in || !c || !gomega || !g) return SPHDEM_E_ARG;
+  if (nquery == 0) return 0;
+  if (!SPHDEM_ADJ_CSR_OK(in) || !in->h || !in->dens || !g->r || !g->h || !g->dens)
+    return SPHDEM_E_ARG;
+  hipLaunchKernelGGL(sphdem_adj::k_omega_bwd, dim3(sphdem_adj::gridFor(nquery)),
+                     dim3(sphdem_adj::kTPB), 0, (hipStream_t)stream, nquery, query, off, b, img,
+                     *in, *c, gomega, *g);
+  return adjLaunched();
+}
+
+int sphdem_adj_dddt(int32_t nquery, const int32_t* query, const int64_t* off, const int32_t* b,
+                    const int32_t* img, const sphdem_adj_fields* in, const sphdem_adj_consts* c,
+                    double* dddt, void* stream) {
+  if (nquery < 0 || !in || !c || !dddt) return SPHDEM_E_ARG;
+  if (nquery == 0) return 0;
+  if (!SPHDEM_ADJ_CSR_OK(in) || !in->h || !in->vhat || !in->omega) return SPHDEM_E_ARG;
+  hipLaunchKernelGGL(sphdem_adj::k_dddt_fwd, dim3(sphdem_adj::gridFor(nquery)),
+                     dim3(sphdem_adj::kTPB), 0, (hipStream_t)stream, nquery, query, off, b, img,
+                     *in, *c, dddt);
+  return adjLaunched();
+}
+
+int sphdem_adj_dddt_bwd(int32_t nquery, const int32_t* query, const int64_t* off,
+                        const int32_t* b, const int32_t* img, const sphdem_adj_fields* in,
+                        const sphdem_adj_consts* c, const double* gdddt,
+                        const sphdem_adj_grads* g, void* stream) {
+  if (nquery < 0 || !in || !c || !gdddt || !g) return SPHDEM_E_ARG;
+  if (nquery == 0) return 0;
+  if (!SPHDEM_ADJ_CSR_OK(in) || !in->h || !in->vhat || !in->omega || !g->r || !g->vhat || !g->h ||
+      !g->omega)
+    return SPHDEM_E_ARG;
+  hipLaunchKernelGGL(sphdem_adj::k_dddt_bwd, dim3(sphdem_adj::gridFor(nquery)),
+                     dim3(sphdem_adj::kTPB), 0, (hipStream_t)stream, nquery, query, off, b, img,
+                     *in, *c, gdddt, *g);
+  return adjLaunched();
+}
+
+int sphdem_adj_coupling(int32_t nquery, const int32_t* query, const int64_t* off,
+                        const int32_t* d, const int32_t* img, const sphdem_adj_fields* in,
+                        const sphdem_adj_consts* c, double* por_sum, double* shep, double* drag,
+                        void* stream) {
+  if (nquery < 0 || !in || !c || !por_sum || !shep || !drag) return SPHDEM_E_ARG;
+  if (nquery == 0) return 0;
+  const int32_t* b = d;
+  if (!SPHDEM_ADJ_CSR_OK(in) || !in->dens || !in->fdrag) return SPHDEM_E_ARG;
+  const dim3 grid(sphdem_adj::gridFor(nquery)), block(sphdem_adj::kTPB);
+  hipLaunchKernelGGL(sphdem_adj::k_cpl_fwd1, grid, block, 0, (hipStream_t)stream, nquery, query,
+                     off, d, img, *in, *c, por_sum, shep);
+  if (adjLaunched()) return SPHDEM_E_HIP;
+  sphdem_adj_fields in2 = *in;
+  in2.shep = shep;
+  hipLaunchKernelGGL(sphdem_adj::k_cpl_fwd2, grid, block, 0, (hipStream_t)stream, nquery, query,
+                     off, d, img, in2, *c, drag);
+  return adjLaunched();
+}
+
+int sphdem_adj_coupling_bwd(int32_t nquery, const int32_t* query, const int64_t* off,
+                            const int32_t* d, const int32_t* img, const sphdem_adj_fields* in,
+                            const sphdem_adj_consts* c, const double* gpor, const double* gdrag,
+                            double* work, const sphdem_adj_grads* g, void* stream) {
+  if (nquery < 0 || !in || !c || !gpor || !gdrag || !work || !g) return SPHDEM_E_ARG;
+  if (nquery == 0) return 0;
+  const int32_t* b = d;
+  if (!SPHDEM_ADJ_CSR_OK(in) || !in->dens || !in->fdrag || !in->shep || !g->r || !g->dens ||
+      !g->fdrag)
+    return SPHDEM_E_ARG;
+  const dim3 grid(sphdem_adj::gridFor(nquery)), block(sphdem_adj::kTPB);
+  hipLaunchKernelGGL(sphdem_adj::k_cpl_bwd1, grid, block, 0, (hipStream_t)stream, nquery, query,
+                     off, d, img, *in, *c, gpor, gdrag, work, *g);
+  if (adjLaunched()) return SPHDEM_E_HIP;
+  hipLaunchKernelGGL(sphdem_adj::k_cpl_bwd2, grid, block, 0, (hipStream_t)stream, nquery, query,
+                     off, d, img, *in, *c, (const double*)work, *g);
   return adjLaunched();
 }
 

@@ -1530,8 +1530,9 @@ int sphdem_get_cell_coords(sphdem_ctx* ctx, int32_t* ijk) {
   return 0;
 }
 
-int sphdem_get_pairs(sphdem_ctx* ctx, int kind, int64_t* count, int32_t* a, int32_t* b,
-                     int32_t* image, int64_t capacity) {
+// device: a, b, image are device pointers filled in place (no host copy)
+static int getPairs(sphdem_ctx* ctx, int kind, int64_t* count, int32_t* a, int32_t* b,
+                    int32_t* image, int64_t capacity, bool device) {
   CHECK(checkLoaded(ctx));
   if (!count || kind < 0 || kind > 3) return SPHDEM_E_ARG;
   if ((kind == SPHDEM_PAIRS_DEM || kind >= SPHDEM_PAIRS_SPH2DEM) && !liqDem(ctx)) {
@@ -1586,22 +1587,42 @@ int sphdem_get_pairs(sphdem_ctx* ctx, int kind, int64_t* count, int32_t* a, int3
     }
     int *da = nullptr, *db = nullptr, *di = nullptr;
     HIPC(hipMalloc(&off, sizeof(long long) * nq));
-    HIPC(hipMalloc(&da, sizeof(int) * tot));
-    HIPC(hipMalloc(&db, sizeof(int) * tot));
-    HIPC(hipMalloc(&di, sizeof(int) * tot));
+    if (device) {
+      da = a;
+      db = b;
+      di = image;
+    } else {
+      HIPC(hipMalloc(&da, sizeof(int) * tot));
+      HIPC(hipMalloc(&db, sizeof(int) * tot));
+      HIPC(hipMalloc(&di, sizeof(int) * tot));
+    }
     HIPC(hipMemcpyAsync(off, ho.data(), sizeof(long long) * nq, hipMemcpyHostToDevice, ctx->stream));
     CHECK(enumerate(1, off, da, db, di));
-    HIPC(hipMemcpyAsync(a, da, sizeof(int) * tot, hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(hipMemcpyAsync(b, db, sizeof(int) * tot, hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(hipMemcpyAsync(image, di, sizeof(int) * tot, hipMemcpyDeviceToHost, ctx->stream));
+    if (!device) {
+      HIPC(hipMemcpyAsync(a, da, sizeof(int) * tot, hipMemcpyDeviceToHost, ctx->stream));
+      HIPC(hipMemcpyAsync(b, db, sizeof(int) * tot, hipMemcpyDeviceToHost, ctx->stream));
+      HIPC(hipMemcpyAsync(image, di, sizeof(int) * tot, hipMemcpyDeviceToHost, ctx->stream));
+    }
     HIPC(hipStreamSynchronize(ctx->stream));
-    (void)hipFree(da);
-    (void)hipFree(db);
-    (void)hipFree(di);
+    if (!device) {
+      (void)hipFree(da);
+      (void)hipFree(db);
+      (void)hipFree(di);
+    }
     (void)hipFree(off);
   }
   (void)hipFree(cnt);
   return 0;
+}
+
+int sphdem_get_pairs(sphdem_ctx* ctx, int kind, int64_t* count, int32_t* a, int32_t* b,
+                     int32_t* image, int64_t capacity) {
+  return getPairs(ctx, kind, count, a, b, image, capacity, false);
+}
+
+int sphdem_get_pairs_device(sphdem_ctx* ctx, int kind, int64_t* count, int32_t* a, int32_t* b,
+                            int32_t* image, int64_t capacity) {
+  return getPairs(ctx, kind, count, a, b, image, capacity, true);
 }
 
 int sphdem_pass_omega(sphdem_ctx* ctx) {

@@ -617,6 +617,8 @@ def load_library(path: str = LIB_PATH):
                                           ctypes.POINTER(ctypes.c_int)]
     lib.sphdem_get_cell_coords.argtypes = [vp, ip]
     lib.sphdem_get_pairs.argtypes = [vp, ctypes.c_int, ctypes.POINTER(i64), ip, ip, ip, i64]
+    lib.sphdem_get_pairs_device.argtypes = [vp, ctypes.c_int, ctypes.POINTER(i64), vp, vp, vp,
+                                            i64]
     lib.sphdem_reduce_dt.argtypes = [vp, dp]
     lib.sphdem_num_deleted.argtypes = [vp, ctypes.POINTER(i64)]
     lib.sphdem_timing_begin.argtypes = [vp, ctypes.c_char_p]

@@ -9,13 +9,14 @@ the sediment3D case -- CsphIncompress::start/middle/end (sphIncompress.cpp:
 differentiable function of the state, in original particle order, and takes
 its gradient by reverse mode:
 
-* The SPH force, the fluid-particle drag (calcFPI) and the DEM contacts run as
-  `torch.autograd.Function`s whose forward and backward are the hand-written
-  HIP kernels of include/sphdem_adjoint.h (`HipPasses`).
-* The remaining per-pair sums (porosity, Shepard sum, DEM->SPH drag, omega,
-  dddt) and the per-particle updates (integrator, EOS, h from density, the
-  dt rule, the hooks) are torch expressions on the device, differentiated by
-  torch.
+* Every pair pass -- the SPH force, omega, dddt, the fluid-side coupling sums
+  (porosity, Shepard sums, DEM->SPH drag), the fluid-particle drag (calcFPI)
+  and the DEM contacts -- runs as a `torch.autograd.Function` whose forward
+  and backward are the hand-written HIP kernels of include/sphdem_adjoint.h
+  (`HipPasses`).
+* The per-particle updates (integrator, EOS, h from density, porosity
+  clamp, the dt rule, the hooks) are torch expressions on the device,
+  differentiated by torch.
 * Neighbour sets are discrete: at every reset they come from the library's own
   searches (`LibraryPairs`: a helper context given the model's positions and
   h), which reproduces the reference's pair sets bit for bit, including the
@@ -33,6 +34,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import time
 
 import numpy as np
 import torch
@@ -48,17 +50,19 @@ GRAVITY = -9.81
 class AdjConsts(ctypes.Structure):
     _fields_ = [(k, ctypes.c_double) for k in (
         "wcon", "viscosity", "dens", "pi", "dem_radius", "dem_vol", "dem_k", "dem_gamma",
-        "drag_a", "drag_b")] + [("shift_len", ctypes.c_double * 3)]
+        "drag_a", "drag_b", "coupling_radius", "porosity_incr")] + [
+            ("shift_len", ctypes.c_double * 3)]
 
 
 class AdjFields(ctypes.Structure):
     _fields_ = [(k, ctypes.c_void_p) for k in (
-        "r", "vhat", "h", "dens", "mass", "omega", "pdr2", "fpfb", "fv", "por", "iam")]
+        "r", "vhat", "h", "dens", "mass", "omega", "pdr2", "fpfb", "fv", "por", "fdrag", "shep",
+        "iam")]
 
 
 class AdjGrads(ctypes.Structure):
     _fields_ = [(k, ctypes.c_void_p) for k in (
-        "r", "vhat", "h", "dens", "omega", "pdr2", "fpfb", "fv", "por", "params")]
+        "r", "vhat", "h", "dens", "omega", "pdr2", "fpfb", "fv", "por", "fdrag", "params")]
 
 
 def _bind(lib):
@@ -66,9 +70,14 @@ def _bind(lib):
         return lib
     vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
     pc, pf, pg = ctypes.POINTER(AdjConsts), ctypes.POINTER(AdjFields), ctypes.POINTER(AdjGrads)
-    lib.sphdem_adj_force.argtypes = [i64, vp, vp, vp, pf, pc, vp, vp, vp]
+    lib.sphdem_adj_force.argtypes = [i32, vp, vp, vp, vp, pf, pc, vp, vp, vp]
+    lib.sphdem_adj_force_bwd.argtypes = [i32, vp, vp, vp, vp, pf, pc, vp, vp, pg, vp]
+    for nm in ("sphdem_adj_omega", "sphdem_adj_dddt"):
+        getattr(lib, nm).argtypes = [i32, vp, vp, vp, vp, pf, pc, vp, vp]
+        getattr(lib, nm + "_bwd").argtypes = [i32, vp, vp, vp, vp, pf, pc, vp, pg, vp]
+    lib.sphdem_adj_coupling.argtypes = [i32, vp, vp, vp, vp, pf, pc, vp, vp, vp, vp]
+    lib.sphdem_adj_coupling_bwd.argtypes = [i32, vp, vp, vp, vp, pf, pc, vp, vp, vp, pg, vp]
     lib.sphdem_adj_contacts.argtypes = [i64, vp, vp, vp, pf, pc, vp, vp]
-    lib.sphdem_adj_force_bwd.argtypes = [i64, vp, vp, vp, pf, pc, vp, vp, pg, vp]
     lib.sphdem_adj_contacts_bwd.argtypes = [i64, vp, vp, vp, pf, pc, vp, pg, vp]
     lib.sphdem_adj_fpi.argtypes = [i32, vp, vp, vp, vp, pf, pc, vp, vp]
     lib.sphdem_adj_fpi_bwd.argtypes = [i32, vp, vp, vp, vp, pf, pc, vp, pg, vp]
@@ -90,17 +99,44 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+# Per-kernel launch timing: when KERNEL_TIMES is a dict, each HIP pass launch is
+# bracketed by CUDA events on the stream it runs on and appended to
+# KERNEL_TIMES[name] as (start, end, work items); kernel_ms() reduces them.
+KERNEL_TIMES = None
+
+
+def _launch(name, items, fn):
+    if KERNEL_TIMES is None:
+        return fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    rc = fn()
+    e1.record()
+    KERNEL_TIMES.setdefault(name, []).append((e0, e1, items))
+    return rc
+
+
+def kernel_ms() -> dict:
+    """{name: (total ms, launches, total work items)} of the recorded launches."""
+    torch.cuda.synchronize()
+    out = {}
+    for k, v in (KERNEL_TIMES or {}).items():
+        out[k] = (sum(a.elapsed_time(b) for a, b, _ in v), len(v), sum(n for _, _, n in v))
+    return out
+
+
 # ---------------------------------------------------------------- pair sets
 class PairSet:
     """One neighbour search: pairs (a, b) with b's periodic image shift, plus
     the CSR form grouped by a (for the per-query kernels)."""
 
-    def __init__(self, abi: torch.Tensor, n: int, shift_len: torch.Tensor):
-        abi = abi.to(torch.int32)
+    def __init__(self, abi, n: int, shift_len: torch.Tensor):
+        """abi: an (m, 3) int tensor of (a, b, image) rows, or the three columns."""
+        if isinstance(abi, torch.Tensor):
+            abi = abi.to(torch.int32)
+            abi = (abi[:, 0], abi[:, 1], abi[:, 2])
         self.n = n
-        self.a = abi[:, 0].contiguous()
-        self.b = abi[:, 1].contiguous()
-        self.img = abi[:, 2].contiguous()
+        self.a, self.b, self.img = (t.to(torch.int32).contiguous() for t in abi)
         self.count = int(self.a.numel())
         if self.count:
             lo = int(torch.minimum(self.a.min(), self.b.min()))
@@ -112,17 +148,29 @@ class PairSet:
         self.shift = unit.to(shift_len.dtype) * shift_len
         self._csr = None
 
-    def csr(self):
-        """(query, off, b, img): the pairs sorted by a (stable)."""
+    def csr(self, queries: torch.Tensor | None = None):
+        """(query, off, b, img): the pairs sorted by a (stable), one segment
+        per query.  queries: every particle to visit, in increasing order (a
+        superset of the a's; those without pairs get empty segments); by
+        default the a's themselves."""
+        key = None if queries is None else "all"
         if self._csr is None:
+            self._csr = {}
+        if key not in self._csr:
             order = torch.sort(self.a.long(), stable=True).indices
             a = self.a[order]
-            q, cnt = torch.unique_consecutive(a, return_counts=True)
+            if queries is None:
+                q, cnt = torch.unique_consecutive(a, return_counts=True)
+            else:
+                q = queries
+                cnt = torch.bincount(a.long(), minlength=self.n)[q.long()]
+                if int(cnt.sum()) != self.count:
+                    raise ValueError("pairs of a particle outside the query set")
             off = torch.zeros(q.numel() + 1, dtype=torch.int64, device=a.device)
             off[1:] = torch.cumsum(cnt, 0)
-            self._csr = (q.to(torch.int32).contiguous(), off.contiguous(),
-                         self.b[order].contiguous(), self.img[order].contiguous())
-        return self._csr
+            self._csr[key] = (q.to(torch.int32).contiguous(), off.contiguous(),
+                              self.b[order].contiguous(), self.img[order].contiguous())
+        return self._csr[key]
 
 
 class LibraryPairs:
@@ -132,14 +180,36 @@ class LibraryPairs:
 
     def __init__(self, case: sd.Case, device: int = 0):
         self.sim = sd.Sim(case, device=device)
+        self.seconds = 0.0  # wall time spent in the searches
 
     def __call__(self, r: torch.Tensor, h: torch.Tensor, kinds) -> dict:
+        t0 = time.perf_counter()
+        out = self._search(r, h, kinds)
+        self.seconds += time.perf_counter() - t0
+        return out
+
+    def _search(self, r, h, kinds):
         self.sim.set("r", r.detach().cpu().numpy())
         self.sim.set("h", h.detach().cpu().numpy())
         self.sim.reset()
         if self.sim.num_deleted():
             raise sd.SphDemError("a particle left the domain during the rollout")
-        return {k: torch.from_numpy(self.sim.pairs(k)).to(r.device) for k in kinds}
+        return {k: self._pairs(k, r.device) for k in kinds}
+
+    def _pairs(self, kind, device):
+        """One search's pairs written straight into device tensors
+        (sphdem_get_pairs_device)."""
+        lib, cnt = self.sim.lib, ctypes.c_int64()
+        self.sim._check(lib.sphdem_get_pairs_device(self.sim.ctx, kind, ctypes.byref(cnt), None,
+                                                    None, None, 0))
+        m = cnt.value
+        cols = [torch.empty(max(m, 1), dtype=torch.int32, device=device) for _ in range(3)]
+        if m:
+            torch.cuda.synchronize(device)  # the buffers exist before the library's stream writes
+            self.sim._check(lib.sphdem_get_pairs_device(
+                self.sim.ctx, kind, ctypes.byref(cnt), *[ctypes.c_void_p(c.data_ptr())
+                                                         for c in cols], m))
+        return tuple(c[:m] for c in cols)
 
     def close(self):
         self.sim.close()
@@ -167,9 +237,15 @@ def kPdWh(c, q, h):
                        torch.zeros_like(q))
 
 
+def _g(t, idx):
+    """t[idx] along dim 0 as index_select, whose backward is a scatter-add
+    (advanced indexing's backward sorts the indices)."""
+    return t.index_select(0, idx)
+
+
 def _dx(ps: PairSet, r):
     a, b = ps.a.long(), ps.b.long()
-    dx = r[a] - (r[b] + ps.shift)
+    dx = _g(r, a) - (_g(r, b) + ps.shift)
     return a, b, dx, torch.sqrt((dx * dx).sum(1))
 
 
@@ -205,11 +281,12 @@ class _ForceFn(torch.autograd.Function):
     def forward(ctx, r, vhat, h, dens, omega, pdr2, visc, ps, C):
         lib = _adj_lib()
         fv, fp = torch.zeros_like(r), torch.zeros_like(r)
-        cs = C.m.consts_with(C.consts, viscosity=float(visc))
+        cs = C.m.consts_with(C.consts, viscosity=float(visc.detach()))
         fl = _fields(C, r=r, vhat=vhat, h=h, dens=dens, omega=omega, pdr2=pdr2)
-        _lib_ok(lib.sphdem_adj_force(ps.count, _ptr(ps.a), _ptr(ps.b), _ptr(ps.img),
-                                     ctypes.byref(fl), ctypes.byref(cs), _ptr(fv), _ptr(fp),
-                                     _stream()), "sphdem_adj_force")
+        q, off, b, img = ps.csr()
+        _lib_ok(_launch("force", ps.count, lambda: lib.sphdem_adj_force(
+            int(q.numel()), _ptr(q), _ptr(off), _ptr(b), _ptr(img), ctypes.byref(fl),
+            ctypes.byref(cs), _ptr(fv), _ptr(fp), _stream())), "sphdem_adj_force")
         ctx.save_for_backward(r, vhat, h, dens, omega, pdr2, visc)
         ctx.ps, ctx.C, ctx.cs = ps, C, cs
         return fv, fp
@@ -225,11 +302,91 @@ class _ForceFn(torch.autograd.Function):
         g = AdjGrads(r=gr.data_ptr(), vhat=gv.data_ptr(), h=gh.data_ptr(), dens=gd.data_ptr(),
                      omega=go.data_ptr(), pdr2=gp.data_ptr(), params=par.data_ptr())
         ps = ctx.ps
-        _lib_ok(_adj_lib().sphdem_adj_force_bwd(
-            ps.count, _ptr(ps.a), _ptr(ps.b), _ptr(ps.img), ctypes.byref(fl),
-            ctypes.byref(ctx.cs), _ptr(gfv), _ptr(gfp), ctypes.byref(g), _stream()),
+        q, off, b, img = ps.csr()
+        _lib_ok(_launch("force_bwd", ps.count, lambda: _adj_lib().sphdem_adj_force_bwd(
+            int(q.numel()), _ptr(q), _ptr(off), _ptr(b), _ptr(img), ctypes.byref(fl),
+            ctypes.byref(ctx.cs), _ptr(gfv), _ptr(gfp), ctypes.byref(g), _stream())),
             "sphdem_adj_force_bwd")
         return gr, gv, gh, gd, go, gp, par[0].reshape(visc.shape), None, None
+
+
+def _csr_call(name, ps, *args, queries=None):
+    q, off, b, img = ps.csr(queries)
+    fn = getattr(_adj_lib(), "sphdem_adj_" + name)
+    _lib_ok(_launch(name, ps.count, lambda: fn(int(q.numel()), _ptr(q), _ptr(off), _ptr(b),
+                                                 _ptr(img), *args, _stream())),
+            "sphdem_adj_" + name)
+
+
+class _OmegaFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, r, h, dens, ps, C):
+        out = torch.ones_like(h)
+        fl = _fields(C, r=r, h=h, dens=dens)
+        # every SPH-group particle, neighbours or not (calcOmega's self term)
+        _csr_call("omega", ps, ctypes.byref(fl), ctypes.byref(C.consts), _ptr(out),
+                  queries=C.m.grp_idx)
+        ctx.save_for_backward(r, h, dens)
+        ctx.ps, ctx.C = ps, C
+        return out
+
+    @staticmethod
+    def backward(ctx, go):
+        r, h, dens = ctx.saved_tensors
+        gr, gh, gd = _zeros_like_all(r, h, dens)
+        fl = _fields(ctx.C, r=r, h=h, dens=dens)
+        g = AdjGrads(r=gr.data_ptr(), h=gh.data_ptr(), dens=gd.data_ptr())
+        _csr_call("omega_bwd", ctx.ps, ctypes.byref(fl), ctypes.byref(ctx.C.consts),
+                  _ptr(go.contiguous()), ctypes.byref(g), queries=ctx.C.m.grp_idx)
+        return gr, gh, gd, None, None
+
+
+class _DddtFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, r, vhat, h, omega, ps, C):
+        out = torch.zeros_like(h)
+        fl = _fields(C, r=r, vhat=vhat, h=h, omega=omega)
+        _csr_call("dddt", ps, ctypes.byref(fl), ctypes.byref(C.consts), _ptr(out))
+        ctx.save_for_backward(r, vhat, h, omega)
+        ctx.ps, ctx.C = ps, C
+        return out
+
+    @staticmethod
+    def backward(ctx, gd_):
+        r, vhat, h, omega = ctx.saved_tensors
+        gr, gv, gh, go = _zeros_like_all(r, vhat, h, omega)
+        fl = _fields(ctx.C, r=r, vhat=vhat, h=h, omega=omega)
+        g = AdjGrads(r=gr.data_ptr(), vhat=gv.data_ptr(), h=gh.data_ptr(), omega=go.data_ptr())
+        _csr_call("dddt_bwd", ctx.ps, ctypes.byref(fl), ctypes.byref(ctx.C.consts),
+                  _ptr(gd_.contiguous()), ctypes.byref(g))
+        return gr, gv, gh, go, None, None
+
+
+class _CouplingFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, r, dens, fdrag, ps, C):
+        por = torch.zeros_like(dens)
+        shep = torch.zeros_like(dens)
+        drag = torch.zeros_like(r)
+        fl = _fields(C, r=r, dens=dens, fdrag=fdrag)
+        _csr_call("coupling", ps, ctypes.byref(fl), ctypes.byref(C.consts), _ptr(por),
+                  _ptr(shep), _ptr(drag))
+        ctx.save_for_backward(r, dens, fdrag, shep)
+        ctx.ps, ctx.C = ps, C
+        return por, drag
+
+    @staticmethod
+    def backward(ctx, gpor, gdrag):
+        r, dens, fdrag, shep = ctx.saved_tensors
+        gpor = torch.zeros_like(dens) if gpor is None else gpor.contiguous()
+        gdrag = torch.zeros_like(r) if gdrag is None else gdrag.contiguous()
+        gr, gd, gf = _zeros_like_all(r, dens, fdrag)
+        work = torch.zeros_like(dens)
+        fl = _fields(ctx.C, r=r, dens=dens, fdrag=fdrag, shep=shep)
+        g = AdjGrads(r=gr.data_ptr(), dens=gd.data_ptr(), fdrag=gf.data_ptr())
+        _csr_call("coupling_bwd", ctx.ps, ctypes.byref(fl), ctypes.byref(ctx.C.consts),
+                  _ptr(gpor), _ptr(gdrag), _ptr(work), ctypes.byref(g))
+        return gr, gd, gf, None, None
 
 
 class _FpiFn(torch.autograd.Function):
@@ -237,13 +394,13 @@ class _FpiFn(torch.autograd.Function):
     def forward(ctx, r, vhat, h, dens, fpfb, fv, por, visc, drag_a, drag_b, ps, C):
         lib = _adj_lib()
         out = torch.zeros_like(r)
-        cs = C.m.consts_with(C.consts, viscosity=float(visc), drag_a=float(drag_a),
-                             drag_b=float(drag_b))
+        cs = C.m.consts_with(C.consts, viscosity=float(visc.detach()),
+                             drag_a=float(drag_a.detach()), drag_b=float(drag_b.detach()))
         q, off, b, img = ps.csr()
         fl = _fields(C, r=r, vhat=vhat, h=h, dens=dens, fpfb=fpfb, fv=fv, por=por)
-        _lib_ok(lib.sphdem_adj_fpi(int(q.numel()), _ptr(q), _ptr(off), _ptr(b), _ptr(img),
-                                   ctypes.byref(fl), ctypes.byref(cs), _ptr(out), _stream()),
-                "sphdem_adj_fpi")
+        _lib_ok(_launch("fpi", ps.count, lambda: lib.sphdem_adj_fpi(
+            int(q.numel()), _ptr(q), _ptr(off), _ptr(b), _ptr(img), ctypes.byref(fl),
+            ctypes.byref(cs), _ptr(out), _stream())), "sphdem_adj_fpi")
         ctx.save_for_backward(r, vhat, h, dens, fpfb, fv, por, visc, drag_a, drag_b)
         ctx.ps, ctx.C, ctx.cs = ps, C, cs
         return out
@@ -259,9 +416,9 @@ class _FpiFn(torch.autograd.Function):
                      fpfb=gpf.data_ptr(), fv=gfv.data_ptr(), por=gpo.data_ptr(),
                      params=par.data_ptr())
         q, off, b, img = ctx.ps.csr()
-        _lib_ok(_adj_lib().sphdem_adj_fpi_bwd(
+        _lib_ok(_launch("fpi_bwd", ctx.ps.count, lambda: _adj_lib().sphdem_adj_fpi_bwd(
             int(q.numel()), _ptr(q), _ptr(off), _ptr(b), _ptr(img), ctypes.byref(fl),
-            ctypes.byref(ctx.cs), _ptr(gout), ctypes.byref(g), _stream()), "sphdem_adj_fpi_bwd")
+            ctypes.byref(ctx.cs), _ptr(gout), ctypes.byref(g), _stream())), "sphdem_adj_fpi_bwd")
         return (gr, gv, gh, gd, gpf, gfv, gpo, par[0].reshape(visc.shape),
                 par[1].reshape(drag_a.shape), par[2].reshape(drag_b.shape), None, None)
 
@@ -271,11 +428,12 @@ class _ContactsFn(torch.autograd.Function):
     def forward(ctx, r, vhat, dem_k, dem_gamma, ps, C):
         lib = _adj_lib()
         out = torch.zeros_like(r)
-        cs = C.m.consts_with(C.consts, dem_k=float(dem_k), dem_gamma=float(dem_gamma))
+        cs = C.m.consts_with(C.consts, dem_k=float(dem_k.detach()),
+                             dem_gamma=float(dem_gamma.detach()))
         fl = _fields(C, r=r, vhat=vhat)
-        _lib_ok(lib.sphdem_adj_contacts(ps.count, _ptr(ps.a), _ptr(ps.b), _ptr(ps.img),
-                                        ctypes.byref(fl), ctypes.byref(cs), _ptr(out),
-                                        _stream()), "sphdem_adj_contacts")
+        _lib_ok(_launch("contacts", ps.count, lambda: lib.sphdem_adj_contacts(
+            ps.count, _ptr(ps.a), _ptr(ps.b), _ptr(ps.img), ctypes.byref(fl), ctypes.byref(cs),
+            _ptr(out), _stream())), "sphdem_adj_contacts")
         ctx.save_for_backward(r, vhat, dem_k, dem_gamma)
         ctx.ps, ctx.C, ctx.cs = ps, C, cs
         return out
@@ -289,9 +447,9 @@ class _ContactsFn(torch.autograd.Function):
         fl = _fields(ctx.C, r=r, vhat=vhat)
         g = AdjGrads(r=gr.data_ptr(), vhat=gv.data_ptr(), params=par.data_ptr())
         ps = ctx.ps
-        _lib_ok(_adj_lib().sphdem_adj_contacts_bwd(
+        _lib_ok(_launch("contacts_bwd", ps.count, lambda: _adj_lib().sphdem_adj_contacts_bwd(
             ps.count, _ptr(ps.a), _ptr(ps.b), _ptr(ps.img), ctypes.byref(fl),
-            ctypes.byref(ctx.cs), _ptr(gout), ctypes.byref(g), _stream()),
+            ctypes.byref(ctx.cs), _ptr(gout), ctypes.byref(g), _stream())),
             "sphdem_adj_contacts_bwd")
         return gr, gv, par[3].reshape(dem_k.shape), par[4].reshape(dem_gamma.shape), None, None
 
@@ -303,11 +461,23 @@ def _dev(*ts):
 
 
 class HipPasses:
-    """The SPH force, fluid-particle drag and DEM contact passes as HIP
-    kernels with hand-written backward kernels."""
+    """Every pair pass of the step as a HIP kernel with a hand-written
+    backward kernel."""
 
     def __init__(self):
         _adj_lib()
+
+    def omega(self, C, ps, r, h, dens):
+        _dev(r, h, dens)
+        return _OmegaFn.apply(r, h, dens, ps, C)
+
+    def dddt(self, C, ps, r, vhat, h, omega):
+        _dev(r, vhat, h, omega)
+        return _DddtFn.apply(r, vhat, h, omega, ps, C)
+
+    def coupling(self, C, ps, r, dens, fdrag):
+        _dev(r, dens, fdrag)
+        return _CouplingFn.apply(r, dens, fdrag, ps, C)
 
     def force(self, C, ps, r, vhat, h, dens, omega, pdr2, visc):
         _dev(r, vhat, h, dens, omega, pdr2)
@@ -359,6 +529,7 @@ class StepModel:
         self.sphb = it == sd.SPH_BOUNDARY
         self.grp = self.sph | self.sphb
         self.dem = it == sd.DEM
+        self.grp_idx = torch.nonzero(self.grp).flatten().to(torch.int32)
         self.mass = torch.as_tensor(case.mass, dtype=f64, device=dev).contiguous()
         self.shift_len = torch.tensor([(P.rmax[i] - P.rmin[i]) if P.periodic[i] else 0.0
                                        for i in range(3)], dtype=f64, device=dev)
@@ -371,7 +542,8 @@ class StepModel:
         P = self.P
         c = AdjConsts(wcon=P.wcon, viscosity=P.viscosity, dens=P.dens, pi=P.pi,
                       dem_radius=P.dem_radius, dem_vol=P.dem_vol, dem_k=P.dem_k,
-                      dem_gamma=P.dem_gamma, drag_a=P.drag_a, drag_b=P.drag_b)
+                      dem_gamma=P.dem_gamma, drag_a=P.drag_a, drag_b=P.drag_b,
+                      coupling_radius=P.coupling_radius, porosity_incr=P.porosity_incr)
         for i in range(3):
             c.shift_len[i] = float(self.shift_len[i])
         return c
@@ -424,42 +596,6 @@ class StepModel:
             cols.append(x)
         return torch.stack(cols, dim=1)
 
-    def _omega(self, ps, r, h, dens):
-        """Nsph::calcOmega (sph.h:437-452) on the SPH group."""
-        c, m = self.P.wcon, self.mass
-        a, b, dx, rr = _dx(ps, r)
-        acc = torch.zeros(self.n, dtype=r.dtype, device=r.device).index_add(
-            0, a, m[b] * kPdWh(c, rr / h[a], h[a]))
-        acc = acc + m * kPdWh(c, torch.zeros_like(h), h)
-        om = 1.0 + h * acc / (dens * 3.0)
-        return torch.where(self.grp, om, torch.ones_like(om))
-
-    def _dddt(self, ps, r, vhat, h, omega):
-        """CsphIncompress::calcDddtDudt (sphIncompress.h:783-851) with gradW on
-        h_a and omega_a (VAR_H_CORRECTION)."""
-        c, m = self.P.wcon, self.mass
-        a, b, dx, rr = _dx(ps, r)
-        F = kF(c, rr / h[a], h[a]) / omega[a]
-        vdr = ((vhat[a] - vhat[b]) * dx).sum(1) * F
-        return torch.zeros(self.n, dtype=r.dtype, device=r.device).index_add(0, a, m[b] * vdr)
-
-    def _coupling_sph(self, ps, r, dens, fdrag):
-        """calcPorosity, calcShepSum (with the reverse sync of ghost sums) and
-        calcDEMtoSPHDrag (sph.h:1303-1362), then finalisePorosity
-        (sph.h:1289-1296), for the SPH group.  Returns (porosity, drag)."""
-        P, m = self.P, self.mass
-        Rc = P.coupling_radius
-        a, d, dx, rr = _dx(ps, r)
-        W = kW(P.wcon, rr * (2.0 / Rc), Rc / 2.0)
-        z = torch.zeros(self.n, dtype=r.dtype, device=r.device)
-        por = z.index_add(0, a, P.porosity_incr * W)
-        shep = z.index_add(0, d, (m[a] / dens[a]) * W)
-        safe = torch.where(shep[d] > 0, shep[d], torch.ones_like(shep[d]))
-        w = torch.where(shep[d] > 0, W / safe, torch.zeros_like(W))
-        drag = torch.zeros_like(r).index_add(0, a, -(m[d] / dens[a] * w)[:, None] * fdrag[d])
-        por = torch.clamp(1.0 - por, min=0.37)
-        return torch.where(self.grp, por, torch.ones_like(por)), drag
-
     def _floor(self, r, vhat, f, th):
         """addCustomBoundaries (sediment3D/customSim.cpp:67-81): the z = RMIN
         floor as a LINEAR spring-dashpot on DEM particles."""
@@ -477,10 +613,15 @@ class StepModel:
         (sph.h:566-568) over the SPH group and DEM."""
         P = self.P
         terms = [torch.tensor(P.maxdt, dtype=h.dtype, device=h.device)]
-        sel = self.sph[ps.a.long()]
+        a, b = ps.a.long(), ps.b.long()
+        sel = _g(self.sph, a)
         if bool(sel.any()):
-            hav = 0.5 * (h[ps.a.long()[sel]] + h[ps.b.long()[sel]])
-            mh = hav.min()
+            # the smallest hav over the pairs, found without a graph over every
+            # pair, then recomputed with its gradient from its two particles
+            with torch.no_grad():
+                hs = 0.5 * (_g(h, a) + _g(h, b))
+                k = torch.where(sel, hs, torch.full_like(hs, math.inf)).argmin()
+            mh = 0.5 * (h[a[k]] + h[b[k]])
             terms.append(0.6 * mh / (2.0 * P.spsound))
             terms.append(0.125 * mh ** 2 / th["viscosity"])
         terms.append(torch.tensor(P.liq_dem_timestep, dtype=h.dtype, device=h.device))
@@ -525,9 +666,11 @@ class StepModel:
         # middle (sphIncompress.cpp:40-165)
         r = self._wrap(r)
         pm = self._pairs((k, "middle"), r, h, (sd.PAIRS_SPH, sd.PAIRS_SPH2DEM))
-        por, drag_s = self._coupling_sph(pm[sd.PAIRS_SPH2DEM], r, dens, fdrag)
+        por_sum, drag_s = pz.coupling(C, pm[sd.PAIRS_SPH2DEM], r, dens, fdrag)
+        por = torch.where(self.grp, torch.clamp(1.0 - por_sum, min=0.37),
+                          torch.ones_like(por_sum))         # finalisePorosity sph.h:1289-1296
         f = torch.where(grp, f + drag_s, f)
-        omega = self._omega(pm[sd.PAIRS_SPH], r, h, dens)
+        omega = pz.omega(C, pm[sd.PAIRS_SPH], r, h, dens)
         press = P.prb * ((dens / (P.refd * por)) ** P.gamma - 1.0)
         pdr2 = torch.where(self.grp, press / (dens * dens), torch.zeros_like(press))
         fv, fp = pz.force(C, pm[sd.PAIRS_SPH], r, vhat, h, dens, omega, pdr2, th["viscosity"])
@@ -555,8 +698,8 @@ class StepModel:
         time = time + half
         r = self._wrap(r)
         pe = self._pairs((k, "end"), r, h, (sd.PAIRS_SPH, sd.PAIRS_DEM2SPH, sd.PAIRS_DEM))
-        omega = self._omega(pe[sd.PAIRS_SPH], r, h, dens)
-        dddt = torch.where(self.grp, self._dddt(pe[sd.PAIRS_SPH], r, vhat, h, omega), dddt)
+        omega = pz.omega(C, pe[sd.PAIRS_SPH], r, h, dens)
+        dddt = torch.where(self.grp, pz.dddt(C, pe[sd.PAIRS_SPH], r, vhat, h, omega), dddt)
         fd = pz.fpi(C, pe[sd.PAIRS_DEM2SPH], r, vhat, h, dens, fp, fv, por, th["viscosity"],
                     th["drag_a"], th["drag_b"])
         f = torch.where(dem, f + fd, f)

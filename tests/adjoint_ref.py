@@ -13,8 +13,40 @@ import sphdem_adjoint as sa
 
 
 class TorchPasses:
-    """sphIncompress.h:899-972 (with sph.h:579-666, 692-734), sph.h:1708-1792
-    (with 1575-1631) and sph.h:1676-1706 as torch expressions over a pair set."""
+    """The pair passes as torch expressions over a pair set: calcForce
+    (sphIncompress.h:899-972 with sph.h:579-666, 692-734), calcOmega
+    (sph.h:437-452), calcDddtDudt (sphIncompress.h:783-851), the fluid-side
+    coupling sums (sph.h:1303-1362), calcFPI (sph.h:1708-1792 with 1575-1631)
+    and calcDEMContacts (sph.h:1676-1706)."""
+
+    def omega(self, C, ps, r, h, dens):
+        c, m, n = C.m.P.wcon, C.m.mass, C.m.n
+        a, b, dx, rr = sa._dx(ps, r)
+        acc = torch.zeros(n, dtype=r.dtype, device=r.device).index_add(
+            0, a, m[b] * sa.kPdWh(c, rr / h[a], h[a]))
+        acc = acc + m * sa.kPdWh(c, torch.zeros_like(h), h)
+        om = 1.0 + h * acc / (dens * 3.0)
+        return torch.where(C.m.grp, om, torch.ones_like(om))
+
+    def dddt(self, C, ps, r, vhat, h, omega):
+        c, m = C.m.P.wcon, C.m.mass
+        a, b, dx, rr = sa._dx(ps, r)
+        F = sa.kF(c, rr / h[a], h[a]) / omega[a]
+        vdr = ((vhat[a] - vhat[b]) * dx).sum(1) * F
+        return torch.zeros_like(h).index_add(0, a, m[b] * vdr)
+
+    def coupling(self, C, ps, r, dens, fdrag):
+        P, m = C.m.P, C.m.mass
+        Rc = P.coupling_radius
+        a, d, dx, rr = sa._dx(ps, r)
+        W = sa.kW(P.wcon, rr * (2.0 / Rc), Rc / 2.0)
+        z = torch.zeros_like(dens)
+        por = z.index_add(0, a, P.porosity_incr * W)
+        shep = z.index_add(0, d, (m[a] / dens[a]) * W)
+        safe = torch.where(shep[d] > 0, shep[d], torch.ones_like(shep[d]))
+        w = torch.where(shep[d] > 0, W / safe, torch.zeros_like(W))
+        drag = torch.zeros_like(r).index_add(0, a, -(m[d] / dens[a] * w)[:, None] * fdrag[d])
+        return por, drag
 
     def force(self, C, ps, r, vhat, h, dens, omega, pdr2, visc):
         m, P = C.m.mass, C.m.P

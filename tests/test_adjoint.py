@@ -52,6 +52,26 @@ def test_adjoint_structs_mirror_the_header():
         assert [f[0] for f in cls._fields_] == names, name
 
 
+def test_adjoint_entry_points_check_their_arguments():
+    """Argument checks return before any device work (no GPU needed)."""
+    lib = sa._adj_lib()
+    fl, cs, g = sa.AdjFields(), sa.AdjConsts(), sa.AdjGrads()
+    buf = ctypes.c_void_p(0x1000)  # never dereferenced: every call below is refused or empty
+    for nm in ("omega", "dddt", "force", "fpi"):
+        fn = getattr(lib, "sphdem_adj_" + nm)
+        extra = [buf, buf] if nm == "force" else [buf]
+        assert fn(-1, buf, buf, buf, buf, ctypes.byref(fl), ctypes.byref(cs), *extra, None) == -1
+        assert fn(0, buf, buf, buf, buf, ctypes.byref(fl), ctypes.byref(cs), *extra, None) == 0
+        # pairs given but the fields they read are missing
+        assert fn(4, buf, buf, buf, buf, ctypes.byref(fl), ctypes.byref(cs), *extra, None) == -1
+    assert lib.sphdem_adj_coupling(3, buf, buf, buf, buf, ctypes.byref(fl), ctypes.byref(cs),
+                                   buf, buf, None, None) == -1
+    assert lib.sphdem_adj_coupling_bwd(3, buf, buf, buf, buf, ctypes.byref(fl), ctypes.byref(cs),
+                                       buf, buf, None, ctypes.byref(g), None) == -1
+    assert lib.sphdem_adj_contacts(-2, buf, buf, buf, ctypes.byref(fl), ctypes.byref(cs), buf,
+                                   None) == -1
+
+
 def test_step_model_follows_the_oracle():
     case = small_case()
     model = sa.StepModel(case, device="cpu", passes=TorchPasses(), pairs=OraclePairs(case))
@@ -146,6 +166,21 @@ def _gpu_state(case, steps):
 
 
 @pytest.mark.gpu
+def test_device_pair_export_equals_the_host_export():
+    case = sd.adjoint_case(nf=12, nz=16, n_dem=200, seed=1007, vscale=0.1, touching=20, floor=3)
+    pairs = sa.LibraryPairs(case)
+    r = torch.as_tensor(case.r, device="cuda")
+    h = torch.as_tensor(case.h, device="cuda")
+    kinds = (sd.PAIRS_SPH, sd.PAIRS_DEM, sd.PAIRS_SPH2DEM, sd.PAIRS_DEM2SPH)
+    got = pairs(r, h, kinds)
+    for k in kinds:
+        host = pairs.sim.pairs(k)
+        dev = np.stack([c.cpu().numpy() for c in got[k]], axis=1)
+        assert len(host) > 0 and np.array_equal(dev, host), k
+    pairs.close()
+
+
+@pytest.mark.gpu
 def test_hip_passes_match_torch_autograd():
     case = sd.adjoint_case(nf=12, nz=16, n_dem=200, seed=1007, vscale=0.1, touching=20, floor=3)
     print(f"adjoint passes: {case.counts()}", flush=True)
@@ -153,19 +188,23 @@ def test_hip_passes_match_torch_autograd():
     th = model.params(requires_grad=True)
     C = sa._Ctx(model, th)
     r, h = S["r"], S["h"]
-    got = pairs(r, h, (sd.PAIRS_SPH, sd.PAIRS_DEM2SPH, sd.PAIRS_DEM))
+    got = pairs(r, h, (sd.PAIRS_SPH, sd.PAIRS_SPH2DEM, sd.PAIRS_DEM2SPH, sd.PAIRS_DEM))
     ps = {k: sa.PairSet(v, model.n, model.shift_len) for k, v in got.items()}
     assert ps[sd.PAIRS_DEM].count > 0
     gen = torch.Generator(device="cuda").manual_seed(5)
     rnd = lambda t: torch.rand(t.shape, generator=gen, dtype=t.dtype, device=t.device) + 0.5
     dens = S["dens"]
-    omega = model._omega(ps[sd.PAIRS_SPH], r, h, dens)
+    omega = TorchPasses().omega(C, ps[sd.PAIRS_SPH], r, h, dens).detach()
     pdr2 = torch.where(model.grp, 1e-3 * rnd(h), torch.zeros_like(h))
     por = torch.where(model.grp, 0.5 + 0.4 * torch.rand(h.shape, generator=gen, dtype=h.dtype,
                                                           device=h.device), torch.ones_like(h))
     fp = torch.where(model.grp[:, None], rnd(r) - 1.0, torch.zeros_like(r))
     fv = torch.where(model.grp[:, None], rnd(r) - 1.0, torch.zeros_like(r))
+    fdrag = torch.where(model.dem[:, None], rnd(r) - 1.0, torch.zeros_like(r))
     cases = {
+        "omega": (lambda P, x: P.omega(C, ps[sd.PAIRS_SPH], *x), [r, h, dens], []),
+        "dddt": (lambda P, x: P.dddt(C, ps[sd.PAIRS_SPH], *x), [r, S["vhat"], h, omega], []),
+        "coupling": (lambda P, x: P.coupling(C, ps[sd.PAIRS_SPH2DEM], *x), [r, dens, fdrag], []),
         "force": (lambda P, x: P.force(C, ps[sd.PAIRS_SPH], *x[:6], th["viscosity"]),
                   [r, S["vhat"], h, dens, omega, pdr2], ["viscosity"]),
         "fpi": (lambda P, x: P.fpi(C, ps[sd.PAIRS_DEM2SPH], *x, th["viscosity"], th["drag_a"],
