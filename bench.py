@@ -128,7 +128,8 @@ def cpu_baseline(case, budget_s):
 
 
 KERNEL_SYMBOL = {"lists": "k_lists", "force": "k_force_l", "omega": "k_omega_l",
-                 "dddt": "k_dddt_l", "coupling": "k_coupling_sph"}
+                 "dddt": "k_dddt_l", "coupling": "k_coupling_sph",
+                 "adj_force_bwd": "k_force_bwd"}
 
 
 def traffic_from_summary(path, kernel):
@@ -383,8 +384,11 @@ def adjoint(a):
         units = counts["fluid"] + counts["wall"]
         b_launch = B_ADJ_FORCE_BWD_PARTICLE * units + B_ADJ_PAIR * it / n
         ach = b_launch / (ms / n / 1e3) / 1e9
+        tfile = os.path.join(ROOT, "profiles", "traffic_adjoint_64x64x16.txt")
+        traffic = (traffic_from_summary(tfile, "adj_force_bwd")
+                   if nf == 64 and ndem == 8192 else None)
         roof = {"bound": "hbm", "kernel": "adj_force_bwd", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
                 "mean_launch_ms": ms / n, "launches": n, "alg_bytes_per_launch": b_launch}
     out = {
         "metric": METRIC,

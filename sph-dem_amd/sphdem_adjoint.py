@@ -149,27 +149,36 @@ class PairSet:
         self._csr = None
 
     def csr(self, queries: torch.Tensor | None = None):
-        """(query, off, b, img): the pairs sorted by a (stable), one segment
-        per query.  queries: every particle to visit, in increasing order (a
-        superset of the a's; those without pairs get empty segments); by
-        default the a's themselves."""
+        """(query, off, b, img): one segment of pairs per query particle.
+
+        The library's searches list each query's pairs contiguously, so the
+        segments are read off as they come; a set where a query appears in
+        two places is first sorted by a (stable).  queries: particles that
+        must each get a segment (those without pairs get empty ones)."""
         key = None if queries is None else "all"
         if self._csr is None:
             self._csr = {}
         if key not in self._csr:
-            order = torch.sort(self.a.long(), stable=True).indices
-            a = self.a[order]
-            if queries is None:
+            a, b, img = self.a, self.b, self.img
+            q, cnt = torch.unique_consecutive(a, return_counts=True)
+            if q.numel() != torch.unique(q).numel():
+                order = torch.sort(a.long(), stable=True).indices
+                a, b, img = a[order], b[order], img[order]
                 q, cnt = torch.unique_consecutive(a, return_counts=True)
-            else:
-                q = queries
-                cnt = torch.bincount(a.long(), minlength=self.n)[q.long()]
-                if int(cnt.sum()) != self.count:
+            if queries is not None:
+                seen = torch.zeros(self.n, dtype=torch.bool, device=a.device)
+                seen[q.long()] = True
+                allq = torch.zeros(self.n, dtype=torch.bool, device=a.device)
+                allq[queries.long()] = True
+                if bool((seen & ~allq).any()):
                     raise ValueError("pairs of a particle outside the query set")
+                extra = torch.nonzero(allq & ~seen).flatten().to(q.dtype)
+                q = torch.cat([q, extra])
+                cnt = torch.cat([cnt, torch.zeros_like(extra, dtype=cnt.dtype)])
             off = torch.zeros(q.numel() + 1, dtype=torch.int64, device=a.device)
             off[1:] = torch.cumsum(cnt, 0)
             self._csr[key] = (q.to(torch.int32).contiguous(), off.contiguous(),
-                              self.b[order].contiguous(), self.img[order].contiguous())
+                              b.contiguous(), img.contiguous())
         return self._csr[key]
 
 
@@ -493,6 +502,23 @@ class HipPasses:
 
 
 # ---------------------------------------------------------------- the step
+def _morton_order(r: np.ndarray, P, edge: float) -> np.ndarray:
+    """Stable order of the particles by the Morton code of their cells (edge
+    `edge`, from the domain's lower corner)."""
+    lo = np.array([P.proc_domain[0], P.proc_domain[2], P.proc_domain[4]])
+    c = np.clip(np.floor((r - lo) / edge), 0, 1023).astype(np.uint64)
+
+    def spread(x):
+        x = (x | (x << np.uint64(16))) & np.uint64(0x030000FF)
+        x = (x | (x << np.uint64(8))) & np.uint64(0x0300F00F)
+        x = (x | (x << np.uint64(4))) & np.uint64(0x030C30C3)
+        x = (x | (x << np.uint64(2))) & np.uint64(0x09249249)
+        return x
+
+    key = spread(c[:, 0]) | (spread(c[:, 1]) << np.uint64(1)) | (spread(c[:, 2]) << np.uint64(2))
+    return np.argsort(key, kind="stable")
+
+
 SEDIMENT_FLAGS = sd.F_VISC_MORRIS | sd.F_VAR_H_CORRECTION | sd.F_LIQ_DEM | sd.F_CUSTOM_WALL
 
 
@@ -523,6 +549,17 @@ class StepModel:
         self.device = torch.device(device)
         dev, f64 = self.device, torch.float64
         self.n = case.n
+        # The model keeps its particles in Morton order of the cells of the
+        # initial positions (cell edge 2H), so that the pair kernels' partner
+        # reads stay local; perm[i] is the original index of row i, inv its
+        # inverse.  Inputs, pair sets, the loss and the gradients are mapped at
+        # the boundary; states returned by step() are in this order
+        # (original() maps a per-particle tensor back).
+        perm = _morton_order(np.asarray(case.r), P, 2.0 * P.h)
+        self.perm = torch.as_tensor(perm, dtype=torch.long, device=dev)
+        self.inv = torch.empty_like(self.perm)
+        self.inv[self.perm] = torch.arange(self.n, device=dev)
+        iam = iam[perm]
         it = torch.as_tensor(iam, device=dev)
         self.iam32 = it.to(torch.int32).contiguous()
         self.sph = it == sd.SPH
@@ -530,7 +567,9 @@ class StepModel:
         self.grp = self.sph | self.sphb
         self.dem = it == sd.DEM
         self.grp_idx = torch.nonzero(self.grp).flatten().to(torch.int32)
-        self.mass = torch.as_tensor(case.mass, dtype=f64, device=dev).contiguous()
+        self.mass = torch.as_tensor(np.asarray(case.mass)[perm], dtype=f64, device=dev).contiguous()
+        dem_orig = np.nonzero(np.asarray(case.iam) == sd.DEM)[0]
+        self.dem_rows = self.inv[torch.as_tensor(dem_orig, dtype=torch.long, device=dev)]
         self.shift_len = torch.tensor([(P.rmax[i] - P.rmin[i]) if P.periodic[i] else 0.0
                                        for i in range(3)], dtype=f64, device=dev)
         self.passes = HipPasses() if passes is None else passes
@@ -565,20 +604,38 @@ class StepModel:
         """The state after upload: r, v = vhat (v_init, default the case's),
         zero f, fdrag and dddt, dt = 0, time = 0."""
         c, dev, f64 = self.case, self.device, torch.float64
-        t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=f64, device=dev)
-        v = t(c.v) if v_init is None else v_init
-        vh = t(c.vhat) if v_init is None else v_init
+        t = lambda a: self.from_original(torch.as_tensor(np.ascontiguousarray(a), dtype=f64,
+                                                         device=dev))
+        v = t(c.v) if v_init is None else self.from_original(v_init)
+        vh = t(c.vhat) if v_init is None else v
         z3 = torch.zeros((self.n, 3), dtype=f64, device=dev)
         return {"r": t(c.r), "v": v, "vhat": vh, "f": z3, "fdrag": z3.clone(), "dens": t(c.dens),
                 "h": t(c.h), "dddt": torch.zeros(self.n, dtype=f64, device=dev),
                 "dt": torch.zeros((), dtype=f64, device=dev),
                 "time": torch.zeros((), dtype=f64, device=dev)}
 
+    def from_original(self, t: torch.Tensor) -> torch.Tensor:
+        """A per-particle tensor in original index order -> the model's order."""
+        return t.index_select(0, self.perm)
+
+    def original(self, t: torch.Tensor) -> torch.Tensor:
+        """A per-particle tensor in the model's order -> original index order."""
+        return t.index_select(0, self.inv)
+
     # ---- pieces
     def _pairs(self, key, r, h, kinds):
+        """The pair sets of a reset at positions r, h (model order), searched in
+        original order and renumbered to the model's rows."""
         if key not in self.cache:
-            got = self.pairs_fn(r, h, kinds)
-            self.cache[key] = {k: PairSet(got[k], self.n, self.shift_len) for k in kinds}
+            got = self.pairs_fn(self.original(r), self.original(h), kinds)
+            out = {}
+            for k in kinds:
+                g = got[k]
+                cols = (g[:, 0], g[:, 1], g[:, 2]) if isinstance(g, torch.Tensor) else g
+                a = self.inv.index_select(0, cols[0].long())
+                b = self.inv.index_select(0, cols[1].long())
+                out[k] = PairSet((a, b, cols[2]), self.n, self.shift_len)
+            self.cache[key] = out
         return self.cache[key]
 
     def _wrap(self, r):
@@ -727,7 +784,7 @@ class StepModel:
 
     def dem_loss(self, S, target: torch.Tensor) -> torch.Tensor:
         """Mean squared DEM z-position against a target profile."""
-        z = S["r"][self.dem, 2]
+        z = S["r"][:, 2].index_select(0, self.dem_rows)   # DEM in original order
         return ((z - target) ** 2).mean()
 
     def gradient(self, steps: int, target: torch.Tensor, v_init=None, every: int = 8):
