@@ -31,6 +31,17 @@ namespace sphdem_adj {
 constexpr int kTPB = 256;
 constexpr int SPH_T = 0;
 
+// Thread index with the grid's blocks dealt to the 8 XCDs in contiguous
+// ranges (the hardware deals consecutive blocks round-robin): consecutive
+// queries are neighbours in space, so each XCD's L2 then holds the partner
+// data of its own range instead of every XCD fetching all of it.  A bijection
+// on the grid.
+__device__ __forceinline__ int xcdIndex() {
+  const int nb = gridDim.x, x = blockIdx.x & 7, r = blockIdx.x >> 3;
+  const int q = nb >> 3, rem = nb & 7;
+  return (x * q + min(x, rem) + r) * blockDim.x + threadIdx.x;
+}
+
 __device__ __forceinline__ double3 ld3(const double* p, int i) {
   return make_double3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
 }
@@ -108,7 +119,7 @@ __global__ void k_force_fwd(int32_t nq, const int32_t* __restrict__ query,
                             const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                             sphdem_adj_consts C, double* __restrict__ fv,
                             double* __restrict__ fp) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcdIndex();
   if (i >= nq) return;
   const int a = query[i];
   const double3 ra = ld3(in.r, a), va = ld3(in.vhat, a);
@@ -154,7 +165,7 @@ __global__ void k_force_bwd(int32_t nq, const int32_t* __restrict__ query,
                             const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                             sphdem_adj_consts C, const double* __restrict__ gfv,
                             const double* __restrict__ gfp, sphdem_adj_grads g) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcdIndex();
   double gnu[1] = {0.0};
   if (i < nq) {
     const int a = query[i];
@@ -269,7 +280,7 @@ __global__ void k_omega_fwd(int32_t nq, const int32_t* __restrict__ query,
                             const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
                             const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                             sphdem_adj_consts C, double* __restrict__ omega) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcdIndex();
   if (i >= nq) return;
   const int a = query[i];
   const double3 ra = ld3(in.r, a);
@@ -292,7 +303,7 @@ __global__ void k_omega_bwd(int32_t nq, const int32_t* __restrict__ query,
                             const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                             sphdem_adj_consts C, const double* __restrict__ gomega,
                             sphdem_adj_grads g) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcdIndex();
   if (i >= nq) return;
   const int a = query[i];
   const double go = gomega[a];
@@ -340,7 +351,7 @@ __global__ void k_dddt_fwd(int32_t nq, const int32_t* __restrict__ query,
                            const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
                            const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                            sphdem_adj_consts C, double* __restrict__ dddt) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcdIndex();
   if (i >= nq) return;
   const int a = query[i];
   const double3 ra = ld3(in.r, a), va = ld3(in.vhat, a);
@@ -365,7 +376,7 @@ __global__ void k_dddt_bwd(int32_t nq, const int32_t* __restrict__ query,
                            const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                            sphdem_adj_consts C, const double* __restrict__ gdddt,
                            sphdem_adj_grads g) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcdIndex();
   if (i >= nq) return;
   const int a = query[i];
   const double gd = gdddt[a];
@@ -421,7 +432,7 @@ __global__ void k_cpl_fwd1(int32_t nq, const int32_t* __restrict__ query,
                            const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                            sphdem_adj_consts C, double* __restrict__ por_sum,
                            double* __restrict__ shep) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcdIndex();
   if (i >= nq) return;
   const int a = query[i];
   const double3 ra = ld3(in.r, a);
@@ -443,7 +454,7 @@ __global__ void k_cpl_fwd2(int32_t nq, const int32_t* __restrict__ query,
                            const int64_t* __restrict__ off, const int32_t* __restrict__ pd,
                            const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                            sphdem_adj_consts C, double* __restrict__ drag) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcdIndex();
   if (i >= nq) return;
   const int a = query[i];
   const double3 ra = ld3(in.r, a);
@@ -476,7 +487,7 @@ __global__ void k_cpl_bwd1(int32_t nq, const int32_t* __restrict__ query,
                            sphdem_adj_consts C, const double* __restrict__ gpor,
                            const double* __restrict__ gdrag, double* __restrict__ shepbar,
                            sphdem_adj_grads g) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcdIndex();
   if (i >= nq) return;
   const int a = query[i];
   const double3 ra = ld3(in.r, a), gdr = ld3(gdrag, a);
@@ -520,7 +531,7 @@ __global__ void k_cpl_bwd2(int32_t nq, const int32_t* __restrict__ query,
                            const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                            sphdem_adj_consts C, const double* __restrict__ shepbar,
                            sphdem_adj_grads g) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcdIndex();
   if (i >= nq) return;
   const int a = query[i];
   const double3 ra = ld3(in.r, a);
@@ -599,7 +610,7 @@ __global__ void k_fpi_fwd(int32_t nq, const int32_t* __restrict__ query,
                           const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
                           const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                           sphdem_adj_consts C, double* __restrict__ fdrag) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcdIndex();
   if (i >= nq) return;
   const int p = query[i];
   const double3 rp = ld3(in.r, p);
@@ -634,7 +645,7 @@ __global__ void k_fpi_bwd(int32_t nq, const int32_t* __restrict__ query,
                           const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                           sphdem_adj_consts C, const double* __restrict__ gfd,
                           sphdem_adj_grads g) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcdIndex();
   double gpar[3] = {0.0, 0.0, 0.0};  // nu, drag_a, drag_b
   const int p = i < nq ? query[i] : 0;
   Fpi o;
