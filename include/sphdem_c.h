@@ -52,6 +52,11 @@ extern "C" {
 #define SPHDEM_F_CUSTOM_WALL (1u << 5)       /* LIQ_DEM_CUSTOM_WALL_CONTACTS     */
 #define SPHDEM_F_NO_DEM_CONTACTS (1u << 6)   /* NO_DEM_CONTACTS                  */
 #define SPHDEM_F_FIXED_DEM (1u << 7)         /* FIXED_DEM                        */
+#define SPHDEM_F_VISC_MONAGHAN (1u << 8)     /* VISC_MONAGHAN                    */
+#define SPHDEM_F_VISC_CLEARY (1u << 9)       /* VISC_CLEARY                      */
+/* At most one of VISC_MORRIS, VISC_CLEARY and VISC_MONAGHAN (calcViscForce,
+ * sph.h:579-631, declares its `visc` once per variant, so the reference does
+ * not compile with two); sphdem_create / sphdem_set_params refuse more. */
 
 /* Precision of the pair arithmetic. */
 #define SPHDEM_FP64 0

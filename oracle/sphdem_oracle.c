@@ -606,6 +606,24 @@ static void calcViscForce(Oracle* o, Particle* pa, Particle* pb, const double* g
     double den = pow(r, 2) + 0.01 * pow(hav, 2);
     for (int i = 0; i < 3; i++) fv[i] = dv[i] * pb->mass * visc * d / den;
   }
+  if (P->flags & SPHDEM_F_VISC_CLEARY) { /* sph.h:598-604 */
+    double visc = 19.8 * P->viscosity * vdr / ((pa->dens + pb->dens) * (pow(r, 2) + 0.01 * pow(hav, 2)));
+    vsig = 2.0 * P->spsound;
+    for (int i = 0; i < 3; i++) fv[i] = pb->mass * visc * gradWa[i];
+  }
+  if (P->flags & SPHDEM_F_VISC_MONAGHAN) { /* sph.h:606-630 (no SLK) */
+    double dr = 0.0;
+    if (r != 0.0) dr = 1 / r;
+    double viss = vdr * dr;
+    /* under LIQ_DEM the signal speed stays 0 (its replacement is commented out);
+     * abs(viss) is the double overload (gsl_math.h includes <math.h>, whose
+     * C++ form declares std::abs in the global namespace) */
+    if (!(P->flags & SPHDEM_F_LIQ_DEM)) vsig = 2.0 * P->spsound + 2.0 * fabs(viss);
+    /* ALPHA = viscToAlpha(VISCOSITY, H, SPSOUND) (sph.h:42, misc.h:41-43) */
+    const double alpha = P->viscosity * (112.0 / 15.0) / (P->spsound * P->h);
+    double visc = viss * vsig * alpha / (pa->dens + pb->dens);
+    for (int i = 0; i < 3; i++) fv[i] = pb->mass * visc * gradWa[i];
+  }
   if (P->flags & SPHDEM_F_VISC_ARTIFICIAL) {
     double artificial_vsig = 0, artificial_visc = 0.0;
     if (vdr < 0) {
@@ -620,7 +638,9 @@ static void calcViscForce(Oracle* o, Particle* pa, Particle* pb, const double* g
     vsig = dmax2(vsig, artificial_vsig);
   }
   o->newDt = dmin2(o->newDt, 0.6 * hav / vsig);                        /* sph.h:545-552 */
-  o->newDt = dmin2(o->newDt, 0.125 * pow(hav, 2) / P->viscosity);      /* sph.h:555-560 */
+  /* viscDiffusionCondition sph.h:554-560 */
+  const double cdiff = (P->flags & SPHDEM_F_VISC_MONAGHAN) ? 0.06 : 0.125;
+  o->newDt = dmin2(o->newDt, cdiff * pow(hav, 2) / P->viscosity);
   for (int i = 0; i < 3; i++) pa->fv[i] += fv[i];
   for (int i = 0; i < 3; i++) pa->f[i] += fv[i];
 }

@@ -10,6 +10,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
+#include <utility>
 #include <atomic>
 #include <cmath>
 #include <thread>
@@ -729,11 +731,12 @@ int computeHinv(sphdem_ctx* ctx) {
   LAUNCHED();
   return 0;
 }
-// calcForce instantiated per flag combination (bit 0 VAR_H_CORRECTION, 1 VISC_MORRIS,
-// 2 VISC_ARTIFICIAL, 3 LIQ_DEM)
+// calcForce instantiated per flag combination (bit 0 VAR_H_CORRECTION, 1 VISC_ARTIFICIAL,
+// 2 LIQ_DEM, bits 3-4 the viscosity variant)
 template <int F>
 void launchForce(sphdem_ctx* ctx) {
-  constexpr bool V = (F & 1) != 0, M = (F & 2) != 0, A = (F & 4) != 0, L = (F & 8) != 0;
+  constexpr bool V = (F & 1) != 0, A = (F & 2) != 0, L = (F & 4) != 0;
+  constexpr int M = F >> 3;
   if (mixed(ctx))
     hipLaunchKernelGGL((k_force_m<V, M, A, L>), dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
                        ctx->cur, ctx->ns, ctx->nbl, ctx->nbcnt, ctx->mixP, ctx->mixV, ctx->mixD,
@@ -743,20 +746,21 @@ void launchForce(sphdem_ctx* ctx) {
                        ctx->cur, ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->hinv, ctx->P,
                        ctx->dg);
 }
+template <int... F>
+constexpr std::array<void (*)(sphdem_ctx*), sizeof...(F)> forceTable(
+    std::integer_sequence<int, F...>) {
+  return {launchForce<F>...};
+}
 // hinvReady: the 1/h table is already current (made beside the list build)
 int passForce(sphdem_ctx* ctx, bool hinvReady = false) {
   CHECK(ensureLists(ctx));
   CHECK(refreshGhosts(ctx, GS_ALL));
   if (!mixed(ctx) && !hinvReady) CHECK(computeHinv(ctx));
-  using Launch = void (*)(sphdem_ctx*);
-  static const Launch table[16] = {
-      launchForce<0>, launchForce<1>, launchForce<2>,  launchForce<3>,
-      launchForce<4>, launchForce<5>, launchForce<6>,  launchForce<7>,
-      launchForce<8>, launchForce<9>, launchForce<10>, launchForce<11>,
-      launchForce<12>, launchForce<13>, launchForce<14>, launchForce<15>};
+  static constexpr auto table = forceTable(std::make_integer_sequence<int, 32>{});
   const uint32_t f = ctx->P.flags;
-  const int sel = ((f & SPHDEM_F_VAR_H_CORRECTION) ? 1 : 0) | ((f & SPHDEM_F_VISC_MORRIS) ? 2 : 0) |
-                  ((f & SPHDEM_F_VISC_ARTIFICIAL) ? 4 : 0) | ((f & SPHDEM_F_LIQ_DEM) ? 8 : 0);
+  const int sel = ((f & SPHDEM_F_VAR_H_CORRECTION) ? 1 : 0) |
+                  ((f & SPHDEM_F_VISC_ARTIFICIAL) ? 2 : 0) | ((f & SPHDEM_F_LIQ_DEM) ? 4 : 0) |
+                  (viscVariant(f) << 3);
   Timed tm(ctx, KID_FORCE);
   if (mixed(ctx)) CHECK(packMix(ctx, MIX_P | MIX_V | MIX_D));
   table[sel](ctx);
@@ -1000,8 +1004,15 @@ void freeAll(sphdem_ctx* ctx) {
 // ============================================================ C ABI
 extern "C" {
 
+// at most one viscosity variant (calcViscForce, sph.h:579-631)
+static bool oneViscosity(uint32_t f) {
+  const uint32_t v = f & (SPHDEM_F_VISC_MORRIS | SPHDEM_F_VISC_CLEARY | SPHDEM_F_VISC_MONAGHAN);
+  return (v & (v - 1)) == 0;
+}
+
 int sphdem_create(const sphdem_params* params, int device, int64_t capacity, sphdem_ctx** out) {
   if (!params || !out || capacity <= 0 || capacity > (int64_t)0x3fffffff) return SPHDEM_E_ARG;
+  if (!oneViscosity(params->flags)) return SPHDEM_E_ARG;
   *out = nullptr;
   sphdem_ctx* ctx = new sphdem_ctx();
   ctx->P = *params;
@@ -1059,6 +1070,8 @@ const char* sphdem_last_error(const sphdem_ctx* ctx) { return ctx ? ctx->err : "
 
 int sphdem_set_params(sphdem_ctx* ctx, const sphdem_params* params) {
   if (!ctx || !params) return SPHDEM_E_ARG;
+  if (!oneViscosity(params->flags))
+    return fail(ctx, SPHDEM_E_ARG, "more than one of VISC_MORRIS, VISC_CLEARY, VISC_MONAGHAN");
   for (int d = 0; d < 3; d++)
     if (params->periodic[d] != ctx->P.periodic[d] || params->proc_domain[2 * d] != ctx->P.proc_domain[2 * d] ||
         params->proc_domain[2 * d + 1] != ctx->P.proc_domain[2 * d + 1])

@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/sphdem_c.h"
+
 namespace sphdem {
 
 enum : int {
@@ -90,6 +92,20 @@ __device__ __forceinline__ int xcdThread() {
   const int q = nb >> 3, rem = nb & 7;
   const int b = x * q + min(x, rem) + r;
   return b * blockDim.x + threadIdx.x;
+}
+
+// Viscosity variant of calcViscForce (sph.h:579-631): at most one is set.
+enum ViscVariant { VISC_NONE = 0, VISC_MOR = 1, VISC_CLE = 2, VISC_MON = 3 };
+__host__ __device__ inline int viscVariant(uint32_t flags) {
+  return (flags & SPHDEM_F_VISC_MORRIS)     ? VISC_MOR
+         : (flags & SPHDEM_F_VISC_CLEARY)   ? VISC_CLE
+         : (flags & SPHDEM_F_VISC_MONAGHAN) ? VISC_MON
+                                            : VISC_NONE;
+}
+// ALPHA = viscToAlpha(VISCOSITY, H, SPSOUND) (sph.h:42, misc.h:41-43), the
+// Monaghan viscosity coefficient
+__host__ __device__ inline double monaghanAlpha(const sphdem_params& P) {
+  return P.viscosity * (112.0 / 15.0) / (P.spsound * P.h);
 }
 
 // Uniform grid: cell_i = (int)((x_i - cmin_i)/edge + off); keys are

@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(256) k_dddt_m(Store S, int ns, const int* __re
 }
 
 // ---- calcForce, mixed --------------------------------------------------------------
-template <bool VARH, bool MORRIS, bool ARTIF, bool LIQDEM>
+template <bool VARH, int VISC, bool ARTIF, bool LIQDEM>
 __global__ void __launch_bounds__(256) k_force_m(Store S, int ns, const int* __restrict__ nbl,
                                                  const int* __restrict__ nbcnt,
                                                  const uint4* __restrict__ Pq,
@@ -224,7 +224,9 @@ __global__ void __launch_bounds__(256) k_force_m(Store S, int ns, const int* __r
     const float rhoa = (float)da.x, pdr2a = (float)da.y;
     const float nu = (float)P.viscosity, viscLiq = (float)(2.0 * P.viscosity * P.dens);
     const float cs2 = (float)(2.0 * P.spsound), alpha = (float)P.alpha_artificial;
-    const double vsigC = MORRIS ? 2.0 * P.spsound : 0.0;
+    const float nu198 = (float)(19.8 * P.viscosity), alphaM = (float)monaghanAlpha(P);
+    const double vsigC = (VISC == VISC_MOR || VISC == VISC_CLE) ? 2.0 * P.spsound : 0.0;
+    const double cdiff = VISC == VISC_MON ? 0.06 : 0.125;  // viscDiffusionCondition
     double minA = inf, minB = inf;
     double fx = 0.0, fy = 0.0, fz = 0.0, px = 0.0, py = 0.0, pz = 0.0, vx = 0.0, vy = 0.0,
            vz = 0.0;
@@ -261,17 +263,29 @@ __global__ void __launch_bounds__(256) k_force_m(Store S, int ns, const int* __r
           const float Gbar = 0.5f * (Ga + Gb);
           const double hav = 0.5 * (ha + (double)v[u].w);
           const float havf = (float)hav;
-          float fvs = 0.0f;  // fv = dv * fvs
+          float fvs = 0.0f;  // fv = dv * fvs (Morris)
+          float fvr = 0.0f;  // fv = dx * fvr (Cleary, Monaghan)
           float vsig = (float)vsigC;
-          if (MORRIS) {
+          const float vdr = (ARTIF || VISC == VISC_CLE || VISC == VISC_MON)
+                                ? dx.x * dv0 + dx.y * dv1 + dx.z * dv2
+                                : 0.0f;
+          if (VISC == VISC_MOR) {
             const float rr = frcp(rhoa * rhob);
             const float visc = LIQDEM ? viscLiq * rr : nu * (rhoa + rhob) * rr;
             const float dd = Gbar * dx.r2;  // dot(dx, dx * Gbar)
             const float den = dx.r2 + 0.01f * (havf * havf);
             fvs = mb * visc * dd * frcp(den);
           }
+          if (VISC == VISC_CLE) {
+            const float den = dx.r2 + 0.01f * (havf * havf);
+            fvr = mb * (nu198 * vdr * frcp((rhoa + rhob) * den)) * Gbar;
+          }
+          if (VISC == VISC_MON) {
+            const float viss = r != 0.0f ? vdr * frcp(r) : 0.0f;
+            if (!LIQDEM) vsig = cs2 + 2.0f * fabsf(viss);
+            fvr = mb * (viss * vsig * alphaM * frcp(rhoa + rhob)) * Gbar;
+          }
           if (ARTIF) {
-            const float vdr = dx.x * dv0 + dx.y * dv1 + dx.z * dv2;
             float avsig = 0.0f, avisc = 0.0f;
             if (vdr < 0.0f) {
               const float viss = r != 0.0f ? vdr * frcp(r) : 0.0f;
@@ -284,11 +298,16 @@ __global__ void __launch_bounds__(256) k_force_m(Store S, int ns, const int* __r
             fz += (double)(s * dx.z);
             vsig = fmaxf(vsig, avsig);
             newDt = dmin2(newDt, 0.6 * hav / (double)vsig);
+          } else if (VISC == VISC_MON) {
+            newDt = dmin2(newDt, 0.6 * hav / (double)vsig);
           } else {
             minA = dmin2(minA, 0.6 * hav);
           }
-          minB = dmin2(minB, 0.125 * (hav * hav));
-          const double f0 = (double)(dv0 * fvs), f1 = (double)(dv1 * fvs), f2 = (double)(dv2 * fvs);
+          minB = dmin2(minB, cdiff * (hav * hav));
+          constexpr bool alongDr = VISC == VISC_CLE || VISC == VISC_MON;
+          const double f0 = (double)(alongDr ? dx.x * fvr : dv0 * fvs);
+          const double f1 = (double)(alongDr ? dx.y * fvr : dv1 * fvs);
+          const double f2 = (double)(alongDr ? dx.z * fvr : dv2 * fvs);
           vx += f0;
           vy += f1;
           vz += f2;
@@ -306,7 +325,7 @@ __global__ void __launch_bounds__(256) k_force_m(Store S, int ns, const int* __r
         fz += q2;
       }
     }
-    if (!ARTIF && minA != inf) newDt = dmin2(newDt, minA / vsigC);
+    if (!ARTIF && VISC != VISC_MON && minA != inf) newDt = dmin2(newDt, minA / vsigC);
     if (minB != inf) newDt = dmin2(newDt, minB / P.viscosity);
     double4 f = S.frc[a], fpa = S.fp[a], fva = S.fv[a];
     f.x += fx;

@@ -454,12 +454,13 @@ __global__ void __launch_bounds__(256) k_dddt_l(Store S, int ns, Box B,
 
 // ---- calcForce (sphIncompress.h:899-972; sph.h:579-666, 692-734) over the list --
 // One instantiation per combination of the flags that change the pair
-// arithmetic.  The per-pair dt conditions (sph.h:545-560) are min-reduced into
-// newDt_bits.  Without artificial viscosity the signal speed is the constant
-// 2*SPSOUND (or 0), and x -> RN(x/c) is monotone, so the minimum over pairs of
-// RN(0.6*hav / vsig) is RN(min(0.6*hav) / vsig): the numerators are min-reduced
-// and divided once (likewise 0.125*hav^2 / VISCOSITY).
-template <bool VARH, bool MORRIS, bool ARTIF, bool LIQDEM>
+// arithmetic (VISC: the viscosity variant, VISC_NONE..VISC_MON).  The per-pair
+// dt conditions (sph.h:545-560) are min-reduced into newDt_bits.  When the
+// signal speed is the constant 2*SPSOUND (or 0) -- Morris or Cleary without
+// artificial viscosity -- x -> RN(x/c) is monotone, so the minimum over pairs
+// of RN(0.6*hav / vsig) is RN(min(0.6*hav) / vsig): the numerators are
+// min-reduced and divided once (likewise c_diff*hav^2 / VISCOSITY).
+template <bool VARH, int VISC, bool ARTIF, bool LIQDEM>
 __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
                                                  const int* __restrict__ nbl,
                                                  const int* __restrict__ nbcnt,
@@ -481,8 +482,10 @@ __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
     const double Ka = P.wcon * (rha * rha) * (rha * rha) * rom;
     const double wrom = P.wcon * rom;
     const double viscLiq = 2.0 * P.viscosity * P.dens;  // numerator under LIQ_DEM
-    const double vsigC = MORRIS ? 2.0 * P.spsound : 0.0;
-    double minA = inf, minB = inf;  // min over pairs of 0.6*hav and 0.125*hav^2
+    const double vsigC = (VISC == VISC_MOR || VISC == VISC_CLE) ? 2.0 * P.spsound : 0.0;
+    const double cdiff = VISC == VISC_MON ? 0.06 : 0.125;  // viscDiffusionCondition
+    const double alphaM = monaghanAlpha(P);
+    double minA = inf, minB = inf;  // min over pairs of 0.6*hav and cdiff*hav^2
     double4 f = S.frc[a], fpa = S.fp[a], fva = S.fv[a];
     const int n = nbcnt[a];
     constexpr int U = 4;
@@ -515,16 +518,30 @@ __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
           const double dv0 = va.x - v[u].x, dv1 = va.y - v[u].y, dv2 = va.z - v[u].z;
           const double Gbar = 0.5 * (Ga + Gb);  // gradW-bar = dr * Gbar
           const double hav = 0.5 * (ha + hb);
-          double fvs = 0.0;  // fv = dv * fvs
+          double fvs = 0.0;  // fv = dv * fvs (Morris)
+          double fvr = 0.0;  // fv = dr * fvr (Cleary, Monaghan: m_b visc gradW-bar)
           double vsig = vsigC;
-          if (MORRIS) {
+          const double vdr = (ARTIF || VISC == VISC_CLE || VISC == VISC_MON)
+                                 ? dot3(dx, dy, dz, dv0, dv1, dv2)
+                                 : 0.0;
+          if (VISC == VISC_MOR) {
             const double rab = 1.0 / (da.x * rhob);
             const double visc = LIQDEM ? viscLiq * rab : P.viscosity * (da.x + rhob) * rab;
             const double den = l2 + 0.01 * (hav * hav);
             fvs = mb * visc * (Gbar * l2) / den;  // dr . gradW-bar = Gbar |dr|^2
           }
+          if (VISC == VISC_CLE) {  // sph.h:598-604
+            const double den = l2 + 0.01 * (hav * hav);
+            const double visc = 19.8 * P.viscosity * vdr / ((da.x + rhob) * den);
+            fvr = mb * visc * Gbar;
+          }
+          if (VISC == VISC_MON) {  // sph.h:606-630, no SLK
+            const double viss = vdr * rinv;
+            if (!LIQDEM) vsig = 2.0 * P.spsound + 2.0 * fabs(viss);
+            const double visc = viss * vsig * alphaM / (da.x + rhob);
+            fvr = mb * visc * Gbar;
+          }
           if (ARTIF) {
-            const double vdr = dot3(dx, dy, dz, dv0, dv1, dv2);
             double avsig = 0, avisc = 0.0;
             if (vdr < 0) {
               const double viss = vdr * rinv;
@@ -537,11 +554,16 @@ __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
             f.z += sg * dz;
             vsig = dmax2(vsig, avsig);
             newDt = dmin2(newDt, 0.6 * hav / vsig);
+          } else if (VISC == VISC_MON) {
+            newDt = dmin2(newDt, 0.6 * hav / vsig);
           } else {
             minA = dmin2(minA, 0.6 * hav);
           }
-          minB = dmin2(minB, 0.125 * (hav * hav));
-          const double fv0 = dv0 * fvs, fv1 = dv1 * fvs, fv2 = dv2 * fvs;
+          minB = dmin2(minB, cdiff * (hav * hav));
+          constexpr bool alongDr = VISC == VISC_CLE || VISC == VISC_MON;
+          const double fv0 = alongDr ? dx * fvr : dv0 * fvs;
+          const double fv1 = alongDr ? dy * fvr : dv1 * fvs;
+          const double fv2 = alongDr ? dz * fvr : dv2 * fvs;
           fva.x += fv0;
           fva.y += fv1;
           fva.z += fv2;
@@ -561,7 +583,7 @@ __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
         f.z += p2;
       }
     }
-    if (!ARTIF && minA != inf) newDt = dmin2(newDt, minA / vsigC);
+    if (!ARTIF && VISC != VISC_MON && minA != inf) newDt = dmin2(newDt, minA / vsigC);
     if (minB != inf) newDt = dmin2(newDt, minB / P.viscosity);
     S.frc[a] = f;
     S.fp[a] = fpa;

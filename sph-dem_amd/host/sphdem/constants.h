@@ -41,6 +41,12 @@ inline sphdem_params paramsFromConstants() {
 #ifdef FIXED_DEM
   f |= SPHDEM_F_FIXED_DEM;
 #endif
+#ifdef VISC_MONAGHAN
+  f |= SPHDEM_F_VISC_MONAGHAN;
+#endif
+#ifdef VISC_CLEARY
+  f |= SPHDEM_F_VISC_CLEARY;
+#endif
   P.flags = f;
   P.precision = SPHDEM_FP64;
   for (int i = 0; i < 3; i++) {

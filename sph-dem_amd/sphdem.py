@@ -34,6 +34,8 @@ F_VISC_ARTIFICIAL = 1 << 4
 F_CUSTOM_WALL = 1 << 5
 F_NO_DEM_CONTACTS = 1 << 6
 F_FIXED_DEM = 1 << 7
+F_VISC_MONAGHAN = 1 << 8
+F_VISC_CLEARY = 1 << 9
 
 FP64, MIXED = 0, 1
 
@@ -354,7 +356,7 @@ _FLAG_DEFINES = [
     (F_VAR_H_CORRECTION, "VAR_H_CORRECTION"), (F_CONST_H, "CONST_H"), (F_LIQ_DEM, "LIQ_DEM"),
     (F_VISC_MORRIS, "VISC_MORRIS"), (F_VISC_ARTIFICIAL, "VISC_ARTIFICIAL"),
     (F_CUSTOM_WALL, "LIQ_DEM_CUSTOM_WALL_CONTACTS"), (F_NO_DEM_CONTACTS, "NO_DEM_CONTACTS"),
-    (F_FIXED_DEM, "FIXED_DEM"),
+    (F_FIXED_DEM, "FIXED_DEM"), (F_VISC_MONAGHAN, "VISC_MONAGHAN"), (F_VISC_CLEARY, "VISC_CLEARY"),
 ]
 
 

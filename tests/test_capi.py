@@ -71,3 +71,13 @@ def test_library_refuses_to_run_without_a_gpu():
     h = ctypes.c_void_p()
     rc = lib.sphdem_create(ctypes.byref(P), 0, 100, ctypes.byref(h))
     assert rc != 0
+
+
+def test_more_than_one_viscosity_variant_is_refused():
+    """calcViscForce takes one variant; two flags are an argument error,
+    reported before any device work."""
+    lib = sd.load_library()
+    P = sd.dambreak_case(nf=6).params
+    P.flags |= sd.F_VISC_CLEARY  # on top of VISC_MORRIS
+    h = ctypes.c_void_p()
+    assert lib.sphdem_create(ctypes.byref(P), 0, 100, ctypes.byref(h)) == -1

@@ -259,6 +259,40 @@ def test_steps(case, nsteps):
     assert_same_pairs(sim.pairs(sd.PAIRS_SPH), orc.pairs(sd.PAIRS_SPH))
 
 
+VISC_VARIANTS = {"cleary": sd.F_VISC_CLEARY, "monaghan": sd.F_VISC_MONAGHAN,
+                 "monaghan+artificial": sd.F_VISC_MONAGHAN | sd.F_VISC_ARTIFICIAL}
+
+
+def visc_case(flags):
+    c = dam_case()
+    c.params.flags = (c.params.flags & ~(sd.F_VISC_MORRIS | sd.F_VISC_ARTIFICIAL)) | flags
+    return c
+
+
+@pytest.mark.parametrize("variant", sorted(VISC_VARIANTS))
+def test_viscosity_variants(variant):
+    """calcViscForce's Cleary and Monaghan branches (sph.h:598-630) and their dt
+    conditions: the force pass on identical state, then three whole steps."""
+    c = visc_case(VISC_VARIANTS[variant])
+    print(f"viscosity variant {variant}", flush=True)
+    sim, orc = both(c)
+    perturb(c, sim, orc, seed=31)
+    fresh_newdt(sim, orc, c.params.maxdt)
+    sph = (c.iam == sd.SPH) | (c.iam == sd.SPH_BOUNDARY)
+    for x in (sim, orc):
+        x.pass_("eos")
+        x.pass_("force")
+    check_fields(sim, orc, ["f", "fp", "fv"], sph)
+    assert np.abs(orc.get("fv")[sph]).max() > 0
+    assert sim.reduce_dt() == pytest.approx(orc.new_dt(), rel=1e-15)
+    sim, orc = both(c)
+    sim.run(3)
+    orc.run(3)
+    assert sim.globals().dt == pytest.approx(orc.globals().dt, rel=1e-12)
+    alive = orc.get("iam") != -1
+    check_fields(sim, orc, ["r", "v", "vhat", "dens", "h", "f", "press"], alive, 1e-9)
+
+
 def test_reinit_every_step():
     """REINIT_DENS_EVERY_N_STEPS = 1 (as damBreak sets): 10 density passes per step."""
     c = dam_case()

@@ -11,8 +11,15 @@ import sphdem as sd
 
 pytestmark = pytest.mark.gpu
 
+def _monaghan():
+    c = sd.dambreak_case(nf=12)
+    c.params.flags = (c.params.flags & ~sd.F_VISC_MORRIS) | sd.F_VISC_MONAGHAN
+    return c
+
+
 CASES = {"dambreak": lambda: sd.dambreak_case(nf=12),
-         "sediment": lambda: sd.sediment_case(nf=12, n_dem=60)}
+         "sediment": lambda: sd.sediment_case(nf=12, n_dem=60),
+         "dambreak_monaghan": _monaghan}
 
 
 def pair_of_sims(mk):

@@ -343,6 +343,53 @@ def test_pair_forces_conserve_momentum():
     assert np.abs(f).max() > 0
 
 
+@pytest.mark.parametrize("visc", ["cleary", "monaghan"])
+def test_viscosity_variants_conserve_dissipate_and_ignore_rotation(visc):
+    """Cleary and Monaghan viscosity (sph.h:598-630) act along dr with a
+    coefficient in dv.dr: per pair antisymmetric (sum_a m_a fv_a = 0 in a
+    periodic box without VAR_H), dissipative (sum_a m_a vhat_a.fv_a < 0, since
+    the gradient factor F < 0), and zero where dv.dr = 0 for every pair (a
+    rigid rotation), which Morris viscosity is not."""
+    flag = {"cleary": sd.F_VISC_CLEARY, "monaghan": sd.F_VISC_MONAGHAN}[visc]
+    c = periodic_lattice_case(12)
+    P = c.params.copy()
+    P.flags = flag
+    c = sd.Case("lat", P, sd._jitter(c.r, P.psep, 5, 0.05) % 1.0, c.v, c.vhat, c.mass, c.h,
+                c.dens, c.iam, sd.HOOKS_NONE)
+    o = oracle.Oracle(c)
+    rng = np.random.default_rng(1)
+    vh = rng.normal(size=(c.n, 3)) * 0.1
+    o.set("vhat", vh)
+    o.set("dens", c.dens * (1 + 0.01 * rng.random(c.n)))
+    o.refresh_ghosts()
+    o.pass_("eos")
+    o.pass_("force")
+    fv = o.get("fv")
+    m = c.mass[:, None]
+    assert np.abs(fv).max() > 0
+    assert np.abs((m * fv).sum(0)).max() < 1e-10 * (m * np.abs(fv)).sum()
+    assert (m[:, 0] * (vh * fv).sum(1)).sum() < 0
+    # rigid rotation about the box centre: interior particles (all partners
+    # real, no periodic images) feel no viscous force; Morris's does not vanish
+    w = np.array([0.3, -0.2, 0.5])
+    rot = np.cross(w, c.r - 0.5)
+    inner = np.all(np.abs(c.r - 0.5) < 0.5 - 3 * P.h, axis=1)
+    scale = np.abs(fv).max()
+    for flags, vanishes in ((flag, True), (sd.F_VISC_MORRIS, False)):
+        P2 = P.copy()
+        P2.flags = flags
+        o2 = oracle.Oracle(c, params=P2)
+        o2.set("vhat", rot)
+        o2.refresh_ghosts()
+        o2.pass_("eos")
+        o2.pass_("force")
+        fr = np.abs(o2.get("fv")[inner]).max()
+        if vanishes:
+            assert fr < 1e-9 * scale
+        else:
+            assert fr > 1e-3 * scale
+
+
 def test_dddt_vanishes_under_rigid_translation(sed):
     o = oracle.Oracle(sed)
     o.set("vhat", np.tile([0.3, -0.2, 0.1], (sed.n, 1)))
