@@ -1,11 +1,19 @@
 // dataLL.h -- CdataLL for host code written against the reference's particle
 // container API (dataLL.h:96-389): constructed over a particleContainer and a
-// CglobalVars, reset(), traverse<F[, If]>() with host functors, getParticles(),
-// numParticles() and the public `globals`.  The particles live on the GPU in
-// the library's own layout (libsphdem.so, include/sphdem_c.h); the container is
-// a host view that is refreshed before a host functor runs and written back
-// after it.  Pair traversals (neighbours, neighboursGroup) run inside the
-// library's passes (CsphIncompress), not as host functors.
+// CglobalVars, reset(), traverse<F[, If]>() (with 0-3 extra arguments),
+// neighbours<F[, If]>(), neighboursGroup<F[, If]>(), neighboursGroupCoupling
+// <F, If>() with host functors, getParticles(), numParticles() and the public
+// `globals`.  The particles live on the GPU in the library's own layout
+// (libsphdem.so, include/sphdem_c.h); the container is a host view that is
+// refreshed before a host functor runs and written back after it.  The pair
+// traversals walk the library's own neighbour sets (sphdem_get_pairs: the
+// cached lists of the last reset, or a fresh coupling search), in its
+// cell-sorted query order and the reference's list order; a partner across a
+// periodic boundary is handed over as a ghost copy shifted by the box length,
+// as the reference's procGhostParticles are, and writes to ghost copies are
+// dropped (the reference keeps them only until its next ghost sync).  The
+// library's timestep passes (CsphIncompress) do not go through these: they run
+// on the GPU.
 //
 // Errors behave as in the reference: a message on cerr, then exit(-1)
 // (dataLL.cpp:200-211).
@@ -13,6 +21,7 @@
 
 #include <cstdint>
 #include <cstdlib>
+#include <deque>
 #include <iostream>
 #include <vector>
 
@@ -30,7 +39,7 @@ class CdataLL {
   // library keeps the particles cell-sorted.
   CdataLL(particleContainer& in_ps, CglobalVars& in_globals, bool opt, const sphdem_params& params,
           int device = 0)
-      : globals(in_globals), ps(in_ps) {
+      : globals(in_globals), ps(in_ps), params(params) {
     (void)opt;
     const int64_t n = (int64_t)ps.size();
     if (sphdem_create(&params, device, n > 0 ? n : 1, &ctx) != SPHDEM_OK) {
@@ -85,6 +94,86 @@ class CdataLL {
     pull();
     for (Cparticle& p : ps)
       if (p.iam != deleted && Iffunct(p)) Thefunct(p, globals, a);
+    push();
+  }
+
+  template <void Thefunct(Cparticle&, CglobalVars&, double)>
+  void traverse(double a) {
+    traverse<Thefunct, always>(a);
+  }
+  template <class T, void Thefunct(Cparticle&, CglobalVars&, T&), bool Iffunct(Cparticle&)>
+  void traverse(T& param) {
+    pull();
+    for (Cparticle& p : ps)
+      if (p.iam != deleted && Iffunct(p)) Thefunct(p, globals, param);
+    push();
+  }
+  template <void Thefunct(Cparticle&, CglobalVars&, double, double), bool Iffunct(Cparticle&)>
+  void traverse(double a, double b) {
+    pull();
+    for (Cparticle& p : ps)
+      if (p.iam != deleted && Iffunct(p)) Thefunct(p, globals, a, b);
+    push();
+  }
+  template <void Thefunct(Cparticle&, CglobalVars&, double, double, double),
+            bool Iffunct(Cparticle&)>
+  void traverse(double a, double b, double c) {
+    pull();
+    for (Cparticle& p : ps)
+      if (p.iam != deleted && Iffunct(p)) Thefunct(p, globals, a, b, c);
+    push();
+  }
+
+  // neighbours<F[, If]>() (dataLL.impl.h:165-204): F(pa, pb, globals) for every
+  // neighbour pb of every particle pa (that passes If), over the cached
+  // neighbour sets of the last reset.
+  template <void Thefunct(Cparticle&, Cparticle&, CglobalVars&)>
+  void neighbours() {
+    neighbours<Thefunct, always>();
+  }
+  template <void Thefunct(Cparticle&, Cparticle&, CglobalVars&), bool Iffunct(Cparticle&)>
+  void neighbours() {
+    pull();
+    Groups G = groups(SPHDEM_PAIRS_SPH, SPHDEM_PAIRS_DEM);
+    for (Group& q : G.list)
+      if (Iffunct(ps[q.a]))
+        for (Cparticle* pb : q.nb) Thefunct(ps[q.a], *pb, globals);
+    push();
+  }
+  // neighboursGroup<F[, If]>() (dataLL.impl.h:206-240): F(pa, neighbours of pa,
+  // globals) for every particle pa that passes If.
+  template <void Thefunct(Cparticle&, std::vector<Cparticle*>&, CglobalVars&)>
+  void neighboursGroup() {
+    neighboursGroup<Thefunct, always>();
+  }
+  template <void Thefunct(Cparticle&, std::vector<Cparticle*>&, CglobalVars&),
+            bool Iffunct(Cparticle&)>
+  void neighboursGroup() {
+    pull();
+    Groups G = groups(SPHDEM_PAIRS_SPH, SPHDEM_PAIRS_DEM);
+    for (Group& q : G.list)
+      if (Iffunct(ps[q.a])) Thefunct(ps[q.a], q.nb, globals);
+    push();
+  }
+  template <class T, void Thefunct(Cparticle&, std::vector<Cparticle*>&, CglobalVars&, T&),
+            bool Iffunct(Cparticle&)>
+  void neighboursGroup(T& param) {
+    pull();
+    Groups G = groups(SPHDEM_PAIRS_SPH, SPHDEM_PAIRS_DEM);
+    for (Group& q : G.list)
+      if (Iffunct(ps[q.a])) Thefunct(ps[q.a], q.nb, globals, param);
+    push();
+  }
+  // neighboursGroupCoupling<F, If>() (dataLL.impl.h:256-267): the coupling
+  // search (calcNeighboursCoupling, dataLL.cpp:1104-1192) done afresh: DEM
+  // partners of a fluid particle, fluid partners of a DEM particle.
+  template <void Thefunct(Cparticle&, std::vector<Cparticle*>&, CglobalVars&),
+            bool Iffunct(Cparticle&)>
+  void neighboursGroupCoupling() {
+    pull();
+    Groups G = groups(SPHDEM_PAIRS_SPH2DEM, SPHDEM_PAIRS_DEM2SPH);
+    for (Group& q : G.list)
+      if (Iffunct(ps[q.a])) Thefunct(ps[q.a], q.nb, globals);
     push();
   }
 
@@ -171,6 +260,53 @@ class CdataLL {
   CglobalVars& globals;
 
  private:
+  static bool always(Cparticle&) { return true; }
+
+  // The neighbour sets of a traversal: one group per live particle of the
+  // two query kinds (fluid / DEM), in the library's query order, then the
+  // particles without partners by index; ghost copies of periodic partners
+  // live in `ghosts` (a deque, so pointers to them stay valid).
+  struct Group {
+    int a;
+    std::vector<Cparticle*> nb;
+  };
+  struct Groups {
+    std::vector<Group> list;
+    std::deque<Cparticle> ghosts;
+  };
+  Groups groups(int kindFluid, int kindDem) {
+    Groups G;
+    const int n = (int)ps.size();
+    std::vector<int> slot(n, -1);
+    for (int kind : {kindFluid, kindDem}) {
+      int64_t cnt = 0;
+      check(sphdem_get_pairs(ctx, kind, &cnt, nullptr, nullptr, nullptr, 0));
+      if (cnt == 0) continue;
+      std::vector<int32_t> a(cnt), b(cnt), img(cnt);
+      check(sphdem_get_pairs(ctx, kind, &cnt, a.data(), b.data(), img.data(), cnt));
+      for (int64_t k = 0; k < cnt; k++) {
+        if (slot[a[k]] < 0) {
+          slot[a[k]] = (int)G.list.size();
+          G.list.push_back(Group{a[k], {}});
+        }
+        Cparticle* pb = &ps[b[k]];
+        if (img[k] != 13) {  // a periodic image: a shifted copy of b
+          G.ghosts.push_back(ps[b[k]]);
+          Cparticle& g = G.ghosts.back();
+          const int s[3] = {img[k] % 3 - 1, (img[k] / 3) % 3 - 1, img[k] / 9 - 1};
+          for (int d = 0; d < 3; d++)
+            if (s[d] != 0) g.r[d] = (s[d] > 0 ? params.rmax[d] - params.rmin[d]
+                                               : params.rmin[d] - params.rmax[d]) + g.r[d];
+          pb = &g;
+        }
+        G.list[slot[a[k]]].nb.push_back(pb);
+      }
+    }
+    for (int i = 0; i < n; i++)
+      if (slot[i] < 0 && ps[i].iam != deleted) G.list.push_back(Group{i, {}});
+    return G;
+  }
+
   struct VecField {
     int id;
     vect Cparticle::*m;
@@ -201,6 +337,7 @@ class CdataLL {
   }
 
   particleContainer& ps;
+  sphdem_params params;
   sphdem_ctx* ctx = nullptr;
 };
 
