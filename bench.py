@@ -23,6 +23,7 @@ Alongside the metric the JSON line carries
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -69,6 +70,8 @@ def parse():
                         "fluid particles advanced together, against the same boxes run "
                         "one after another")
     p.add_argument("--boxes", type=int, default=256, help="ensemble: number of boxes")
+    p.add_argument("--no-tune", action="store_true",
+                   help="ensemble: keep the default tile shape (no autotuning)")
     p.add_argument("--workers", type=int, default=8,
                    help="ensemble: host threads stepping the boxes")
     p.add_argument("--precision", choices=("fp64", "mixed"), default=None,
@@ -237,6 +240,26 @@ def ensemble(a):
     sd.run_many(sims, a.warmup, workers=a.workers)
     for s in sims:
         s.sync()
+    untuned = None
+    shapes = {}
+    if not a.no_tune:
+        # the default shape first, then the autotuner: one representative box
+        # per power-of-two fluid-count bucket is tuned (sphdem_tune_tile_shape,
+        # state restored) and its shape given to the bucket
+        t0 = time.perf_counter()
+        sd.run_many(sims, a.steps, workers=a.workers)
+        for s in sims:
+            s.sync()
+        untuned = time.perf_counter() - t0
+        buckets = {}
+        for s, c in zip(sims, cases):
+            buckets.setdefault(int(math.log2(int(np.sum(c.iam == sd.SPH)))), []).append(s)
+        for k, group in sorted(buckets.items()):
+            seg, blk, ms = group[0].tune_tile_shape(steps=3)
+            for s in group:
+                s.set_tile_shape(seg, blk)
+            shapes[f"fluid 2^{k}"] = {"boxes": len(group), "seg_particles": seg,
+                                      "walker_block": blk, "tuned_ms_per_step": ms}
     t0 = time.perf_counter()
     sd.run_many(sims, a.steps, workers=a.workers)
     for s in sims:
@@ -280,6 +303,11 @@ def ensemble(a):
             "what": "the same boxes one after another (sphdem_run_steps per box)",
         },
         "speedup_vs_sequential": seq / el,
+        "tile_shapes": shapes or None,
+        "default_shape": None if untuned is None else {
+            "value": counted * a.steps / untuned, "unit": "particle-steps/s",
+            "what": "the same boxes together with the default tile shape (64 particles per "
+                    "segment, 256-thread walker blocks), timed before tuning"},
         "roofline": None,
         "step_roofline": {
             "alg_bytes_per_step": step_bytes,

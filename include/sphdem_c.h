@@ -279,6 +279,23 @@ int sphdem_checkpoint_size(const sphdem_ctx* ctx, int64_t* bytes);
 int sphdem_checkpoint_save(sphdem_ctx* ctx, void* host, int64_t capacity);
 int sphdem_checkpoint_restore(sphdem_ctx* ctx, const void* host, int64_t bytes);
 
+/* ---- tile shape ----------------------------------------------------------- */
+/* No counterpart in the reference (a CPU code has no launch shape).  The list
+ * build packs whole cells of one (x,y) column into segments of at most
+ * seg_particles (8..64; one 64-lane wavefront per segment) and the SPH pair
+ * walkers run walker_block (64, 128 or 256) threads per workgroup.  Results do
+ * not depend on the shape (pair sets and list order are the reference's for
+ * any segment size); the defaults are 64 and 256. */
+int sphdem_set_tile_shape(sphdem_ctx* ctx, int seg_particles, int walker_block);
+int sphdem_get_tile_shape(const sphdem_ctx* ctx, int* seg_particles, int* walker_block);
+/* Autotuner: times `steps` steps (after one untimed step) of each shape of
+ * {64, 48, 32} x {256, 128, 64} from the context's current state, restored
+ * from a full-state checkpoint before each trial and after the last, keeps the
+ * fastest and returns it (and its ms per step if ms_per_step is not NULL).
+ * The simulation state is left as it was. */
+int sphdem_tune_tile_shape(sphdem_ctx* ctx, int steps, int hookset, int* seg_particles,
+                           int* walker_block, double* ms_per_step);
+
 /* Number of particles deleted as out of range since the context was created. */
 int sphdem_num_deleted(const sphdem_ctx* ctx, int64_t* n);
 

@@ -13,6 +13,7 @@
 #include <array>
 #include <utility>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <thread>
 #include <vector>
@@ -43,6 +44,9 @@ struct sphdem_ctx {
   int *segCnt = nullptr, *segOff = nullptr;
   Segment* segs = nullptr;
   int segColCap = 0, segCap = 0;
+  // tile shape: particles per list-build segment (<= 64, one wavefront) and
+  // threads per workgroup of the SPH pair walkers (sphdem_set_tile_shape)
+  int tileSeg = 64, walkTPB = 256;
   // cached SPH neighbour lists (valid from their first use until the next reset)
   int *nbl = nullptr, *nbcnt = nullptr;
   bool listsValid = false;
@@ -153,6 +157,10 @@ int fail(sphdem_ctx* c, int code, const char* fmt, ...) {
 
 constexpr int TPB = 256;
 inline int blocks(int n) { return (n + TPB - 1) / TPB; }
+// workgroups of an SPH pair walker over n particles (the tile shape's block)
+inline int walkBlocks(const sphdem_ctx* ctx, int n) {
+  return (n + ctx->walkTPB - 1) / ctx->walkTPB;
+}
 // grid of a grid-stride kernel that ends in one atomic per block on a single
 // address: those atomics serialise, so at most two blocks per CU
 inline int reduceBlocks(int n) { return std::min(blocks(n), 512); }
@@ -333,11 +341,12 @@ int buildSegments(sphdem_ctx* ctx) {
   // one wavefront per column; the count kernel's waves cover ncol + 1 (the
   // last one writes the scan's closing zero)
   hipLaunchKernelGGL(k_seg_count, dim3((ncol + 1 + kSegWaves - 1) / kSegWaves),
-                     dim3(64 * kSegWaves), 0, ctx->stream, ctx->start, g, ctx->segCnt);
+                     dim3(64 * kSegWaves), 0, ctx->stream, ctx->start, g, ctx->tileSeg,
+                     ctx->segCnt);
   LAUNCHED();
   CHECK(scanInts(ctx, ctx->segCnt, ctx->segOff, ncol + 1));
   hipLaunchKernelGGL(k_seg_write, dim3((ncol + kSegWaves - 1) / kSegWaves), dim3(64 * kSegWaves),
-                     0, ctx->stream, ctx->start, g, ctx->segOff, ctx->segs);
+                     0, ctx->stream, ctx->start, g, ctx->tileSeg, ctx->segOff, ctx->segs);
   LAUNCHED();
   return 0;
 }
@@ -702,11 +711,11 @@ int passOmega(sphdem_ctx* ctx) {
   Timed tm(ctx, KID_OMEGA);
   if (mixed(ctx)) {
     CHECK(packMix(ctx, MIX_P));
-    hipLaunchKernelGGL(k_omega_m, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
+    hipLaunchKernelGGL(k_omega_m, dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0, ctx->stream, ctx->cur,
                        ctx->ns, ctx->nbl, ctx->nbcnt, ctx->mixP, ctx->mixRes, ctx->P.wcon);
   } else {
     CHECK(packPosm(ctx));
-    hipLaunchKernelGGL(k_omega_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
+    hipLaunchKernelGGL(k_omega_l, dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0, ctx->stream, ctx->cur,
                        ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->posm, ctx->P.wcon);
   }
   LAUNCHED();
@@ -738,11 +747,11 @@ void launchForce(sphdem_ctx* ctx) {
   constexpr bool V = (F & 1) != 0, A = (F & 2) != 0, L = (F & 4) != 0;
   constexpr int M = F >> 3;
   if (mixed(ctx))
-    hipLaunchKernelGGL((k_force_m<V, M, A, L>), dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
+    hipLaunchKernelGGL((k_force_m<V, M, A, L>), dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0, ctx->stream,
                        ctx->cur, ctx->ns, ctx->nbl, ctx->nbcnt, ctx->mixP, ctx->mixV, ctx->mixD,
                        ctx->mixRes, ctx->P, ctx->dg);
   else
-    hipLaunchKernelGGL((k_force_l<V, M, A, L>), dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
+    hipLaunchKernelGGL((k_force_l<V, M, A, L>), dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0, ctx->stream,
                        ctx->cur, ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->hinv, ctx->P,
                        ctx->dg);
 }
@@ -774,22 +783,22 @@ int passDddt(sphdem_ctx* ctx) {
     Timed tm(ctx, KID_DDDT);
     CHECK(packMix(ctx, MIX_P | MIX_V));
     if (varH(ctx))
-      hipLaunchKernelGGL(k_dddt_m<true>, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
+      hipLaunchKernelGGL(k_dddt_m<true>, dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0, ctx->stream,
                          ctx->cur, ctx->ns, ctx->nbl, ctx->nbcnt, ctx->mixP, ctx->mixV,
                          ctx->mixRes, ctx->P.wcon);
     else
-      hipLaunchKernelGGL(k_dddt_m<false>, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream,
+      hipLaunchKernelGGL(k_dddt_m<false>, dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0, ctx->stream,
                          ctx->cur, ctx->ns, ctx->nbl, ctx->nbcnt, ctx->mixP, ctx->mixV,
                          ctx->mixRes, ctx->P.wcon);
   } else if (varH(ctx)) {
     Timed tm(ctx, KID_DDDT);
-    hipLaunchKernelGGL(k_dddt_l<true>, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
+    hipLaunchKernelGGL(k_dddt_l<true>, dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0, ctx->stream, ctx->cur,
                        ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, (const double2*)nullptr,
                        ctx->P.wcon);
   } else {
     CHECK(computeHinv(ctx));
     Timed tm(ctx, KID_DDDT);
-    hipLaunchKernelGGL(k_dddt_l<false>, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
+    hipLaunchKernelGGL(k_dddt_l<false>, dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0, ctx->stream, ctx->cur,
                        ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->hinv, ctx->P.wcon);
   }
   LAUNCHED();
@@ -836,11 +845,11 @@ int passDensity(sphdem_ctx* ctx) {
   CHECK(refreshGhosts(ctx, GS_POSH | GS_VHM));
   if (mixed(ctx)) {
     CHECK(packMix(ctx, MIX_P));
-    hipLaunchKernelGGL(k_density_m, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
+    hipLaunchKernelGGL(k_density_m, dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0, ctx->stream, ctx->cur,
                        ctx->ns, ctx->nbl, ctx->nbcnt, ctx->mixP, ctx->mixRes, ctx->P.wcon);
   } else {
     CHECK(packPosm(ctx));
-    hipLaunchKernelGGL(k_density_l, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
+    hipLaunchKernelGGL(k_density_l, dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0, ctx->stream, ctx->cur,
                        ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->posm, ctx->P.wcon);
   }
   LAUNCHED();
@@ -1148,7 +1157,7 @@ int sphdem_output_vars(sphdem_ctx* ctx, sphdem_output* out) {
   double* densCalc = ctx->fieldbuf;
   if (ns > 0) {
     CHECK(packPosm(ctx));
-    hipLaunchKernelGGL(k_density_l, dim3(blocks(ns)), dim3(TPB), 0, ctx->stream, ctx->cur, ns,
+    hipLaunchKernelGGL(k_density_l, dim3(walkBlocks(ctx, ns)), dim3(ctx->walkTPB), 0, ctx->stream, ctx->cur, ns,
                        ctx->box, ctx->nbl, ctx->nbcnt, ctx->posm, ctx->P.wcon, densCalc);
     LAUNCHED();
   }
@@ -1311,6 +1320,81 @@ int sphdem_checkpoint_restore(sphdem_ctx* ctx, const void* host, int64_t bytes) 
   // the store is in original-index order: sort it with the saved search
   // radius (the grid of the saved state)
   return doReset(ctx, true);
+}
+
+int sphdem_set_tile_shape(sphdem_ctx* ctx, int seg_particles, int walker_block) {
+  if (!ctx) return SPHDEM_E_ARG;
+  if (seg_particles < 8 || seg_particles > 64 ||
+      (walker_block != 64 && walker_block != 128 && walker_block != 256))
+    return fail(ctx, SPHDEM_E_ARG,
+                "tile shape %d particles per segment, %d threads per walker block: segments "
+                "hold 8..64 particles, walker blocks 64, 128 or 256 threads",
+                seg_particles, walker_block);
+  // the segments of the last reset stay a valid partition until the next
+  // reset rebuilds them with the new size
+  ctx->tileSeg = seg_particles;
+  ctx->walkTPB = walker_block;
+  return 0;
+}
+
+int sphdem_get_tile_shape(const sphdem_ctx* ctx, int* seg_particles, int* walker_block) {
+  if (!ctx || !seg_particles || !walker_block) return SPHDEM_E_ARG;
+  *seg_particles = ctx->tileSeg;
+  *walker_block = ctx->walkTPB;
+  return 0;
+}
+
+int sphdem_tune_tile_shape(sphdem_ctx* ctx, int steps, int hookset, int* seg_particles,
+                           int* walker_block, double* ms_per_step) {
+  CHECK(checkLoaded(ctx));
+  if (steps < 1 || steps > 1000 || !seg_particles || !walker_block) return SPHDEM_E_ARG;
+  HIPC(hipSetDevice(ctx->device));
+  int64_t bytes = 0;
+  CHECK(sphdem_checkpoint_size(ctx, &bytes));
+  std::vector<char> blob((size_t)bytes);
+  CHECK(sphdem_checkpoint_save(ctx, blob.data(), bytes));
+  static const int segs[] = {64, 48, 32};
+  static const int tpbs[] = {256, 128, 64};
+  const int seg0 = ctx->tileSeg, tpb0 = ctx->walkTPB;
+  double best = INFINITY;
+  int bestSeg = seg0, bestTpb = tpb0;
+  int rc = 0;
+  for (int sg : segs) {
+    for (int tb : tpbs) {
+      ctx->tileSeg = sg;
+      ctx->walkTPB = tb;
+      // the restore's reset rebuilds the segments with this size
+      if ((rc = sphdem_checkpoint_restore(ctx, blob.data(), bytes)) != 0) break;
+      if ((rc = sphdem_run_steps(ctx, 1, hookset)) != 0) break;  // first launches untimed
+      if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        rc = SPHDEM_E_HIP;
+        break;
+      }
+      const auto t0 = std::chrono::steady_clock::now();
+      if ((rc = sphdem_run_steps(ctx, steps, hookset)) != 0) break;
+      if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        rc = SPHDEM_E_HIP;
+        break;
+      }
+      const double el =
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (el < best) {
+        best = el;
+        bestSeg = sg;
+        bestTpb = tb;
+      }
+    }
+    if (rc) break;
+  }
+  ctx->tileSeg = rc ? seg0 : bestSeg;
+  ctx->walkTPB = rc ? tpb0 : bestTpb;
+  const int rrc = sphdem_checkpoint_restore(ctx, blob.data(), bytes);
+  if (rc) return rc;
+  if (rrc) return rrc;
+  *seg_particles = bestSeg;
+  *walker_block = bestTpb;
+  if (ms_per_step) *ms_per_step = 1e3 * best / steps;
+  return 0;
 }
 
 int sphdem_num_deleted(const sphdem_ctx* ctx, int64_t* n) {

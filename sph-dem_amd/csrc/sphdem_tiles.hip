@@ -21,10 +21,11 @@ namespace sphdem {
 constexpr int kWavesPerBlock = 4;
 
 // ---- segments -------------------------------------------------------------------
-// Greedy packing of one column's cells into segments of <= 64 particles;
-// s[z] = store start of cell z of the column, z = 0..Z (s[Z] the column end).
+// Greedy packing of one column's cells into segments of <= cap particles
+// (cap <= 64, the tile shape's segment size); s[z] = store start of cell z of
+// the column, z = 0..Z (s[Z] the column end).
 template <class E>
-__device__ __forceinline__ void walkColumn(const int* s, int Z, E&& emit) {
+__device__ __forceinline__ void walkColumn(const int* s, int Z, int cap, E&& emit) {
   int z = 0;
   while (z < Z) {
     const int c0 = s[z], c1 = s[z + 1];
@@ -32,8 +33,8 @@ __device__ __forceinline__ void walkColumn(const int* s, int Z, E&& emit) {
       z++;
       continue;
     }
-    if (c1 - c0 > 64) {  // an over-full cell is split into several segments
-      for (int p = c0; p < c1; p += 64) emit(p, min(p + 64, c1), z, z);
+    if (c1 - c0 > cap) {  // an over-full cell is split into several segments
+      for (int p = c0; p < c1; p += cap) emit(p, min(p + cap, c1), z, z);
       z++;
       continue;
     }
@@ -42,7 +43,7 @@ __device__ __forceinline__ void walkColumn(const int* s, int Z, E&& emit) {
     z++;
     while (z < Z) {
       const int k = s[z + 1] - s[z];
-      if (acc + k > 64) break;
+      if (acc + k > cap) break;
       acc += k;
       if (k) last = z;
       z++;
@@ -69,26 +70,27 @@ __device__ __forceinline__ bool stageColumn(const int* __restrict__ start, const
   return true;
 }
 __global__ void __launch_bounds__(64 * kSegWaves) k_seg_count(const int* __restrict__ start,
-                                                               Grid g, int* __restrict__ cnt) {
+                                                               Grid g, int cap,
+                                                               int* __restrict__ cnt) {
   __shared__ int lds[kSegWaves][kMaxZ + 1];
   int col;
   const int ncol = g.dim[0] * g.dim[1];
   if (!stageColumn(start, g, lds, col, [&](const int* s, int Z, int lane) {
         int c = 0;
-        walkColumn(s, Z, [&](int, int, int, int) { c++; });
+        walkColumn(s, Z, cap, [&](int, int, int, int) { c++; });
         if (lane == 0) cnt[col] = c;
       }))
     if (col == ncol && (threadIdx.x & 63) == 0) cnt[ncol] = 0;  // the scan's closing zero
 }
 __global__ void __launch_bounds__(64 * kSegWaves) k_seg_write(const int* __restrict__ start,
-                                                               Grid g,
+                                                               Grid g, int cap,
                                                                const int* __restrict__ off,
                                                                Segment* __restrict__ segs) {
   __shared__ int lds[kSegWaves][kMaxZ + 1];
   int col;
   stageColumn(start, g, lds, col, [&](const int* s, int Z, int lane) {
     int w = off[col];
-    walkColumn(s, Z, [&](int p0, int p1, int z0, int z1) {
+    walkColumn(s, Z, cap, [&](int p0, int p1, int z0, int z1) {
       if (lane == 0) segs[w] = Segment{p0, p1, col, z0 | (z1 << 16)};
       w++;
     });

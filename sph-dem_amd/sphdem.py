@@ -623,6 +623,12 @@ def load_library(path: str = LIB_PATH):
                                             i64]
     lib.sphdem_reduce_dt.argtypes = [vp, dp]
     lib.sphdem_num_deleted.argtypes = [vp, ctypes.POINTER(i64)]
+    lib.sphdem_set_tile_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    lib.sphdem_get_tile_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int)]
+    lib.sphdem_tune_tile_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_int),
+                                           ctypes.POINTER(ctypes.c_int), dp]
     lib.sphdem_timing_begin.argtypes = [vp, ctypes.c_char_p]
     lib.sphdem_timing_end.argtypes = [vp, dp, ctypes.POINTER(i64)]
     _lib = lib
@@ -822,6 +828,25 @@ class Sim:
         v = ctypes.c_int64()
         self._check(self.lib.sphdem_num_deleted(self.ctx, ctypes.byref(v)))
         return v.value
+
+    # ---- tile shape (sphdem_set/get/tune_tile_shape)
+    def set_tile_shape(self, seg_particles: int, walker_block: int):
+        self._check(self.lib.sphdem_set_tile_shape(self.ctx, seg_particles, walker_block))
+
+    def tile_shape(self) -> tuple[int, int]:
+        s, b = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.sphdem_get_tile_shape(self.ctx, ctypes.byref(s), ctypes.byref(b)))
+        return s.value, b.value
+
+    def tune_tile_shape(self, steps: int = 2, hookset: int | None = None):
+        """Times each candidate shape over `steps` steps from the current state
+        (restored afterwards), keeps the fastest; returns (segment particles,
+        walker block, ms per step)."""
+        hs = self.case.hookset if hookset is None else hookset
+        s, b, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self._check(self.lib.sphdem_tune_tile_shape(self.ctx, steps, hs, ctypes.byref(s),
+                                                    ctypes.byref(b), ctypes.byref(ms)))
+        return s.value, b.value, ms.value
 
     # ---- measurement: HIP events around each launch of one kernel
     def timing_begin(self, kernel: str):
