@@ -107,7 +107,11 @@ bool recordEvent(sphdem_ctx* c) {
   if (c->evUsed == c->evPool.size()) {
     if (c->evPool.size() >= kMaxTimedEvents) return false;
     hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return false;
+    // timestamps only (nothing synchronizes with these events but the final
+    // elapsed-time readout after a stream synchronize): no system-scope
+    // acquire/release, whose cache writeback and invalidation would cost the
+    // stream more than the kernels timed
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return false;
     c->evPool.push_back(e);
   }
   return hipEventRecord(c->evPool[c->evUsed++], c->stream) == hipSuccess;
@@ -481,8 +485,10 @@ struct Side {
   explicit Side(sphdem_ctx* c_, bool create = true) : c(c_), mainStream(c_->stream) {
     if (c->overlap && !c->side && create &&
         (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
-         hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming) != hipSuccess ||
-         hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming) != hipSuccess))
+         hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming | hipEventReleaseToDevice) !=
+             hipSuccess ||
+         hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming | hipEventReleaseToDevice) !=
+             hipSuccess))
       c->overlap = false;
     if (c->overlap && c->side && hipEventRecord(c->evFork, mainStream) == hipSuccess &&
         hipStreamWaitEvent(c->side, c->evFork, 0) == hipSuccess) {
@@ -830,7 +836,9 @@ int passFpi(sphdem_ctx* ctx) {
 int passContacts(sphdem_ctx* ctx) {
   if (ctx->nd == 0) return 0;
   Timed tm(ctx, KID_CONTACTS);
-  hipLaunchKernelGGL(k_contacts, dim3(blocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur,
+  // one wavefront per workgroup: the few DEM wavefronts (one thread per
+  // sphere, each walking 27 cells of dependent loads) spread over as many CUs
+  hipLaunchKernelGGL(k_contacts, dim3((ctx->nd + 63) / 64), dim3(64), 0, ctx->stream, ctx->cur,
                      ctx->ns, ctx->nd, ctx->gc, ctx->start, ctx->box, ctx->P);
   LAUNCHED();
   return 0;  // writes DEM fields only
