@@ -44,6 +44,7 @@ struct sphdem_ctx {
   int *segCnt = nullptr, *segOff = nullptr;
   Segment* segs = nullptr;
   int segColCap = 0, segCap = 0;
+  DevGlobals* hostG = nullptr;  // pinned host copy of dg for the reset's readback
   // tile shape: particles per list-build segment (<= 64, one wavefront) and
   // threads per workgroup of the SPH pair walkers (sphdem_set_tile_shape)
   int tileSeg = 64, walkTPB = 256;
@@ -544,8 +545,10 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false) {
     LAUNCHED();
   }
   if (pre) CHECK(scanInts(ctx, ctx->gcnt, ctx->goff, n + 1, &ctx->dg->ghostTotal));
-  DevGlobals hg;
-  HIPC(hipMemcpyAsync(&hg, ctx->dg, sizeof(DevGlobals), hipMemcpyDeviceToHost, ctx->stream));
+  // into pinned memory: a direct copy, no staging through a runtime buffer
+  HIPC(hipMemcpyAsync(ctx->hostG, ctx->dg, sizeof(DevGlobals), hipMemcpyDeviceToHost,
+                      ctx->stream));
+  const DevGlobals& hg = *ctx->hostG;
   HIPC(hipStreamSynchronize(ctx->stream));
   if (pre) ngPre = hg.ghostTotal;
   const double oldSsr = ctx->ssr;
@@ -1010,6 +1013,8 @@ void freeAll(sphdem_ctx* ctx) {
     if (p) (void)hipFree(p);
   for (hipEvent_t e : ctx->evPool) (void)hipEventDestroy(e);
   ctx->evPool.clear();
+  if (ctx->hostG) (void)hipHostFree(ctx->hostG);
+  ctx->hostG = nullptr;
   if (ctx->evFork) (void)hipEventDestroy(ctx->evFork);
   if (ctx->evJoin) (void)hipEventDestroy(ctx->evJoin);
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
@@ -1060,6 +1065,7 @@ int sphdem_create(const sphdem_params* params, int device, int64_t capacity, sph
     HIPC(hipMalloc(&ctx->errflag, sizeof(int)));
     HIPC(hipMalloc(&ctx->dg, sizeof(DevGlobals)));
     HIPC(hipMemset(ctx->dg, 0, sizeof(DevGlobals)));
+    HIPC(hipHostMalloc((void**)&ctx->hostG, sizeof(DevGlobals), hipHostMallocDefault));
     return 0;
   };
   rc = setup();
