@@ -293,6 +293,41 @@ def test_viscosity_variants(variant):
     check_fields(sim, orc, ["r", "v", "vhat", "dens", "h", "f", "press"], alive, 1e-9)
 
 
+def _with_flags(c, add=0, damptime=None):
+    c.params.flags |= add
+    if damptime is not None:
+        c.params.damptime = damptime
+    return c
+
+
+SWITCH_CASES = {
+    "const_h": lambda: _with_flags(dam_case(), sd.F_CONST_H),
+    "damptime": lambda: _with_flags(dam_case(), damptime=1),
+    "fixed_dem": lambda: _with_flags(contact_case(), sd.F_FIXED_DEM),
+    "no_dem_contacts": lambda: _with_flags(contact_case(), sd.F_NO_DEM_CONTACTS),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SWITCH_CASES))
+def test_step_switches(name):
+    """CONST_H (h kept), DAMPTIME (velocities x 0.98 while time < DAMPTIME,
+    sphIncompress.h:716-721), FIXED_DEM (DEM velocities restored after every
+    kick, no contacts) and NO_DEM_CONTACTS: three whole steps against the
+    oracle."""
+    c = SWITCH_CASES[name]()
+    print(f"switch case {name}", flush=True)
+    sim, orc = both(c)
+    sim.run(3)
+    orc.run(3)
+    assert sim.globals().dt == pytest.approx(orc.globals().dt, rel=1e-12)
+    assert sim.globals().time == pytest.approx(orc.globals().time, rel=1e-12)
+    alive = orc.get("iam") != -1
+    names = ["r", "v", "vhat", "dens", "h", "f", "press"]
+    if c.params.flags & sd.F_LIQ_DEM:
+        names += ["porosity", "fdrag", "fp"]
+    check_fields(sim, orc, names, alive, 1e-9)
+
+
 def test_reinit_every_step():
     """REINIT_DENS_EVERY_N_STEPS = 1 (as damBreak sets): 10 density passes per step."""
     c = dam_case()

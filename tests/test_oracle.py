@@ -390,6 +390,72 @@ def test_viscosity_variants_conserve_dissipate_and_ignore_rotation(visc):
             assert fr > 1e-3 * scale
 
 
+def _contact_sediment():
+    c = sd.sediment_case(nf=12, n_dem=40, seed=77)
+    dem = np.where(c.iam == sd.DEM)[0]
+    R = c.params.dem_radius
+    for k in range(0, len(dem) - 1, 2):
+        c.r[dem[k + 1]] = c.r[dem[k]] + np.array([1.9 * R, 0.0, 0.0])
+    return c
+
+
+def test_const_h_keeps_h():
+    """CONST_H: driftRest leaves h alone (sphIncompress.h:699)."""
+    c = sd.dambreak_case(nf=8)
+    c.params.flags |= sd.F_CONST_H
+    o = oracle.Oracle(c)
+    o.run(3)
+    alive = o.get("iam") != -1
+    assert np.array_equal(o.get("h")[alive], c.h[alive])
+    assert not np.array_equal(o.get("dens")[alive], c.dens[alive])
+
+
+def test_damptime_scales_the_first_kick():
+    """While time < DAMPTIME every kicked velocity is multiplied by 0.98
+    (sphIncompress.h:716-721): after the first step (dt = 0 in start(), so the
+    only kick is middle()'s) the damped SPH velocities are 0.98 times the
+    undamped ones, bit for bit."""
+    c = sd.dambreak_case(nf=8)
+    a = oracle.Oracle(c)
+    P = c.params.copy()
+    P.damptime = 1
+    b = oracle.Oracle(c, params=P)
+    a.run(1)
+    b.run(1)
+    sph = c.iam == sd.SPH
+    va, vb = a.get("v")[sph], b.get("v")[sph]
+    assert np.abs(va).max() > 0
+    assert np.array_equal(vb, va * 0.98)
+
+
+def test_fixed_dem_never_moves_and_feels_no_contacts():
+    """FIXED_DEM: DEM velocities are restored after each kick and contacts
+    are skipped (sphIncompress.h:666-671, 737-741; sphIncompress.cpp:229)."""
+    c = _contact_sediment()
+    c.params.flags |= sd.F_FIXED_DEM
+    o = oracle.Oracle(c)
+    dem = c.iam == sd.DEM
+    o.run(1)  # the first reset wraps spheres placed past the periodic faces
+    r1 = o.get("r")[dem]
+    o.run(2)
+    assert np.array_equal(o.get("v")[dem], np.zeros((dem.sum(), 3)))
+    assert np.array_equal(o.get("r")[dem], r1)
+
+
+def test_no_dem_contacts_skips_the_contact_force():
+    """NO_DEM_CONTACTS: overlapping spheres get no contact force (fp stays
+    zero, while the default run pushes them apart)."""
+    c = _contact_sediment()
+    dem = c.iam == sd.DEM
+    P = c.params.copy()
+    P.flags |= sd.F_NO_DEM_CONTACTS
+    with_c, without = oracle.Oracle(c), oracle.Oracle(c, params=P)
+    with_c.run(2)
+    without.run(2)
+    assert np.abs(with_c.get("fp")[dem]).max() > 0
+    assert np.abs(without.get("fp")[dem]).max() == 0
+
+
 def test_dddt_vanishes_under_rigid_translation(sed):
     o = oracle.Oracle(sed)
     o.set("vhat", np.tile([0.3, -0.2, 0.1], (sed.n, 1)))
