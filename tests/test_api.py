@@ -1,0 +1,84 @@
+"""The C ABI used the way a driver uses it: the phase calls with device hooks
+between them against sphdem_run_steps (whose hooks are fused), parameter
+changes on a live context against the oracle given the same changes, and the
+per-kernel timing window."""
+import numpy as np
+import pytest
+
+import oracle
+import sphdem as sd
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("r", "v", "vhat", "f", "fp", "fb", "fdrag", "dens", "h", "press", "porosity")
+G = np.array([0.0, 0.0, -9.81])
+
+
+def test_phase_calls_with_hooks_equal_run_steps():
+    """run.cpp's loop spelled out -- sphStep++, start, gravity on sph, middle,
+    the sediment3D beforeEnd hooks, end, afterEnd freeze -- ends where
+    sphdem_run_steps with the sediment hook set ends, bit for bit."""
+    case = sd.sediment_case(nf=12, n_dem=60)
+    a, b = sd.Sim(case), sd.Sim(case)
+    a.run(3)
+    P = case.params
+    for _ in range(3):
+        g = b.globals()
+        g.sphStep += 1
+        b.set_globals(g)
+        b.start()
+        b.hook_body_force(1 << sd.SPH, G)
+        b.middle()
+        b.hook_sum_drag()
+        b.hook_body_force(1 << sd.DEM, G)
+        b.hook_dem_floor()
+        b.hook_freeze(1 << sd.DEM, P.time_drop)
+        b.end()
+        b.hook_freeze(1 << sd.DEM, P.time_drop)
+    ga, gb = a.globals(), b.globals()
+    assert (ga.time, ga.dt, ga.sphStep) == (gb.time, gb.dt, gb.sphStep)
+    assert list(ga.custom) == list(gb.custom)
+    for nm in FIELDS:
+        assert np.array_equal(a.get(nm), b.get(nm)), nm
+
+
+def test_parameter_change_mid_run_matches_the_oracle():
+    """sphdem_set_params between steps (drag law, stiffness, viscosity) takes
+    effect from the next step as in the oracle given the same change."""
+    case = sd.sediment_case(nf=12, n_dem=60)
+    sim, orc = sd.Sim(case), oracle.Oracle(case)
+    sim.run(1)
+    orc.run(1)
+    P = case.params.copy()
+    P.drag_a *= 1.1
+    P.dem_k *= 0.9
+    P.viscosity *= 1.5
+    sim.set_params(P)
+    orc.set_params(P)
+    sim.run(2)
+    orc.run(2)
+    assert sim.globals().dt == pytest.approx(orc.globals().dt, rel=1e-12)
+    for nm in ("r", "vhat", "dens", "f", "fdrag", "porosity"):
+        a, b = sim.get(nm), orc.get(nm)
+        err = np.abs(a - b).max() / np.abs(b).max()
+        assert err <= 1e-9, (nm, err)
+    # the domain of a live context is fixed
+    Q = P.copy()
+    Q.proc_domain[0] -= 0.1
+    with pytest.raises(sd.SphDemError, match="domain"):
+        sim.set_params(Q)
+
+
+def test_kernel_timing_window_counts_launches():
+    sim = sd.Sim(sd.sediment_case(nf=12, n_dem=60))
+    sim.run(1)
+    sim.timing_begin("lists")
+    sim.run(3)
+    ms, launches = sim.timing_end()
+    assert launches == 6 and ms > 0  # two list builds per step (middle and end)
+    sim.timing_begin("force")
+    sim.run(2)
+    ms, launches = sim.timing_end()
+    assert launches == 2 and ms > 0
+    with pytest.raises(sd.SphDemError, match="unknown kernel"):
+        sim.timing_begin("no_such_kernel")
