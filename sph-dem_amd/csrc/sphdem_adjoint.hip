@@ -160,6 +160,12 @@ __global__ void k_force_fwd(int32_t nq, const int32_t* __restrict__ query,
   fp[3 * a + 2] += pz;
 }
 
+// Backward over the symmetric SPH lists, gathered: the thread of a adds the
+// adjoints of a's inputs from its own force terms (a the query) and from the
+// terms of its partners' forces in which a is the partner (b the query: the
+// same r, dx and dv with their signs flipped, F taken on both h, divided by
+// omega_b), so every adjoint is written by one thread, without atomics.  The
+// viscosity adjoint is summed over the own terms only (each pair once).
 __global__ void k_force_bwd(int32_t nq, const int32_t* __restrict__ query,
                             const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
                             const int32_t* __restrict__ pimg, sphdem_adj_fields in,
@@ -171,9 +177,9 @@ __global__ void k_force_bwd(int32_t nq, const int32_t* __restrict__ query,
     const int a = query[i];
     const double3 ra = ld3(in.r, a), va = ld3(in.vhat, a);
     const double ha = in.h[a], ia = 1.0 / in.omega[a], rhoa = in.dens[a], pda = in.pdr2[a];
+    const double ma = in.mass[a];
     const bool sphA = in.iam[a] == SPH_T;
     const double3 gv = ld3(gfv, a), gp = ld3(gfp, a);
-    // adjoints of a's inputs, summed over its pairs
     double grx = 0.0, gry = 0.0, grz = 0.0, gvx = 0.0, gvy = 0.0, gvz = 0.0;
     double gha = 0.0, gda = 0.0, gia = 0.0, gpa = 0.0;
     for (int64_t j = off[i]; j < off[i + 1]; j++) {
@@ -181,76 +187,101 @@ __global__ void k_force_bwd(int32_t nq, const int32_t* __restrict__ query,
       const double3 rb = imageOf(in.r, b, pimg[j], C.shift_len);
       const double dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
       const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
-      const double hb = in.h[b];
+      const double hb = in.h[b], rhob = in.dens[b], pdb = in.pdr2[b], mb = in.mass[b];
       double Fa, dFa_r, dFa_h, Fb, dFb_r, dFb_h;
       wendF(C.wcon, r, ha, Fa, dFa_r, dFa_h);
       wendF(C.wcon, r, hb, Fb, dFb_r, dFb_h);
-      const double mb = in.mass[b];
-      double r2bar = 0.0, Fabar = 0.0, Fbbar = 0.0, hbbar = 0.0;
+      const double3 vb = ld3(in.vhat, b);
+      const double dvx = va.x - vb.x, dvy = va.y - vb.y, dvz = va.z - vb.z;
+      const double visc = 2.0 * C.viscosity * C.dens / (rhoa * rhob);
+      const double Fs = Fa + Fb;
+      const double hav = 0.5 * (ha + hb);
+      const double den = r2 + 0.01 * hav * hav;
+      // adjoints gathered for this pair: of r^2, F(r, h_a), F(r, h_b), and
+      // the direct dx term; a's own h, dens, pdr2 in the running sums
+      double r2bar = 0.0, Fabar = 0.0, Fbbar = 0.0;
+      double dgx = 0.0, dgy = 0.0, dgz = 0.0;
+      // ---- a's own terms (a the query)
       if (sphA) {
-        const double rhob = in.dens[b];
-        const double visc = 2.0 * C.viscosity * C.dens / (rhoa * rhob);
-        const double Fs = Fa + Fb;
         const double d = r2 * Fs * 0.5 * ia;
-        const double hav = 0.5 * (ha + hb);
-        const double den = r2 + 0.01 * hav * hav;
         const double A = mb * visc * d / den;
-        const double3 vb = ld3(in.vhat, b);
-        const double dvx = va.x - vb.x, dvy = va.y - vb.y, dvz = va.z - vb.z;
-        // fv = A dv
-        const double Abar = gv.x * dvx + gv.y * dvy + gv.z * dvz;
+        const double Abar = gv.x * dvx + gv.y * dvy + gv.z * dvz;  // fv_a = A dv
         gvx += A * gv.x;
         gvy += A * gv.y;
         gvz += A * gv.z;
-        add3(g.vhat, b, -A * gv.x, -A * gv.y, -A * gv.z);
-        // A = m_b visc d / den
         const double viscbar = Abar * mb * d / den;
         const double dbar = Abar * mb * visc / den;
         const double denbar = -Abar * A / den;
-        // visc = 2 nu rho0 / (rho_a rho_b)
         gnu[0] += viscbar * visc / C.viscosity;
         gda += -viscbar * visc / rhoa;
-        add1(g.dens, b, -viscbar * visc / rhob);
-        // d = r^2 Fs ia / 2
-        r2bar += dbar * Fs * 0.5 * ia;
+        r2bar += dbar * Fs * 0.5 * ia + denbar;
         const double Fsbar = dbar * r2 * 0.5 * ia;
         gia += dbar * r2 * Fs * 0.5;
         Fabar += Fsbar;
         Fbbar += Fsbar;
-        // den = r^2 + 0.01 hav^2
-        r2bar += denbar;
-        const double havbar = denbar * 0.02 * hav;
-        gha += 0.5 * havbar;
-        hbbar += 0.5 * havbar;
+        gha += 0.5 * (denbar * 0.02 * hav);
       }
-      // fp = B dx, B = -m_b (pdr2_a Fa + pdr2_b Fb) ia
-      const double pdb = in.pdr2[b];
-      const double B = -mb * (pda * Fa + pdb * Fb) * ia;
-      const double Bbar = gp.x * dx + gp.y * dy + gp.z * dz;
-      gpa += -mb * Fa * ia * Bbar;
-      add1(g.pdr2, b, -mb * Fb * ia * Bbar);
-      Fabar += -mb * pda * ia * Bbar;
-      Fbbar += -mb * pdb * ia * Bbar;
-      gia += -mb * (pda * Fa + pdb * Fb) * Bbar;
-      // F(r, h)
+      {  // pressure: fp_a = B dx, B = -m_b (pdr2_a Fa + pdr2_b Fb) ia
+        const double B = -mb * (pda * Fa + pdb * Fb) * ia;
+        const double Bbar = gp.x * dx + gp.y * dy + gp.z * dz;
+        gpa += -mb * Fa * ia * Bbar;
+        Fabar += -mb * pda * ia * Bbar;
+        Fbbar += -mb * pdb * ia * Bbar;
+        gia += -mb * (pda * Fa + pdb * Fb) * Bbar;
+        dgx += B * gp.x;
+        dgy += B * gp.y;
+        dgz += B * gp.z;
+      }
+      // ---- b's terms with a as the partner (dx, dv flip sign; ib = 1/omega_b)
+      const double3 gvb = ld3(gfv, b), gpb = ld3(gfp, b);
+      const double ib = 1.0 / in.omega[b];
+      if (in.iam[b] == SPH_T) {
+        const double d = r2 * Fs * 0.5 * ib;
+        const double A = ma * visc * d / den;
+        // fv_b = A (vhat_b - vhat_a) = -A dv
+        const double Abar = -(gvb.x * dvx + gvb.y * dvy + gvb.z * dvz);
+        gvx += -A * gvb.x;
+        gvy += -A * gvb.y;
+        gvz += -A * gvb.z;
+        const double viscbar = Abar * ma * d / den;
+        const double dbar = Abar * ma * visc / den;
+        const double denbar = -Abar * A / den;
+        gda += -viscbar * visc / rhoa;
+        r2bar += dbar * Fs * 0.5 * ib + denbar;
+        const double Fsbar = dbar * r2 * 0.5 * ib;
+        Fabar += Fsbar;
+        Fbbar += Fsbar;
+        gha += 0.5 * (denbar * 0.02 * hav);
+      }
+      {  // fp_b = B (-dx), B = -m_a (pdr2_b Fb + pdr2_a Fa) ib
+        const double B = -ma * (pdb * Fb + pda * Fa) * ib;
+        const double Bbar = -(gpb.x * dx + gpb.y * dy + gpb.z * dz);
+        gpa += -ma * Fa * ib * Bbar;
+        Fabar += -ma * pda * ib * Bbar;
+        Fbbar += -ma * pdb * ib * Bbar;
+        // d fp_b / d r_a through -dx: -B gp_b
+        dgx += -B * gpb.x;
+        dgy += -B * gpb.y;
+        dgz += -B * gpb.z;
+      }
+      // F(r, h): the r dependence goes to r_a; the h_a dependence of F(r, h_a)
       const double rbar = Fabar * dFa_r + Fbbar * dFb_r;
       gha += Fabar * dFa_h;
-      hbbar += Fbbar * dFb_h;
-      add1(g.h, b, hbbar);
-      // dx: direct term B gp, through r^2 (2 dx) and through r (dx / r)
-      const double s = 2.0 * r2bar + (r > 0.0 ? rbar / r : 0.0);
-      const double gx = B * gp.x + s * dx, gy = B * gp.y + s * dy, gz = B * gp.z + s * dz;
-      grx += gx;
-      gry += gy;
-      grz += gz;
-      add3(g.r, b, -gx, -gy, -gz);
+      const double sc = 2.0 * r2bar + (r > 0.0 ? rbar / r : 0.0);
+      grx += dgx + sc * dx;
+      gry += dgy + sc * dy;
+      grz += dgz + sc * dz;
     }
-    add3(g.r, a, grx, gry, grz);
-    add3(g.vhat, a, gvx, gvy, gvz);
-    add1(g.h, a, gha);
-    add1(g.dens, a, gda);
-    add1(g.omega, a, -gia * ia * ia);  // ia = 1/omega_a
-    add1(g.pdr2, a, gpa);
+    g.r[3 * a] += grx;
+    g.r[3 * a + 1] += gry;
+    g.r[3 * a + 2] += grz;
+    g.vhat[3 * a] += gvx;
+    g.vhat[3 * a + 1] += gvy;
+    g.vhat[3 * a + 2] += gvz;
+    g.h[a] += gha;
+    g.dens[a] += gda;
+    g.omega[a] += -gia * ia * ia;  // ia = 1/omega_a
+    g.pdr2[a] += gpa;
   }
   blockAdd<1>(gnu, g.params);
 }
@@ -298,6 +329,12 @@ __global__ void k_omega_fwd(int32_t nq, const int32_t* __restrict__ query,
   omega[a] += ha * S / (in.dens[a] * 3.0);
 }
 
+// Backward of omega over the symmetric SPH lists (b in the list of a iff a is
+// in the list of b, with the opposite image): the thread of a adds the terms
+// of its own sum and, from the same entries, those of its partners' sums in
+// which a appears, so every adjoint is written by one thread, without atomics.
+//   own:     Sbar_a = go_a h_a / (3 rho_a),  r_a += Sbar_a m_b dP(r, h_a)/dr dx/r
+//   partner: r_a   += Sbar_b m_a dP(r, h_b)/dr dx/r     (dx = r_a - r_b')
 __global__ void k_omega_bwd(int32_t nq, const int32_t* __restrict__ query,
                             const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
                             const int32_t* __restrict__ pimg, sphdem_adj_fields in,
@@ -307,41 +344,55 @@ __global__ void k_omega_bwd(int32_t nq, const int32_t* __restrict__ query,
   if (i >= nq) return;
   const int a = query[i];
   const double go = gomega[a];
-  if (go == 0.0) return;
   const double3 ra = ld3(in.r, a);
-  const double ha = in.h[a], rhoa = in.dens[a];
+  const double ha = in.h[a], rhoa = in.dens[a], ma = in.mass[a];
   double P0, dP0r, dP0h, P, dPr, dPh;
   wendP(C.wcon, 0.0, ha, P0, dP0r, dP0h);
-  double S = in.mass[a] * P0;
-  for (int64_t j = off[i]; j < off[i + 1]; j++) {
-    const int b = pb[j];
-    const double3 rb = imageOf(in.r, b, pimg[j], C.shift_len);
-    const double dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
-    wendP(C.wcon, sqrt(dx * dx + dy * dy + dz * dz), ha, P, dPr, dPh);
-    S += in.mass[b] * P;
+  double S = ma * P0;
+  if (go != 0.0) {
+    for (int64_t j = off[i]; j < off[i + 1]; j++) {
+      const int b = pb[j];
+      const double3 rb = imageOf(in.r, b, pimg[j], C.shift_len);
+      const double dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
+      wendP(C.wcon, sqrt(dx * dx + dy * dy + dz * dz), ha, P, dPr, dPh);
+      S += in.mass[b] * P;
+    }
   }
   const double Sbar = go * ha / (3.0 * rhoa);
-  double gha = go * S / (3.0 * rhoa) + Sbar * in.mass[a] * dP0h;
-  add1(g.dens, a, -go * ha * S / (3.0 * rhoa * rhoa));
+  double gha = go * S / (3.0 * rhoa) + Sbar * ma * dP0h;
+  const double gda = -go * ha * S / (3.0 * rhoa * rhoa);
   double grx = 0.0, gry = 0.0, grz = 0.0;
   for (int64_t j = off[i]; j < off[i + 1]; j++) {
     const int b = pb[j];
     const double3 rb = imageOf(in.r, b, pimg[j], C.shift_len);
     const double dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
     const double r = sqrt(dx * dx + dy * dy + dz * dz);
-    wendP(C.wcon, r, ha, P, dPr, dPh);
-    const double c = Sbar * in.mass[b];
-    gha += c * dPh;
-    if (r > 0.0) {
-      const double k = c * dPr / r;
-      grx += k * dx;
-      gry += k * dy;
-      grz += k * dz;
-      add3(g.r, b, -k * dx, -k * dy, -k * dz);
+    double k = 0.0;
+    if (Sbar != 0.0) {  // a's own sum
+      wendP(C.wcon, r, ha, P, dPr, dPh);
+      const double c = Sbar * in.mass[b];
+      gha += c * dPh;
+      k += c * dPr;
+    }
+    const double gob = gomega[b];
+    if (gob != 0.0) {  // b's sum, in which a is the partner
+      const double hb = in.h[b];
+      double Pb, dPbr, dPbh;
+      wendP(C.wcon, r, hb, Pb, dPbr, dPbh);
+      k += gob * hb / (3.0 * in.dens[b]) * ma * dPbr;
+    }
+    if (r > 0.0 && k != 0.0) {
+      const double kr = k / r;
+      grx += kr * dx;
+      gry += kr * dy;
+      grz += kr * dz;
     }
   }
-  add3(g.r, a, grx, gry, grz);
-  add1(g.h, a, gha);
+  g.r[3 * a] += grx;
+  g.r[3 * a + 1] += gry;
+  g.r[3 * a + 2] += grz;
+  g.h[a] += gha;
+  g.dens[a] += gda;
 }
 
 // ---------------------------------------------------------------- dddt
@@ -371,6 +422,10 @@ __global__ void k_dddt_fwd(int32_t nq, const int32_t* __restrict__ query,
   dddt[a] += sum;
 }
 
+// Backward of dddt over the symmetric SPH lists, gathered as for omega: the
+// thread of a adds the terms of dddt_a and of every dddt_b in which a is the
+// partner (u = dv.dx and dx change sign between the two, F is taken on the
+// query's h, divided by the query's omega).
 __global__ void k_dddt_bwd(int32_t nq, const int32_t* __restrict__ query,
                            const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
                            const int32_t* __restrict__ pimg, sphdem_adj_fields in,
@@ -380,42 +435,54 @@ __global__ void k_dddt_bwd(int32_t nq, const int32_t* __restrict__ query,
   if (i >= nq) return;
   const int a = query[i];
   const double gd = gdddt[a];
-  if (gd == 0.0) return;
   const double3 ra = ld3(in.r, a), va = ld3(in.vhat, a);
-  const double ha = in.h[a], ia = 1.0 / in.omega[a];
+  const double ha = in.h[a], ia = 1.0 / in.omega[a], ma = in.mass[a];
   double grx = 0.0, gry = 0.0, grz = 0.0, gvx = 0.0, gvy = 0.0, gvz = 0.0, gha = 0.0, gia = 0.0;
   for (int64_t j = off[i]; j < off[i + 1]; j++) {
     const int b = pb[j];
+    const double gdb = gdddt[b];
+    if (gd == 0.0 && gdb == 0.0) continue;
     const double3 rb = imageOf(in.r, b, pimg[j], C.shift_len);
     const double dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
     const double r = sqrt(dx * dx + dy * dy + dz * dz);
-    double F, dFr, dFh;
-    wendF(C.wcon, r, ha, F, dFr, dFh);
     const double3 vb = ld3(in.vhat, b);
     const double dvx = va.x - vb.x, dvy = va.y - vb.y, dvz = va.z - vb.z;
-    const double u = dvx * dx + dvy * dy + dvz * dz;
-    const double mb = in.mass[b];
-    // term = m_b u F ia
-    const double ubar = gd * mb * F * ia;
-    const double Fbar = gd * mb * u * ia;
-    gia += gd * mb * u * F;
-    gha += Fbar * dFh;
-    // u = dv . dx
+    const double u = dvx * dx + dvy * dy + dvz * dz;  // the same for (b, a)
+    double ubar = 0.0, k = 0.0;
+    if (gd != 0.0) {  // term m_b u F(r, h_a) / omega_a of dddt_a
+      double F, dFr, dFh;
+      wendF(C.wcon, r, ha, F, dFr, dFh);
+      const double mb = in.mass[b];
+      ubar += gd * mb * F * ia;
+      const double Fbar = gd * mb * u * ia;
+      gia += gd * mb * u * F;
+      gha += Fbar * dFh;
+      k += Fbar * dFr;
+    }
+    if (gdb != 0.0) {  // term m_a u F(r, h_b) / omega_b of dddt_b
+      double Fb, dFbr, dFbh;
+      wendF(C.wcon, r, in.h[b], Fb, dFbr, dFbh);
+      const double ib = 1.0 / in.omega[b];
+      ubar += gdb * ma * Fb * ib;
+      k += gdb * ma * u * ib * dFbr;
+    }
+    // u = dv.dx: d/dvhat_a = dx, d/dr_a = dv; r = |dx|
     gvx += ubar * dx;
     gvy += ubar * dy;
     gvz += ubar * dz;
-    add3(g.vhat, b, -ubar * dx, -ubar * dy, -ubar * dz);
-    const double k = r > 0.0 ? Fbar * dFr / r : 0.0;
-    const double gx = ubar * dvx + k * dx, gy = ubar * dvy + k * dy, gz = ubar * dvz + k * dz;
-    grx += gx;
-    gry += gy;
-    grz += gz;
-    add3(g.r, b, -gx, -gy, -gz);
+    const double kr = r > 0.0 ? k / r : 0.0;
+    grx += ubar * dvx + kr * dx;
+    gry += ubar * dvy + kr * dy;
+    grz += ubar * dvz + kr * dz;
   }
-  add3(g.r, a, grx, gry, grz);
-  add3(g.vhat, a, gvx, gvy, gvz);
-  add1(g.h, a, gha);
-  add1(g.omega, a, -gia * ia * ia);
+  g.r[3 * a] += grx;
+  g.r[3 * a + 1] += gry;
+  g.r[3 * a + 2] += grz;
+  g.vhat[3 * a] += gvx;
+  g.vhat[3 * a + 1] += gvy;
+  g.vhat[3 * a + 2] += gvz;
+  g.h[a] += gha;
+  g.omega[a] += -gia * ia * ia;
 }
 
 // ------------------------------------------------------ fluid-side coupling
