@@ -31,8 +31,13 @@
  *     b[off[i] .. off[i+1]) with their images; one thread per query keeps a's
  *     sums in registers.
  *   - Forward outputs and backward adjoints are ADDED to the arrays given (the
- *     caller zeroes them); the sums use atomics, so the last bits vary run to
- *     run.
+ *     caller zeroes them).  The backward kernels of force, omega and dddt
+ *     gather: the thread of a particle also adds the terms of the pairs in
+ *     which it is the partner, which requires a symmetric pair set (every
+ *     (a, b, img) has its (b, a, 26 - img)); the SPH set of sphdem_get_pairs
+ *     is symmetric (the criterion uses max(h_a, h_b)).  The coupling, fpi
+ *     and contact backward kernels add partner terms with FP64 atomics, so
+ *     their last bits vary run to run.
  *   - Every call is ordered on `stream` (a hipStream_t; NULL = the null
  *     stream) and returns 0, or SPHDEM_E_ARG for a null pointer or a negative
  *     count, or SPHDEM_E_HIP if a launch fails.
