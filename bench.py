@@ -44,12 +44,12 @@ B_ALG_KERNEL = {
     # calcNeighbours: r h -> the list (4 B per entry, filled in below from the
     # measured mean list length) + its count
     "lists": 32 + 4,
-    # calcForce: r vhat h mass dens pdr2 iam omega porosity -> f fp fv fb
-    "force": 84 + 16 + 24 + 72,
-    # calcOmega: r h mass dens iam -> omega
-    "omega": 60,
-    # calcDddtDudt: r vhat h mass iam omega -> dddt
-    "dddt": 76 + 8,
+    # calcForce with calcOmega in the same walk (the step's omega pass, VAR_H):
+    # r vhat h mass dens pdr2 iam porosity -> f fp fv fb omega
+    "force": 76 + 16 + 24 + 72 + 8,
+    # calcDddtDudt with calcOmega in the same walk: r vhat h mass dens iam ->
+    # dddt omega
+    "dddt": 76 + 8 + 8,
     # porosity (28 -> 8) + DEM->SPH drag (56 -> 48)
     "coupling": 36 + 104,
 }
@@ -130,7 +130,7 @@ def cpu_baseline(case, budget_s):
     }
 
 
-KERNEL_SYMBOL = {"lists": "k_lists", "force": "k_force_l", "omega": "k_omega_l",
+KERNEL_SYMBOL = {"lists": "k_lists", "force": "k_force_l",
                  "dddt": "k_dddt_l", "coupling": "k_coupling_sph",
                  "adj_force_bwd": "k_force_bwd"}
 

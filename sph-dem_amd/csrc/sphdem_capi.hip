@@ -750,48 +750,63 @@ int computeHinv(sphdem_ctx* ctx) {
   return 0;
 }
 // calcForce instantiated per flag combination (bit 0 VAR_H_CORRECTION, 1 VISC_ARTIFICIAL,
-// 2 LIQ_DEM, bits 3-4 the viscosity variant)
-template <int F>
+// 2 LIQ_DEM, bits 3-4 the viscosity variant); OMG: the walkers with calcOmega fused
+// in (VAR_H only: the OMG entries without VAR_H are empty and never selected)
+template <int F, bool OMG>
 void launchForce(sphdem_ctx* ctx) {
   constexpr bool V = (F & 1) != 0, A = (F & 2) != 0, L = (F & 4) != 0;
   constexpr int M = F >> 3;
-  if (mixed(ctx))
-    hipLaunchKernelGGL((k_force_m<V, M, A, L>), dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0, ctx->stream,
-                       ctx->cur, ctx->ns, ctx->nbl, ctx->nbcnt, ctx->mixP, ctx->mixV, ctx->mixD,
-                       ctx->mixRes, ctx->P, ctx->dg);
-  else
-    hipLaunchKernelGGL((k_force_l<V, M, A, L>), dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0, ctx->stream,
-                       ctx->cur, ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->hinv, ctx->P,
-                       ctx->dg);
+  if constexpr (!OMG || V) {
+    if (mixed(ctx))
+      hipLaunchKernelGGL((k_force_m<V, M, A, L, OMG>), dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0,
+                         ctx->stream, ctx->cur, ctx->ns, ctx->nbl, ctx->nbcnt, ctx->mixP, ctx->mixV,
+                         ctx->mixD, ctx->mixRes, ctx->P, ctx->dg);
+    else
+      hipLaunchKernelGGL((k_force_l<V, M, A, L, OMG>), dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0,
+                         ctx->stream, ctx->cur, ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->hinv,
+                         ctx->P, ctx->dg);
+  }
 }
-template <int... F>
+template <bool OMG, int... F>
 constexpr std::array<void (*)(sphdem_ctx*), sizeof...(F)> forceTable(
     std::integer_sequence<int, F...>) {
-  return {launchForce<F>...};
+  return {launchForce<F, OMG>...};
 }
-// hinvReady: the 1/h table is already current (made beside the list build)
-int passForce(sphdem_ctx* ctx, bool hinvReady = false) {
+// hinvReady: the 1/h table is already current (made beside the list build);
+// withOmega (VAR_H only): calcOmega summed in the same walk and omega stored --
+// the step's omega pass, which only the force and dddt of omega's own particle
+// read, fused into the walk that follows it
+int passForce(sphdem_ctx* ctx, bool hinvReady = false, bool withOmega = false) {
+  if (withOmega && !varH(ctx)) return fail(ctx, SPHDEM_E_STATE, "fused omega needs VAR_H");
   CHECK(ensureLists(ctx));
   CHECK(refreshGhosts(ctx, GS_ALL));
   if (!mixed(ctx) && !hinvReady) CHECK(computeHinv(ctx));
-  static constexpr auto table = forceTable(std::make_integer_sequence<int, 32>{});
+  static constexpr auto table = forceTable<false>(std::make_integer_sequence<int, 32>{});
+  static constexpr auto tableOmega = forceTable<true>(std::make_integer_sequence<int, 32>{});
   const uint32_t f = ctx->P.flags;
   const int sel = ((f & SPHDEM_F_VAR_H_CORRECTION) ? 1 : 0) |
                   ((f & SPHDEM_F_VISC_ARTIFICIAL) ? 2 : 0) | ((f & SPHDEM_F_LIQ_DEM) ? 4 : 0) |
                   (viscVariant(f) << 3);
   Timed tm(ctx, KID_FORCE);
   if (mixed(ctx)) CHECK(packMix(ctx, MIX_P | MIX_V | MIX_D));
-  table[sel](ctx);
+  (withOmega ? tableOmega : table)[sel](ctx);
   LAUNCHED();
+  if (withOmega) markStale(ctx, GS_DPO);
   return 0;
 }
-int passDddt(sphdem_ctx* ctx) {
+// withOmega: as in passForce
+int passDddt(sphdem_ctx* ctx, bool withOmega = false) {
+  if (withOmega && !varH(ctx)) return fail(ctx, SPHDEM_E_STATE, "fused omega needs VAR_H");
   CHECK(ensureLists(ctx));
   CHECK(refreshGhosts(ctx, GS_POSH | GS_VHM));
   if (mixed(ctx)) {
     Timed tm(ctx, KID_DDDT);
     CHECK(packMix(ctx, MIX_P | MIX_V));
-    if (varH(ctx))
+    if (varH(ctx) && withOmega)
+      hipLaunchKernelGGL((k_dddt_m<true, true>), dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0,
+                         ctx->stream, ctx->cur, ctx->ns, ctx->nbl, ctx->nbcnt, ctx->mixP, ctx->mixV,
+                         ctx->mixRes, ctx->P.wcon);
+    else if (varH(ctx))
       hipLaunchKernelGGL(k_dddt_m<true>, dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0, ctx->stream,
                          ctx->cur, ctx->ns, ctx->nbl, ctx->nbcnt, ctx->mixP, ctx->mixV,
                          ctx->mixRes, ctx->P.wcon);
@@ -801,9 +816,14 @@ int passDddt(sphdem_ctx* ctx) {
                          ctx->mixRes, ctx->P.wcon);
   } else if (varH(ctx)) {
     Timed tm(ctx, KID_DDDT);
-    hipLaunchKernelGGL(k_dddt_l<true>, dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0, ctx->stream, ctx->cur,
-                       ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, (const double2*)nullptr,
-                       ctx->P.wcon);
+    if (withOmega)
+      hipLaunchKernelGGL((k_dddt_l<true, true>), dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0,
+                         ctx->stream, ctx->cur, ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt,
+                         (const double2*)nullptr, ctx->P.wcon);
+    else
+      hipLaunchKernelGGL(k_dddt_l<true>, dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0, ctx->stream, ctx->cur,
+                         ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, (const double2*)nullptr,
+                         ctx->P.wcon);
   } else {
     CHECK(computeHinv(ctx));
     Timed tm(ctx, KID_DDDT);
@@ -811,6 +831,7 @@ int passDddt(sphdem_ctx* ctx) {
                        ctx->ns, ctx->box, ctx->nbl, ctx->nbcnt, ctx->hinv, ctx->P.wcon);
   }
   LAUNCHED();
+  if (withOmega) markStale(ctx, GS_DPO);
   return 0;
 }
 int passCoupling(sphdem_ctx* ctx) {
@@ -892,12 +913,11 @@ int stepMiddle(sphdem_ctx* ctx) {
   CHECK(doReset(ctx, false, true));
   bool hinvReady = false;
   {
-    // On the side stream, beside the list build and omega: the coupling pass
-    // (porosity, Shepard sums, DEM->SPH drag), then the EOS (reads dens and
-    // porosity, stores pdr2 and press) and the FP64 force pass's 1/h table.
-    // None of them writes what the list build or omega read, and omega
-    // stores only omega.  (The ghosts are fresh after the reset, so nothing
-    // on the side stream refreshes them.)
+    // On the side stream, beside the list build: the coupling pass (porosity,
+    // Shepard sums, DEM->SPH drag), then the EOS (reads dens and porosity,
+    // stores pdr2 and press) and the FP64 force pass's 1/h table.  None of
+    // them writes what the list build reads.  (The ghosts are fresh after the
+    // reset, so nothing on the side stream refreshes them.)
     CHECK(refreshGhosts(ctx, GS_POSH));
     Side side(ctx);
     if (liqDem(ctx)) CHECK(passCoupling(ctx));
@@ -908,10 +928,10 @@ int stepMiddle(sphdem_ctx* ctx) {
     }
     side.back();
     CHECK(ensureLists(ctx));
-    if (varH(ctx)) CHECK(passOmega(ctx));
     CHECK(side.join());
   }
-  CHECK(passForce(ctx, hinvReady));
+  // calcOmega (VAR_H) is summed in the force walk: the force of a reads omega_a only
+  CHECK(passForce(ctx, hinvReady, varH(ctx)));
   const int na = ctx->ns + ctx->nd;
   hipLaunchKernelGGL(k_accel, dim3(reduceBlocks(std::max(na, 1))), dim3(TPB), 0, ctx->stream,
                      ctx->cur, na, ctx->dg);
@@ -931,12 +951,13 @@ int stepEnd(sphdem_ctx* ctx) {
   LAUNCHED();
   markStale(ctx, GS_ALL);
   CHECK(doReset(ctx));
-  if (varH(ctx)) CHECK(passOmega(ctx));
+  // calcOmega (VAR_H) is summed in the dddt walk: the dddt of a reads omega_a only
+  const bool fuse = varH(ctx);
   if (liqDem(ctx)) {
     // calcFPI, DEM contacts and the DEM kick on the side stream, beside dddt:
     // they read SPH positions, h, dens, vhat, porosity and forces and write DEM
-    // fields only; dddt writes the SPH dddt only.  driftRest (k_end_rest)
-    // changes SPH dens and h, so it waits for both.
+    // fields only; dddt writes the SPH dddt (and omega, which they do not read)
+    // only.  driftRest (k_end_rest) changes SPH dens and h, so it waits for both.
     Side side(ctx);
     CHECK(passFpi(ctx));
     if (!(ctx->P.flags & (SPHDEM_F_FIXED_DEM | SPHDEM_F_NO_DEM_CONTACTS))) CHECK(passContacts(ctx));
@@ -946,10 +967,10 @@ int stepEnd(sphdem_ctx* ctx) {
       LAUNCHED();
     }
     side.back();
-    CHECK(passDddt(ctx));
+    CHECK(passDddt(ctx, fuse));
     CHECK(side.join());
   } else {
-    CHECK(passDddt(ctx));
+    CHECK(passDddt(ctx, fuse));
   }
   Timed tm(ctx, KID_REST);
   hipLaunchKernelGGL(k_end_rest, dim3(blocks(std::max(ctx->ns, 1))), dim3(TPB), 0, ctx->stream,

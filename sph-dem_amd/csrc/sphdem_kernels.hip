@@ -741,7 +741,8 @@ __global__ void __launch_bounds__(256) k_fpi(Store S, int ns, int nd, Grid gs,
     if (S.iam[b] == T_BOUNDARY) return;
     const double r = sqrt(l2);
     const double4 vb = S.vhm[b];
-    const double4 db = S.dpo[b];
+    // dens and porosity only: omega may be written meanwhile (the fused dddt walk)
+    const double2 db = make_double2(S.dpo[b].x, S.dpo[b].w);
     const double dvol = vb.w / db.x;
     const double q = r / pb.w;
     const double Wab = wendW(P.wcon, q, pb.w);
@@ -766,7 +767,7 @@ __global__ void __launch_bounds__(256) k_fpi(Store S, int ns, int nd, Grid gs,
     fvel[0] += dvWab * vb.x;
     fvel[1] += dvWab * vb.y;
     fvel[2] += dvWab * vb.z;
-    por += dvWab * db.w;
+    por += dvWab * db.y;
     fdens += Wab * vb.w;
     sum += dvWab;
   }, lane, 64);

@@ -153,12 +153,15 @@ __global__ void __launch_bounds__(256) k_density_m(Store S, int ns, const int* _
 }
 
 // ---- calcDddtDudt, mixed -----------------------------------------------------------
-template <bool VARH>
+// OMG (VAR_H only): calcOmega fused into the walk (the k_omega_m sum, in list
+// order); the dddt terms are summed without 1/omega_a and scaled once in FP64.
+template <bool VARH, bool OMG = false>
 __global__ void __launch_bounds__(256) k_dddt_m(Store S, int ns, const int* __restrict__ nbl,
                                                 const int* __restrict__ nbcnt,
                                                 const uint4* __restrict__ P,
                                                 const float4* __restrict__ V, float res,
                                                 double wcon) {
+  static_assert(VARH || !OMG, "omega is only fused under VAR_H");
   const int a = xcdThread();
   if (a >= ns) return;
   const uint4 qa = P[a];
@@ -166,8 +169,11 @@ __global__ void __launch_bounds__(256) k_dddt_m(Store S, int ns, const int* __re
   const double ha = S.posh[a].w;
   const float wc10 = (float)(10.0 * wcon);
   const float rha = (float)(1.0 / ha), i5a = (float)(1.0 / pow5(ha));
-  const float rom = (float)(1.0 / S.dpo[a].z);
-  double dddt = S.vh0d[a].w;
+  const float rom = (VARH && !OMG) ? (float)(1.0 / S.dpo[a].z) : 1.0f;
+  const float wc = (float)wcon;
+  const float i3 = OMG ? (float)(1.0 / cube(ha)) : 0.0f, c3 = OMG ? (float)(3.0 / pow4(ha)) : 0.0f;
+  double om = OMG ? S.vhm[a].w * (double)wendPdWh_m(wc, 0.0f, rha, i3, c3) : 0.0;
+  double dddt = OMG ? 0.0 : S.vh0d[a].w;
   const int n = nbcnt[a];
   constexpr int U = 8;
   for (int k0 = 0; k0 < n; k0 += U) {
@@ -185,6 +191,7 @@ __global__ void __launch_bounds__(256) k_dddt_m(Store S, int ns, const int* __re
       const DxF d = dxq(qa, q[u], res);
       const float dv0 = va.x - v[u].x, dv1 = va.y - v[u].y, dv2 = va.z - v[u].z;
       const float r = fsqrt(d.r2);
+      if (OMG) om += (double)__uint_as_float(q[u].w) * (double)wendPdWh_m(wc, r * rha, rha, i3, c3);
       float gF;  // gradW = dx * gF
       if (VARH) {
         gF = wendF_m(wc10, r * rha, i5a) * rom;
@@ -198,17 +205,25 @@ __global__ void __launch_bounds__(256) k_dddt_m(Store S, int ns, const int* __re
       dddt += (double)__uint_as_float(q[u].w) * (double)(gF * (dv0 * d.x + dv1 * d.y + dv2 * d.z));
     }
   }
+  if (OMG) {
+    const double omega = 1 + ha * om / (S.dpo[a].x * 3);
+    S.dpo[a].z = omega;
+    dddt = S.vh0d[a].w + (1.0 / omega) * dddt;
+  }
   S.vh0d[a].w = dddt;
 }
 
 // ---- calcForce, mixed --------------------------------------------------------------
-template <bool VARH, int VISC, bool ARTIF, bool LIQDEM>
+// OMG (VAR_H only): calcOmega fused into the walk, as in k_dddt_m; the force
+// sums are taken without 1/omega_a and scaled once in FP64.
+template <bool VARH, int VISC, bool ARTIF, bool LIQDEM, bool OMG = false>
 __global__ void __launch_bounds__(256) k_force_m(Store S, int ns, const int* __restrict__ nbl,
                                                  const int* __restrict__ nbcnt,
                                                  const uint4* __restrict__ Pq,
                                                  const float4* __restrict__ V,
                                                  const float4* __restrict__ D, float res,
                                                  sphdem_params P, DevGlobals* g) {
+  static_assert(VARH || !OMG, "omega is only fused under VAR_H");
   const int a = xcdThread();
   const double inf = __longlong_as_double(0x7ff0000000000000ll);
   double newDt = inf;
@@ -220,7 +235,10 @@ __global__ void __launch_bounds__(256) k_force_m(Store S, int ns, const int* __r
     const double ha = S.posh[a].w;
     const float wc10 = (float)(10.0 * P.wcon);
     const float rha = (float)(1.0 / ha), i5a = (float)(1.0 / pow5(ha));
-    const float rom = VARH ? (float)(1.0 / da.z) : 1.0f;
+    const float rom = (VARH && !OMG) ? (float)(1.0 / da.z) : 1.0f;
+    const float wc = (float)P.wcon;
+    const float i3 = OMG ? (float)(1.0 / cube(ha)) : 0.0f, c3 = OMG ? (float)(3.0 / pow4(ha)) : 0.0f;
+    double om = OMG ? S.vhm[a].w * (double)wendPdWh_m(wc, 0.0f, rha, i3, c3) : 0.0;
     const float rhoa = (float)da.x, pdr2a = (float)da.y;
     const float nu = (float)P.viscosity, viscLiq = (float)(2.0 * P.viscosity * P.dens);
     const float cs2 = (float)(2.0 * P.spsound), alpha = (float)P.alpha_artificial;
@@ -248,6 +266,7 @@ __global__ void __launch_bounds__(256) k_force_m(Store S, int ns, const int* __r
         const DxF dx = dxq(qa, q[u], res);
         const float mb = __uint_as_float(q[u].w), rhob = d[u].x, pdr2b = d[u].y;
         const float r = fsqrt(dx.r2);
+        if (OMG) om += (double)mb * (double)wendPdWh_m(wc, r * rha, rha, i3, c3);
         const float Fa = wendF_m(wc10, r * rha, i5a);
         const float Fb = wendF_m(wc10, r * d[u].z, d[u].w);
         // gradients are dx * G; under VAR_H both divide by pa.omega (sph.h:331-332)
@@ -327,6 +346,20 @@ __global__ void __launch_bounds__(256) k_force_m(Store S, int ns, const int* __r
     }
     if (!ARTIF && VISC != VISC_MON && minA != inf) newDt = dmin2(newDt, minA / vsigC);
     if (minB != inf) newDt = dmin2(newDt, minB / P.viscosity);
+    if (OMG) {
+      const double omega = 1 + ha * om / (da.x * 3);
+      S.dpo[a].z = omega;
+      const double rw = 1.0 / omega;
+      fx *= rw;
+      fy *= rw;
+      fz *= rw;
+      px *= rw;
+      py *= rw;
+      pz *= rw;
+      vx *= rw;
+      vy *= rw;
+      vz *= rw;
+    }
     double4 f = S.frc[a], fpa = S.fp[a], fva = S.fv[a];
     f.x += fx;
     f.y += fy;

@@ -404,22 +404,28 @@ __global__ void __launch_bounds__(256) k_density_l(Store S, int ns, Box B,
 }
 
 // ---- calcDddtDudt (sphIncompress.h:783-851) over the cached list ----------------
-template <bool VARH>
+// OMG (VAR_H only): calcOmega fused into the walk, as in k_force_l -- every
+// dddt term of a is linear in 1/omega_a, so the terms are summed without it
+// beside the omega sum and scaled once.
+template <bool VARH, bool OMG = false>
 __global__ void __launch_bounds__(256) k_dddt_l(Store S, int ns, Box B,
                                                 const int* __restrict__ nbl,
                                                 const int* __restrict__ nbcnt,
                                                 const double2* __restrict__ hinv, double wcon) {
+  static_assert(VARH || !OMG, "omega is only fused under VAR_H");
   const int a = xcdThread();
   if (a >= ns) return;
   const double4 pa = S.posh[a];
   const double4 va = S.vhm[a];
   const double ha = pa.w;
   const double rha = 1.0 / ha;
-  const double rom = VARH ? 1.0 / S.dpo[a].z : 1.0;
+  const double rom = (VARH && !OMG) ? 1.0 / S.dpo[a].z : 1.0;
+  const double i3 = OMG ? 1.0 / cube(ha) : 0.0, c3 = OMG ? 3.0 / pow4(ha) : 0.0;
+  double om = OMG ? va.w * wendPdWhi(wcon, 0, ha, rha, i3, c3) : 0.0;
   // gradW = dr * G with G = poly(q) K(h) / r, K(h) = WCON h^-4 (/ omega_a
   // under VAR_H, sph.h:331-332): F(q, h) = WCON poly(q) / (q h^5) and 1/q = h/r
   const double Ka = wcon * (rha * rha) * (rha * rha) * rom;
-  double dddt = S.vh0d[a].w;
+  double dddt = OMG ? 0.0 : S.vh0d[a].w;
   const int n = nbcnt[a];
   constexpr int U = 8;
   for (int k0 = 0; k0 < n; k0 += U) {
@@ -441,6 +447,7 @@ __global__ void __launch_bounds__(256) k_dddt_l(Store S, int ns, Box B,
       const double r = sqrt(dot3(dx, dy, dz, dx, dy, dz));
       const double rinv = r > 0.0 ? 1.0 / r : 0.0;  // r = 0: dr = 0, gradW = 0
       const double qa = divr(r, ha, rha);
+      if (OMG) om += v[u].w * wendPdWhi(wcon, qa, ha, rha, i3, c3);
       double G = qa <= 2.0 ? wendPoly(qa) * Ka * rinv : 0.0;
       if (!VARH) {
         const double qb = divr(r, p[u].w, hi[u].x);
@@ -450,6 +457,11 @@ __global__ void __launch_bounds__(256) k_dddt_l(Store S, int ns, Box B,
       }
       dddt += v[u].w * (dot3(dv0, dv1, dv2, dx, dy, dz) * G);
     }
+  }
+  if (OMG) {
+    const double omega = 1 + ha * om / (S.dpo[a].x * 3);
+    S.dpo[a].z = omega;
+    dddt = S.vh0d[a].w + (1.0 / omega) * dddt;
   }
   S.vh0d[a].w = dddt;
 }
@@ -462,12 +474,20 @@ __global__ void __launch_bounds__(256) k_dddt_l(Store S, int ns, Box B,
 // artificial viscosity -- x -> RN(x/c) is monotone, so the minimum over pairs
 // of RN(0.6*hav / vsig) is RN(min(0.6*hav) / vsig): the numerators are
 // min-reduced and divided once (likewise c_diff*hav^2 / VISCOSITY).
-template <bool VARH, int VISC, bool ARTIF, bool LIQDEM>
+//
+// OMG (VAR_H only): calcOmega fused into the same walk.  Under VAR_H every
+// force term of a is linear in 1/omega_a and in nothing else of omega
+// (sph.h:331-332), and omega_a is a sum over a's own list (sph.h:437-452),
+// so the walk sums the omega terms (in list order, as k_omega_l) beside the
+// force terms taken without 1/omega_a, stores omega_a, and adds the force
+// sums times 1/omega_a to the incoming f, fp, fv.
+template <bool VARH, int VISC, bool ARTIF, bool LIQDEM, bool OMG = false>
 __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
                                                  const int* __restrict__ nbl,
                                                  const int* __restrict__ nbcnt,
                                                  const double2* __restrict__ hinv,
                                                  sphdem_params P, DevGlobals* g) {
+  static_assert(VARH || !OMG, "omega is only fused under VAR_H");
   const int a = xcdThread();
   const double inf = __longlong_as_double(0x7ff0000000000000ll);
   double newDt = inf;
@@ -478,7 +498,9 @@ __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
     const double4 da = S.dpo[a];
     const double ha = pa.w;
     const double rha = 1.0 / ha;
-    const double rom = VARH ? 1.0 / da.z : 1.0;
+    const double rom = (VARH && !OMG) ? 1.0 / da.z : 1.0;
+    const double i3 = OMG ? 1.0 / cube(ha) : 0.0, c3 = OMG ? 3.0 / pow4(ha) : 0.0;
+    double om = OMG ? va.w * wendPdWhi(P.wcon, 0, ha, rha, i3, c3) : 0.0;
     // gradW = dr * F(q, h) (/ omega_a under VAR_H), F = WCON poly(q) / (q h^5)
     // = dr * poly(q) * K(h) / r with K(h) = WCON h^-4 (/ omega_a)
     const double Ka = P.wcon * (rha * rha) * (rha * rha) * rom;
@@ -488,7 +510,8 @@ __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
     const double cdiff = VISC == VISC_MON ? 0.06 : 0.125;  // viscDiffusionCondition
     const double alphaM = monaghanAlpha(P);
     double minA = inf, minB = inf;  // min over pairs of 0.6*hav and cdiff*hav^2
-    double4 f = S.frc[a], fpa = S.fp[a], fva = S.fv[a];
+    const double4 zero4 = make_double4(0.0, 0.0, 0.0, 0.0);
+    double4 f = OMG ? zero4 : S.frc[a], fpa = OMG ? zero4 : S.fp[a], fva = OMG ? zero4 : S.fv[a];
     const int n = nbcnt[a];
     constexpr int U = 4;
     for (int k0 = 0; k0 < n; k0 += U) {
@@ -512,6 +535,7 @@ __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
         const double r = sqrt(l2);
         const double rinv = r > 0.0 ? 1.0 / r : 0.0;  // r = 0: dr = 0, gradW = 0
         const double qa = divr(r, ha, rha), qb = divr(r, hb, hi[u].x);
+        if (OMG) om += mb * wendPdWhi(P.wcon, qa, ha, rha, i3, c3);
         const double Kb = wrom * (hi[u].y * hb);  // WCON hb^-4 (/ omega_a)
         double Ga = qa <= 2.0 ? wendPoly(qa) * Ka * rinv : 0.0;
         double Gb = qb <= 2.0 ? wendPoly(qb) * Kb * rinv : 0.0;
@@ -587,6 +611,15 @@ __global__ void __launch_bounds__(256) k_force_l(Store S, int ns, Box B,
     }
     if (!ARTIF && VISC != VISC_MON && minA != inf) newDt = dmin2(newDt, minA / vsigC);
     if (minB != inf) newDt = dmin2(newDt, minB / P.viscosity);
+    if (OMG) {
+      const double omega = 1 + ha * om / (da.x * 3);
+      S.dpo[a].z = omega;
+      const double rw = 1.0 / omega;
+      const double4 f0 = S.frc[a], fp0 = S.fp[a], fv0 = S.fv[a];
+      f = make_double4(f0.x + rw * f.x, f0.y + rw * f.y, f0.z + rw * f.z, f0.w);
+      fpa = make_double4(fp0.x + rw * fpa.x, fp0.y + rw * fpa.y, fp0.z + rw * fpa.z, fp0.w);
+      fva = make_double4(fv0.x + rw * fva.x, fv0.y + rw * fva.y, fv0.z + rw * fva.z, fv0.w);
+    }
     S.frc[a] = f;
     S.fp[a] = fpa;
     S.fv[a] = fva;
