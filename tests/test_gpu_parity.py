@@ -252,6 +252,11 @@ def test_steps(case, nsteps):
     if case.params.flags & sd.F_LIQ_DEM:
         names += ["porosity", "fdrag", "shepSum"]
     check_fields(sim, orc, names, alive, tol)
+    # end()'s dddt walk and, under VAR_H, the omega summed in it
+    iam = orc.get("iam")
+    sphg = (iam == sd.SPH) | (iam == sd.SPH_BOUNDARY)
+    extra = ["dddt"] + (["omega"] if case.params.flags & sd.F_VAR_H_CORRECTION else [])
+    check_fields(sim, orc, extra, sphg, tol)
     # the neighbour sets of the evolved state (lists rebuilt by a reset: the
     # cached ones predate end()'s driftRest, which changes h) are identical
     sim.reset()
