@@ -306,7 +306,9 @@ int sphdem_num_deleted(const sphdem_ctx* ctx, int64_t* n);
  * "kick", "rest", "bin", "rank", "permute", "wrap", "lists" = the SPH
  * neighbour-list build) is bracketed by HIP events
  * on the context's stream; sphdem_timing_end synchronises and returns the
- * summed kernel time and the number of launches timed. */
+ * summed kernel time and the number of launches timed.  Under
+ * VAR_H_CORRECTION the steps sum omega inside their "force" and "dddt"
+ * launches, so "omega" times only sphdem_pass_omega. */
 int sphdem_timing_begin(sphdem_ctx* ctx, const char* kernel);
 int sphdem_timing_end(sphdem_ctx* ctx, double* total_ms, int64_t* launches);
 
