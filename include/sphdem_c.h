@@ -174,6 +174,12 @@ int sphdem_upload(sphdem_ctx* ctx, int64_t n, const double* r, const double* v,
                   const double* dens, const int32_t* iam);
 int sphdem_num_particles(const sphdem_ctx* ctx, int64_t* n);
 int sphdem_set_field(sphdem_ctx* ctx, int field, const void* host);
+/* sphdem_set_field from device memory of this context's device (n doubles, or
+ * n*3 for a vector field, in original order; the caller's writes to it must be
+ * complete), for callers whose state lives on the GPU (the adjoint's pair
+ * searches); no counterpart in the reference, whose particles live on the
+ * host.  Returns once the field is written. */
+int sphdem_set_field_device(sphdem_ctx* ctx, int field, const void* dev);
 int sphdem_get_field(sphdem_ctx* ctx, int field, void* host);
 int sphdem_set_globals(sphdem_ctx* ctx, const sphdem_globals* g);
 int sphdem_get_globals(sphdem_ctx* ctx, sphdem_globals* g);

@@ -40,6 +40,10 @@ struct sphdem_ctx {
   int *key = nullptr, *slot = nullptr, *tmp = nullptr, *perm = nullptr;
   int *counts = nullptr, *start = nullptr, *tiles = nullptr;
   int Kcap = 0, tileCap = 0;
+  // the pair export's per-query counts and offsets (grown on demand)
+  int* pairCnt = nullptr;
+  long long* pairOff = nullptr;
+  int pairCap = 0;
   // segments of the SPH grid (the wavefront work units of the pair passes)
   int *segCnt = nullptr, *segOff = nullptr;
   Segment* segs = nullptr;
@@ -1029,7 +1033,7 @@ void freeAll(sphdem_ctx* ctx) {
                   ctx->hinv,        ctx->gcnt,        ctx->goff,    ctx->gsrc,   ctx->gord,
                   ctx->gcell,       ctx->gorig,       ctx->gcode,   ctx->inv,
                   ctx->mixP,        ctx->mixV,        ctx->mixD,    ctx->okey,
-                  ctx->posm};
+                  ctx->posm,        ctx->pairCnt,     ctx->pairOff};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : ctx->evPool) (void)hipEventDestroy(e);
@@ -1465,13 +1469,16 @@ int sphdem_get_field(sphdem_ctx* ctx, int field, void* host) {
   return 0;
 }
 
-int sphdem_set_field(sphdem_ctx* ctx, int field, const void* host) {
+// src: n values (n*3 for vectors) in original order, in host memory or
+// (device) in device memory of the context's device
+static int setField(sphdem_ctx* ctx, int field, const void* src, bool device) {
   CHECK(checkLoaded(ctx));
-  if (!host) return SPHDEM_E_ARG;
+  if (!src) return SPHDEM_E_ARG;
   HIPC(hipSetDevice(ctx->device));
   const int n = ctx->n;
   if (field == SPHDEM_FIELD_IAM) {
-    const int32_t* t = (const int32_t*)host;
+    if (device) return fail(ctx, SPHDEM_E_ARG, "particle types are fixed at upload");
+    const int32_t* t = (const int32_t*)src;
     for (int i = 0; i < n; i++)
       if (t[i] != SPHDEM_SPH && t[i] != SPHDEM_SPH_BOUNDARY && t[i] != SPHDEM_DEM &&
           t[i] != SPHDEM_DELETED)
@@ -1480,15 +1487,26 @@ int sphdem_set_field(sphdem_ctx* ctx, int field, const void* host) {
   }
   FieldLoc L;
   if (!fieldLoc(field, &L)) return fail(ctx, SPHDEM_E_ARG, "unknown or read-only field %d", field);
-  HIPC(hipMemcpyAsync(ctx->fieldbuf, host, sizeof(double) * L.width * n, hipMemcpyHostToDevice,
-                      ctx->stream));
+  const double* from = (const double*)src;
+  if (!device) {
+    HIPC(hipMemcpyAsync(ctx->fieldbuf, src, sizeof(double) * L.width * n, hipMemcpyHostToDevice,
+                        ctx->stream));
+    from = (const double*)ctx->fieldbuf;
+  }
   hipLaunchKernelGGL(k_set_field, dim3(blocks(n)), dim3(TPB), 0, ctx->stream,
-                     (double*)storeArr(ctx->cur, L.arr), L.off, L.width, ctx->cur.orig, n,
-                     (const double*)ctx->fieldbuf);
+                     (double*)storeArr(ctx->cur, L.arr), L.off, L.width, ctx->cur.orig, n, from);
   LAUNCHED();
   markStale(ctx, GS_ALL);
   HIPC(hipStreamSynchronize(ctx->stream));
   return 0;
+}
+
+int sphdem_set_field(sphdem_ctx* ctx, int field, const void* host) {
+  return setField(ctx, field, host, false);
+}
+
+int sphdem_set_field_device(sphdem_ctx* ctx, int field, const void* dev) {
+  return setField(ctx, field, dev, true);
 }
 
 int sphdem_set_globals(sphdem_ctx* ctx, const sphdem_globals* g) {
@@ -1680,9 +1698,19 @@ static int getPairs(sphdem_ctx* ctx, int kind, int64_t* count, int32_t* a, int32
   // the SPH set is read from the cached lists the SPH passes use (built now
   // if no pass has built them since the last reset)
   if (kind == SPHDEM_PAIRS_SPH) CHECK(ensureLists(ctx));
-  int* cnt = nullptr;
-  long long* off = nullptr;
-  HIPC(hipMalloc(&cnt, sizeof(int) * nq));
+  if (nq > ctx->pairCap) {
+    if (ctx->pairCnt) HIPC(hipFree(ctx->pairCnt));
+    if (ctx->pairOff) HIPC(hipFree(ctx->pairOff));
+    ctx->pairCnt = nullptr;
+    ctx->pairOff = nullptr;
+    ctx->pairCap = 0;
+    const int cap = nq + nq / 4;
+    HIPC(hipMalloc(&ctx->pairCnt, sizeof(int) * cap));
+    HIPC(hipMalloc(&ctx->pairOff, sizeof(long long) * cap));
+    ctx->pairCap = cap;
+  }
+  int* cnt = ctx->pairCnt;
+  long long* off = ctx->pairOff;
   HIPC(hipMemsetAsync(cnt, 0, sizeof(int) * nq, ctx->stream));
   auto enumerate = [&](int mode, const long long* o, int* da, int* db, int* di) -> int {
     if (kind == SPHDEM_PAIRS_SPH) {
@@ -1713,12 +1741,9 @@ static int getPairs(sphdem_ctx* ctx, int kind, int64_t* count, int32_t* a, int32
   }
   *count = tot;
   if (a && b && image && tot > 0) {
-    if (tot > capacity) {
-      (void)hipFree(cnt);
+    if (tot > capacity)
       return fail(ctx, SPHDEM_E_CAPACITY, "pair buffer of %lld < %lld", (long long)capacity, tot);
-    }
     int *da = nullptr, *db = nullptr, *di = nullptr;
-    HIPC(hipMalloc(&off, sizeof(long long) * nq));
     if (device) {
       da = a;
       db = b;
@@ -1741,9 +1766,7 @@ static int getPairs(sphdem_ctx* ctx, int kind, int64_t* count, int32_t* a, int32
       (void)hipFree(db);
       (void)hipFree(di);
     }
-    (void)hipFree(off);
   }
-  (void)hipFree(cnt);
   return 0;
 }
 

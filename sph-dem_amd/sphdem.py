@@ -38,6 +38,8 @@ F_VISC_MONAGHAN = 1 << 8
 F_VISC_CLEARY = 1 << 9
 
 FP64, MIXED = 0, 1
+# return codes of the C ABI (SPHDEM_E_*)
+E_ARG, E_HIP, E_STATE, E_CAPACITY = -1, -2, -3, -4
 
 HOOKS_NONE, HOOKS_DAMBREAK, HOOKS_SEDIMENT = 0, 1, 2
 
@@ -602,6 +604,7 @@ def load_library(path: str = LIB_PATH):
     lib.sphdem_upload.argtypes = [vp, i64, dp, dp, dp, dp, dp, dp, ip]
     lib.sphdem_num_particles.argtypes = [vp, ctypes.POINTER(i64)]
     lib.sphdem_set_field.argtypes = [vp, ctypes.c_int, vp]
+    lib.sphdem_set_field_device.argtypes = [vp, ctypes.c_int, vp]
     lib.sphdem_get_field.argtypes = [vp, ctypes.c_int, vp]
     lib.sphdem_set_globals.argtypes = [vp, ctypes.POINTER(Globals)]
     lib.sphdem_get_globals.argtypes = [vp, ctypes.POINTER(Globals)]
@@ -722,6 +725,12 @@ class Sim:
         dt = np.int32 if name in INT_FIELDS else np.float64
         a = np.ascontiguousarray(value, dtype=dt)
         self._check(self.lib.sphdem_set_field(self.ctx, FIELD[name], a.ctypes.data))
+
+    def set_device(self, name: str, ptr: int):
+        """sphdem_set_field_device: `ptr` addresses n (n*3 for a vector field)
+        float64 values in original order in this context's device memory, with
+        the writes that made them complete."""
+        self._check(self.lib.sphdem_set_field_device(self.ctx, FIELD[name], ctypes.c_void_p(ptr)))
 
     def globals(self) -> Globals:
         g = Globals()

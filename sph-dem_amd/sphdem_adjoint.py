@@ -189,7 +189,10 @@ class LibraryPairs:
 
     def __init__(self, case: sd.Case, device: int = 0):
         self.sim = sd.Sim(case, device=device)
+        self.n = case.n
+        self.device = device
         self.seconds = 0.0  # wall time spent in the searches
+        self.cap = {}       # pair buffer size per search kind, grown on demand
 
     def __call__(self, r: torch.Tensor, h: torch.Tensor, kinds) -> dict:
         t0 = time.perf_counter()
@@ -198,27 +201,42 @@ class LibraryPairs:
         return out
 
     def _search(self, r, h, kinds):
-        self.sim.set("r", r.detach().cpu().numpy())
-        self.sim.set("h", h.detach().cpu().numpy())
+        """r (n, 3) and h (n) in original order, on the helper's device: set
+        device to device (sphdem_set_field_device), reset, export."""
+        rr = r.detach().to(torch.float64).contiguous()
+        hh = h.detach().to(torch.float64).contiguous()
+        if (not rr.is_cuda or rr.device.index != self.device or tuple(rr.shape) != (self.n, 3)
+                or tuple(hh.shape) != (self.n,) or hh.device != rr.device):
+            raise ValueError(f"positions {tuple(rr.shape)} and h {tuple(hh.shape)} must be "
+                             f"({self.n}, 3) and ({self.n},) on cuda:{self.device}")
+        torch.cuda.synchronize(rr.device)  # rr, hh complete before the library's stream reads
+        self.sim.set_device("r", rr.data_ptr())
+        self.sim.set_device("h", hh.data_ptr())
         self.sim.reset()
         if self.sim.num_deleted():
             raise sd.SphDemError("a particle left the domain during the rollout")
-        return {k: self._pairs(k, r.device) for k in kinds}
+        return {k: self._pairs(k, rr.device) for k in kinds}
 
     def _pairs(self, kind, device):
         """One search's pairs written straight into device tensors
-        (sphdem_get_pairs_device)."""
+        (sphdem_get_pairs_device): one call that counts and writes into
+        buffers of the kind's last size; a second one, with buffers of the
+        count it returned, when those were too small."""
         lib, cnt = self.sim.lib, ctypes.c_int64()
-        self.sim._check(lib.sphdem_get_pairs_device(self.sim.ctx, kind, ctypes.byref(cnt), None,
-                                                    None, None, 0))
-        m = cnt.value
-        cols = [torch.empty(max(m, 1), dtype=torch.int32, device=device) for _ in range(3)]
-        if m:
+        cap = self.cap.get(kind, 0)
+        for _ in range(2):
+            cols = [torch.empty(max(cap, 1), dtype=torch.int32, device=device) for _ in range(3)]
             torch.cuda.synchronize(device)  # the buffers exist before the library's stream writes
-            self.sim._check(lib.sphdem_get_pairs_device(
-                self.sim.ctx, kind, ctypes.byref(cnt), *[ctypes.c_void_p(c.data_ptr())
-                                                         for c in cols], m))
-        return tuple(c[:m] for c in cols)
+            rc = lib.sphdem_get_pairs_device(self.sim.ctx, kind, ctypes.byref(cnt),
+                                             *[ctypes.c_void_p(c.data_ptr()) for c in cols], cap)
+            m = cnt.value
+            if rc == sd.E_CAPACITY and m > cap:
+                cap = m + m // 16 + 1024
+                self.cap[kind] = cap
+                continue
+            self.sim._check(rc)
+            return tuple(c[:m] for c in cols)
+        raise sd.SphDemError(f"pair export of kind {kind}: {m} pairs after a buffer of {cap}")
 
     def close(self):
         self.sim.close()

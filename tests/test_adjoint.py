@@ -173,12 +173,41 @@ def test_device_pair_export_equals_the_host_export():
     r = torch.as_tensor(case.r, device="cuda")
     h = torch.as_tensor(case.h, device="cuda")
     kinds = (sd.PAIRS_SPH, sd.PAIRS_DEM, sd.PAIRS_SPH2DEM, sd.PAIRS_DEM2SPH)
-    got = pairs(r, h, kinds)
+    got = pairs(r, h, kinds)  # first search: buffers sized from the counts
+    for k in kinds:
+        host = pairs.sim.pairs(k)
+        dev = np.stack([c.cpu().numpy() for c in got[k]], axis=1)
+        assert len(host) > 0 and np.array_equal(dev, host), k
+    # a second search at moved positions reuses (or regrows) those buffers
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    r2 = r + 0.2 * float(case.params.dem_radius) * (
+        torch.rand(r.shape, generator=gen, device="cuda", dtype=torch.float64) - 0.5)
+    got = pairs(r2, h, kinds)
     for k in kinds:
         host = pairs.sim.pairs(k)
         dev = np.stack([c.cpu().numpy() for c in got[k]], axis=1)
         assert len(host) > 0 and np.array_equal(dev, host), k
     pairs.close()
+
+
+@pytest.mark.gpu
+def test_set_field_device_equals_set_field():
+    case = sd.adjoint_case(nf=12, nz=16, n_dem=200, seed=1007)
+    a, b = sd.Sim(case), sd.Sim(case)
+    rng = np.random.default_rng(8)
+    for nm, shape in (("r", (case.n, 3)), ("h", (case.n,)), ("vhat", (case.n, 3))):
+        x = np.asarray(getattr(case, nm), float) + 1e-4 * rng.standard_normal(shape)
+        a.set(nm, x)
+        t = torch.as_tensor(x, device="cuda").contiguous()
+        torch.cuda.synchronize()
+        b.set_device(nm, t.data_ptr())
+        assert np.array_equal(a.get(nm), b.get(nm)), nm
+        assert np.array_equal(b.get(nm), x), nm
+    with pytest.raises(sd.SphDemError):
+        t = torch.zeros(case.n, dtype=torch.int32, device="cuda")
+        b.set_device("iam", t.data_ptr())
+    a.close()
+    b.close()
 
 
 @pytest.mark.gpu
