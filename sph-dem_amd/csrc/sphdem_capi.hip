@@ -519,6 +519,33 @@ struct Side {
   ~Side() { (void)join(); }
 };
 
+// Error bits the kernels raise in DevGlobals::errflags (bit 0: k_bin found a
+// particle outside its cell grid; bit 1: k_lists clipped a list at kMaxNb
+// entries).  Reported once, then cleared, so that a caller that restores a
+// checkpoint can continue.
+int flagError(sphdem_ctx* ctx, unsigned flags) {
+  if (!flags) return 0;
+  HIPC(hipMemsetAsync(&ctx->dg->errflags, 0, sizeof(unsigned), ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  if (flags & 1u)
+    return fail(ctx, SPHDEM_E_CAPACITY,
+                "a particle fell outside the cell grid; the pair sets since the last reset are "
+                "incomplete");
+  return fail(ctx, SPHDEM_E_CAPACITY,
+              "a particle had more than %d SPH neighbours; the passes since the last list build "
+              "used a clipped list",
+              kMaxNb);
+}
+// Reads the error bits after everything issued so far (one host round trip):
+// the end of every entry point that builds neighbour lists, so that no call
+// returns 0 after a pass used a clipped list.
+int checkFlags(sphdem_ctx* ctx) {
+  HIPC(hipMemcpyAsync(&ctx->hostG->errflags, &ctx->dg->errflags, sizeof(unsigned),
+                      hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return flagError(ctx, ctx->hostG->errflags);
+}
+
 // midInit: middle()'s reset, which also starts the dt minimum at maxdt
 int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false) {
   const sphdem_params& P = ctx->P;
@@ -556,10 +583,7 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false) {
   HIPC(hipStreamSynchronize(ctx->stream));
   if (pre) ngPre = hg.ghostTotal;
   const double oldSsr = ctx->ssr;
-  if (hg.errflags & 1u)
-    return fail(ctx, SPHDEM_E_CAPACITY, "a particle fell outside the cell grid at the last reset");
-  if (hg.errflags & 2u)
-    return fail(ctx, SPHDEM_E_CAPACITY, "a particle had more than %d SPH neighbours", kMaxNb);
+  CHECK(flagError(ctx, hg.errflags));
   ctx->ns = ctx->ns0 - (int)hg.ndel_sph;
   ctx->nd = ctx->nd0 - (int)hg.ndel_dem;
   double hmax;
@@ -1192,6 +1216,7 @@ int sphdem_output_vars(sphdem_ctx* ctx, sphdem_output* out) {
   HIPC(hipSetDevice(ctx->device));
   const int ns = ctx->ns;
   CHECK(ensureLists(ctx));
+  CHECK(checkFlags(ctx));
   CHECK(refreshGhosts(ctx, GS_POSH | GS_VHM));
   double* densCalc = ctx->fieldbuf;
   if (ns > 0) {
@@ -1552,17 +1577,20 @@ int sphdem_sync(sphdem_ctx* ctx) {
 int sphdem_step_start(sphdem_ctx* ctx) {
   CHECK(checkLoaded(ctx));
   HIPC(hipSetDevice(ctx->device));
-  return stepStart(ctx);
+  CHECK(stepStart(ctx));
+  return checkFlags(ctx);
 }
 int sphdem_step_middle(sphdem_ctx* ctx) {
   CHECK(checkLoaded(ctx));
   HIPC(hipSetDevice(ctx->device));
-  return stepMiddle(ctx);
+  CHECK(stepMiddle(ctx));
+  return checkFlags(ctx);
 }
 int sphdem_step_end(sphdem_ctx* ctx) {
   CHECK(checkLoaded(ctx));
   HIPC(hipSetDevice(ctx->device));
-  return stepEnd(ctx);
+  CHECK(stepEnd(ctx));
+  return checkFlags(ctx);
 }
 
 int sphdem_hook_body_force(sphdem_ctx* ctx, uint32_t type_mask, const double accel[3]) {
@@ -1587,7 +1615,10 @@ int sphdem_hook_sum_drag(sphdem_ctx* ctx) {
   return hookSumDrag(ctx);
 }
 
-int sphdem_run_steps(sphdem_ctx* ctx, int nsteps, int hookset) {
+namespace {
+// nsteps of run.cpp:102-118; the error bits of the last step's lists are left
+// to the caller's checkFlags (each reset reads those of the step before)
+int runSteps(sphdem_ctx* ctx, int nsteps, int hookset) {
   CHECK(checkLoaded(ctx));
   if (nsteps < 0 || hookset < SPHDEM_HOOKS_NONE || hookset > SPHDEM_HOOKS_SEDIMENT)
     return fail(ctx, SPHDEM_E_ARG, "bad nsteps/hookset");
@@ -1618,6 +1649,12 @@ int sphdem_run_steps(sphdem_ctx* ctx, int nsteps, int hookset) {
   }
   return 0;
 }
+}  // namespace
+
+int sphdem_run_steps(sphdem_ctx* ctx, int nsteps, int hookset) {
+  CHECK(runSteps(ctx, nsteps, hookset));
+  return checkFlags(ctx);
+}
 
 int sphdem_run_steps_many(sphdem_ctx* const* ctxs, int nbox, int nsteps, int hookset,
                           int workers, int* failed_box) {
@@ -1643,7 +1680,7 @@ int sphdem_run_steps_many(sphdem_ctx* const* ctxs, int nbox, int nsteps, int hoo
     for (int s = 0; s < nsteps; s++)
       for (int b = w; b < nbox; b += workers) {
         if (err.load(std::memory_order_relaxed)) return;
-        const int rc = sphdem_run_steps(ctxs[b], 1, hookset);
+        const int rc = runSteps(ctxs[b], 1, hookset);
         if (rc) {
           int none = 0;
           if (err.compare_exchange_strong(none, rc)) errBox.store(b);
@@ -1657,6 +1694,20 @@ int sphdem_run_steps_many(sphdem_ctx* const* ctxs, int nbox, int nsteps, int hoo
   work(0);
   for (std::thread& t : pool) t.join();
   for (int b = 0; b < nbox; b++) ctxs[b]->overlap = overlap[b];
+  // the last step's lists of every box (one round trip per box, after all ran)
+  for (int b = 0; b < nbox && !err.load(); b++) {
+    sphdem_ctx* ctx = ctxs[b];
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+      err.store(SPHDEM_E_HIP);
+      errBox.store(b);
+      break;
+    }
+    const int rc = checkFlags(ctx);
+    if (rc) {
+      err.store(rc);
+      errBox.store(b);
+    }
+  }
   if (failed_box) *failed_box = errBox.load();
   return err.load();
 }
@@ -1697,7 +1748,10 @@ static int getPairs(sphdem_ctx* ctx, int kind, int64_t* count, int32_t* a, int32
   }
   // the SPH set is read from the cached lists the SPH passes use (built now
   // if no pass has built them since the last reset)
-  if (kind == SPHDEM_PAIRS_SPH) CHECK(ensureLists(ctx));
+  if (kind == SPHDEM_PAIRS_SPH) {
+    CHECK(ensureLists(ctx));
+    CHECK(checkFlags(ctx));
+  }
   if (nq > ctx->pairCap) {
     if (ctx->pairCnt) HIPC(hipFree(ctx->pairCnt));
     if (ctx->pairOff) HIPC(hipFree(ctx->pairOff));
@@ -1783,7 +1837,8 @@ int sphdem_get_pairs_device(sphdem_ctx* ctx, int kind, int64_t* count, int32_t* 
 int sphdem_pass_omega(sphdem_ctx* ctx) {
   CHECK(checkLoaded(ctx));
   HIPC(hipSetDevice(ctx->device));
-  return passOmega(ctx);
+  CHECK(passOmega(ctx));
+  return checkFlags(ctx);
 }
 int sphdem_pass_eos(sphdem_ctx* ctx) {
   CHECK(checkLoaded(ctx));
@@ -1793,17 +1848,20 @@ int sphdem_pass_eos(sphdem_ctx* ctx) {
 int sphdem_pass_force(sphdem_ctx* ctx) {
   CHECK(checkLoaded(ctx));
   HIPC(hipSetDevice(ctx->device));
-  return passForce(ctx);
+  CHECK(passForce(ctx));
+  return checkFlags(ctx);
 }
 int sphdem_pass_dddt(sphdem_ctx* ctx) {
   CHECK(checkLoaded(ctx));
   HIPC(hipSetDevice(ctx->device));
-  return passDddt(ctx);
+  CHECK(passDddt(ctx));
+  return checkFlags(ctx);
 }
 int sphdem_pass_density(sphdem_ctx* ctx) {
   CHECK(checkLoaded(ctx));
   HIPC(hipSetDevice(ctx->device));
-  return passDensity(ctx);
+  CHECK(passDensity(ctx));
+  return checkFlags(ctx);
 }
 int sphdem_pass_porosity(sphdem_ctx* ctx) {
   CHECK(checkLoaded(ctx));

@@ -574,7 +574,11 @@ def ensemble_cases(nbox: int = 256, seed: int = 1005, nmin: int = 4096,
 # the GPU library
 # ----------------------------------------------------------------------------
 class SphDemError(RuntimeError):
-    pass
+    """A failed C-ABI call; `code` is its SPHDEM_E_* return value."""
+
+    def __init__(self, msg: str, code: int = 0):
+        super().__init__(msg)
+        self.code = code
 
 
 _lib = None
@@ -696,7 +700,7 @@ class Sim:
             msg = b""
             if ctx is not None and getattr(self, "ctx", None):
                 msg = self.lib.sphdem_last_error(self.ctx) or b""
-            raise SphDemError(f"sphdem call failed ({rc}): {msg.decode(errors='replace')}")
+            raise SphDemError(f"sphdem call failed ({rc}): {msg.decode(errors='replace')}", rc)
 
     def close(self):
         if getattr(self, "ctx", None):

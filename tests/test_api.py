@@ -82,3 +82,30 @@ def test_kernel_timing_window_counts_launches():
     assert launches == 2 and ms > 0
     with pytest.raises(sd.SphDemError, match="unknown kernel"):
         sim.timing_begin("no_such_kernel")
+
+
+def test_clipped_list_is_reported_by_the_call_that_used_it():
+    """A particle with more than the list capacity of SPH neighbours makes the
+    call whose pass walked the clipped list fail with SPHDEM_E_CAPACITY (not a
+    later reset); the error is reported once and cleared, so the same context
+    continues once h is sane again."""
+    case = sd.sediment_case(nf=12, n_dem=60)
+    print(f"clipped list: sediment nf=12, {case.n} particles", flush=True)
+    sim = sd.Sim(case)
+    sim.run(1)
+    h0 = sim.get("h")
+    # h x 2.5: about 15x the ~120 neighbours, far above the 256-entry lists
+    sim.set("h", np.where(case.iam == sd.DEM, h0, 2.5 * h0))
+    sim.reset()
+    with pytest.raises(sd.SphDemError) as e:
+        sim.pass_("force")
+    assert e.value.code == sd.E_CAPACITY and "neighbours" in str(e.value)
+    sim.set("h", h0)
+    sim.reset()
+    sim.pass_("force")
+    # the same through a whole step: the failing call is the one of the step
+    sim.set("h", np.where(case.iam == sd.DEM, h0, 2.5 * h0))
+    with pytest.raises(sd.SphDemError) as e:
+        sim.run(1)
+    assert e.value.code == sd.E_CAPACITY
+    sim.close()
