@@ -48,6 +48,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "../include/sphdem_c.h"
 
@@ -591,7 +594,7 @@ static void calcNeighboursCoupling(Oracle* o, Particle* p, PList* out) {
 /* -------------------------------------------------------- pair operators */
 /* calcViscForce sph.h:579-666 */
 static void calcViscForce(Oracle* o, Particle* pa, Particle* pb, const double* gradWa,
-                          const double* dv, double r) {
+                          const double* dv, double r, double* newDt) {
   const sphdem_params* P = &o->P;
   double dx[3], fv[3] = {0, 0, 0};
   for (int i = 0; i < 3; i++) dx[i] = pa->r[i] - pb->r[i];
@@ -637,10 +640,10 @@ static void calcViscForce(Oracle* o, Particle* pa, Particle* pb, const double* g
     for (int i = 0; i < 3; i++) pa->f[i] += s * gradWa[i];
     vsig = dmax2(vsig, artificial_vsig);
   }
-  o->newDt = dmin2(o->newDt, 0.6 * hav / vsig);                        /* sph.h:545-552 */
+  *newDt = dmin2(*newDt, 0.6 * hav / vsig);                            /* sph.h:545-552 */
   /* viscDiffusionCondition sph.h:554-560 */
   const double cdiff = (P->flags & SPHDEM_F_VISC_MONAGHAN) ? 0.06 : 0.125;
-  o->newDt = dmin2(o->newDt, cdiff * pow(hav, 2) / P->viscosity);
+  *newDt = dmin2(*newDt, cdiff * pow(hav, 2) / P->viscosity);
   for (int i = 0; i < 3; i++) pa->fv[i] += fv[i];
   for (int i = 0; i < 3; i++) pa->f[i] += fv[i];
 }
@@ -659,7 +662,7 @@ static void calcPressForce(Particle* pa, Particle* pb, const double* gWa, const 
 }
 
 /* CsphIncompress::calcForce sphIncompress.h:899-972 */
-static void calcForce(Oracle* o, Particle* pa, Particle* pb) {
+static void calcForce(Oracle* o, Particle* pa, Particle* pb, double* newDt) {
   double dx[3], gWa[3], gWb[3];
   for (int i = 0; i < 3; i++) dx[i] = pa->r[i] - pb->r[i];
   const double r = lenv(dx);
@@ -676,7 +679,7 @@ static void calcForce(Oracle* o, Particle* pa, Particle* pb) {
     double dv[3], gbar[3];
     for (int i = 0; i < 3; i++) dv[i] = pa->vhat[i] - pb->vhat[i];
     for (int i = 0; i < 3; i++) gbar[i] = 0.5 * (gWa[i] + gWb[i]);
-    calcViscForce(o, pa, pb, gbar, dv, r);
+    calcViscForce(o, pa, pb, gbar, dv, r, newDt);
   }
   if (ifBoundary(pb)) {
     /* calcBoundaryForces (type boundary walls) is outside the supported flag set */
@@ -972,14 +975,19 @@ static void calcEnergies(Oracle* o, Particle* p) {
   p->u += o->dt * p->dudt;
 }
 /* Nsph::accelCondition sph.h:566-568 */
-static void accelCondition(Oracle* o, Particle* p) {
-  o->newDt = dmin2(o->newDt, 0.25 * sqrt(p->h / lenv(p->f)));
+static void accelCondition(Particle* p, double* newDt) {
+  *newDt = dmin2(*newDt, 0.25 * sqrt(p->h / lenv(p->f)));
 }
 
 /* ---------------------------------------------------------- pass drivers */
 static int isLiqDem(const Oracle* o) { return (o->P.flags & SPHDEM_F_LIQ_DEM) != 0; }
 
+/* Pass drivers.  The gathers run over the particles with OpenMP (when built
+ * with it): each particle's sums keep the reference's order, so the results
+ * are those of a serial sweep; the Shepard scatter stays serial and the dt
+ * minimum is exact in any order. */
 static void passOmega(Oracle* o) { /* neighboursGroup<calcOmega,ifSph..> */
+#pragma omp parallel for schedule(dynamic, 256)
   for (int k = 0; k < o->n; k++) {
     Particle* p = &o->ps[k];
     if (!ifSphOrSphBoundary(p)) continue;
@@ -991,20 +999,25 @@ static void syncOmega(Oracle* o) { /* syncParticlesBetweenProcs<readOmega,...> *
   for (int g = 0; g < o->ng; g++) o->gh[g].omega = o->ghost_origin[g]->omega;
 }
 static void passEos(Oracle* o) { /* traverse<calcPressSpsoundPdr2,...OrGhost> */
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < o->n; k++)
     if (ifSphOrSphBoundary(&o->ps[k])) calcPressSpsoundPdr2(o, &o->ps[k]);
   for (int g = 0; g < o->ng; g++)
     if (ifSphOrSphBoundary(&o->gh[g])) calcPressSpsoundPdr2(o, &o->gh[g]);
 }
 static void passForce(Oracle* o) { /* neighbours<calcForce,ifSphOrSphBoundary> */
+  double newDt = o->newDt;
+#pragma omp parallel for schedule(dynamic, 256) reduction(min : newDt)
   for (int k = 0; k < o->n; k++) {
     Particle* p = &o->ps[k];
     if (!ifSphOrSphBoundary(p)) continue;
     ensureNeighbours(o, p);
-    for (int i = 0; i < p->nnb; i++) calcForce(o, p, p->nb[i]);
+    for (int i = 0; i < p->nnb; i++) calcForce(o, p, p->nb[i], &newDt);
   }
+  o->newDt = newDt;
 }
 static void passDddt(Oracle* o) { /* neighbours<calcDddtDudt,...> */
+#pragma omp parallel for schedule(dynamic, 256)
   for (int k = 0; k < o->n; k++) {
     Particle* p = &o->ps[k];
     if (!ifSphOrSphBoundary(p)) continue;
@@ -1013,11 +1026,13 @@ static void passDddt(Oracle* o) { /* neighbours<calcDddtDudt,...> */
   }
 }
 static void passDensityOnce(Oracle* o) { /* one iteration of sphIncompress.cpp:29-34 */
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < o->n; k++)
     if (ifSphOrSphBoundary(&o->ps[k])) calcHFromDens(o, &o->ps[k]);
   for (int g = 0; g < o->ng; g++)
     if (ifSphOrSphBoundary(&o->gh[g])) calcHFromDens(o, &o->gh[g]);
   for (int g = 0; g < o->ng; g++) o->gh[g].h = o->ghost_origin[g]->h;
+#pragma omp parallel for schedule(dynamic, 256)
   for (int k = 0; k < o->n; k++) {
     Particle* p = &o->ps[k];
     if (!ifSphOrSphBoundary(p)) continue;
@@ -1028,23 +1043,27 @@ static void passDensityOnce(Oracle* o) { /* one iteration of sphIncompress.cpp:2
 }
 /* sphIncompress.cpp:64-78: porosity, Shepard sum, DEM->SPH drag, finalise */
 static void passPorosity(Oracle* o) {
-  PList nb = {0, 0, 0};
+  /* The SPH->DEM search of each SPH particle (it depends on positions only,
+   * which none of the three sweeps below changes), made once, in parallel;
+   * each sweep then reads the particle's list in the search's order. */
+  PList* cl = (PList*)calloc((size_t)o->n, sizeof(PList));
+#pragma omp parallel for schedule(dynamic, 256)
   for (int k = 0; k < o->n; k++) {
     Particle* p = &o->ps[k];
     if (!ifSphOrSphBoundary(p)) continue;
-    calcNeighboursCoupling(o, p, &nb);
-    calcPorosity(o, p, &nb);
+    calcNeighboursCoupling(o, p, &cl[k]);
+    calcPorosity(o, p, &cl[k]);
   }
   /* traverse<initShepSum,ifDemOrGhost>: DEM and DEM ghosts */
   for (int k = 0; k < o->n; k++)
     if (ifDem(&o->ps[k])) o->ps[k].shepSum = 0.0;
   for (int g = 0; g < o->ng; g++)
     if (ifDem(&o->gh[g])) o->gh[g].shepSum = 0.0;
+  /* the scatter, serial in container order */
   for (int k = 0; k < o->n; k++) {
     Particle* p = &o->ps[k];
     if (!ifSphOrSphBoundary(p)) continue;
-    calcNeighboursCoupling(o, p, &nb);
-    calcShepSum(o, p, &nb);
+    calcShepSum(o, p, &cl[k]);
   }
   /* reverseSyncParticlesBetweenProcs<readShepSum,sumShepSum> dataLL.impl.h:414-477:
    * ghost values return to their origin, summed in direction order. */
@@ -1052,18 +1071,19 @@ static void passPorosity(Oracle* o) {
     for (int g = 0; g < o->ng; g++)
       if (o->ghost_dir[g] == d) o->ghost_origin[g]->shepSum += o->gh[g].shepSum;
   for (int g = 0; g < o->ng; g++) o->gh[g].shepSum = o->ghost_origin[g]->shepSum;
+#pragma omp parallel for schedule(dynamic, 256)
   for (int k = 0; k < o->n; k++) {
     Particle* p = &o->ps[k];
     if (!ifSphOrSphBoundary(p)) continue;
-    calcNeighboursCoupling(o, p, &nb);
-    calcDEMtoSPHDrag(o, p, &nb);
+    calcDEMtoSPHDrag(o, p, &cl[k]);
   }
+  for (int k = 0; k < o->n; k++) free(cl[k].e);
+  free(cl);
   for (int k = 0; k < o->n; k++)
     if (ifSphOrSphBoundary(&o->ps[k])) finalisePorosity(&o->ps[k]);
   for (int k = 0; k < o->n; k++)
     if (ifBoundary(&o->ps[k])) o->ps[k].porosity = 1.0; /* setPorosityForBoundary */
   for (int g = 0; g < o->ng; g++) o->gh[g].porosity = o->ghost_origin[g]->porosity;
-  free(nb.e);
 }
 static void passFpi(Oracle* o) { /* sync f fb fp fv, then neighboursGroupCoupling<calcFPI,ifDem> */
   for (int g = 0; g < o->ng; g++) {
@@ -1075,16 +1095,21 @@ static void passFpi(Oracle* o) { /* sync f fb fp fv, then neighboursGroupCouplin
       o->gh[g].fv[i] = src->fv[i];
     }
   }
-  PList nb = {0, 0, 0};
-  for (int k = 0; k < o->n; k++) {
-    Particle* p = &o->ps[k];
-    if (!ifDem(p)) continue;
-    calcNeighboursCoupling(o, p, &nb);
-    calcFPI(o, p, &nb);
+#pragma omp parallel
+  {
+    PList nb = {0, 0, 0};
+#pragma omp for schedule(dynamic, 16)
+    for (int k = 0; k < o->n; k++) {
+      Particle* p = &o->ps[k];
+      if (!ifDem(p)) continue;
+      calcNeighboursCoupling(o, p, &nb);
+      calcFPI(o, p, &nb);
+    }
+    free(nb.e);
   }
-  free(nb.e);
 }
 static void passDemContacts(Oracle* o) { /* neighboursGroup<calcDEMContacts,ifDem> */
+#pragma omp parallel for schedule(dynamic, 64)
   for (int k = 0; k < o->n; k++) {
     Particle* p = &o->ps[k];
     if (!ifDem(p)) continue;
@@ -1128,10 +1153,13 @@ static int stepMiddle(Oracle* o) {
     if (!(o->P.flags & (SPHDEM_F_FIXED_DEM | SPHDEM_F_NO_DEM_CONTACTS)))
       if (o->time > o->P.time_drop) o->newDt = dmin2(o->newDt, o->P.dem_timestep);
   }
+  double newDt = o->newDt;
+#pragma omp parallel for schedule(static) reduction(min : newDt)
   for (int k = 0; k < o->n; k++) {
     Particle* p = &o->ps[k];
-    if (ifSphOrSphBoundary(p) || ifDem(p)) accelCondition(o, p);
+    if (ifSphOrSphBoundary(p) || ifDem(p)) accelCondition(p, &newDt);
   }
+  o->newDt = newDt;
   double oldDt = o->dt;
   o->dt = o->newDt; /* both branches of sphIncompress.cpp:145-152 */
   for (int k = 0; k < o->n; k++)
@@ -1279,6 +1307,18 @@ OEXPORT void orc_destroy(void* vo) {
 }
 
 OEXPORT void orc_set_params(void* vo, const sphdem_params* P) { ((Oracle*)vo)->P = *P; }
+
+/* Host threads of the parallel gathers (1 = the serial sweep); returns the
+ * number in use afterwards. */
+OEXPORT int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
 
 OEXPORT int orc_num_particles(void* vo) { return ((Oracle*)vo)->n; }
 OEXPORT long orc_num_deleted(void* vo) { return ((Oracle*)vo)->ndeleted; }

@@ -63,8 +63,16 @@ def load():
         getattr(lib, "orc_kernel_" + nm).argtypes = [vp, ctypes.c_double, ctypes.c_double]
         getattr(lib, "orc_kernel_" + nm).restype = ctypes.c_double
     lib.orc_particle_drag.argtypes = [vp, ctypes.c_double, dp, dp, ctypes.c_double, dp]
+    lib.orc_set_threads.argtypes = [ctypes.c_int]
+    lib.orc_set_threads.restype = ctypes.c_int
     _lib = lib
     return lib
+
+
+def set_threads(n: int) -> int:
+    """Host threads of the oracle's parallel gathers (0: leave as is); returns
+    the number in use.  Results do not depend on it."""
+    return load().orc_set_threads(n)
 
 
 def _dp(a):

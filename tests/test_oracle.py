@@ -623,3 +623,22 @@ def test_sediment_run_is_finite_and_dt_is_bounded(sed):
         assert np.all(np.isfinite(x)), name
     P = sed.params
     assert o.globals().dt <= P.dem_timestep * (1 + 1e-15)
+
+
+def test_parallel_oracle_equals_the_serial_sweep():
+    """The oracle's OpenMP gathers leave every result bit for bit as the
+    one-thread sweep leaves it (each particle's sums keep their order; the dt
+    minimum is exact in any order)."""
+    c = sd.sediment_case(nf=10, n_dem=40, seed=3)
+    out = {}
+    for nt in (1, 4):
+        oracle.set_threads(nt)
+        o = oracle.Oracle(c)
+        o.run(2)
+        out[nt] = {k: o.get(k) for k in ("r", "v", "vhat", "dens", "h", "f", "porosity",
+                                        "fdrag", "shepSum", "omega", "dddt")}
+        out[nt]["dt"] = o.globals().dt
+        o.close()
+    oracle.set_threads(1)
+    for k in out[1]:
+        assert np.array_equal(out[1][k], out[4][k], equal_nan=True), k
