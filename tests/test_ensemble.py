@@ -86,3 +86,35 @@ def test_a_failing_box_is_reported():
         sd.run_many(sims, 1, hookset=7, workers=2)  # no such hook set
     for s in sims:
         s.close()
+
+
+@pytest.mark.gpu
+def test_ensemble_boxes_match_the_oracle():
+    """Boxes of the ensemble generator (their own drag, stiffness and damping
+    draws) advanced together by sphdem_run_steps_many in the FP64 form, each
+    compared with the oracle run of that box: pair sets after the run
+    bit-exact, fields within the whole-step bound of the parity tests."""
+    import oracle
+
+    cases = sd.ensemble_cases(nbox=3, seed=41, nmin=1000, nmax=3000)
+    sims = [sd.Sim(c) for c in cases]
+    sd.run_many(sims, 2, workers=3)
+    for c, s in zip(cases, sims):
+        print(f"ensemble box {c.info['box']}: {c.counts()}", flush=True)
+        orc = oracle.Oracle(c)
+        orc.run(2)
+        alive = orc.get("iam") != -1
+        for nm in ("r", "vhat", "dens", "h", "f", "porosity", "fdrag", "shepSum"):
+            a, b = s.get(nm)[alive], orc.get(nm)[alive]
+            err = float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+            assert err <= 1e-10, (c.info["box"], nm, err)
+        # the searches of the evolved state (a reset on both first: the cached
+        # lists predate end()'s driftRest)
+        s.reset()
+        orc.reset()
+        for kind in (sd.PAIRS_SPH, sd.PAIRS_SPH2DEM, sd.PAIRS_DEM2SPH):
+            g = set(map(tuple, s.pairs(kind).tolist()))
+            o = set(map(tuple, orc.pairs(kind).tolist()))
+            assert g == o, (c.info["box"], kind)
+        orc.close()
+        s.close()

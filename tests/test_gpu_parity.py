@@ -61,7 +61,14 @@ def contact_case():
     return c
 
 
-CASES = {"dambreak": dam_case, "sediment": sed_case}
+def closedbox_case():
+    """The large mixed-precision workload's geometry at 12^3: the dam-break box
+    closed on every side with the SPH-DEM flag set and DEM spheres in the upper
+    half of the fluid block (1.6 % of the fluid count, as 65,536 of 160^3)."""
+    return sd.dambreak_case(nf=12, with_dem=28, liq_dem_flags=True)
+
+
+CASES = {"dambreak": dam_case, "sediment": sed_case, "closedbox": closedbox_case}
 
 
 @pytest.fixture(scope="module", params=list(CASES))

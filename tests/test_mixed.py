@@ -19,7 +19,9 @@ def _monaghan():
 
 CASES = {"dambreak": lambda: sd.dambreak_case(nf=12),
          "sediment": lambda: sd.sediment_case(nf=12, n_dem=60),
-         "dambreak_monaghan": _monaghan}
+         "dambreak_monaghan": _monaghan,
+         # the large mixed-precision workload's geometry (closed box, SPH-DEM) at 12^3
+         "closedbox": lambda: sd.dambreak_case(nf=12, with_dem=28, liq_dem_flags=True)}
 
 
 def pair_of_sims(mk):
