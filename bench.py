@@ -1,25 +1,37 @@
 """Benchmark: SPH-DEM particle-steps/s of the gfx950 timestep loop.
 
-Workload (one GPU): SPH-DEM sedimentation, 64^3 = 262,144 fluid particles
-(dx = 1/64 m, periodic in x and y), 2 x 3 layers of sphBoundary walls
-(24,576) and 4,096 DEM spheres (diameter dx/2, density 2500), the sediment3D
-flag set, FP64.  A step is one run.cpp iteration (sphStep++, the sediment3D
-hooks, start/middle/end) over all particles with the inputs already resident
-in HBM.  particle-steps/s = (N_fluid + N_DEM) * steps / wall time of the
-timed steps (walls are processed but not counted).
+Default workload (one GPU): the large mixed-precision SPH-DEM case, 160^3 =
+4,096,000 fluid particles (dx = 0.002 m) in the dam-break box (2L x L x L,
+L = 0.32 m) closed by 3 layers of sphBoundary walls (791,256), with 65,536
+DEM spheres (diameter dx/2, density 2500) in the upper half of the fluid
+block, the SPH-DEM flag set (Morris viscosity, VAR_H_CORRECTION, LIQ_DEM,
+Di Felice drag, custom DEM floor).  The same initial state is run in both
+forms, FP64 and mixed (FP32 pair arithmetic over FP64 positions, lists,
+sums, integration and dt); `value` is the mixed form's throughput and the
+line carries the FP64 form's, their ratio and the mixed form's error against
+FP64.  A step is one run.cpp iteration (sphStep++, the sediment3D hooks,
+start/middle/end) over all particles with the inputs already resident in HBM.
+particle-steps/s = (N_fluid + N_DEM) * steps / wall time of the timed steps
+(walls are processed but not counted).  `--workload sediment` runs the 64^3
+sedimentation box (periodic x/y) the same way.
 
 Alongside the metric the JSON line carries
-  roofline      : the dominant kernel's algorithmic bytes per launch divided by
-                  its mean duration, timed with HIP events on the library's own
-                  stream over the timed steps, against the 8 TB/s HBM peak
-                  (default kernel: the SPH neighbour-list build, two launches a
-                  step); `traffic` is the kernel's HBM bytes per launch from the
-                  rocprofv3 FETCH_SIZE / WRITE_SIZE summary committed under
-                  profiles/, scaled by the copy-kernel calibration it records;
-  step_roofline : whole-step algorithmic bytes (1152 B per SPH/wall particle +
-                  650 B per DEM particle per step) x steps/s against 8 TB/s;
-  cpu_baseline  : the CPU FP64 restatement timed on this host on a bounded
-                  sample of the same workload.
+  roofline      : the dominant kernel of the headline form (default: the SPH
+                  neighbour-list build, two launches a step) with its
+                  algorithmic bytes per launch -- the cell-build row of the
+                  per-pass table, 32 B per SPH/wall particle (read r, write
+                  cell key and index) -- divided by its mean duration, timed
+                  with HIP events on the library's own stream over the timed
+                  steps, against the 8 TB/s HBM peak; `traffic` is the kernel's
+                  HBM bytes per launch from the rocprofv3 FETCH_SIZE /
+                  WRITE_SIZE summary committed under profiles/, scaled by the
+                  copy-kernel calibration it records; the list entries the
+                  kernel writes (4 B each) are reported beside it, not in it;
+  step_roofline : whole-step algorithmic bytes of the headline form (FP64:
+                  1152 B per SPH/wall particle + 650 B per DEM particle per
+                  step; mixed: 1040 + 650) x steps/s against 8 TB/s;
+  cpu_baseline  : the CPU FP64 restatement timed on this host's cores on one
+                  step of the same workload (all host threads, OpenMP).
 """
 import argparse
 import json
@@ -38,37 +50,38 @@ import sphdem as sd  # noqa: E402
 METRIC = "SPH-DEM particle-steps/s, mixed vs FP64 at stated error; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per particle per launch (FP64 widths), from the pass table:
-# each input field read once, each output field written once per pass.
+# Algorithmic bytes per SPH/wall particle per launch, from the per-pass table
+# (each input field of a pass read once, each output field written once, at
+# the widths the form reads and stores them; neighbour re-reads not counted).
+# A kernel that does the work of several passes counts each of them.
 B_ALG_KERNEL = {
-    # calcNeighbours: r h -> the list (4 B per entry, filled in below from the
-    # measured mean list length) + its count
-    "lists": 32 + 4,
-    # calcForce with calcOmega in the same walk (the step's omega pass, VAR_H):
-    # r vhat h mass dens pdr2 iam porosity -> f fp fv fb omega
-    "force": 76 + 16 + 24 + 72 + 8,
-    # calcDddtDudt with calcOmega in the same walk: r vhat h mass dens iam ->
-    # dddt omega
-    "dddt": 76 + 8 + 8,
+    # the cell build (read r, write cell key + index): the list build's row
+    "lists": {"fp64": 32, "mixed": 32},
+    # calcForce (r vhat h mass dens pdr2 iam, omega porosity -> f fp fv fb)
+    # with calcOmega summed in the same walk (r h mass dens iam -> omega)
+    "force": {"fp64": 196 + 60, "mixed": 156 + 40},
+    # calcDddtDudt (r vhat h mass iam, omega -> dddt) with calcOmega
+    "dddt": {"fp64": 84 + 60, "mixed": 52 + 40},
     # porosity (28 -> 8) + DEM->SPH drag (56 -> 48)
-    "coupling": 36 + 104,
+    "coupling": {"fp64": 36 + 104, "mixed": 36 + 104},
 }
-B_ALG_STEP_SPH = 1152  # per SPH / wall particle per step (SPH-DEM flag set)
-B_ALG_STEP_DEM = 650   # per DEM particle per step
+# whole step per SPH/wall particle and per DEM particle
+B_ALG_STEP_SPH = {"fp64": 1152, "mixed": 1040}
+B_ALG_STEP_DEM = {"fp64": 650, "mixed": 650}
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=200)
-    p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--steps", type=int, default=100)
+    p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--workload", choices=("sediment", "large", "ensemble", "adjoint"),
-                   default="sediment",
-                   help="sediment: 64^3 sedimentation (the headline); large: the 160^3 "
-                        "dam-break geometry with 65,536 DEM and the SPH-DEM flag set; "
-                        "ensemble: 256 independent sedimentation boxes of 4,096-65,536 "
-                        "fluid particles advanced together, against the same boxes run "
-                        "one after another")
+                   default="large",
+                   help="large (the headline): 160^3 fluid + 65,536 DEM in the closed "
+                        "dam-break box with the SPH-DEM flag set, FP64 and mixed forms; "
+                        "sediment: the 64^3 sedimentation box; ensemble: 256 independent "
+                        "sedimentation boxes of 4,096-65,536 fluid particles advanced "
+                        "together, against the same boxes run one after another")
     p.add_argument("--boxes", type=int, default=256, help="ensemble: number of boxes")
     p.add_argument("--no-tune", action="store_true",
                    help="ensemble: keep the default tile shape (no autotuning)")
@@ -82,8 +95,10 @@ def parse():
     p.add_argument("--traffic-file", default=None,
                    help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary to take `traffic` from "
                         "(default: the committed summary of the default workload)")
-    p.add_argument("--cpu-seconds", type=float, default=12.0,
-                   help="budget of the CPU baseline sample")
+    p.add_argument("--cpu-steps", type=int, default=1,
+                   help="steps of the CPU baseline sample")
+    p.add_argument("--cpu-threads", type=int, default=0,
+                   help="host threads of the CPU baseline (0: all)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-mixed", action="store_true",
                    help="skip the mixed-precision form (its timing and error)")
@@ -105,37 +120,38 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 3}), flush=True)
 
 
-def cpu_baseline(case, budget_s):
+def cpu_baseline(case, steps, threads):
+    """The CPU FP64 restatement (oracle/, the checker of the GPU library) on
+    this host: `steps` steps of the same case, its gather passes on `threads`
+    OpenMP threads (0: all the host offers).  Only the steps are timed."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle
 
+    used = oracle.set_threads(threads if threads > 0 else (os.cpu_count() or 1))
     orc = oracle.Oracle(case)
-    done = 0
     t0 = time.perf_counter()
-    while True:
-        orc.run(1)
-        done += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or done >= 5:
-            break
+    orc.run(steps)
+    el = time.perf_counter() - t0
     counted = int(np.sum((case.iam == sd.SPH) | (case.iam == sd.DEM)))
     orc.close()
     return {
-        "value": counted * done / el,
+        "value": counted * steps / el,
         "unit": "particle-steps/s",
-        "cores": 1,
+        "cores": used,
         "kind": "port",
-        "sample": f"{done} step(s) of the same {case.n}-particle case "
-                  f"({el:.1f} s, single-threaded CPU FP64 restatement)",
+        "sample": f"{steps} step(s) of the same {case.n}-particle case ({el:.1f} s, the CPU "
+                  f"FP64 restatement, gather passes on {used} OpenMP threads, the resets "
+                  "and the Shepard scatter serial)",
     }
 
 
 KERNEL_SYMBOL = {"lists": "k_lists", "force": "k_force_l",
                  "dddt": "k_dddt_l", "coupling": "k_coupling_sph",
                  "adj_force_bwd": "k_force_bwd"}
+KERNEL_SYMBOL_MIXED = {"force": "k_force_m", "dddt": "k_dddt_m"}
 
 
-def traffic_from_summary(path, kernel):
+def traffic_from_summary(path, kernel, form="fp64"):
     """HBM bytes per launch of `kernel` from a committed rocprofv3 summary:
     lines `calibration FETCH_SIZE <factor>` / `calibration WRITE_SIZE <factor>`
     and `<symbol> <dispatches> <FETCH_SIZE KiB> <WRITE_SIZE KiB>` per kernel."""
@@ -153,6 +169,8 @@ def traffic_from_summary(path, kernel):
             elif len(w) >= 4 and w[1].isdigit():
                 rows[w[0]] = (float(w[2]), float(w[3]))
     sym = KERNEL_SYMBOL.get(kernel)
+    if form == "mixed":
+        sym = KERNEL_SYMBOL_MIXED.get(kernel, sym)
     if sym not in rows or "FETCH_SIZE" not in fac or "WRITE_SIZE" not in fac:
         return None
     fetch, write = rows[sym]
@@ -171,10 +189,10 @@ def make_case(a):
 
 def workload_name(a, counts):
     if a.workload == "large":
-        return (f"SPH-DEM dam-break geometry {round(counts['fluid'] ** (1 / 3))}^3 fluid + "
-                f"{counts['wall']} wall + {counts['dem']} DEM, closed box, SPH-DEM flags, FP64")
+        return (f"SPH-DEM closed dam-break box {round(counts['fluid'] ** (1 / 3))}^3 fluid + "
+                f"{counts['wall']} wall + {counts['dem']} DEM, SPH-DEM flags")
     return (f"SPH-DEM sedimentation {round(counts['fluid'] ** (1 / 3))}^3 fluid + "
-            f"{counts['wall']} wall + {counts['dem']} DEM, periodic x/y, sediment3D flags, FP64")
+            f"{counts['wall']} wall + {counts['dem']} DEM, periodic x/y, sediment3D flags")
 
 
 def _errors(mix, ref, iam):
@@ -192,37 +210,141 @@ def _errors(mix, ref, iam):
     return e
 
 
-def mixed_form(a, counts, counted, final64):
-    """The mixed-precision form (FP32 pair arithmetic over FP64 positions, lists,
-    sums, integration and dt) on the same workload: its throughput over the same
-    timed steps, and its error against the FP64 form after one step (arithmetic
-    error) and after warmup + timed steps (including the growth of differences
-    along the trajectory)."""
-    one = {}
-    for prec in (sd.FP64, sd.MIXED):
-        s = sd.Sim(make_case(a), device=0, precision=prec)
-        s.run(1)
-        one[prec] = {k: s.get(k) for k in ("r", "dens", "f")}
-        iam = s.get("iam")
-        s.close()
-    sim = sd.Sim(make_case(a), device=0, precision=sd.MIXED)
-    sim.run(a.warmup)
+def time_form(a, case, prec, kernel):
+    """One form on the workload: the first warm-up step (its state is the
+    one-step error reference), the other warm-up steps, then `steps` timed
+    steps with every launch of `kernel` bracketed by HIP events on the
+    library's stream."""
+    sim = sd.Sim(case, device=0, precision=prec)
+    one = None
+    if a.warmup > 0:
+        sim.run(1)
+        one = {k: sim.get(k) for k in ("r", "dens", "f")}
+        sim.run(a.warmup - 1)
     sim.sync()
+    sim.timing_begin(kernel)
     t0 = time.perf_counter()
     sim.run(a.steps)
     sim.sync()
     el = time.perf_counter() - t0
-    fin = {k: sim.get(k) for k in ("r", "dens")}
-    sim.close()
-    return {
-        "value": counted * a.steps / el,
-        "unit": "particle-steps/s",
-        "ms_per_step": 1e3 * el / a.steps,
-        "dtype": "f32 pair arithmetic, f64 positions/lists/sums/integration/dt",
-        "error_vs_fp64_1_step": _errors(one[sd.MIXED], one[sd.FP64], iam),
-        "error_vs_fp64_after_run": _errors(fin, final64, iam),
-        "steps_compared_after_run": a.warmup + a.steps,
+    kms, launches = sim.timing_end()
+    for nm in ("r", "dens"):
+        if not np.all(np.isfinite(sim.get(nm))):
+            raise RuntimeError(f"non-finite {nm} after the run")
+    counts = case.counts()
+    out = {
+        "seconds": el,
+        "kernel_ms": kms,
+        "launches": launches,
+        "one_step": one,
+        "final": {k: sim.get(k) for k in ("r", "dens")},
+        "iam": sim.get("iam"),
+        "globals": sim.globals(),
+        "mean_nb": sim.pair_count(sd.PAIRS_SPH) / max(counts["fluid"] + counts["wall"], 1),
     }
+    sim.close()
+    return out
+
+
+def run_forms(a):
+    """The FP64 and mixed forms on the same initial state; the mixed form is
+    the headline unless --no-mixed."""
+    case = make_case(a)
+    counts = case.counts()
+    counted = counts["fluid"] + counts["dem"]
+    units = counts["fluid"] + counts["wall"]  # SPH-group particles a pair kernel processes
+    kernel = a.roofline_kernel
+    print(f"bench: {workload_name(a, counts)}; FP64 form, {a.warmup} + {a.steps} steps",
+          file=sys.stderr, flush=True)
+    f64 = time_form(a, case, sd.FP64, kernel)
+    mix = None
+    if not a.no_mixed:
+        print("bench: mixed form", file=sys.stderr, flush=True)
+        mix = time_form(a, case, sd.MIXED, kernel)
+    head, form = (mix, "mixed") if mix is not None else (f64, "fp64")
+
+    def rate(r):
+        return counted * a.steps / r["seconds"]
+
+    if kernel in ("fpi", "contacts", "shepsum"):
+        units = counts["dem"]
+    per_unit = B_ALG_KERNEL.get(kernel, {}).get(form, 0)
+    b_launch = per_unit * units
+    mean_s = (head["kernel_ms"] / 1e3) / max(head["launches"], 1)
+    achieved = b_launch / mean_s / 1e9 if head["launches"] else None
+    tfile = a.traffic_file
+    if tfile is None and a.nf is None and a.ndem is None:
+        tfile = os.path.join(ROOT, "profiles", {
+            "large": "traffic_closedbox_160cubed.txt",
+            "sediment": "traffic_sediment_64cubed.txt"}[a.workload])
+    traffic = traffic_from_summary(tfile, kernel, form) if tfile else None
+    step_bytes = B_ALG_STEP_SPH[form] * units + B_ALG_STEP_DEM[form] * counts["dem"]
+    step_gbs = step_bytes * (a.steps / head["seconds"]) / 1e9
+    g = head["globals"]
+    out = {
+        "metric": METRIC,
+        "value": rate(head),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * head["seconds"] / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32/f64 mixed" if form == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": workload_name(a, counts),
+            "form": form,
+            "fluid": counts["fluid"], "wall": counts["wall"], "dem": counts["dem"],
+            "seed": 1002 if a.workload == "large" else 1001, "sim_time_s": g.time,
+            "dt_last": g.dt,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": kernel,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": traffic,
+            "mean_launch_ms": mean_s * 1e3,
+            "launches": head["launches"],
+            "alg_bytes_per_launch": b_launch,
+            "alg_bytes_per_unit": per_unit,
+            "units_per_launch": units,
+            "mean_sph_neighbours": head["mean_nb"],
+            "list_entry_bytes_per_launch": (4.0 * head["mean_nb"] * units
+                                            if kernel == "lists" else None),
+        },
+        "step_roofline": {
+            "alg_bytes_per_step": step_bytes,
+            "achieved": step_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": step_gbs / HBM_PEAK_GBS,
+        },
+    }
+    f64_bytes = B_ALG_STEP_SPH["fp64"] * units + B_ALG_STEP_DEM["fp64"] * counts["dem"]
+    out["fp64"] = {
+        "value": rate(f64),
+        "unit": "particle-steps/s",
+        "ms_per_step": 1e3 * f64["seconds"] / a.steps,
+        "step_roofline_frac": f64_bytes * (a.steps / f64["seconds"]) / 1e9 / HBM_PEAK_GBS,
+        "kernel_mean_launch_ms": (f64["kernel_ms"] / max(f64["launches"], 1)),
+    }
+    if mix is not None:
+        out["mixed_over_fp64"] = f64["seconds"] / mix["seconds"]
+        err = {"after_run": _errors(mix["final"], f64["final"], f64["iam"]),
+               "steps_compared_after_run": a.warmup + a.steps}
+        if mix["one_step"] is not None:
+            err["after_1_step"] = _errors(mix["one_step"], f64["one_step"], f64["iam"])
+        out["mixed_error_vs_fp64"] = err
+    if not a.no_cpu:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(case, a.cpu_steps, a.cpu_threads)
+    print(json.dumps(out), flush=True)
 
 
 def ensemble(a):
@@ -274,7 +396,8 @@ def ensemble(a):
         if not np.all(np.isfinite(s.get("dens"))):
             raise RuntimeError("non-finite density after the run")
         s.close()
-    step_bytes = B_ALG_STEP_SPH * (fluid + wall) + B_ALG_STEP_DEM * dem
+    fm = "fp64" if prec == sd.FP64 else "mixed"
+    step_bytes = B_ALG_STEP_SPH[fm] * (fluid + wall) + B_ALG_STEP_DEM[fm] * dem
     step_gbs = step_bytes * (a.steps / el) / 1e9
     out = {
         "metric": METRIC,
@@ -496,89 +619,7 @@ def main():
     if a.workload == "adjoint":
         adjoint(a)
         return
-    case = make_case(a)
-    counts = case.counts()
-    counted = counts["fluid"] + counts["dem"]
-    sim = sd.Sim(case, device=0)
-    sim.run(a.warmup)
-    sim.sync()
-    sim.timing_begin(a.roofline_kernel)
-    t0 = time.perf_counter()
-    sim.run(a.steps)
-    sim.sync()
-    t1 = time.perf_counter()
-    kms, launches = sim.timing_end()
-    el = t1 - t0
-    # sanity: the state is finite
-    for nm in ("r", "dens"):
-        if not np.all(np.isfinite(sim.get(nm))):
-            raise RuntimeError(f"non-finite {nm} after the run")
-    g = sim.globals()
-    units = counts["fluid"] + counts["wall"]  # SPH-group particles a pair kernel processes
-    mean_nb = sim.pair_count(sd.PAIRS_SPH) / max(units, 1)
-    final64 = {k: sim.get(k) for k in ("r", "dens")}
-    sim.close()
-    mixed = None if a.no_mixed else mixed_form(a, counts, counted, final64)
-    value = counted * a.steps / el
-    if a.roofline_kernel in ("fpi", "contacts", "shepsum"):
-        units = counts["dem"]
-    per_unit = B_ALG_KERNEL.get(a.roofline_kernel, 0)
-    if a.roofline_kernel == "lists":
-        per_unit += 4 * mean_nb
-    b_launch = per_unit * units
-    tfile = a.traffic_file
-    if tfile is None and a.workload == "sediment" and a.nf is None and a.ndem is None:
-        tfile = os.path.join(ROOT, "profiles", "traffic_sediment_64cubed.txt")
-    traffic = traffic_from_summary(tfile, a.roofline_kernel) if tfile else None
-    mean_s = (kms / 1e3) / max(launches, 1)
-    achieved = b_launch / mean_s / 1e9 if launches else None
-    step_bytes = B_ALG_STEP_SPH * (counts["fluid"] + counts["wall"]) + B_ALG_STEP_DEM * counts["dem"]
-    step_gbs = step_bytes * (a.steps / el) / 1e9
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "particle-steps/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": 1e3 * el / a.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {
-            "workload": workload_name(a, counts),
-            "fluid": counts["fluid"], "wall": counts["wall"], "dem": counts["dem"],
-            "seed": 1002 if a.workload == "large" else 1001, "sim_time_s": g.time,
-            "dt_last": g.dt,
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": a.roofline_kernel,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": traffic,
-            "mean_launch_ms": mean_s * 1e3,
-            "launches": launches,
-            "alg_bytes_per_launch": b_launch,
-            "mean_sph_neighbours": mean_nb,
-        },
-        "step_roofline": {
-            "alg_bytes_per_step": step_bytes,
-            "achieved": step_gbs,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": step_gbs / HBM_PEAK_GBS,
-        },
-    }
-    if mixed is not None:
-        out["mixed"] = mixed
-    if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(case, a.cpu_seconds)
-    print(json.dumps(out), flush=True)
+    run_forms(a)
 
 
 if __name__ == "__main__":
