@@ -21,7 +21,8 @@
  * Compiled by oracle/Makefile into oracle/liboracle.so and driven through
  * ctypes (tests/oracle.py).  The reference itself does not build in this image
  * (vect.h:7 includes GSL headers, which are absent), so this restatement is
- * pinned by closed forms, invariants and brute force in tests/test_oracle_*.py.
+ * pinned by closed forms, invariants and brute force in tests/test_oracle.py
+ * and tests/test_oracle_physics.py.
  *
  * Parity status of each function is listed in the table at the end of this
  * header comment in the form  function : how it is pinned.
@@ -29,19 +30,36 @@
  *   cells getCellI/dem_getCellI  : formula restated; brute-force bin check
  *   calcNeighbours               : brute force O(N^2) with the same predicate
  *   periodic ghosts/updateDomain : brute-force minimum image pair set
- *   calcNeighboursCoupling (SPH) : brute-force evaluation of the cell rule
  *   calcOmega, calcDensity       : lattice sums in closed form
- *   calcForce (press+visc)       : antisymmetry / momentum conservation,
- *                                  zero force on a uniform periodic lattice
- *   calcDddtDudt                 : zero under rigid translation
+ *   calcPressForce               : -grad(p)/rho of a linear pressure field on
+ *                                  a lattice; zero on a uniform lattice;
+ *                                  antisymmetry / momentum conservation
+ *   calcViscForce (Morris)       : nu lap(v) of a quadratic shear (plain and
+ *                                  LIQ_DEM coefficients), zero for a linear
+ *                                  shear; Cleary/Monaghan: conservation,
+ *                                  dissipation, zero under rigid rotation
+ *   calcDddtDudt                 : 3 alpha rho under uniform compression
+ *                                  (1/omega_a included); zero under rigid
+ *                                  translation and rotation
+ *   calcPressSpsoundPdr2         : closed form; LIQ_DEM zero at REFD*eps and
+ *                                  slope PRB*GAMMA/(REFD*eps)
  *   integrator, dt rule          : hand-computed single-particle trajectory
+ *                                  (also fixes the first-step dt = 0)
  *   calcParticleDrag             : Di Felice closed form
+ *   calcFPI                      : uniform-fluid closed forms of every
+ *                                  Shepard-normalised output, the fv double
+ *                                  count included, plus the Di Felice term
  *   calcDEMContacts              : two-sphere restitution closed form
- *   calcFPI fv double count, calcShepSum scatter+reverse sync,
- *   gradW using pa.omega for both sides, first-step dt = 0 logic
- *                                : parity unpinned (only the compiled
- *                                  reference could pin them; restated from
- *                                  the cited lines)
+ *   porosity, calcShepSum,
+ *   calcDEMtoSPHDrag             : porosity/Shepard-sum consistency; Newton's
+ *                                  third law of the drag pair
+ *   parity unpinned (only the compiled reference could pin them; restated
+ *   from the cited lines):
+ *     calcNeighboursCoupling's SPH-side cell rule (checked only against a
+ *     brute-force evaluation of the same rule); which particle's omega
+ *     divides gradW under VAR_H (pa's for both sides: equal on a uniform
+ *     lattice, so no closed form tells them apart); the order of the
+ *     reverse-sync additions of the Shepard scatter.
  */
 #include <math.h>
 #include <stdint.h>
