@@ -8,8 +8,14 @@ in 32 KiB blocks (the XML format's "binary" mode with vtkZLibDataCompressor and
 UInt32 block headers).  The reader also accepts uncompressed binary and ascii
 arrays, as ParaView may write them.
 
-Array names follow the reference's writer: dens, h, u, mass, iam, tag, v, vhat,
-press, porosity, shepSum, fdrag, fb, fp, fv.
+Two layouts follow the reference's writers: the restart file of
+writeRestart (io_data_vtk.cpp:231-411: FP64 dens h eBForce eViscF eViscB eFF u
+mass colour, Int32 iam tag, FP64 3-vectors v vhat vort norm1 norm2 norm3, with
+iam, v and norm1 the active scalars, vectors and normals, one piece per rank
+under a .pvtu index) and the output file of writeOutput (dens, h, u, mass, iam,
+tag, v, vhat, press, porosity, shepSum, fdrag, fb, fp, fv).  Positions are
+written as Float64 (the reference's vtkPoints default to Float32, which rounds
+a restart's positions).
 """
 from __future__ import annotations
 
@@ -46,17 +52,21 @@ def _array_xml(name, a, ncomp, compress, indent):
     return f'{indent}<DataArray {attrs}>\n{indent}  {_encode(a, compress)}\n{indent}</DataArray>\n'
 
 
-def write_vtu(path: str, points: np.ndarray, point_data: dict, compress: bool = True):
-    """points: (n, 3) float64; point_data: name -> (n,) or (n, k) array."""
+def write_vtu(path: str, points: np.ndarray, point_data: dict, compress: bool = True,
+              active: dict | None = None):
+    """points: (n, 3) float64; point_data: name -> (n,) or (n, k) array;
+    active: PointData attributes naming the active arrays (Scalars, Vectors,
+    Normals)."""
     pts = np.ascontiguousarray(points, dtype=np.float64)
     n = len(pts)
     comp = ' compressor="vtkZLibDataCompressor"' if compress else ""
+    attrs = "".join(f' {k}="{v}"' for k, v in (active or {}).items())
     out = ['<?xml version="1.0"?>\n',
            f'<VTKFile type="UnstructuredGrid" version="0.1" byte_order="LittleEndian"'
            f' header_type="UInt32"{comp}>\n',
            "  <UnstructuredGrid>\n",
            f'    <Piece NumberOfPoints="{n}" NumberOfCells="{n}">\n',
-           "      <PointData>\n"]
+           f"      <PointData{attrs}>\n"]
     for name, a in point_data.items():
         a = np.asarray(a)
         if a.dtype == np.float32 or a.dtype == np.float64:
@@ -103,6 +113,14 @@ def _decode(text: str, dtype, compressed: bool, header_fmt: str) -> np.ndarray:
     return np.frombuffer(b"".join(parts), dtype=dtype).copy()
 
 
+def point_data_layout(path: str) -> list:
+    """(name, VTK type, components) of every PointData array of a .vtu file,
+    in file order, without decoding them."""
+    root = ET.parse(path).getroot()
+    return [(el.get("Name"), el.get("type"), int(el.get("NumberOfComponents", "1")))
+            for el in root.findall("UnstructuredGrid/Piece/PointData/DataArray")]
+
+
 def read_vtu(path: str):
     """Returns (points (n, 3), {name: array}) of the first piece."""
     root = ET.parse(path).getroot()
@@ -143,6 +161,83 @@ def write_particles(sim, path: str, compress: bool = True):
     data["iam"] = sim.get("iam").astype(np.int32)
     data["tag"] = sim.get("tag").astype(np.int32)
     write_vtu(path, sim.get("r"), data, compress)
+
+
+# writeRestart's arrays in its order (io_data_vtk.cpp:283-300, 3D), with the
+# Cparticle field each holds; fields the library does not carry (colour,
+# vorticity, the boundary normals, the per-particle energies other than the
+# viscous one) are written as zeros, as for particles that never set them
+RESTART_ARRAYS = (("dens", 1, np.float64), ("h", 1, np.float64), ("eBForce", 1, np.float64),
+                  ("eViscF", 1, np.float64), ("eViscB", 1, np.float64), ("eFF", 1, np.float64),
+                  ("u", 1, np.float64), ("mass", 1, np.float64), ("colour", 1, np.float64),
+                  ("iam", 1, np.int32), ("tag", 1, np.int32), ("v", 3, np.float64),
+                  ("vhat", 3, np.float64), ("vort", 3, np.float64), ("norm1", 3, np.float64),
+                  ("norm2", 3, np.float64), ("norm3", 3, np.float64))
+RESTART_ACTIVE = {"Scalars": "iam", "Vectors": "v", "Normals": "norm1"}
+_CARRIED = {"dens", "h", "u", "mass", "iam", "tag", "v", "vhat"}
+
+
+def restart_arrays(get, n: int) -> dict:
+    """The restart file's point data from `get(name)` (a live Sim's or any
+    object with the library's field names)."""
+    out = {}
+    for name, k, dt in RESTART_ARRAYS:
+        if name in _CARRIED:
+            out[name] = np.asarray(get(name)).astype(dt)
+        else:
+            out[name] = np.zeros((n, k) if k > 1 else n, dtype=dt)
+    return out
+
+
+def write_restart(src, path: str, compress: bool = True):
+    """The reference's restart file (writeRestart) of a live simulation or the
+    oracle: the .vtu of one piece, particles in upload order."""
+    n = len(src.get("iam"))
+    write_vtu(path, src.get("r"), restart_arrays(src.get, n), compress, RESTART_ACTIVE)
+
+
+def write_pvtu(path: str, pieces: list, layout: list, active: dict | None = None):
+    """The parallel index file vtkXMLPUnstructuredGridWriter writes beside the
+    pieces: the arrays' names, types and components, and one Piece per rank."""
+    attrs = "".join(f' {k}="{v}"' for k, v in (active or {}).items())
+    out = ['<?xml version="1.0"?>\n',
+           '<VTKFile type="PUnstructuredGrid" version="0.1" byte_order="LittleEndian"'
+           ' compressor="vtkZLibDataCompressor">\n',
+           '  <PUnstructuredGrid GhostLevel="0">\n', f"    <PPointData{attrs}>\n"]
+    for name, vtype, k in layout:
+        nc = f' NumberOfComponents="{k}"' if k > 1 else ""
+        out.append(f'      <PDataArray type="{vtype}" Name="{name}"{nc}/>\n')
+    out.append("    </PPointData>\n    <PPoints>\n"
+               '      <PDataArray type="Float64" NumberOfComponents="3"/>\n    </PPoints>\n')
+    for p in pieces:
+        out.append(f'    <Piece Source="{p}"/>\n')
+    out.append("  </PUnstructuredGrid>\n</VTKFile>\n")
+    with open(path, "w") as f:
+        f.write("".join(out))
+
+
+def read_pvtu(path: str):
+    """(piece file names, [(name, VTK type, components)]) of a .pvtu index."""
+    root = ET.parse(path).getroot()
+    if root.get("type") != "PUnstructuredGrid":
+        raise ValueError(f"{path}: not a PUnstructuredGrid file")
+    g = root.find("PUnstructuredGrid")
+    layout = [(el.get("Name"), el.get("type"), int(el.get("NumberOfComponents", "1")))
+              for el in g.findall("PPointData/PDataArray")]
+    return [p.get("Source") for p in g.findall("Piece")], layout
+
+
+def write_restart_set(src, base: str, timestep: int, compress: bool = True):
+    """A one-rank restart as the reference names it: base + "Restart" +
+    %07d + ".pvtu" indexing base + "Restart" + %07d + "_0.vtu"
+    (io_data_vtk.cpp:26-28, 381)."""
+    import os
+
+    stem = f"{base}Restart{timestep:07d}"
+    write_restart(src, stem + "_0.vtu", compress)
+    layout = [(nm, "Int32" if dt == np.int32 else "Float64", k) for nm, k, dt in RESTART_ARRAYS]
+    write_pvtu(stem + ".pvtu", [os.path.basename(stem) + "_0.vtu"], layout, RESTART_ACTIVE)
+    return stem + ".pvtu"
 
 
 def case_from_restart(path: str, params, hookset: int, name: str = "restart"):
@@ -191,5 +286,10 @@ def append_globals(path: str, outstep: int, sim, wall_times=(0.0,) * 6):
 
 
 def read_globals(path: str) -> np.ndarray:
-    """The numeric table of a Globals file (header skipped)."""
-    return np.loadtxt(path, comments="#", ndmin=2)
+    """The numeric table of a Globals file (header skipped; a file with only
+    the header gives an empty table)."""
+    with open(path) as f:
+        rows = [ln for ln in f if ln.strip() and not ln.lstrip().startswith("#")]
+    if not rows:
+        return np.zeros((0, len(GLOBALS_HEADER.split()) - 1))
+    return np.loadtxt(rows, ndmin=2)
