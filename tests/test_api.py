@@ -103,9 +103,22 @@ def test_clipped_list_is_reported_by_the_call_that_used_it():
     sim.set("h", h0)
     sim.reset()
     sim.pass_("force")
-    # the same through a whole step: the failing call is the one of the step
+    # the same through the phase calls: start() recomputes h from the density
+    # (driftRest), so the large h is set after it; middle() is the call whose
+    # force pass walks the clipped lists
+    sim.start()
     sim.set("h", np.where(case.iam == sd.DEM, h0, 2.5 * h0))
     with pytest.raises(sd.SphDemError) as e:
-        sim.run(1)
+        sim.middle()
     assert e.value.code == sd.E_CAPACITY
     sim.close()
+    # and through whole steps under CONST_H (h as set stays): sphdem_run_steps
+    # fails in the call that built the clipped lists
+    c2 = sd.sediment_case(nf=12, n_dem=60)
+    c2.params.flags |= sd.F_CONST_H
+    c2.h = np.where(c2.iam == sd.DEM, c2.h, 2.5 * c2.h)
+    sim2 = sd.Sim(c2)
+    with pytest.raises(sd.SphDemError) as e:
+        sim2.run(1)
+    assert e.value.code == sd.E_CAPACITY
+    sim2.close()
