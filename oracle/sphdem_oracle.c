@@ -89,6 +89,8 @@ typedef struct Particle {
   double r[3], v[3], v0[3], vhat[3], vhat0[3];
   double f[3], fp[3], fv[3], fb[3], ff[3], fdrag[3], fvel[3];
   double mass, h, dens, dddt, press, pdr2, omega, porosity, shepSum, u, dudt;
+  /* per-particle energies of calcEnergies (particle.h:62-316) */
+  double eViscF, deViscFdt, eViscB, eBForce, eFF, deFFdt;
   int iam, tag;
   int shift[3]; /* ghosts: periodic image shift (-1,0,1) per axis; 0 for real */
   /* cached neighbour list (CpInfo::neighbrs, dataLL.h:76-84) */
@@ -986,8 +988,16 @@ static void initSumsMiddle(Particle* p) {
 }
 /* initSumsEnd sphIncompress.h:765-767 */
 static void initSumsEnd(Particle* p) { p->dddt = 0.0; }
-/* calcEnergies sphIncompress.h:769-781 (energy diagnostics omitted) */
+/* calcEnergies sphIncompress.h:769-781 */
 static void calcEnergies(Oracle* o, Particle* p) {
+  if (p->iam == IAM_SPH) {
+    p->eViscB = 0;
+    p->eBForce = 0;
+    p->deViscFdt = -dot3(p->vhat, p->fv);
+    p->eViscF += o->dt * p->deViscFdt;
+    p->deFFdt = -dot3(p->vhat, p->ff);
+    p->eFF += o->dt * p->deFFdt;
+  }
   calcPressSpsoundPdr2(o, p);
   p->dudt = p->pdr2 * p->dddt;
   p->u += o->dt * p->dudt;
@@ -1126,6 +1136,50 @@ static void passFpi(Oracle* o) { /* sync f fb fp fv, then neighboursGroupCouplin
     free(nb.e);
   }
 }
+/* calcDEMWallContacts sph.h:1634-1674: the nearest boundary or sphBoundary
+ * particle in the DEM's neighbour list.  The list of a DEM particle comes from
+ * the DEM cells, which hold DEM particles only (DEM are binned in dem_cells
+ * alone, dataLL.cpp:979-990), so the scan never finds a wall particle and the
+ * force is never applied; restated as the scan. */
+static void calcDEMWallContacts(Oracle* o, Particle* p) {
+  (void)o;
+  int found = 0;
+  for (int i = 0; i < p->nnb; i++) {
+    const Particle* pn = p->nb[i];
+    if ((pn->iam != IAM_BOUNDARY) && (pn->iam != IAM_SPHB)) continue;
+    found = 1;
+  }
+  if (found) {
+    fprintf(stderr, "oracle: a wall particle in a DEM neighbour list (dem cells hold DEM only)\n");
+    abort();
+  }
+}
+static void passDemWallContacts(Oracle* o) { /* neighboursGroup<calcDEMWallContacts,ifDem> */
+  for (int k = 0; k < o->n; k++) {
+    Particle* p = &o->ps[k];
+    if (!ifDem(p)) continue;
+    ensureNeighbours(o, p);
+    calcDEMWallContacts(o, p);
+  }
+}
+/* DEM_BLOCK sphIncompress.cpp:231-238: calcAvForce then setAvForce
+ * (sphIncompress.h:107-114) over the DEM of the container, in its order */
+static void demBlock(Oracle* o) {
+  double av[3] = {0, 0, 0}, tmp = 0;
+  for (int k = 0; k < o->n; k++) {
+    Particle* p = &o->ps[k];
+    if (!ifDem(p)) continue;
+    for (int i = 0; i < 3; i++) av[i] += p->f[i];
+    tmp++;
+  }
+  if (tmp == 0) return;
+  for (int i = 0; i < 3; i++) av[i] = av[i] / tmp;
+  for (int k = 0; k < o->n; k++) {
+    Particle* p = &o->ps[k];
+    if (!ifDem(p)) continue;
+    for (int i = 0; i < 3; i++) p->f[i] = av[i];
+  }
+}
 static void passDemContacts(Oracle* o) { /* neighboursGroup<calcDEMContacts,ifDem> */
 #pragma omp parallel for schedule(dynamic, 64)
   for (int k = 0; k < o->n; k++) {
@@ -1202,7 +1256,11 @@ static int stepEnd(Oracle* o) {
   passDddt(o);
   if (isLiqDem(o)) {
     passFpi(o);
-    if (!(o->P.flags & (SPHDEM_F_FIXED_DEM | SPHDEM_F_NO_DEM_CONTACTS))) passDemContacts(o);
+    if (!(o->P.flags & (SPHDEM_F_FIXED_DEM | SPHDEM_F_NO_DEM_CONTACTS))) {
+      passDemContacts(o);
+      if (!(o->P.flags & SPHDEM_F_CUSTOM_WALL)) passDemWallContacts(o);
+    }
+    if (o->P.flags & SPHDEM_F_DEM_BLOCK) demBlock(o);
     for (int k = 0; k < o->n; k++)
       if (ifDem(&o->ps[k])) kick(o, &o->ps[k], 0.5 * o->dt, 0.5 * o->dt);
   }
@@ -1262,6 +1320,65 @@ static void hookSumDrag(Oracle* o) {
     else if (p->iam == IAM_DEM)
       for (int i = 3; i < 6; i++) o->custom[i] += p->mass * p->fdrag[i - 3];
   }
+}
+
+/* ------------------------------------------------ output-step diagnostics */
+/* CsphIncompress::calcOutputVars sphIncompress.h:271-319 with calcOutput
+ * (:233-260), calcVarDens (:262-264) and calcAveDensFromMass (:266-272), at
+ * one rank (sumOverProcs is the identity).  Sums run in container order. */
+static void outputVars(Oracle* o, sphdem_output* g) {
+  memset(g, 0, sizeof(*g));
+  double nSph = 0, rmsFF = 0, aveDensFromMass = 0;
+  for (int k = 0; k < o->n; k++) { /* traverse<calcOutput,ifSphOrSphBoundary> */
+    Particle* p = &o->ps[k];
+    if (!ifSphOrSphBoundary(p)) continue;
+    if (p->iam == IAM_SPH) {
+      for (int i = 0; i < 3; i++) g->linMom[i] += p->v[i] * p->mass;
+      g->angMom += (p->r[0] * p->v[1] - p->r[1] * p->v[0]) * p->mass;
+      g->eKE += 0.5 * p->mass * len2v(p->vhat);
+      g->eViscF += p->mass * p->eViscF;
+      g->edViscFdt += p->mass * p->deViscFdt;
+      g->eViscB += p->mass * p->eViscB;
+      g->eBForce += p->mass * p->eBForce;
+      g->eFF += p->mass * p->eFF;
+      g->edFFdt += p->mass * p->deFFdt;
+      nSph++;
+    }
+    const double v = lenv(p->v);
+    if (v > g->maxV) g->maxV = v;
+    const double f = lenv(p->f);
+    if (f > g->maxF) g->maxF = f;
+    const double ff = lenv(p->ff);
+    if (ff > g->maxFF) g->maxFF = ff;
+    rmsFF += ff * ff; /* len2(ff) of the scalar ff: its square */
+    g->aveDens += p->dens;
+    g->eElast += p->mass * p->u;
+    g->edElastdt += p->mass * p->dudt;
+    double denfac = p->dens / o->P.refd;
+    denfac = o->P.prb * ((pow(denfac, 6) - 1) / 6 + 1 / denfac - 1);
+    g->eElastExact += p->mass * denfac / o->P.refd;
+  }
+  for (int k = 0; k < o->n; k++) { /* neighboursGroup<calcAveDensFromMass,...> */
+    Particle* p = &o->ps[k];
+    if (!ifSphOrSphBoundary(p)) continue;
+    ensureNeighbours(o, p);
+    const double oldDens = p->dens;
+    calcDensity(o, p);
+    aveDensFromMass += pow(p->dens - oldDens, 2);
+    p->dens = oldDens;
+  }
+  const int n = o->n; /* getParticles()->size(): deleted particles are gone */
+  g->n = n;
+  g->nSph = (int64_t)nSph;
+  g->aveDens = g->aveDens / n;
+  g->aveDensFromMass = sqrt(aveDensFromMass / n);
+  g->rmsFF = sqrt(rmsFF / n);
+  for (int k = 0; k < o->n; k++) { /* traverse<calcVarDens,ifSphOrSphBoundary> */
+    Particle* p = &o->ps[k];
+    if (ifSphOrSphBoundary(p)) g->varDens += pow(p->dens - g->aveDens, 2);
+  }
+  g->varDens /= n;
+  g->eTotal = g->eKE + g->eElast + g->eViscF + g->eViscB + g->eBForce + g->eFF;
 }
 
 /* ============================================================ public API */
@@ -1328,6 +1445,8 @@ OEXPORT void orc_set_params(void* vo, const sphdem_params* P) { ((Oracle*)vo)->P
 
 /* Host threads of the parallel gathers (1 = the serial sweep); returns the
  * number in use afterwards. */
+OEXPORT void orc_output_vars(void* vo, sphdem_output* out) { outputVars((Oracle*)vo, out); }
+
 OEXPORT int orc_set_threads(int n) {
 #ifdef _OPENMP
   if (n > 0) omp_set_num_threads(n);

@@ -63,6 +63,7 @@ def load():
         getattr(lib, "orc_kernel_" + nm).argtypes = [vp, ctypes.c_double, ctypes.c_double]
         getattr(lib, "orc_kernel_" + nm).restype = ctypes.c_double
     lib.orc_particle_drag.argtypes = [vp, ctypes.c_double, dp, dp, ctypes.c_double, dp]
+    lib.orc_output_vars.argtypes = [vp, ctypes.POINTER(sd.Output)]
     lib.orc_set_threads.argtypes = [ctypes.c_int]
     lib.orc_set_threads.restype = ctypes.c_int
     _lib = lib
@@ -193,6 +194,14 @@ class Oracle:
         self.lib.orc_pairs(self.h, kind, a.ctypes.data_as(ip), b.ctypes.data_as(ip),
                            im.ctypes.data_as(ip), n)
         return np.stack([a[:n], b[:n], im[:n]], axis=1)
+
+    def output_vars(self) -> dict:
+        """calcOutputVars diagnostics as a dict (the fields of sphdem_output)."""
+        o = sd.Output()
+        self.lib.orc_output_vars(self.h, ctypes.byref(o))
+        d = {k: getattr(o, k) for k, _ in sd.Output._fields_}
+        d["linMom"] = list(o.linMom)
+        return d
 
     def num_ghosts(self) -> int:
         return self.lib.orc_num_ghosts(self.h)

@@ -60,3 +60,32 @@ def test_density_from_mass_vanishes_after_a_density_pass():
     assert o["aveDensFromMass"] == 0.0
     sim.run(1)
     assert sim.output_vars()["aveDensFromMass"] > 0.0
+
+
+@pytest.mark.parametrize("case", ["sediment", "closedbox"])
+def test_output_vars_match_the_oracle(case):
+    """sphdem_output_vars after several steps against the oracle's
+    calcOutputVars with its calcEnergies (sphIncompress.h:233-319, 769-781):
+    the running viscous energy, the elastic energies, momenta, maxima and the
+    density statistics, within the whole-step bound of the parity tests; the
+    counts exactly."""
+    import oracle
+
+    c = (sd.sediment_case(nf=12, n_dem=60) if case == "sediment"
+         else sd.dambreak_case(nf=12, with_dem=28, liq_dem_flags=True))
+    sim, orc = sd.Sim(c), oracle.Oracle(c)
+    sim.run(3)
+    orc.run(3)
+    g, o = sim.output_vars(), orc.output_vars()
+    assert (g["n"], g["nSph"]) == (o["n"], o["nSph"])
+    for k in ("angMom", "eKE", "eViscF", "edViscFdt", "eElast", "edElastdt", "eElastExact",
+              "eTotal", "maxV", "maxF", "aveDens", "varDens", "aveDensFromMass"):
+        assert g[k] == pytest.approx(o[k], rel=1e-9, abs=1e-12 * max(abs(o["eKE"]), 1.0)), k
+    for i in range(3):
+        assert g["linMom"][i] == pytest.approx(o["linMom"][i], rel=1e-9,
+                                               abs=1e-12 * abs(o["linMom"][2])), i
+    assert o["eViscF"] != 0.0 and o["eElast"] != 0.0
+    for k in ("eViscB", "eBForce", "eFF", "edFFdt", "maxFF", "rmsFF"):
+        assert g[k] == 0.0 and o[k] == 0.0, k
+    sim.close()
+    orc.close()
