@@ -54,6 +54,7 @@ extern "C" {
 #define SPHDEM_F_FIXED_DEM (1u << 7)         /* FIXED_DEM                        */
 #define SPHDEM_F_VISC_MONAGHAN (1u << 8)     /* VISC_MONAGHAN                    */
 #define SPHDEM_F_VISC_CLEARY (1u << 9)       /* VISC_CLEARY                      */
+#define SPHDEM_F_DEM_BLOCK (1u << 10)        /* DEM_BLOCK                        */
 /* At most one of VISC_MORRIS, VISC_CLEARY and VISC_MONAGHAN (calcViscForce,
  * sph.h:579-631, declares its `visc` once per variant, so the reference does
  * not compile with two); sphdem_create / sphdem_set_params refuse more. */

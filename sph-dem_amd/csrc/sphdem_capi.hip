@@ -989,6 +989,15 @@ int stepEnd(sphdem_ctx* ctx) {
     Side side(ctx);
     CHECK(passFpi(ctx));
     if (!(ctx->P.flags & (SPHDEM_F_FIXED_DEM | SPHDEM_F_NO_DEM_CONTACTS))) CHECK(passContacts(ctx));
+    // calcDEMWallContacts (without LIQ_DEM_CUSTOM_WALL_CONTACTS) scans the
+    // DEM's own neighbour list, which holds DEM particles only (DEM are binned
+    // in dem_cells alone, dataLL.cpp:979-990), for boundary particles: it never
+    // finds one and adds nothing, so there is no kernel for it
+    if ((ctx->P.flags & SPHDEM_F_DEM_BLOCK) && ctx->nd > 0) {
+      hipLaunchKernelGGL(k_dem_block, dim3(1), dim3(kDemBlockThreads), 0, ctx->stream, ctx->cur,
+                         ctx->ns, ctx->nd);
+      LAUNCHED();
+    }
     if (ctx->nd > 0) {
       hipLaunchKernelGGL(k_dem_kick, dim3(blocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur,
                          ctx->ns, ctx->nd, ctx->P, ctx->dg);

@@ -1014,6 +1014,47 @@ __global__ void k_end_drift(Store S, int ns, DevGlobals* g) {
 }
 
 // kick on dem with (dt/2, dt/2) (sphIncompress.cpp:239)
+// DEM_BLOCK (sphIncompress.cpp:231-238 with calcAvForce/setAvForce,
+// sphIncompress.h:107-114): every DEM particle's f becomes the mean f of the
+// DEM particles.  One workgroup: each thread sums a strided share of the DEM
+// range in store order, a fixed tree adds the shares, then every thread writes
+// the mean to its share (deterministic run to run).
+constexpr int kDemBlockThreads = 1024;
+__global__ void __launch_bounds__(kDemBlockThreads) k_dem_block(Store S, int ns, int nd) {
+  __shared__ double red[3][kDemBlockThreads];
+  double sx = 0.0, sy = 0.0, sz = 0.0, cnt = 0.0;
+  for (int a = ns + threadIdx.x; a < ns + nd; a += kDemBlockThreads)
+    if (S.iam[a] == T_DEM) {
+      const double4 f = S.frc[a];
+      sx += f.x;
+      sy += f.y;
+      sz += f.z;
+      cnt += 1.0;
+    }
+  __shared__ double cred[kDemBlockThreads];
+  red[0][threadIdx.x] = sx;
+  red[1][threadIdx.x] = sy;
+  red[2][threadIdx.x] = sz;
+  cred[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int s = kDemBlockThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      for (int c = 0; c < 3; c++) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+      cred[threadIdx.x] += cred[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  const double n = cred[0];
+  if (n == 0.0) return;
+  const double ax = red[0][0] / n, ay = red[1][0] / n, az = red[2][0] / n;
+  for (int a = ns + threadIdx.x; a < ns + nd; a += kDemBlockThreads)
+    if (S.iam[a] == T_DEM) {
+      S.frc[a].x = ax;
+      S.frc[a].y = ay;
+      S.frc[a].z = az;
+    }
+}
+
 __global__ void k_dem_kick(Store S, int ns, int nd, sphdem_params P, DevGlobals* g) {
   const int a = ns + blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= ns + nd || S.iam[a] != T_DEM) return;

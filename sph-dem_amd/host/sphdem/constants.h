@@ -47,6 +47,9 @@ inline sphdem_params paramsFromConstants() {
 #ifdef VISC_CLEARY
   f |= SPHDEM_F_VISC_CLEARY;
 #endif
+#ifdef DEM_BLOCK
+  f |= SPHDEM_F_DEM_BLOCK;
+#endif
   P.flags = f;
   P.precision = SPHDEM_FP64;
   for (int i = 0; i < 3; i++) {

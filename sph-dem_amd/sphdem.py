@@ -36,6 +36,7 @@ F_NO_DEM_CONTACTS = 1 << 6
 F_FIXED_DEM = 1 << 7
 F_VISC_MONAGHAN = 1 << 8
 F_VISC_CLEARY = 1 << 9
+F_DEM_BLOCK = 1 << 10
 
 FP64, MIXED = 0, 1
 # return codes of the C ABI (SPHDEM_E_*)
@@ -359,6 +360,7 @@ _FLAG_DEFINES = [
     (F_VISC_MORRIS, "VISC_MORRIS"), (F_VISC_ARTIFICIAL, "VISC_ARTIFICIAL"),
     (F_CUSTOM_WALL, "LIQ_DEM_CUSTOM_WALL_CONTACTS"), (F_NO_DEM_CONTACTS, "NO_DEM_CONTACTS"),
     (F_FIXED_DEM, "FIXED_DEM"), (F_VISC_MONAGHAN, "VISC_MONAGHAN"), (F_VISC_CLEARY, "VISC_CLEARY"),
+    (F_DEM_BLOCK, "DEM_BLOCK"),
 ]
 
 

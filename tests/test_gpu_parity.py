@@ -312,7 +312,28 @@ def _with_flags(c, add=0, damptime=None):
     return c
 
 
+def _without_custom_wall(c):
+    """LIQ_DEM_CUSTOM_WALL_CONTACTS off: the reference then runs
+    calcDEMWallContacts, whose scan of the DEM list finds no wall particle."""
+    c.params.flags &= ~sd.F_CUSTOM_WALL
+    return c
+
+
+def floor_case():
+    """sediment with DEM spheres overlapping the z = RMIN floor (centre at
+    0.8 R): the sediment3D floor contact (addCustomBoundaries,
+    sediment3D/customSim.cpp:67-81) acts on them from the first step."""
+    c = sd.sediment_case(nf=12, n_dem=40, seed=78)
+    dem = np.where(c.iam == sd.DEM)[0]
+    R = c.params.dem_radius
+    c.r[dem[:12], 2] = 0.8 * R
+    return c
+
+
 SWITCH_CASES = {
+    "dem_floor": floor_case,
+    "dem_block": lambda: _with_flags(contact_case(), sd.F_DEM_BLOCK),
+    "no_custom_wall": lambda: _without_custom_wall(floor_case()),
     "const_h": lambda: _with_flags(dam_case(), sd.F_CONST_H),
     "damptime": lambda: _with_flags(dam_case(), damptime=1),
     "fixed_dem": lambda: _with_flags(contact_case(), sd.F_FIXED_DEM),
@@ -324,7 +345,9 @@ SWITCH_CASES = {
 def test_step_switches(name):
     """CONST_H (h kept), DAMPTIME (velocities x 0.98 while time < DAMPTIME,
     sphIncompress.h:716-721), FIXED_DEM (DEM velocities restored after every
-    kick, no contacts) and NO_DEM_CONTACTS: three whole steps against the
+    kick, no contacts), NO_DEM_CONTACTS, DEM_BLOCK (every DEM force the mean,
+    sphIncompress.cpp:231-238), spheres on the custom floor, and the flag set
+    without LIQ_DEM_CUSTOM_WALL_CONTACTS: three whole steps against the
     oracle."""
     c = SWITCH_CASES[name]()
     print(f"switch case {name}", flush=True)

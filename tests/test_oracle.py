@@ -642,3 +642,45 @@ def test_parallel_oracle_equals_the_serial_sweep():
     oracle.set_threads(1)
     for k in out[1]:
         assert np.array_equal(out[1][k], out[4][k], equal_nan=True), k
+
+
+def test_dem_block_gives_every_sphere_the_mean_force():
+    """DEM_BLOCK (sphIncompress.cpp:231-238): after end() every DEM particle
+    carries the mean of the forces the DEM particles had before it, and the
+    kick used that mean (v = v0 + dt f)."""
+    c = _contact_sediment()
+    dem = c.iam == sd.DEM
+    P = c.params.copy()
+    P.flags |= sd.F_DEM_BLOCK
+    plain, block = oracle.Oracle(c), oracle.Oracle(c)
+    for o in (plain, block):
+        o.run(1)
+        if o is block:
+            o.set_params(P)  # the same state, then one step with the block
+        g = o.globals()
+        g.sphStep += 1
+        o.set_globals(g)
+        o.start()
+        o.middle()
+        o.end()
+    fb, fp = block.get("f")[dem], plain.get("f")[dem]
+    assert np.all(fb == fb[0])
+    assert np.allclose(fb[0], fp.mean(axis=0), rtol=1e-12, atol=1e-15)
+    dt = block.globals().dt
+    assert np.allclose(block.get("v")[dem], block.get("v0")[dem] + dt * fb, rtol=1e-12,
+                       atol=1e-15)
+
+
+def test_wall_contact_scan_of_the_dem_lists_changes_nothing():
+    """Without LIQ_DEM_CUSTOM_WALL_CONTACTS the reference adds
+    calcDEMWallContacts (sph.h:1634-1674), which looks for wall particles in
+    the DEM's own neighbour list; DEM lists hold DEM only, so the run is the
+    same bit for bit."""
+    c = _contact_sediment()
+    P = c.params.copy()
+    P.flags &= ~sd.F_CUSTOM_WALL
+    a, b = oracle.Oracle(c), oracle.Oracle(c, params=P)
+    a.run(3)
+    b.run(3)
+    for k in ("r", "v", "f", "fp", "fb", "dens"):
+        assert np.array_equal(a.get(k), b.get(k), equal_nan=True), k
