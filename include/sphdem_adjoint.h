@@ -154,6 +154,25 @@ int sphdem_adj_coupling_bwd(int32_t nquery, const int32_t* query, const int64_t*
                             const sphdem_adj_consts* c, const double* gpor, const double* gdrag,
                             double* work, const sphdem_adj_grads* g, void* stream);
 
+/* sphdem_adj_coupling_bwd without the work argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * Shepard-sum adjoints go to a scratch array the library holds in device
+ * memory, one per GPU for the life of the process (8 MiB: at most
+ * SPHDEM_ADJ_NOALLOC_MAX_PARTICLES doubles); n is the particle count, which
+ * must exceed every index in query and d.  The call clears scratch[0, n) with
+ * a kernel, then runs the same two kernels as sphdem_adj_coupling_bwd; it
+ * calls no hipMalloc, hipFree or allocating library routine.  Returns
+ * SPHDEM_E_CAPACITY when n exceeds the maximum.  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (other streams, other host
+ * threads) race and give wrong adjoints; calls ordered on one stream do not. */
+#define SPHDEM_ADJ_NOALLOC_MAX_PARTICLES (1 << 20)
+int sphdem_adj_coupling_bwd_noalloc(int32_t nquery, const int32_t* query, const int64_t* off,
+                                    const int32_t* d, const int32_t* img,
+                                    const sphdem_adj_fields* in, const sphdem_adj_consts* c,
+                                    const double* gpor, const double* gdrag, int64_t n,
+                                    const sphdem_adj_grads* g, void* stream);
+
 /* Fluid-particle interaction and drag of each DEM particle q = query[i]
  * over its fluid partners b[off[i] .. off[i+1]) (the DEM->SPH coupling set):
  * fdrag[q] = the calcFPI drag (zero when the Shepard sum is <= 0.5).

@@ -23,6 +23,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "../../include/sphdem_adjoint.h"
 #include "../../include/sphdem_c.h"
 
@@ -895,6 +897,41 @@ namespace {
 int adjLaunched() { return hipGetLastError() == hipSuccess ? 0 : SPHDEM_E_HIP; }
 }  // namespace
 
+// ---- device-resident scratch of sphdem_adj_coupling_bwd_noalloc
+// The Shepard-sum adjoints of the coupling backward (one double per particle),
+// for callers that cannot allocate.  Every GPU the library runs on holds its
+// own copy for the life of the process; host code reaches it only through the
+// per-device address hipGetSymbolAddress returns (the symbol itself is a
+// host-side stand-in, not a device address).
+static __device__ __align__(256) double scratch[SPHDEM_ADJ_NOALLOC_MAX_PARTICLES];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratchMu;
+double* g_scratchPtr[kScratchDevices] = {};
+
+// The current device's scratch address, looked up once per device (with that
+// device current) on the first call made on it.
+int scratchOfCurrentDevice(double** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kScratchDevices) return SPHDEM_E_DEVICE;
+  std::lock_guard<std::mutex> lock(g_scratchMu);
+  if (!g_scratchPtr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return SPHDEM_E_HIP;
+    g_scratchPtr[dev] = (double*)p;
+  }
+  *out = g_scratchPtr[dev];
+  return 0;
+}
+
+// zero fill (a kernel, not a memset: the call stays a plain kernel sequence)
+__global__ void k_zero_doubles(double* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+}  // namespace
+
 extern "C" {
 
 #define SPHDEM_ADJ_CSR_OK(in)                                                       \
@@ -1011,6 +1048,36 @@ int sphdem_adj_coupling_bwd(int32_t nquery, const int32_t* query, const int64_t*
   if (!SPHDEM_ADJ_CSR_OK(in) || !in->dens || !in->fdrag || !in->shep || !g->r || !g->dens ||
       !g->fdrag)
     return SPHDEM_E_ARG;
+  const dim3 grid(sphdem_adj::gridFor(nquery)), block(sphdem_adj::kTPB);
+  hipLaunchKernelGGL(sphdem_adj::k_cpl_bwd1, grid, block, 0, (hipStream_t)stream, nquery, query,
+                     off, d, img, *in, *c, gpor, gdrag, work, *g);
+  if (adjLaunched()) return SPHDEM_E_HIP;
+  hipLaunchKernelGGL(sphdem_adj::k_cpl_bwd2, grid, block, 0, (hipStream_t)stream, nquery, query,
+                     off, d, img, *in, *c, (const double*)work, *g);
+  return adjLaunched();
+}
+
+int sphdem_adj_coupling_bwd_noalloc(int32_t nquery, const int32_t* query, const int64_t* off,
+                                    const int32_t* d, const int32_t* img,
+                                    const sphdem_adj_fields* in, const sphdem_adj_consts* c,
+                                    const double* gpor, const double* gdrag, int64_t n,
+                                    const sphdem_adj_grads* g, void* stream) {
+  if (nquery < 0 || n < 0 || !in || !c || !gpor || !gdrag || !g) return SPHDEM_E_ARG;
+  if (n > SPHDEM_ADJ_NOALLOC_MAX_PARTICLES) return SPHDEM_E_CAPACITY;
+  if (nquery == 0) return 0;
+  const int32_t* b = d;
+  if (!SPHDEM_ADJ_CSR_OK(in) || !in->dens || !in->fdrag || !in->shep || !g->r || !g->dens ||
+      !g->fdrag)
+    return SPHDEM_E_ARG;
+  double* work = nullptr;
+  const int rc = scratchOfCurrentDevice(&work);
+  if (rc) return rc;
+  // the array keeps the last call's contents: clear the part this call uses
+  if (n > 0) {
+    hipLaunchKernelGGL(k_zero_doubles, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, work, n);
+    if (adjLaunched()) return SPHDEM_E_HIP;
+  }
   const dim3 grid(sphdem_adj::gridFor(nquery)), block(sphdem_adj::kTPB);
   hipLaunchKernelGGL(sphdem_adj::k_cpl_bwd1, grid, block, 0, (hipStream_t)stream, nquery, query,
                      off, d, img, *in, *c, gpor, gdrag, work, *g);

@@ -77,6 +77,8 @@ def _bind(lib):
         getattr(lib, nm + "_bwd").argtypes = [i32, vp, vp, vp, vp, pf, pc, vp, pg, vp]
     lib.sphdem_adj_coupling.argtypes = [i32, vp, vp, vp, vp, pf, pc, vp, vp, vp, vp]
     lib.sphdem_adj_coupling_bwd.argtypes = [i32, vp, vp, vp, vp, pf, pc, vp, vp, vp, pg, vp]
+    lib.sphdem_adj_coupling_bwd_noalloc.argtypes = [i32, vp, vp, vp, vp, pf, pc, vp, vp, i64,
+                                                    pg, vp]
     lib.sphdem_adj_contacts.argtypes = [i64, vp, vp, vp, pf, pc, vp, vp]
     lib.sphdem_adj_contacts_bwd.argtypes = [i64, vp, vp, vp, pf, pc, vp, pg, vp]
     lib.sphdem_adj_fpi.argtypes = [i32, vp, vp, vp, vp, pf, pc, vp, vp]
