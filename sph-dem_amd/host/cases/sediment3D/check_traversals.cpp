@@ -12,9 +12,9 @@
 #include <cstring>
 #include <vector>
 
-#include "customConstants.h"  // first: constants.h and the core read its names
-#include "../../sphdem/constants.h"
-#include "../../sphdem/dataLL.h"
+#include "customConstants.h"
+#include "dataLL.h"
+#include "sphIncompress.h"
 
 using namespace sphdem_host;
 
@@ -35,7 +35,7 @@ static double criterion(const Cparticle& a, const Cparticle& b) {
 }
 
 static bool ifSphGroup(Cparticle& p) { return p.iam == sph || p.iam == sphBoundary; }
-static bool ifDem(Cparticle& p) { return p.iam == dem; }
+static bool isDem(Cparticle& p) { return p.iam == dem; }
 static bool ifAny(Cparticle&) { return true; }
 
 static void countPair(Cparticle& a, Cparticle& b, CglobalVars&) {
@@ -52,6 +52,16 @@ static void countCoupling(Cparticle& p, std::vector<Cparticle*>& nb, CglobalVars
   for (Cparticle* q : nb) gCplWorst = std::max(gCplWorst, criterion(p, *q));
 }
 static void sumMass(Cparticle& p, CglobalVars&, double& acc) { acc += p.mass; }
+// a scatter into the partners (ghost copies included), carried back to the
+// origins by reverseSyncParticlesBetweenProcs
+static void zeroTmp(Cparticle& p, CglobalVars&) { p.tmp = 0.0; }
+static void scatterOne(Cparticle& p, std::vector<Cparticle*>& nb, CglobalVars&) {
+  (void)p;
+  for (Cparticle* q : nb) q->tmp += 1.0;
+}
+static double readTmp(Cparticle& p) { return p.tmp; }
+static void sumTmp(Cparticle& p, double x) { p.tmp += x; }
+static void kickZ(Cparticle& p, CglobalVars&) { p.f[2] += 1.0; }
 
 int main(int argc, char** argv) {
   if (argc != 2) {
@@ -86,14 +96,13 @@ int main(int argc, char** argv) {
     ps[i].dens = d[i];
     ps[i].iam = iam[i];
   }
-  const sphdem_params P = paramsFromConstants();
   CglobalVars globals;
-  globals.maxdt = P.maxdt;
-  CdataLL data(ps, globals, false, P);
+  globals.maxdt = MAXDT;
+  CdataLL data(ps, globals, false);
   data.reset();
   data.neighbours<countPair>();
   data.neighboursGroup<countGroup, ifSphGroup>();
-  data.neighboursGroupCoupling<countCoupling, ifDem>();
+  data.neighboursGroupCoupling<countCoupling, isDem>();
   double mass = 0.0;
   data.traverse<double, sumMass, ifAny>(mass);
   // the group sizes written by countGroup came back from the GPU
@@ -101,8 +110,46 @@ int main(int argc, char** argv) {
   long written = 0;
   for (const Cparticle& p : ps)
     if (p.iam == sph || p.iam == sphBoundary) written += (long)p.u;
+  // the scatter: every sph/sphBoundary particle is the partner of exactly
+  // the particles in its own list (the lists are symmetric)
+  data.traverse<zeroTmp, ifAny>();
+  data.neighboursGroup<scatterOne, ifSphGroup>();
+  data.reverseSyncParticlesBetweenProcs<double, readTmp, sumTmp>();
+  long scatterBad = 0;
+  for (const Cparticle& p : ps)
+    if ((p.iam == sph || p.iam == sphBoundary) && p.tmp != p.u) scatterBad++;
+  // lazy sync: a traversal that writes f uploads that one field
+  const long up0 = data.fieldUploads();
+  data.traverse<kickZ, ifAny>();
+  data.reset();
+  const long fieldUploads = data.fieldUploads() - up0;
+  // the output diagnostics through CsphIncompress
+  CsphIncompress sphInc(&data);
+  sphInc.calcOutputVars();
+  // insertion, deletion and tags
+  const int n0 = data.numParticles();
+  int demTag = -1;
+  for (const Cparticle& p : ps)
+    if (p.iam == dem) demTag = p.tag;
+  Cparticle* victim = data.getParticleFromTag(demTag);
+  const bool tagOk = victim && victim->tag == demTag && victim->iam == dem;
+  Cparticle copy = *victim;
+  data.markForDeletion(*victim);
+  data.deleteParticles();
+  const int nAfterDelete = data.numParticles();
+  copy.tag = n0 + 1;
+  copy.r[2] += 0.5 * PSEP;
+  data.insertNewParticle(copy);
+  data.reset();
+  int64_t onDevice = 0;
+  sphdem_num_particles(data.context(), &onDevice);
+  Cparticle* found = data.getParticleFromTag(n0 + 1);
+  const bool insertedOk = found && found->iam == dem && found->r[2] == copy.r[2];
   printf("pairs=%ld worst=%.9f groups=%ld groupnb=%ld written=%ld coupling=%ld cplworst=%.9f "
-         "mass=%.17g\n",
-         gPairs, gWorst, gGroups, gGroupNb, written, gCpl, gCplWorst, mass);
+         "mass=%.17g scatterbad=%ld fielduploads=%ld nsph=%d outn=%d tagok=%d ndel=%d "
+         "ndev=%lld insertok=%d\n",
+         gPairs, gWorst, gGroups, gGroupNb, written, gCpl, gCplWorst, mass, scatterBad,
+         fieldUploads, globals.nSph, globals.n, tagOk ? 1 : 0, n0 - nAfterDelete,
+         (long long)onDevice, insertedOk ? 1 : 0);
   return 0;
 }

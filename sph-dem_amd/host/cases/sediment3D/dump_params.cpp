@@ -3,8 +3,8 @@
 // binding's parameters of the same case).
 #include <cstdio>
 
-#include "customConstants.h"  // first: constants.h reads its names
-#include "../../sphdem/constants.h"
+#include "customConstants.h"
+#include "constants.h"
 
 int main() {
   const sphdem_params P = sphdem_host::paramsFromConstants();

@@ -1,29 +1,34 @@
-// run.cpp -- the sediment3D case driven like the reference's run.cpp:80-119:
-// CdataLL over the particle container, CsphIncompress, CcustomSim, then per
-// step sphStep++, beforeStart, start, beforeMiddle, middle, beforeEnd, end,
-// afterEnd.
+// run.cpp -- the sediment3D case driven as the reference's run.cpp:73-160
+// drives a case on one rank: CdataLL over the particle container,
+// CsphIncompress, CcustomSim and CcustomOutput, then per step sphStep++,
+// beforeStart, start, beforeMiddle, middle, beforeEnd, end, afterEnd, and at
+// the end an output step (calcOutputVars, calcOutput, the output, globals and
+// restart files, calcTraverseOrder).
 //
-// usage: run <initial condition file> <steps> <state file>
+// usage: run <initial condition file> <steps> <state file> [<output base name>]
 //   initial condition: magic "SPHDEMIC", int64 n, r v vhat (3n doubles),
 //   mass h dens (n doubles), iam (n int32)
 //   state (written): magic "SPHDEMST", int64 n, time, dt, then r v vhat f
 //   (3n), dens h press omega porosity (n), per particle in input order
+//   output base name: if given, the output step writes <base>0000001.pvtu
+//   (+ _0.vtu), <base>Globals0.dat and <base>Restart0000001.pvtu (+ _0.vtu)
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
-#include "customConstants.h"  // first: constants.h and the core read its names
-#include "../../sphdem/constants.h"
-#include "../../sphdem/sphIncompress.h"
+#include "customConstants.h"
+#include "customOutput.h"
 #include "customSim.h"
-
-using namespace sphdem_host;
+#include "io_data_vtk.h"
+#include "sphIncompress.h"
 
 static bool readAll(FILE* f, void* p, size_t bytes) { return fread(p, 1, bytes, f) == bytes; }
 
 int main(int argc, char** argv) {
-  if (argc != 4) {
-    fprintf(stderr, "usage: %s <initial condition> <steps> <state out>\n", argv[0]);
+  if (argc != 4 && argc != 5) {
+    fprintf(stderr, "usage: %s <initial condition> <steps> <state out> [<output base>]\n",
+            argv[0]);
     return 2;
   }
   FILE* f = fopen(argv[1], "rb");
@@ -45,6 +50,7 @@ int main(int argc, char** argv) {
     fprintf(stderr, "run: %s is truncated\n", argv[1]);
     return 2;
   }
+  CglobalVars globals;
   particleContainer ps(n);
   for (int64_t i = 0; i < n; i++) {
     for (int k = 0; k < 3; k++) {
@@ -57,14 +63,17 @@ int main(int argc, char** argv) {
     ps[i].dens = d[i];
     ps[i].iam = iam[i];
   }
-  const sphdem_params P = paramsFromConstants();
-  CglobalVars globals;
-  globals.maxdt = P.maxdt;
+  const sphdem_params P = sphdem_host::paramsFromConstants();
   for (int i = 0; i < 6; i++) globals.procDomain[i] = P.proc_domain[i];
 
-  CdataLL data(ps, globals, false, P);
-  CsphIncompress sph(&data);
-  CcustomSim customSim(&data, globals.time);
+  // run.cpp:77-87
+  CdataLL* data = new CdataLL(ps, globals, true);
+  CsphIncompress sph(data);
+  CcustomSim customSim(data, globals.time);
+  CcustomOutput customOutput(data);
+  globals.maxdt = MAXDT;
+
+  // run.cpp:102-119
   const int steps = atoi(argv[2]);
   for (int s = 0; s < steps; s++) {
     globals.sphStep++;
@@ -76,7 +85,23 @@ int main(int argc, char** argv) {
     sph.end();
     customSim.afterEnd(globals.time);
   }
-  data.pull();
+
+  // an output step (run.cpp:120-141)
+  int outstep = 1;
+  globals.outstep = outstep;
+  globals.endOutStep();
+  sph.calcOutputVars();
+  if (argc == 5) {
+    Cio_data_vtk io_data(argv[4], &globals);
+    customOutput.calcOutput(outstep, &customSim, &io_data);
+    io_data.writeOutput(outstep, *data->getParticles(), &globals);
+    io_data.writeGlobals(outstep, &globals);
+    io_data.writeRestart(outstep, *data->getParticles(), &globals);
+    io_data.writeDomain(outstep, &globals);
+  }
+  data->calcTraverseOrder();
+  globals.startOutStep();
+  data->getParticles();  // the host view, current with the GPU
 
   FILE* o = fopen(argv[3], "wb");
   if (!o) {
@@ -101,6 +126,8 @@ int main(int argc, char** argv) {
     fwrite(buf.data(), 8, n, o);
   }
   fclose(o);
-  printf("run: %d steps, time %.9g, dt %.9g\n", steps, globals.time, globals.dt);
+  printf("run: %d steps, time %.9g, dt %.9g, eKE %.9g, field uploads %ld\n", steps, globals.time,
+         globals.dt, globals.eKE, data->fieldUploads());
+  delete data;
   return 0;
 }

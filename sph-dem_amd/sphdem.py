@@ -364,14 +364,17 @@ _FLAG_DEFINES = [
 ]
 
 
-def custom_constants_header(P: Params, title: str) -> str:
+def custom_constants_header(P: Params, title: str, max_particles: int = 1) -> str:
     """A customConstants.h with the names the reference's core headers read,
-    holding exactly the doubles of P (17 significant digits round-trip)."""
+    holding exactly the doubles of P (17 significant digits round-trip).  It
+    defines SPHDEM_DERIVED_CONSTANTS and carries the constants the reference
+    derives in its core headers (WCON_WENDLAND, DEM_TIMESTEP, ...) as the
+    exact values of P, and MAX_NUM_PARTICLES_PER_CPU = max_particles."""
     def d(x):
         return repr(float(x))
 
     out = [f"// customConstants.h -- {title}.", "// Generated by sphdem.custom_constants_header().",
-           "#pragma once", ""]
+           "#pragma once", "", "#define SPHDEM_DERIVED_CONSTANTS", "#define LINEAR"]
     for bit, name in _FLAG_DEFINES:
         if P.flags & bit:
             out.append(f"#define {name}")
@@ -380,7 +383,7 @@ def custom_constants_header(P: Params, title: str) -> str:
     out.append("const double RMIN[3] = {%s, %s, %s};" % tuple(d(x) for x in P.rmin))
     out.append("const double RMAX[3] = {%s, %s, %s};" % tuple(d(x) for x in P.rmax))
     ints = [("REINIT_DENS_EVERY_N_STEPS", P.reinit_every), ("REINIT_DENS_AT_N_STEPS", P.reinit_at),
-            ("DAMPTIME", P.damptime)]
+            ("DAMPTIME", P.damptime), ("MAX_NUM_PARTICLES_PER_CPU", max_particles)]
     for name, v in ints:
         out.append(f"const int {name} = {int(v)};")
     dbl = [("PI", P.pi), ("DENS", P.dens), ("REFD", P.refd), ("GAMMA", P.gamma),
@@ -389,7 +392,7 @@ def custom_constants_header(P: Params, title: str) -> str:
            ("WCON_WENDLAND", P.wcon), ("MAXDT", P.maxdt), ("DEM_RADIUS", P.dem_radius),
            ("DEM_DENS", P.dem_dens), ("DEM_VOL", P.dem_vol), ("DEM_K", P.dem_k),
            ("DEM_GAMMA", P.dem_gamma), ("DEM_SEARCH_RADIUS", P.dem_search_radius),
-           ("COUPLING_RADIUS", P.coupling_radius), ("TIME_DROP_PARTICLE", P.time_drop),
+           ("LIQ_DEM_COUPLING_RADIUS", P.coupling_radius), ("TIME_DROP_PARTICLE", P.time_drop),
            ("DEM_TIMESTEP", P.dem_timestep), ("LIQ_DEM_TIMESTEP", P.liq_dem_timestep),
            ("POROSITY_INCR", P.porosity_incr), ("DRAG_A", P.drag_a), ("DRAG_B", P.drag_b)]
     for name, v in dbl:

@@ -29,7 +29,8 @@ def built(path):
 def test_committed_constants_match_the_generator():
     c = host_case()
     want = sd.custom_constants_header(
-        c.params, "SPH-DEM sedimentation, 12^3 fluid + 60 DEM (the host test case)")
+        c.params, "SPH-DEM sedimentation, 12^3 fluid + 60 DEM (the host test case)",
+        max_particles=2 * c.n)
     with open(os.path.join(CASE_DIR, "customConstants.h")) as f:
         assert f.read() == want
 
@@ -87,7 +88,9 @@ def test_host_pair_traversals_walk_the_library_sets(tmp_path):
     and traverse<T, F, If>(T&) of the C++ drop-in with host functors: every pair
     of the library's sets is visited once, periodic partners arrive as shifted
     ghost copies (all within the neighbour criterion), and a write made inside
-    a group functor reaches the GPU."""
+    a group functor reaches the GPU.  Then reverseSyncParticlesBetweenProcs of
+    a scatter, the field-by-field lazy upload, calcOutputVars,
+    getParticleFromTag, deletion and insertion (dataLL.h:185-257)."""
     c = host_case()
     ic = tmp_path / "ic.bin"
     sd.write_ic(c, str(ic))
@@ -107,4 +110,46 @@ def test_host_pair_traversals_walk_the_library_sets(tmp_path):
     assert int(got["coupling"]) == sim.pair_count(sd.PAIRS_DEM2SPH) > 0
     assert float(got["cplworst"]) <= 1.0 + 1e-12
     assert float(got["mass"]) == pytest.approx(float(c.mass.sum()), rel=1e-12)
+    # reverseSyncParticlesBetweenProcs carried the ghosts' scatter back
+    assert int(got["scatterbad"]) == 0
+    # a traversal that wrote f uploaded f alone
+    assert int(got["fielduploads"]) == 1
+    # calcOutputVars through CsphIncompress counts the container
+    assert int(got["nsph"]) == int((c.iam == sd.SPH).sum())
+    assert int(got["outn"]) == c.n
+    # getParticleFromTag, markForDeletion + deleteParticles, insertNewParticle
+    assert int(got["tagok"]) == 1 and int(got["ndel"]) == 1
+    assert int(got["ndev"]) == c.n and int(got["insertok"]) == 1
     sim.close()
+
+
+REF_CASE = "/root/reference/sediment3D"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_CASE),
+                    reason="the reference's case directory is present only where it was surveyed")
+def test_reference_case_code_builds_unchanged(tmp_path):
+    """The reference's sediment3D plug-in (customSim.cpp, customOutput.cpp with
+    their headers, customConstants.h and parameters.h, compiled from where they
+    lie) and a driver in the shape of run.cpp:15-160 build and link against the
+    drop-in (sph-dem_amd/host/sphdem) and libsphdem.so with include paths only:
+    CdataLL(particleContainer&, CglobalVars&, bool), CsphIncompress with
+    calcOutputVars, CcustomSimBase / CcustomOutputBase, Cio_data_vtk, Nsph and
+    the traversal templates the plug-in instantiates."""
+    host = os.path.join(os.path.dirname(sd.LIB_PATH), "host", "sphdem")
+    driver = os.path.join(os.path.dirname(os.path.abspath(__file__)), "dropin", "run_shaped.cpp")
+    objs = []
+    for src in (os.path.join(REF_CASE, "customSim.cpp"), os.path.join(REF_CASE, "customOutput.cpp"),
+                driver):
+        obj = str(tmp_path / (os.path.basename(src) + ".o"))
+        r = subprocess.run(["g++", "-std=c++17", "-O1", "-c", f"-I{REF_CASE}", f"-I{host}", src,
+                            "-o", obj], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr[-4000:]
+        objs.append(obj)
+    exe = str(tmp_path / "run")
+    libdir = os.path.dirname(sd.LIB_PATH)
+    r = subprocess.run(["g++", *objs, "-o", exe, f"-L{libdir}", "-lsphdem", "-lz",
+                        f"-Wl,-rpath,{libdir}"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-4000:]
+    assert os.path.exists(exe)
+
