@@ -73,7 +73,8 @@ B_ALG_STEP_DEM = {"fp64": 650, "mixed": 650}
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=100)
+    p.add_argument("--steps", type=int, default=None,
+                   help="timed steps (default 100; the adjoint's rollout 32)")
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--workload", choices=("sediment", "large", "ensemble", "adjoint"),
                    default="large",
@@ -461,7 +462,7 @@ def adjoint(a):
     import torch
     import sphdem_adjoint as sa
 
-    steps = a.steps if a.steps != 200 else 32
+    steps = a.steps
     nf = a.nf or 64
     ndem = a.ndem or 8192
     case = sd.adjoint_case(nf=nf, nz=16, n_dem=ndem)
@@ -483,8 +484,30 @@ def adjoint(a):
     sa.KERNEL_TIMES = None
     search_s = pairs.seconds
     peak_mem = torch.cuda.max_memory_allocated() / 2 ** 30
-    # the library's own forward over the same rollout, and finite differences of it
-    def prod_loss(P=None, v=None):
+    # the library's own forward over the same rollout, and central finite
+    # differences of it.  Each tracked run steps through the phase calls with
+    # the sediment3D hooks (as sphdem_run_steps does, bit for bit) and keeps,
+    # after every reset, the sorted pair keys of each search kind; a perturbed
+    # run then reports how many resets saw a pair set other than the base
+    # run's.  Where none did, the loss is a smooth function of the perturbation
+    # and the FD must agree with the adjoint to 1e-4.
+    kinds = (sd.PAIRS_SPH, sd.PAIRS_DEM, sd.PAIRS_SPH2DEM, sd.PAIRS_DEM2SPH)
+
+    class _Exporter:
+        _pairs = sa.LibraryPairs._pairs
+
+        def __init__(self, sim):
+            self.sim, self.cap = sim, {}
+
+    def snapshot(exp):
+        out = []
+        for k in kinds:
+            pa, pb, pi = exp._pairs(k, "cuda")
+            key = (pa.long() * case.n + pb.long()) * 27 + pi.long()
+            out.append(torch.sort(key).values)
+        return out
+
+    def prod_loss(P=None, v=None, track=False):
         import dataclasses
         c = dataclasses.replace(case, v=case.v.copy(), vhat=case.vhat.copy(),
                                 params=(P or case.params).copy())
@@ -494,14 +517,55 @@ def adjoint(a):
         sim = sd.Sim(c)
         sim.sync()
         t = time.perf_counter()
-        sim.run(steps)
+        keys = None
+        if not track:
+            sim.run(steps)
+        else:
+            keys = []
+            exp = _Exporter(sim)
+            G = np.array([0.0, 0.0, sa.GRAVITY])
+            td = c.params.time_drop
+            for _ in range(steps):
+                gl = sim.globals()
+                gl.sphStep += 1
+                sim.set_globals(gl)
+                sim.start()
+                sim.hook_body_force(1 << sd.SPH, G)
+                sim.middle()
+                keys.append(snapshot(exp))
+                sim.hook_sum_drag()
+                sim.hook_body_force(1 << sd.DEM, G)
+                sim.hook_dem_floor()
+                sim.hook_freeze(1 << sd.DEM, td)
+                sim.end()
+                keys.append(snapshot(exp))
+                sim.hook_freeze(1 << sd.DEM, td)
         sim.sync()
         dt = time.perf_counter() - t
         z = sim.get("r")[case.iam == sd.DEM, 2]
         sim.close()
-        return float(np.mean((z - target_np) ** 2)), dt
-    base, fwd_s = prod_loss()
-    fd = {}
+        return float(np.mean((z - target_np) ** 2)), dt, keys
+
+    base, fwd_s, _ = prod_loss()
+    base_t, _, base_keys = prod_loss(track=True)
+
+    def changed(keys):
+        """Resets (of 2 per step) where some pair set differs from the base run's."""
+        return sum(any(not torch.equal(x, y) for x, y in zip(kb, kn))
+                   for kb, kn in zip(base_keys, keys))
+
+    def central(run_p, run_m, denom, adj):
+        lp, _, kp = run_p()
+        lm, _, km = run_m()
+        d = (lp - lm) / denom
+        cp, cm = changed(kp), changed(km)
+        err = abs(adj - d) / abs(d) if d else None
+        return {"fd": d, "rel_err": err, "resets_with_other_pairs": [cp, cm],
+                "pair_sets_identical": cp == 0 and cm == 0,
+                "within_1e-4": bool(err is not None and err <= 1e-4)}
+
+    fd = {"tracked_base_loss_equals_library_forward": base_t == base,
+          "resets_per_run": 2 * steps}
     for k in ("drag_a", "dem_k", "viscosity"):
         b0 = getattr(case.params, k)
         row = {"adjoint": g[k]}
@@ -509,24 +573,26 @@ def adjoint(a):
             Pp, Pm = case.params.copy(), case.params.copy()
             setattr(Pp, k, b0 * (1 + rel))
             setattr(Pm, k, b0 * (1 - rel))
-            d = (prod_loss(Pp)[0] - prod_loss(Pm)[0]) / (2 * rel * abs(b0))
-            row[f"fd_rel_step_{rel:g}"] = d
-            row[f"rel_err_{rel:g}"] = abs(g[k] - d) / abs(d) if d else None
+            row[f"rel_step_{rel:g}"] = central(lambda: prod_loss(Pp, track=True),
+                                               lambda: prod_loss(Pm, track=True),
+                                               2 * rel * abs(b0), g[k])
         fd[k] = row
     dirn = sd.splitmix64(7, case.n * 3).reshape(case.n, 3) - 0.5
     dirn[case.iam == sd.SPH_BOUNDARY] = 0.0
-    adj_dir = float((g["v_init"].cpu().numpy() * dirn).sum())
-    eps = 1e-6
-    d = (prod_loss(v=case.v + eps * dirn)[0] - prod_loss(v=case.v - eps * dirn)[0]) / (2 * eps)
-    fd["v_init_direction"] = {"adjoint": adj_dir, "fd_step_1e-06": d,
-                              "rel_err_1e-06": abs(adj_dir - d) / abs(d) if d else None,
-                              "what": "all fluid and DEM velocities"}
-    dd = np.where((case.iam == sd.DEM)[:, None], dirn, 0.0)
-    adj_dd = float((g["v_init"].cpu().numpy() * dd).sum())
-    d = (prod_loss(v=case.v + eps * dd)[0] - prod_loss(v=case.v - eps * dd)[0]) / (2 * eps)
-    fd["v_init_dem_direction"] = {"adjoint": adj_dd, "fd_step_1e-06": d,
-                                  "rel_err_1e-06": abs(adj_dd - d) / abs(d) if d else None,
-                                  "what": "DEM velocities only"}
+    vin = g["v_init"].cpu().numpy()
+    for name, sel, what in (("v_init_direction", case.iam != sd.SPH_BOUNDARY,
+                             "all fluid and DEM velocities"),
+                            ("v_init_fluid_direction", case.iam == sd.SPH,
+                             "fluid velocities only"),
+                            ("v_init_dem_direction", case.iam == sd.DEM, "DEM velocities only")):
+        dd = np.where(sel[:, None], dirn, 0.0)
+        adj = float((vin * dd).sum())
+        row = {"adjoint": adj, "what": what}
+        for eps in (1e-6, 1e-5, 1e-4):
+            row[f"step_{eps:g}"] = central(lambda: prod_loss(v=case.v + eps * dd, track=True),
+                                           lambda: prod_loss(v=case.v - eps * dd, track=True),
+                                           2 * eps, adj)
+        fd[name] = row
     kernels = {k: {"ms_per_launch": ms / n, "launches": n, "items_per_launch": it / n}
                for k, (ms, n, it) in kt.items()}
     roof = None
@@ -609,6 +675,8 @@ def main():
     if a.copy_calibration:
         copy_calibration()
         return
+    if a.steps is None:
+        a.steps = 32 if a.workload == "adjoint" else 100
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(the path shards by space, see the design notes)", file=sys.stderr)

@@ -132,8 +132,11 @@ class PairSet:
     """One neighbour search: pairs (a, b) with b's periodic image shift, plus
     the CSR form grouped by a (for the per-query kernels)."""
 
-    def __init__(self, abi, n: int, shift_len: torch.Tensor):
-        """abi: an (m, 3) int tensor of (a, b, image) rows, or the three columns."""
+    def __init__(self, abi, n: int, shift_len: torch.Tensor, symmetric: bool = False):
+        """abi: an (m, 3) int tensor of (a, b, image) rows, or the three columns.
+        symmetric: the set is known to hold (b, a, 26 - image) for every
+        (a, b, image) -- the library's SPH search, whose criterion uses
+        max(h_a, h_b) -- so require_symmetric() need not check it."""
         if isinstance(abi, torch.Tensor):
             abi = abi.to(torch.int32)
             abi = (abi[:, 0], abi[:, 1], abi[:, 2])
@@ -149,6 +152,24 @@ class PairSet:
         unit = torch.stack([im % 3 - 1, (im // 3) % 3 - 1, im // 9 - 1], dim=1)
         self.shift = unit.to(shift_len.dtype) * shift_len
         self._csr = None
+        self._symmetric = symmetric or self.count == 0
+
+    def require_symmetric(self):
+        """Raises ValueError unless every (a, b, image) has its (b, a, 26 - image):
+        the backward kernels of force, omega and dddt gather the partner terms
+        from the query's own list and are correct only then.  Checked once per
+        set (two sorts of the pair keys)."""
+        if self._symmetric:
+            return
+        m = 27 * self.n
+        a, b, im = self.a.long(), self.b.long(), self.img.long()
+        fwd = torch.sort(a * m + b * 27 + im).values
+        rev = torch.sort(b * m + a * 27 + (26 - im)).values
+        if not torch.equal(fwd, rev):
+            raise ValueError("the pair set is not symmetric: some (a, b, image) has no "
+                             "(b, a, 26 - image); the gathered backward of force, omega and "
+                             "dddt needs both")
+        self._symmetric = True
 
     def csr(self, queries: torch.Tensor | None = None):
         """(query, off, b, img): one segment of pairs per query particle.
@@ -188,6 +209,10 @@ class LibraryPairs:
     """Pair sets of the library's own searches: a helper context holding the
     case is given positions and h, reset (CdataLL::reset) and asked for the
     pairs of each search kind (sphdem_get_pairs)."""
+
+    # the library's SPH search is symmetric: its criterion uses max(h_a, h_b)
+    # and one set of positions (its pair sets equal the oracle's brute force)
+    symmetric_sph = True
 
     def __init__(self, case: sd.Case, device: int = 0):
         self.sim = sd.Sim(case, device=device)
@@ -496,11 +521,15 @@ class HipPasses:
     def __init__(self):
         _adj_lib()
 
+    # force, omega and dddt: their backward kernels gather over the query's
+    # own list, so the set must be symmetric (checked before any launch)
     def omega(self, C, ps, r, h, dens):
+        ps.require_symmetric()
         _dev(r, h, dens)
         return _OmegaFn.apply(r, h, dens, ps, C)
 
     def dddt(self, C, ps, r, vhat, h, omega):
+        ps.require_symmetric()
         _dev(r, vhat, h, omega)
         return _DddtFn.apply(r, vhat, h, omega, ps, C)
 
@@ -509,6 +538,7 @@ class HipPasses:
         return _CouplingFn.apply(r, dens, fdrag, ps, C)
 
     def force(self, C, ps, r, vhat, h, dens, omega, pdr2, visc):
+        ps.require_symmetric()
         _dev(r, vhat, h, dens, omega, pdr2)
         return _ForceFn.apply(r, vhat, h, dens, omega, pdr2, visc, ps, C)
 
@@ -654,7 +684,8 @@ class StepModel:
                 cols = (g[:, 0], g[:, 1], g[:, 2]) if isinstance(g, torch.Tensor) else g
                 a = self.inv.index_select(0, cols[0].long())
                 b = self.inv.index_select(0, cols[1].long())
-                out[k] = PairSet((a, b, cols[2]), self.n, self.shift_len)
+                sym = k == sd.PAIRS_SPH and getattr(self.pairs_fn, "symmetric_sph", False)
+                out[k] = PairSet((a, b, cols[2]), self.n, self.shift_len, symmetric=sym)
             self.cache[key] = out
         return self.cache[key]
 

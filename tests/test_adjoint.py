@@ -315,3 +315,21 @@ def test_hip_adjoint_matches_finite_differences_of_the_library():
         assert g[k] == pytest.approx(g_t[k], rel=1e-8, abs=1e-300), k
     assert normwise(g["v_init"].cpu(), g_t["v_init"].cpu()) < 1e-9
     pairs.close()
+
+
+def test_gathered_backward_refuses_an_asymmetric_pair_set():
+    """The backward kernels of force, omega and dddt gather the partner terms
+    from the query's own list: a set missing a mirror pair (b, a, 26 - image)
+    is refused before any launch, and a symmetric one passes the check."""
+    L = torch.tensor([1.0, 1.0, 0.0], dtype=torch.float64)
+    sym = sa.PairSet(torch.tensor([[0, 1, 13], [1, 0, 13], [2, 3, 12], [3, 2, 14]]), 4, L)
+    sym.require_symmetric()
+    lop = sa.PairSet(torch.tensor([[0, 1, 13], [1, 0, 13], [2, 3, 12], [3, 2, 12]]), 4, L)
+    with pytest.raises(ValueError, match="not symmetric"):
+        lop.require_symmetric()
+    r = torch.zeros((4, 3), dtype=torch.float64)
+    h = torch.ones(4, dtype=torch.float64)
+    for call in (lambda P: P.omega(None, lop, r, h, h), lambda P: P.dddt(None, lop, r, r, h, h),
+                 lambda P: P.force(None, lop, r, r, h, h, h, h, None)):
+        with pytest.raises(ValueError, match="not symmetric"):
+            call(sa.HipPasses())
