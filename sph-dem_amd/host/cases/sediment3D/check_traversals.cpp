@@ -110,14 +110,18 @@ int main(int argc, char** argv) {
   long written = 0;
   for (const Cparticle& p : ps)
     if (p.iam == sph || p.iam == sphBoundary) written += (long)p.u;
-  // the scatter: every sph/sphBoundary particle is the partner of exactly
-  // the particles in its own list (the lists are symmetric)
+  // the scatter: each particle ends with the number of lists it appears in,
+  // ghost appearances included once reverseSyncParticlesBetweenProcs has
+  // carried them to their origins (printed as sum and index-weighted sum, for
+  // the caller to compare with the library's pair set)
   data.traverse<zeroTmp, ifAny>();
   data.neighboursGroup<scatterOne, ifSphGroup>();
   data.reverseSyncParticlesBetweenProcs<double, readTmp, sumTmp>();
-  long scatterBad = 0;
-  for (const Cparticle& p : ps)
-    if ((p.iam == sph || p.iam == sphBoundary) && p.tmp != p.u) scatterBad++;
+  long scatterSum = 0, scatterW = 0;
+  for (size_t i = 0; i < ps.size(); i++) {
+    scatterSum += (long)ps[i].tmp;
+    scatterW += (long)(i + 1) * (long)ps[i].tmp;
+  }
   // lazy sync: a traversal that writes f uploads that one field
   const long up0 = data.fieldUploads();
   data.traverse<kickZ, ifAny>();
@@ -146,9 +150,9 @@ int main(int argc, char** argv) {
   Cparticle* found = data.getParticleFromTag(n0 + 1);
   const bool insertedOk = found && found->iam == dem && found->r[2] == copy.r[2];
   printf("pairs=%ld worst=%.9f groups=%ld groupnb=%ld written=%ld coupling=%ld cplworst=%.9f "
-         "mass=%.17g scatterbad=%ld fielduploads=%ld nsph=%d outn=%d tagok=%d ndel=%d "
+         "mass=%.17g scattersum=%ld scatterw=%ld fielduploads=%ld nsph=%d outn=%d tagok=%d ndel=%d "
          "ndev=%lld insertok=%d\n",
-         gPairs, gWorst, gGroups, gGroupNb, written, gCpl, gCplWorst, mass, scatterBad,
+         gPairs, gWorst, gGroups, gGroupNb, written, gCpl, gCplWorst, mass, scatterSum, scatterW,
          fieldUploads, globals.nSph, globals.n, tagOk ? 1 : 0, n0 - nAfterDelete,
          (long long)onDevice, insertedOk ? 1 : 0);
   return 0;

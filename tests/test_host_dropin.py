@@ -110,8 +110,13 @@ def test_host_pair_traversals_walk_the_library_sets(tmp_path):
     assert int(got["coupling"]) == sim.pair_count(sd.PAIRS_DEM2SPH) > 0
     assert float(got["cplworst"]) <= 1.0 + 1e-12
     assert float(got["mass"]) == pytest.approx(float(c.mass.sum()), rel=1e-12)
-    # reverseSyncParticlesBetweenProcs carried the ghosts' scatter back
-    assert int(got["scatterbad"]) == 0
+    # reverseSyncParticlesBetweenProcs carried the ghosts' scatter back: each
+    # particle counts its appearances in the lists (not its own list length:
+    # lattice pairs at exactly 2h through a periodic image are accepted from
+    # one side only when the spacing is not dyadic, as in the reference)
+    app = np.bincount(sim.pairs(sd.PAIRS_SPH)[:, 1], minlength=c.n).astype(np.int64)
+    assert int(got["scattersum"]) == int(app.sum()) == n_sph
+    assert int(got["scatterw"]) == int((app * np.arange(1, c.n + 1)).sum())
     # a traversal that wrote f uploaded f alone
     assert int(got["fielduploads"]) == 1
     # calcOutputVars through CsphIncompress counts the container
