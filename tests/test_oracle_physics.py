@@ -250,3 +250,50 @@ def test_fluid_particle_interaction_adds_the_di_felice_drag():
     got = o.get("fdrag")[dem][0]
     assert got[2] == pytest.approx(F / m_p, rel=1e-10)
     assert abs(got[0]) < 1e-12 * abs(got[2]) and abs(got[1]) < 1e-12 * abs(got[2])
+
+
+def test_output_vars_of_a_rigidly_moving_compressed_lattice():
+    """calcOutputVars (sphIncompress.h:233-319) on a lattice moving rigidly
+    with v = U (vhat = 1.5 U), every sph density REFD (1 + d), and one DEM
+    sphere moving faster: the momenta are those of the rigid motion of the
+    fluid mass alone, the kinetic energy is the fluid's in vhat (calcOutput
+    sums 0.5 m vhat^2), maxV is |U| (the DEM sphere is not an sph/sphBoundary
+    particle), the density mean and variance divide by every particle of the
+    container, and the exact elastic energy is m times the integral of p /
+    rho^2 from REFD to rho (by quadrature here).  After a density pass the
+    density-from-mass statistic is zero."""
+    from scipy.integrate import quad
+
+    c = open_lattice(sd.F_VISC_MORRIS, dem_at=(0.5 + 0.013, 0.5 - 0.021, 0.5 + 0.007))
+    P = c.params
+    n = c.n
+    sph = c.iam == sd.SPH
+    U = np.array([0.3, -0.2, 0.1])
+    v = np.tile(U, (n, 1))
+    v[~sph] = 10.0 * U
+    d = 0.01
+    dens = np.where(sph, P.refd * (1.0 + d), c.dens)
+    o = oracle.Oracle(c)
+    o.set("v", v)
+    o.set("vhat", 1.5 * v)
+    o.set("dens", dens)
+    out = o.output_vars()
+    m, r = c.mass[sph], c.r[sph]
+    M = m.sum()
+    assert np.allclose(out["linMom"], M * U, rtol=1e-12)
+    assert out["angMom"] == pytest.approx(float((m * (r[:, 0] * U[1] - r[:, 1] * U[0])).sum()),
+                                          rel=1e-11)
+    assert out["eKE"] == pytest.approx(0.5 * M * 2.25 * float(U @ U), rel=1e-12)
+    assert out["maxV"] == pytest.approx(float(np.linalg.norm(U)), rel=1e-14)
+    assert out["n"] == n and out["nSph"] == n - 1
+    rho = P.refd * (1.0 + d)
+    ave = (n - 1) * rho / n
+    assert out["aveDens"] == pytest.approx(ave, rel=1e-12)
+    assert out["varDens"] == pytest.approx((n - 1) * (rho - ave) ** 2 / n, rel=1e-9)
+    e_unit, _ = quad(lambda x: P.prb * ((x / P.refd) ** 7 - 1.0) / x ** 2, P.refd, rho,
+                     epsabs=0.0, epsrel=1e-13)
+    assert out["eElastExact"] == pytest.approx(M * e_unit, rel=1e-9)
+    assert out["eTotal"] == pytest.approx(out["eKE"] + out["eElast"] + out["eViscF"]
+                                          + out["eViscB"] + out["eBForce"] + out["eFF"], rel=1e-14)
+    o.pass_("density")
+    assert o.output_vars()["aveDensFromMass"] == 0.0
