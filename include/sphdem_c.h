@@ -287,16 +287,17 @@ int sphdem_checkpoint_save(sphdem_ctx* ctx, void* host, int64_t capacity);
 int sphdem_checkpoint_restore(sphdem_ctx* ctx, const void* host, int64_t bytes);
 
 /* ---- tile shape ----------------------------------------------------------- */
-/* No counterpart in the reference (a CPU code has no launch shape).  The list
- * build packs whole cells of one (x,y) column into segments of at most
- * seg_particles (8..64; one 64-lane wavefront per segment) and the SPH pair
- * walkers run walker_block (64, 128 or 256) threads per workgroup.  Results do
- * not depend on the shape (pair sets and list order are the reference's for
- * any segment size); the defaults are 64 and 256. */
+/* No counterpart in the reference (a CPU code has no launch shape).  With
+ * seg_particles = 0 the list build runs one lane per target particle (the
+ * per-target search); with 8..64 it packs whole cells of one (x,y) column into
+ * segments of at most seg_particles, one 64-lane wavefront per segment.  The
+ * SPH pair walkers run walker_block (64, 128 or 256) threads per workgroup.
+ * Results do not depend on the shape (pair sets and list order are the
+ * reference's for every shape); the defaults are 0 and 256. */
 int sphdem_set_tile_shape(sphdem_ctx* ctx, int seg_particles, int walker_block);
 int sphdem_get_tile_shape(const sphdem_ctx* ctx, int* seg_particles, int* walker_block);
 /* Autotuner: times `steps` steps (after one untimed step) of each shape of
- * {64, 48, 32} x {256, 128, 64} from the context's current state, restored
+ * {0, 64, 48, 32} x {256, 128, 64} from the context's current state, restored
  * from a full-state checkpoint before each trial and after the last, keeps the
  * fastest and returns it (and its ms per step if ms_per_step is not NULL).
  * The simulation state is left as it was. */

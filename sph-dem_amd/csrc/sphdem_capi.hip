@@ -48,12 +48,19 @@ struct sphdem_ctx {
   int *segCnt = nullptr, *segOff = nullptr;
   Segment* segs = nullptr;
   int segColCap = 0, segCap = 0;
+  bool segsValid = false;  // the segments partition the current store
   DevGlobals* hostG = nullptr;  // pinned host copy of dg for the reset's readback
-  // tile shape: particles per list-build segment (<= 64, one wavefront) and
-  // threads per workgroup of the SPH pair walkers (sphdem_set_tile_shape)
-  int tileSeg = 64, walkTPB = 256;
+  // tile shape: particles per list-build segment (<= 64, one wavefront; 0 for
+  // the per-target search) and threads per workgroup of the SPH pair walkers
+  // (sphdem_set_tile_shape)
+  int tileSeg = 0, walkTPB = 256;
   // cached SPH neighbour lists (valid from their first use until the next reset)
   int *nbl = nullptr, *nbcnt = nullptr;
+  // the per-target search's quantised records (one per entry, plus slack for
+  // a batch read past a range's end) and its two generation flag slots
+  uint2* lrec = nullptr;
+  int* lflag = nullptr;
+  int lgen = 0;
   bool listsValid = false;
   double2* hinv = nullptr;  // {1/h, 1/h^5} per SPH-group particle
   long long* okey = nullptr;  // order key of each entry, in scatter order
@@ -223,6 +230,8 @@ int allocEntries(sphdem_ctx* ctx, int64_t ghostCap) {
   HIPC(hipMalloc(&tmp, sizeof(int) * ent));
   HIPC(hipMalloc(&perm, sizeof(int) * ent));
   HIPC(hipMalloc(&hinv, sizeof(double2) * ent));
+  uint2* lrec = nullptr;
+  HIPC(hipMalloc(&lrec, sizeof(uint2) * (ent + 64)));
   if (ctx->storeMem[0]) {
     const size_t n = (size_t)ctx->n;
     for (int k = 0; k < kNumVec; k++)
@@ -237,7 +246,7 @@ int allocEntries(sphdem_ctx* ctx, int64_t ghostCap) {
   }
   void* old[] = {ctx->storeMem[0], ctx->storeMem[1], ctx->key, ctx->slot, ctx->tmp, ctx->perm,
                  ctx->hinv, ctx->okey, ctx->gsrc, ctx->gord, ctx->gcell, ctx->gorig,
-                 ctx->gcode};
+                 ctx->gcode, ctx->lrec};
   for (void* q : old)
     if (q) HIPC(hipFree(q));
   ctx->cur = cur;
@@ -250,6 +259,7 @@ int allocEntries(sphdem_ctx* ctx, int64_t ghostCap) {
   ctx->perm = perm;
   ctx->hinv = hinv;
   ctx->okey = okey;
+  ctx->lrec = lrec;
   ctx->ghostCap = ghostCap;
   HIPC(hipMalloc(&ctx->gsrc, sizeof(int) * ghostCap));
   HIPC(hipMalloc(&ctx->gord, sizeof(int) * ghostCap));
@@ -331,6 +341,8 @@ int scanCounts(sphdem_ctx* ctx, int m) { return scanInts(ctx, ctx->counts, ctx->
 // Segments of the SPH grid after a rebuild: per-column counts, a scan, then
 // the segment records in column order (so segment order follows space).
 int buildSegments(sphdem_ctx* ctx) {
+  ctx->segsValid = ctx->tileSeg != 0;
+  if (ctx->tileSeg == 0) return 0;  // the per-target search needs no segments
   const Grid& g = ctx->gs;
   const int ncol = g.dim[0] * g.dim[1];
   if (ncol + 1 > ctx->segColCap) {
@@ -379,8 +391,34 @@ int refreshGhosts(sphdem_ctx* ctx, unsigned need) {
   ctx->gstale = 0;
   return 0;
 }
+// The quantisation unit of the per-target search: a power of two u with
+// edge / u in [1024, 2048), origin the grid's lower corner.
+ListQuant listQuant(const Grid& g) {
+  int e = 0;
+  (void)frexp(g.edge, &e);  // edge = m 2^e, m in [0.5, 1)
+  ListQuant q;
+  for (int d = 0; d < 3; d++) q.o[d] = g.cmin[d];
+  q.uinv = ldexp(1.0, 11 - e);
+  return q;
+}
 int launchLists(sphdem_ctx* ctx) {
   CHECK(refreshGhosts(ctx, GS_POSH));
+  if (ctx->tileSeg == 0) {
+    if (ctx->ns == 0) return 0;
+    const ListQuant lq = listQuant(ctx->gs);
+    const int gen = ctx->lgen++ & 1;
+    hipLaunchKernelGGL(k_lrec, dim3(blocks(ctx->ns + ctx->ng)), dim3(TPB), 0, ctx->stream,
+                       ctx->cur.posh, ctx->ns, ctx->n, ctx->ng, lq, ctx->lrec, ctx->lflag, gen);
+    LAUNCHED();
+    const int per = (ctx->ns + 7) / 8;
+    hipLaunchKernelGGL(k_lists_t<32>, dim3(8 * ((per + TPB - 1) / TPB)), dim3(TPB), 0,
+                       ctx->stream, ctx->cur.posh, ctx->lrec, ctx->cur.cell, ctx->gs, ctx->gbase,
+                       ctx->start, ctx->ns, lq.uinv, ctx->lflag, gen, ctx->nbl, ctx->nbcnt,
+                       ctx->dg);
+    LAUNCHED();
+    return 0;
+  }
+  if (!ctx->segsValid) CHECK(buildSegments(ctx));
   hipLaunchKernelGGL(k_lists, dim3(tileBlocks(ctx)), dim3(TPB), 0, ctx->stream, ctx->cur.posh,
                      ctx->cur.cell, ctx->gs, ctx->gbase, ctx->start, ctx->segs,
                      ctx->segOff + ctx->gs.dim[0] * ctx->gs.dim[1], ctx->ns, ctx->nbl,
@@ -550,6 +588,7 @@ int checkFlags(sphdem_ctx* ctx) {
 int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false) {
   const sphdem_params& P = ctx->P;
   const int n = ctx->n;
+  ctx->segsValid = false;
   // hmax_bits is cleared by the previous reset's k_bin (or by the upload);
   // gcnt[n], the scans' closing zero, is written by nothing else
   if (n == 0) HIPC(hipMemsetAsync(&ctx->dg->hmax_bits, 0, sizeof(unsigned long long), ctx->stream));
@@ -1066,7 +1105,8 @@ void freeAll(sphdem_ctx* ctx) {
                   ctx->hinv,        ctx->gcnt,        ctx->goff,    ctx->gsrc,   ctx->gord,
                   ctx->gcell,       ctx->gorig,       ctx->gcode,   ctx->inv,
                   ctx->mixP,        ctx->mixV,        ctx->mixD,    ctx->okey,
-                  ctx->posm,        ctx->pairCnt,     ctx->pairOff};
+                  ctx->posm,        ctx->pairCnt,     ctx->pairOff, ctx->lrec,
+                  ctx->lflag};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : ctx->evPool) (void)hipEventDestroy(e);
@@ -1121,6 +1161,8 @@ int sphdem_create(const sphdem_params* params, int device, int64_t capacity, sph
     HIPC(hipMalloc(&ctx->nbcnt, sizeof(int) * capacity));
     HIPC(hipMalloc(&ctx->fieldbuf, sizeof(double) * 3 * capacity));
     HIPC(hipMalloc(&ctx->errflag, sizeof(int)));
+    HIPC(hipMalloc(&ctx->lflag, sizeof(int) * 2));
+    HIPC(hipMemset(ctx->lflag, 0, sizeof(int) * 2));
     HIPC(hipMalloc(&ctx->dg, sizeof(DevGlobals)));
     HIPC(hipMemset(ctx->dg, 0, sizeof(DevGlobals)));
     HIPC(hipHostMalloc((void**)&ctx->hostG, sizeof(DevGlobals), hipHostMallocDefault));
@@ -1397,14 +1439,16 @@ int sphdem_checkpoint_restore(sphdem_ctx* ctx, const void* host, int64_t bytes) 
 
 int sphdem_set_tile_shape(sphdem_ctx* ctx, int seg_particles, int walker_block) {
   if (!ctx) return SPHDEM_E_ARG;
-  if (seg_particles < 8 || seg_particles > 64 ||
+  if ((seg_particles != 0 && (seg_particles < 8 || seg_particles > 64)) ||
       (walker_block != 64 && walker_block != 128 && walker_block != 256))
     return fail(ctx, SPHDEM_E_ARG,
                 "tile shape %d particles per segment, %d threads per walker block: segments "
-                "hold 8..64 particles, walker blocks 64, 128 or 256 threads",
+                "hold 8..64 particles (0: per-target search), walker blocks 64, 128 or 256 "
+                "threads",
                 seg_particles, walker_block);
   // the segments of the last reset stay a valid partition until the next
-  // reset rebuilds them with the new size
+  // reset rebuilds them with the new size; after a reset without segments
+  // (the per-target search) the next segmented build makes them
   ctx->tileSeg = seg_particles;
   ctx->walkTPB = walker_block;
   return 0;
@@ -1426,7 +1470,7 @@ int sphdem_tune_tile_shape(sphdem_ctx* ctx, int steps, int hookset, int* seg_par
   CHECK(sphdem_checkpoint_size(ctx, &bytes));
   std::vector<char> blob((size_t)bytes);
   CHECK(sphdem_checkpoint_save(ctx, blob.data(), bytes));
-  static const int segs[] = {64, 48, 32};
+  static const int segs[] = {0, 64, 48, 32};
   static const int tpbs[] = {256, 128, 64};
   const int seg0 = ctx->tileSeg, tpb0 = ctx->walkTPB;
   double best = INFINITY;

@@ -278,6 +278,169 @@ __global__ void __launch_bounds__(256) k_lists(const double4* __restrict__ pos,
   }
 }
 
+// ---- per-target search -------------------------------------------------------------
+// The same lists from one lane per target particle.  A lane walks its own
+// 27-cell stencil (x-outer, then y, then z; per column the cells cz-1..cz+1 as
+// one store range, or, in a column holding ghosts, cell by cell: the real part,
+// then the ghosts), so each lane tests only its own ~27 cells of candidates
+// instead of the union a segment's wavefront streams.  Lanes of one wavefront
+// are 64 consecutive targets of the cell-sorted store: they share most of their
+// stencil and read the same candidates at the same time.
+//
+// The test runs on quantised records first: a candidate's position as 16-bit
+// fixed point modulo 2^16 in a unit u (a power of two with edge/u in
+// [1024, 2048)), so a stencil separation (< 2 cell edges per axis) is an exact
+// int16 difference of at most 4097 units, and its squared length D an exact
+// int32 (one packed 16-bit subtraction, one 16-bit dot product and one 16-bit
+// multiply-add).  Quantisation moves |dr| by less than sqrt(3) u, so with
+// band = 1.74 > sqrt(3):
+//   D <= (2 h_a / u - band)^2            is_neighbr holds (2 max(h) >= 2 h_a);
+//   D >  (2 h / u + band)^2 for h_a, h_b is_neighbr fails;
+// anything between is decided by the FP64 test on the store positions, in the
+// reference's arithmetic.  Each bound keeps a margin of more than 0.007 u,
+// far above the FP64 rounding of |dr|^2 (about 1e-15 relative), so the list is
+// the FP64 test's exactly.  The record's upper half carries the candidate's
+// bound as B16 = ceil(((2 h_b / u + band)^2 + 1) / 1024) in bits 16..31 above
+// its z: D * 64 <= that word whenever D <= 1024 * B16.  A non-finite position
+// or a non-positive h anywhere sets a flag that sends every candidate to the
+// FP64 test.
+constexpr double kQBand = 1.74;
+struct ListQuant {
+  double o[3];  // any common origin (the grid's cmin)
+  double uinv;  // 1/u, a power of two
+};
+__device__ __forceinline__ unsigned q16(double x, double o, double uinv) {
+  return (unsigned)(long long)floor((x - o) * uinv) & 0xffffu;
+}
+// Records of the SPH-group entries [0, ns) and the ghosts [n, n + ng).  The
+// flag slot of generation gen collects non-finite input; the next
+// generation's slot is cleared for the next build.
+__global__ void k_lrec(const double4* __restrict__ posh, int ns, int n, int ng, ListQuant lq,
+                       uint2* __restrict__ rec, int* __restrict__ lflag, int gen) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) lflag[(gen + 1) & 1] = 0;
+  if (i >= ns + ng) return;
+  const int s = i < ns ? i : n + (i - ns);
+  const double4 p = posh[s];
+  const bool ok = isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(p.w) && p.w > 0.0;
+  if (!ok) {
+    atomicOr(&lflag[gen & 1], 1);
+    rec[s] = make_uint2(0u, 0u);
+    return;
+  }
+  const double b = 2.0 * p.w * lq.uinv + kQBand;
+  const double bq = fmin(b * b + 1.0, 1.0e9);
+  const unsigned b16 = (unsigned)fmin(ceil(bq * (1.0 / 1024.0)), 65535.0);
+  rec[s] = make_uint2(q16(p.x, lq.o[0], lq.uinv) | (q16(p.y, lq.o[1], lq.uinv) << 16),
+                      q16(p.z, lq.o[2], lq.uinv) | (b16 << 16));
+}
+
+typedef short short2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ short2v asShort2(unsigned v) { return __builtin_bit_cast(short2v, v); }
+
+// Grid: 8 * ceil(ceil(ns / 8) / blockDim) workgroups; XCD blockIdx % 8 takes
+// the corresponding eighth of the targets (a spatially coherent slab for its
+// L2).  NB candidates per batch (16 or 32).
+template <int NB>
+__global__ void __launch_bounds__(256) k_lists_t(const double4* __restrict__ pos,
+                                                 const uint2* __restrict__ rec,
+                                                 const int* __restrict__ cellc, Grid g, int gbase,
+                                                 const int* __restrict__ start, int ns,
+                                                 double uinv, const int* __restrict__ lflag,
+                                                 int gen, int* __restrict__ nbl,
+                                                 int* __restrict__ nbcnt, DevGlobals* dg) {
+  const int per = (ns + 7) >> 3;
+  const int local = (int)(blockIdx.x >> 3) * (int)blockDim.x + (int)threadIdx.x;
+  const int t = (int)(blockIdx.x & 7) * per + local;
+  if (local >= per || t >= ns) return;
+  const double4 p = pos[t];
+  const uint2 own = rec[t];
+  const short2v ownxy = asShort2(own.x);
+  int aq = -1, bq = 0x7fffffff;
+  if (lflag[gen & 1] == 0) {
+    const double a = 2.0 * p.w * uinv - kQBand;
+    if (a > 0.0) aq = (int)floor(a * a) - 1;
+    const double b = 2.0 * p.w * uinv + kQBand;
+    bq = (int)fmin(ceil(b * b) + 1.0, 2.0e9);
+  }
+  const double lima2 = (2.0 * p.w) * (2.0 * p.w);
+  const int cell = cellc[t];
+  const int cx = cell & 2047, cy = (cell >> 11) & 2047, cz = cellZ(cell);
+  int c = 0;  // entries found
+  int b0 = 0, b1 = 0, b2 = 0, b3 = 0;  // the chunk being filled, b0 the newest
+  auto push = [&](int e) {
+    b3 = b2;
+    b2 = b1;
+    b1 = b0;
+    b0 = e;
+    if ((c & (kChunk - 1)) == kChunk - 1 && c < kMaxNb)
+      *reinterpret_cast<int4*>(nbl + nbChunk(c >> 2, t, ns)) = make_int4(b3, b2, b1, b0);
+    c++;
+  };
+  // candidates [J0, J1) in batches of NB: bit k of acc / may for candidate
+  // jb + k (reads past J1 land in the records' slack and are masked off)
+  auto scan = [&](int J0, int J1) {
+    for (int jb = J0; jb < J1; jb += NB) {
+      const uint2* rp = rec + jb;
+      uint2 r[NB];
+#pragma unroll
+      for (int k = 0; k < NB; k++) r[k] = rp[k];
+      unsigned acc = 0, may = 0;
+#pragma unroll
+      for (int k = 0; k < NB; k++) {
+        const short2v dxy = asShort2(r[k].x) - ownxy;
+        const int dz = (int)(short)(r[k].y - own.y);
+        const int D = __builtin_amdgcn_sdot2(dxy, dxy, dz * dz, false);
+        acc |= (D <= aq ? 1u : 0u) << k;
+        may |= (((D <= bq) | ((unsigned)D * 64u <= r[k].y)) ? 1u : 0u) << k;
+      }
+      unsigned win = lowBits(min(J1 - jb, NB));
+      const int self = t - jb;
+      if ((unsigned)self < (unsigned)NB) win &= ~(1u << self);
+      unsigned amb = may & ~acc & win;
+      while (amb) {  // the FP64 test of is_neighbr (particle.h:236-238)
+        const int k = __builtin_ctz(amb);
+        amb &= amb - 1;
+        const double4 q = pos[jb + k];
+        const double dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+        const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+        const double hb2 = __longlong_as_double(__double_as_longlong(q.w) + (1ll << 52));
+        if (l2 <= fmax(lima2, hb2 * hb2)) acc |= 1u << k;
+      }
+      unsigned m = acc & win;
+      while (m) {
+        const int k = __builtin_ctz(m);
+        m &= m - 1;
+        push(jb + k);
+      }
+    }
+  };
+  const int z0 = max(cz - 1, 0), z1 = min(cz + 1, g.dim[2] - 1);
+  for (int X = max(cx - 1, 0); X <= min(cx + 1, g.dim[0] - 1); X++)
+    for (int Y = max(cy - 1, 0); Y <= min(cy + 1, g.dim[1] - 1); Y++) {
+      const int col = (X * g.dim[1] + Y) * g.dim[2];
+      const int k0 = g.base + col, q0 = gbase + col;
+      if (gbase > 0 && start[q0 + z0] != start[q0 + z1 + 1]) {
+        for (int z = z0; z <= z1; z++) {
+          scan(start[k0 + z], start[k0 + z + 1]);
+          scan(start[q0 + z], start[q0 + z + 1]);
+        }
+      } else {
+        scan(start[k0 + z0], start[k0 + z1 + 1]);
+      }
+    }
+  if (c > kMaxNb) {
+    atomicOr(&dg->errflags, 2u);
+    c = kMaxNb;
+  }
+  const int rem = c & (kChunk - 1);  // entries of the last, partial chunk
+  if (rem)
+    *reinterpret_cast<int4*>(nbl + nbChunk(c >> 2, t, ns)) =
+        rem == 1 ? make_int4(b0, 0, 0, 0)
+                 : (rem == 2 ? make_int4(b1, b0, 0, 0) : make_int4(b2, b1, b0, 0));
+  nbcnt[t] = c;
+}
+
 // The cached lists as (a, b, image) triples of original indices, for the
 // parity enumeration of the SPH pair set: a ghost neighbour reports its
 // origin and its image code.
