@@ -146,7 +146,7 @@ def cpu_baseline(case, steps, threads):
     }
 
 
-KERNEL_SYMBOL = {"lists": "k_lists", "force": "k_force_l",
+KERNEL_SYMBOL = {"lists": "k_lists_t", "force": "k_force_l",
                  "dddt": "k_dddt_l", "coupling": "k_coupling_sph",
                  "adj_force_bwd": "k_force_bwd"}
 KERNEL_SYMBOL_MIXED = {"force": "k_force_m", "dddt": "k_dddt_m"}

@@ -586,6 +586,103 @@ __global__ void k_hfromdens(Store S, int ns, double hfac) {
   S.posh[a].w = h;
 }
 
+// The candidate ranges of forCells for a wave-uniform query point: visit(j0,
+// j1, sx, sy, sz) once per column range, in forCells' order.
+template <class V>
+__device__ __forceinline__ void forCellRanges(const Grid& g, const int* __restrict__ start,
+                                              const Box& B, double qx, double qy, double qz,
+                                              int reach, V&& visit) {
+  const int nsx = B.periodic[0] ? 3 : 1, nsy = B.periodic[1] ? 3 : 1, nsz = B.periodic[2] ? 3 : 1;
+  const double q[3] = {qx, qy, qz};
+  for (int ix = 0; ix < nsx; ix++) {
+    const int sx = ix == 0 ? 0 : (ix == 1 ? -1 : 1);
+    const double tx = sx == 0 ? q[0] : (sx == 1 ? q[0] - B.Lp[0] : q[0] - B.Lm[0]);
+    const int cx = cellOf(tx, g.cmin[0], g.edge, g.off);
+    const int x0 = max(cx - reach, 0), x1 = min(cx + reach, g.dim[0] - 1);
+    if (x0 > x1) continue;
+    for (int iy = 0; iy < nsy; iy++) {
+      const int sy = iy == 0 ? 0 : (iy == 1 ? -1 : 1);
+      const double ty = sy == 0 ? q[1] : (sy == 1 ? q[1] - B.Lp[1] : q[1] - B.Lm[1]);
+      const int cy = cellOf(ty, g.cmin[1], g.edge, g.off);
+      const int y0 = max(cy - reach, 0), y1 = min(cy + reach, g.dim[1] - 1);
+      if (y0 > y1) continue;
+      for (int iz = 0; iz < nsz; iz++) {
+        const int sz = iz == 0 ? 0 : (iz == 1 ? -1 : 1);
+        const double tz = sz == 0 ? q[2] : (sz == 1 ? q[2] - B.Lp[2] : q[2] - B.Lm[2]);
+        const int cz = cellOf(tz, g.cmin[2], g.edge, g.off);
+        const int z0 = max(cz - reach, 0), z1 = min(cz + reach, g.dim[2] - 1);
+        if (z0 > z1) continue;
+        for (int a = x0; a <= x1; a++)
+          for (int b = y0; b <= y1; b++) {
+            const int k0 = g.base + (a * g.dim[1] + b) * g.dim[2];
+            visit(start[k0 + z0], start[k0 + z1 + 1], sx, sy, sz);
+          }
+      }
+    }
+  }
+}
+
+// Accepted candidates of one wavefront, compacted in LDS so that the pair
+// arithmetic runs on full wavefronts: lanes test 64 consecutive candidates,
+// the accepted ones are appended in lane order (ballot + prefix count), and
+// every 64 appended entries are processed together, lane i taking entry i.
+// The order of the entries, and so of every lane's partial sums, depends
+// only on the candidate order: results are the same run to run.
+constexpr int kCompactCap = 128;  // 64 carried over + 64 appended
+// ranges(visit) calls visit(j0, j1, code) for each candidate range (code: an
+// image code the body receives with the candidate); test(j, code) decides a
+// candidate, body(j, code) runs for each accepted one.
+template <class Ranges, class Test, class Body>
+__device__ __forceinline__ void compactedPairs(int* __restrict__ lb, int* __restrict__ lc,
+                                               int lane, Ranges&& ranges, Test&& test,
+                                               Body&& body) {
+  int cnt = 0;  // wave-uniform
+  auto flush = [&](int m) {
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < m) body(lb[lane], lc[lane]);
+    __builtin_amdgcn_wave_barrier();
+  };
+  ranges([&](int j0, int j1, int code) {
+    for (int jj = j0; jj < j1; jj += 64) {
+      const int j = jj + lane;
+      const bool ok = j < j1 && test(j, code);
+      const unsigned long long m = __ballot(ok);
+      if (ok) {
+        const int at = cnt + (int)__builtin_amdgcn_mbcnt_hi(
+                                 (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        lb[at] = j;
+        lc[at] = code;
+      }
+      cnt += __popcll(m);
+      if (cnt >= 64) {
+        flush(64);
+        // carry the entries past the first 64 to the front
+        int cb = 0, cc = 0;
+        if (lane < cnt - 64) {
+          cb = lb[64 + lane];
+          cc = lc[64 + lane];
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (lane < cnt - 64) {
+          lb[lane] = cb;
+          lc[lane] = cc;
+        }
+        cnt -= 64;
+      }
+    }
+  });
+  if (cnt > 0) flush(cnt);
+}
+__device__ __forceinline__ int imageCode(int sx, int sy, int sz) {
+  return (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+}
+__device__ __forceinline__ void imageShift(int code, int& sx, int& sy, int& sz) {
+  sx = code % 3 - 1;
+  sy = (code / 3) % 3 - 1;
+  sz = code / 9 - 1;
+}
+
 // ============================================================ SPH <-> DEM coupling
 // calcNeighboursCoupling's SPH-side rule (dataLL.cpp:1154-1190) as a predicate
 // on (DEM cell of the SPH particle, DEM cell of the DEM image, distance).
@@ -640,36 +737,54 @@ __global__ void __launch_bounds__(256) k_shepsum(Store S, int ns, int nd, Grid g
                                                  const int* __restrict__ start, Box B,
                                                  CouplingRule R, int reach, double wcon,
                                                  double Rc) {
-  const int lane = threadIdx.x & 63;
+  __shared__ int lbuf[4][kCompactCap], lcode[4][kCompactCap];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int j = ns + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (j >= ns + nd) return;  // uniform per wavefront
   const double4 pj = S.posh[j];
   double sum = 0.0;
+  Box B0 = B;
+  B0.periodic[0] = B0.periodic[1] = B0.periodic[2] = 0;
   // candidates a see the image r_j + s L; enumerate them around that image
-  const int nsx = B.periodic[0] ? 3 : 1, nsy = B.periodic[1] ? 3 : 1, nsz = B.periodic[2] ? 3 : 1;
-  for (int ix = 0; ix < nsx; ix++)
-    for (int iy = 0; iy < nsy; iy++)
-      for (int iz = 0; iz < nsz; iz++) {
-        const int sx = ix == 0 ? 0 : (ix == 1 ? -1 : 1);
-        const int sy = iy == 0 ? 0 : (iy == 1 ? -1 : 1);
-        const int sz = iz == 0 ? 0 : (iz == 1 ? -1 : 1);
-        if ((sx | sy | sz) != 0 && !demImageExists(R, B, pj, sx, sy, sz)) continue;
-        const Img im = imagePos(pj, sx, sy, sz, B);
-        Box B0 = B;
-        B0.periodic[0] = B0.periodic[1] = B0.periodic[2] = 0;
-        forCells(gs, start, B0, im.x, im.y, im.z, reach, [&](int a, int, int, int) {
-          if (!isSphT(S.iam[a])) return;
-          const double4 pa = S.posh[a];
-          const double dx = pa.x - im.x, dy = pa.y - im.y, dz = pa.z - im.z;
-          const double l2 = dot3(dx, dy, dz, dx, dy, dz);
-          if (l2 > R.skip2) return;
-          const int ca[3] = {demCell(R, pa.x, 0), demCell(R, pa.y, 1), demCell(R, pa.z, 2)};
-          if (!couplingAccept(R, ca, im.x, im.y, im.z, l2, false)) return;
-          const double dv = S.vhm[a].w / S.dpo[a].x;
-          const double r = sqrt(l2);
-          sum += dv * wendW(wcon, r * (2.0 / Rc), Rc / 2.0);
-        }, lane, 64);
-      }
+  auto ranges = [&](auto&& visit) {
+    const int nsx = B.periodic[0] ? 3 : 1, nsy = B.periodic[1] ? 3 : 1,
+              nsz = B.periodic[2] ? 3 : 1;
+    for (int ix = 0; ix < nsx; ix++)
+      for (int iy = 0; iy < nsy; iy++)
+        for (int iz = 0; iz < nsz; iz++) {
+          const int sx = ix == 0 ? 0 : (ix == 1 ? -1 : 1);
+          const int sy = iy == 0 ? 0 : (iy == 1 ? -1 : 1);
+          const int sz = iz == 0 ? 0 : (iz == 1 ? -1 : 1);
+          if ((sx | sy | sz) != 0 && !demImageExists(R, B, pj, sx, sy, sz)) continue;
+          const Img im = imagePos(pj, sx, sy, sz, B);
+          forCellRanges(gs, start, B0, im.x, im.y, im.z, reach,
+                        [&](int j0, int j1, int, int, int) { visit(j0, j1, imageCode(sx, sy, sz)); });
+        }
+  };
+  auto dist2 = [&](int a, int code, Img& im) {
+    int sx, sy, sz;
+    imageShift(code, sx, sy, sz);
+    im = imagePos(pj, sx, sy, sz, B);
+    const double4 pa = S.posh[a];
+    const double dx = pa.x - im.x, dy = pa.y - im.y, dz = pa.z - im.z;
+    return dot3(dx, dy, dz, dx, dy, dz);
+  };
+  // the test keeps the cheap distance cut; the coupling cell rule runs on the
+  // compacted candidates
+  compactedPairs(lbuf[w], lcode[w], lane, ranges, [&](int a, int code) {
+    if (!isSphT(S.iam[a])) return false;
+    Img im;
+    return dist2(a, code, im) <= R.skip2;
+  }, [&](int a, int code) {
+    Img im;
+    const double l2 = dist2(a, code, im);
+    const double4 pa = S.posh[a];
+    const int ca[3] = {demCell(R, pa.x, 0), demCell(R, pa.y, 1), demCell(R, pa.z, 2)};
+    if (!couplingAccept(R, ca, im.x, im.y, im.z, l2, false)) return;
+    const double dv = S.vhm[a].w / S.dpo[a].x;
+    const double r = sqrt(l2);
+    sum += dv * wendW(wcon, r * (2.0 / Rc), Rc / 2.0);
+  });
   sum = waveSum(sum);
   if (lane == 0) S.fp[j].w = sum;
 }
@@ -719,11 +834,14 @@ __global__ void k_coupling_sph(Store S, int ns, int nd, Grid gc, const int* __re
 // calcFPI (sph.h:1708-1792) with calcParticleDrag (sph.h:1575-1631, Di Felice):
 // each DEM particle gathers its SPH partners (the 27 SPH cells around it,
 // dataLL.cpp:1106-1126).
-// One wavefront per DEM particle, as k_shepsum.
+// One wavefront per DEM particle: the lanes test the candidates, the accepted
+// ones are compacted and summed on full wavefronts, and the partial sums meet
+// in a fixed butterfly.
 __global__ void __launch_bounds__(256) k_fpi(Store S, int ns, int nd, Grid gs,
                                              const int* __restrict__ start, Box B,
                                              sphdem_params P) {
-  const int lane = threadIdx.x & 63;
+  __shared__ int lbuf[4][kCompactCap], lcode[4][kCompactCap];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int j = ns + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (j >= ns + nd) return;  // uniform per wavefront
   const double4 pj = S.posh[j];
@@ -731,15 +849,27 @@ __global__ void __launch_bounds__(256) k_fpi(Store S, int ns, int nd, Grid gs,
   double sum = 0, fdens = 0, por = 0;
   double ff[3] = {0, 0, 0}, fpfb[3] = {0, 0, 0}, fv[3] = {0, 0, 0}, fdr[3] = {0, 0, 0},
          fvel[3] = {0, 0, 0};
-  forCells(gs, start, B, pj.x, pj.y, pj.z, 1, [&](int b, int sx, int sy, int sz) {
-    const double4 pb = S.posh[b];
+  auto dist2 = [&](const double4& pb, int sx, int sy, int sz) {
     const Img im = imagePos(pb, sx, sy, sz, B);
     const double dx = pj.x - im.x, dy = pj.y - im.y, dz = pj.z - im.z;
-    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+    return dot3(dx, dy, dz, dx, dy, dz);
+  };
+  auto ranges = [&](auto&& visit) {
+    forCellRanges(gs, start, B, pj.x, pj.y, pj.z, 1, [&](int j0, int j1, int sx, int sy, int sz) {
+      visit(j0, j1, imageCode(sx, sy, sz));
+    });
+  };
+  compactedPairs(lbuf[w], lcode[w], lane, ranges, [&](int b, int code) {
+    int sx, sy, sz;
+    imageShift(code, sx, sy, sz);
+    const double4 pb = S.posh[b];
     const double lim = 2.0 * dmax2(pj.w, pb.w);
-    if (!(l2 <= lim * lim)) return;
-    if (S.iam[b] == T_BOUNDARY) return;
-    const double r = sqrt(l2);
+    return dist2(pb, sx, sy, sz) <= lim * lim && S.iam[b] != T_BOUNDARY;
+  }, [&](int b, int code) {
+    int sx, sy, sz;
+    imageShift(code, sx, sy, sz);
+    const double4 pb = S.posh[b];
+    const double r = sqrt(dist2(pb, sx, sy, sz));
     const double4 vb = S.vhm[b];
     // dens and porosity only: omega may be written meanwhile (the fused dddt walk)
     const double2 db = make_double2(S.dpo[b].x, S.dpo[b].w);
@@ -770,7 +900,7 @@ __global__ void __launch_bounds__(256) k_fpi(Store S, int ns, int nd, Grid gs,
     por += dvWab * db.y;
     fdens += Wab * vb.w;
     sum += dvWab;
-  }, lane, 64);
+  });
   sum = waveSum(sum);
   fdens = waveSum(fdens);
   por = waveSum(por);

@@ -301,7 +301,8 @@ __global__ void __launch_bounds__(256) k_lists(const double4* __restrict__ pos,
 // far above the FP64 rounding of |dr|^2 (about 1e-15 relative), so the list is
 // the FP64 test's exactly.  The record's upper half carries the candidate's
 // bound as B16 = ceil(((2 h_b / u + band)^2 + 1) / 1024) in bits 16..31 above
-// its z: D * 64 <= that word whenever D <= 1024 * B16.  A non-finite position
+// its z: the word shifted right by 6 is at least 1024 * B16, so D at most that
+// is "maybe" whenever D <= 1024 * B16.  A non-finite position
 // or a non-positive h anywhere sets a flag that sends every candidate to the
 // FP64 test.
 constexpr double kQBand = 1.74;
@@ -335,12 +336,22 @@ __global__ void k_lrec(const double4* __restrict__ posh, int ns, int n, int ng, 
                       q16(p.z, lq.o[2], lq.uinv) | (b16 << 16));
 }
 
+// acc = 2 acc + (x >= D ? 1 : 0): a compare into VCC and an add with carry,
+// where (acc << 1) | (x >= D) compiles to a compare, a select and a merge.
+// After NB steps candidate k of a batch is bit NB - 1 - k.
+__device__ __forceinline__ void shiftInGe(unsigned& acc, int x, int D) {
+  asm("v_cmp_ge_i32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc"
+      : "+v"(acc)
+      : "v"(x), "v"(D)
+      : "vcc");
+}
+
 typedef short short2v __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ short2v asShort2(unsigned v) { return __builtin_bit_cast(short2v, v); }
 
 // Grid: 8 * ceil(ceil(ns / 8) / blockDim) workgroups; XCD blockIdx % 8 takes
 // the corresponding eighth of the targets (a spatially coherent slab for its
-// L2).  NB candidates per batch (16 or 32).
+// L2).  NB = 32 candidates per batch (the masks' width).
 template <int NB>
 __global__ void __launch_bounds__(256) k_lists_t(const double4* __restrict__ pos,
                                                  const uint2* __restrict__ rec,
@@ -349,6 +360,7 @@ __global__ void __launch_bounds__(256) k_lists_t(const double4* __restrict__ pos
                                                  double uinv, const int* __restrict__ lflag,
                                                  int gen, int* __restrict__ nbl,
                                                  int* __restrict__ nbcnt, DevGlobals* dg) {
+  static_assert(NB == 32, "the masks are 32 bits wide");
   const int per = (ns + 7) >> 3;
   const int local = (int)(blockIdx.x >> 3) * (int)blockDim.x + (int)threadIdx.x;
   const int t = (int)(blockIdx.x & 7) * per + local;
@@ -391,26 +403,27 @@ __global__ void __launch_bounds__(256) k_lists_t(const double4* __restrict__ pos
         const short2v dxy = asShort2(r[k].x) - ownxy;
         const int dz = (int)(short)(r[k].y - own.y);
         const int D = __builtin_amdgcn_sdot2(dxy, dxy, dz * dz, false);
-        acc |= (D <= aq ? 1u : 0u) << k;
-        may |= (((D <= bq) | ((unsigned)D * 64u <= r[k].y)) ? 1u : 0u) << k;
+        shiftInGe(acc, aq, D);
+        shiftInGe(may, max(bq, (int)(r[k].y >> 6)), D);
       }
-      unsigned win = lowBits(min(J1 - jb, NB));
+      // candidate jb + k is bit 31 - k
+      unsigned win = __builtin_bitreverse32(lowBits(min(J1 - jb, NB)));
       const int self = t - jb;
-      if ((unsigned)self < (unsigned)NB) win &= ~(1u << self);
+      if ((unsigned)self < (unsigned)NB) win &= ~(0x80000000u >> self);
       unsigned amb = may & ~acc & win;
       while (amb) {  // the FP64 test of is_neighbr (particle.h:236-238)
-        const int k = __builtin_ctz(amb);
-        amb &= amb - 1;
+        const int k = __builtin_clz(amb);
+        amb &= ~(0x80000000u >> k);
         const double4 q = pos[jb + k];
         const double dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         const double l2 = dot3(dx, dy, dz, dx, dy, dz);
         const double hb2 = __longlong_as_double(__double_as_longlong(q.w) + (1ll << 52));
-        if (l2 <= fmax(lima2, hb2 * hb2)) acc |= 1u << k;
+        if (l2 <= fmax(lima2, hb2 * hb2)) acc |= 0x80000000u >> k;
       }
       unsigned m = acc & win;
       while (m) {
-        const int k = __builtin_ctz(m);
-        m &= m - 1;
+        const int k = __builtin_clz(m);
+        m &= ~(0x80000000u >> k);
         push(jb + k);
       }
     }
