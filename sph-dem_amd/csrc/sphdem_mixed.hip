@@ -245,8 +245,13 @@ __global__ void __launch_bounds__(256) k_force_m(Store S, int ns, const int* __r
     const float nu198 = (float)(19.8 * P.viscosity), alphaM = (float)monaghanAlpha(P);
     const double vsigC = (VISC == VISC_MOR || VISC == VISC_CLE) ? 2.0 * P.spsound : 0.0;
     const double cdiff = VISC == VISC_MON ? 0.06 : 0.125;  // viscDiffusionCondition
-    double minA = inf, minB = inf;
-    double fx = 0.0, fy = 0.0, fz = 0.0, px = 0.0, py = 0.0, pz = 0.0, vx = 0.0, vy = 0.0,
+    // the dt conditions 0.6 hav and cdiff hav^2 are increasing in hav, and so
+    // are their correctly rounded values: their minima are those of the
+    // smallest hav, taken once at the end
+    double minHav = inf;
+    // f is the sum of the pressure, viscous and artificial-viscosity parts,
+    // added once at the end
+    double ax = 0.0, ay = 0.0, az = 0.0, px = 0.0, py = 0.0, pz = 0.0, vx = 0.0, vy = 0.0,
            vz = 0.0;
     const int n = nbcnt[a];
     constexpr int U = 4;
@@ -312,17 +317,15 @@ __global__ void __launch_bounds__(256) k_force_m(Store S, int ns, const int* __r
               avisc = viss * avsig * alpha * frcp(rhoa + rhob);
             }
             const float s = mb * avisc * Gbar;
-            fx += (double)(s * dx.x);
-            fy += (double)(s * dx.y);
-            fz += (double)(s * dx.z);
+            ax += (double)(s * dx.x);
+            ay += (double)(s * dx.y);
+            az += (double)(s * dx.z);
             vsig = fmaxf(vsig, avsig);
             newDt = dmin2(newDt, 0.6 * hav / (double)vsig);
           } else if (VISC == VISC_MON) {
             newDt = dmin2(newDt, 0.6 * hav / (double)vsig);
-          } else {
-            minA = dmin2(minA, 0.6 * hav);
           }
-          minB = dmin2(minB, cdiff * (hav * hav));
+          minHav = dmin2(minHav, hav);
           constexpr bool alongDr = VISC == VISC_CLE || VISC == VISC_MON;
           const double f0 = (double)(alongDr ? dx.x * fvr : dv0 * fvs);
           const double f1 = (double)(alongDr ? dx.y * fvr : dv1 * fvs);
@@ -330,22 +333,19 @@ __global__ void __launch_bounds__(256) k_force_m(Store S, int ns, const int* __r
           vx += f0;
           vy += f1;
           vz += f2;
-          fx += f0;
-          fy += f1;
-          fz += f2;
         }
         const float pp = -mb * (pdr2a * Ga + pdr2b * Gb);  // pressure force = dx * pp
         const double q0 = (double)(pp * dx.x), q1 = (double)(pp * dx.y), q2 = (double)(pp * dx.z);
         px += q0;
         py += q1;
         pz += q2;
-        fx += q0;
-        fy += q1;
-        fz += q2;
       }
     }
-    if (!ARTIF && VISC != VISC_MON && minA != inf) newDt = dmin2(newDt, minA / vsigC);
-    if (minB != inf) newDt = dmin2(newDt, minB / P.viscosity);
+    if (minHav != inf) {
+      if (!ARTIF && VISC != VISC_MON) newDt = dmin2(newDt, (0.6 * minHav) / vsigC);
+      newDt = dmin2(newDt, (cdiff * (minHav * minHav)) / P.viscosity);
+    }
+    double fx = px + vx + ax, fy = py + vy + ay, fz = pz + vz + az;
     if (OMG) {
       const double omega = 1 + ha * om / (da.x * 3);
       S.dpo[a].z = omega;
