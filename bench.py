@@ -430,8 +430,8 @@ def ensemble(a):
         "tile_shapes": shapes or None,
         "default_shape": None if untuned is None else {
             "value": counted * a.steps / untuned, "unit": "particle-steps/s",
-            "what": "the same boxes together with the default tile shape (64 particles per "
-                    "segment, 256-thread walker blocks), timed before tuning"},
+            "what": "the same boxes together with the default tile shape (per-target list "
+                    "search, 256-thread walker blocks), timed before tuning"},
         "roofline": None,
         "step_roofline": {
             "alg_bytes_per_step": step_bytes,

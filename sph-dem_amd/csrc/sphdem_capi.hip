@@ -61,6 +61,9 @@ struct sphdem_ctx {
   uint2* lrec = nullptr;
   int* lflag = nullptr;
   int lgen = 0;
+  // DEM cells with a sphere within one cell, marked with the reset's number
+  int* demNear = nullptr;
+  int demNearCap = 0, demNearGen = 0;
   bool listsValid = false;
   double2* hinv = nullptr;  // {1/h, 1/h^5} per SPH-group particle
   long long* okey = nullptr;  // order key of each entry, in scatter order
@@ -909,9 +912,25 @@ int passCoupling(sphdem_ctx* ctx) {
                        ctx->P.wcon, ctx->P.coupling_radius);
     LAUNCHED();
   }
+  // the DEM cells with a sphere around them (no memset: a new mark each reset)
+  const int cells = ctx->gc.dim[0] * ctx->gc.dim[1] * ctx->gc.dim[2];
+  if (cells > ctx->demNearCap) {
+    if (ctx->demNear) HIPC(hipFree(ctx->demNear));
+    ctx->demNearCap = cells + cells / 4;
+    HIPC(hipMalloc(&ctx->demNear, sizeof(int) * ctx->demNearCap));
+    HIPC(hipMemsetAsync(ctx->demNear, 0, sizeof(int) * ctx->demNearCap, ctx->stream));
+    ctx->demNearGen = 0;
+  }
+  const int gen = ++ctx->demNearGen;
+  if (ctx->nd > 0) {
+    hipLaunchKernelGGL(k_dem_near, dim3(blocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                       ctx->ns, ctx->nd, ctx->gc, ctx->demNear, gen);
+    LAUNCHED();
+  }
   Timed tm(ctx, KID_COUPLING);
   hipLaunchKernelGGL(k_coupling_sph, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
-                     ctx->ns, ctx->nd, ctx->gc, ctx->start, ctx->box, ctx->rule, ctx->P);
+                     ctx->ns, ctx->nd, ctx->gc, ctx->start, ctx->box, ctx->rule, ctx->P,
+                     ctx->demNear, gen);
   LAUNCHED();
   markStale(ctx, GS_DPO);
   return 0;
@@ -1106,7 +1125,7 @@ void freeAll(sphdem_ctx* ctx) {
                   ctx->gcell,       ctx->gorig,       ctx->gcode,   ctx->inv,
                   ctx->mixP,        ctx->mixV,        ctx->mixD,    ctx->okey,
                   ctx->posm,        ctx->pairCnt,     ctx->pairOff, ctx->lrec,
-                  ctx->lflag};
+                  ctx->lflag,       ctx->demNear};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : ctx->evPool) (void)hipEventDestroy(e);

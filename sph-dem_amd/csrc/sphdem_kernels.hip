@@ -74,10 +74,14 @@ __device__ __forceinline__ void atomicMaxPos(unsigned long long* p, double v) {
 // real particles whose image r_b + s*L may lie near q.  Cells along z of one
 // (x,y) column are contiguous in the sorted store, so each column is one range.
 // `first`/`stride` split each contiguous range over the lanes of a wavefront.
-template <class V>
+struct KeepAll {
+  __device__ bool operator()(int, int, int) const { return true; }
+};
+template <class V, class K = KeepAll>
 __device__ __forceinline__ void forCells(const Grid& g, const int* __restrict__ start,
                                          const Box& B, double qx, double qy, double qz,
-                                         int reach, V&& visit, int first = 0, int stride = 1) {
+                                         int reach, V&& visit, int first = 0, int stride = 1,
+                                         K&& keep = K()) {
   const int nsx = B.periodic[0] ? 3 : 1, nsy = B.periodic[1] ? 3 : 1, nsz = B.periodic[2] ? 3 : 1;
   const double q[3] = {qx, qy, qz};
   for (int ix = 0; ix < nsx; ix++) {
@@ -97,7 +101,7 @@ __device__ __forceinline__ void forCells(const Grid& g, const int* __restrict__ 
         const double tz = sz == 0 ? q[2] : (sz == 1 ? q[2] - B.Lp[2] : q[2] - B.Lm[2]);
         const int cz = cellOf(tz, g.cmin[2], g.edge, g.off);
         const int z0 = max(cz - reach, 0), z1 = min(cz + reach, g.dim[2] - 1);
-        if (z0 > z1) continue;
+        if (z0 > z1 || !keep(cx, cy, cz)) continue;
         for (int a = x0; a <= x1; a++)
           for (int b = y0; b <= y1; b++) {
             const int k0 = g.base + (a * g.dim[1] + b) * g.dim[2];
@@ -792,8 +796,23 @@ __global__ void __launch_bounds__(256) k_shepsum(Store S, int ns, int nd, Grid g
 // SPH side of middle()'s coupling block (sphIncompress.cpp:64-78): calcPorosity
 // (sph.h:1323-1341), calcDEMtoSPHDrag (sph.h:1343-1362) and finalisePorosity
 // (sph.h:1289-1296), one gather over the DEM partners.
+// DEM cells with a DEM particle within one cell (the 3x3x3 block around them):
+// near[c] == gen.  A coupling search centred on any other cell of the DEM grid
+// finds no candidate and is skipped.
+__global__ void k_dem_near(Store S, int ns, int nd, Grid gc, int* __restrict__ near, int gen) {
+  const int j = ns + blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns + nd || S.iam[j] != T_DEM) return;
+  const int c = S.cell[j];
+  const int x = c & 2047, y = (c >> 11) & 2047, z = cellZ(c);
+  for (int a = max(x - 1, 0); a <= min(x + 1, gc.dim[0] - 1); a++)
+    for (int b = max(y - 1, 0); b <= min(y + 1, gc.dim[1] - 1); b++)
+      for (int d = max(z - 1, 0); d <= min(z + 1, gc.dim[2] - 1); d++)
+        near[(a * gc.dim[1] + b) * gc.dim[2] + d] = gen;
+}
+
 __global__ void k_coupling_sph(Store S, int ns, int nd, Grid gc, const int* __restrict__ start,
-                               Box B, CouplingRule R, sphdem_params P) {
+                               Box B, CouplingRule R, sphdem_params P,
+                               const int* __restrict__ near, int gen) {
   const int a = xcdThread();
   if (a >= ns || !isSphT(S.iam[a])) return;
   const double4 pa = S.posh[a];
@@ -802,6 +821,13 @@ __global__ void k_coupling_sph(Store S, int ns, int nd, Grid gc, const int* __re
   const int ca[3] = {demCell(R, pa.x, 0), demCell(R, pa.y, 1), demCell(R, pa.z, 2)};
   double por = 0.0;
   double fd0 = 0.0, fd1 = 0.0, fd2 = 0.0;
+  // forCells over the DEM grid, skipping each image shift whose centre cell
+  // lies in the grid with no DEM particle around it
+  auto keep = [&](int cx, int cy, int cz) {
+    if (cx < 0 || cy < 0 || cz < 0 || cx >= gc.dim[0] || cy >= gc.dim[1] || cz >= gc.dim[2])
+      return true;
+    return near[(cx * gc.dim[1] + cy) * gc.dim[2] + cz] == gen;
+  };
   forCells(gc, start, B, pa.x, pa.y, pa.z, 1, [&](int j, int sx, int sy, int sz) {
     const double4 pj = S.posh[j];
     if ((sx | sy | sz) != 0 && !demImageExists(R, B, pj, sx, sy, sz)) return;
@@ -819,7 +845,7 @@ __global__ void k_coupling_sph(Store S, int ns, int nd, Grid gc, const int* __re
     fd0 -= s * fdj.x * w / shep;
     fd1 -= s * fdj.y * w / shep;
     fd2 -= s * fdj.z * w / shep;
-  });
+  }, 0, 1, keep);
   S.fdrag[a] = make_double4(fd0, fd1, fd2, 0.0);
   double4 f = S.frc[a];
   f.x += fd0;

@@ -25,7 +25,7 @@ def test_traffic_summary_parser(tmp_path):
                  "calibration FETCH_SIZE 2.0\n"
                  "calibration WRITE_SIZE 1.0\n"
                  "kernel dispatches FETCH_SIZE_KiB WRITE_SIZE_KiB\n"
-                 "k_lists 8 100.0 50.0\n")
+                 "k_lists_t 8 100.0 50.0\n")
     assert bench.traffic_from_summary(str(f), "lists") == pytest.approx(1024 * (200.0 + 50.0))
     assert bench.traffic_from_summary(str(f), "force") is None
     assert bench.traffic_from_summary(str(tmp_path / "missing.txt"), "lists") is None
