@@ -554,13 +554,20 @@ def adjoint(a):
         return sum(any(not torch.equal(x, y) for x, y in zip(kb, kn))
                    for kb, kn in zip(base_keys, keys))
 
+    def changed_by_kind(keys):
+        """The same count for each pair kind alone."""
+        return {name: sum(not torch.equal(kb[i], kn[i]) for kb, kn in zip(base_keys, keys))
+                for i, name in enumerate(("sph", "dem", "sph2dem", "dem2sph"))}
+
     def central(run_p, run_m, denom, adj):
         lp, _, kp = run_p()
         lm, _, km = run_m()
         d = (lp - lm) / denom
         cp, cm = changed(kp), changed(km)
         err = abs(adj - d) / abs(d) if d else None
+        bp, bm = changed_by_kind(kp), changed_by_kind(km)
         return {"fd": d, "rel_err": err, "resets_with_other_pairs": [cp, cm],
+                "by_kind": {k: [bp[k], bm[k]] for k in bp},
                 "pair_sets_identical": cp == 0 and cm == 0,
                 "within_1e-4": bool(err is not None and err <= 1e-4)}
 

@@ -114,15 +114,18 @@ __device__ __forceinline__ void segRange(int nseg, int w, int& s0, int& s1) {
 
 // ---- cached neighbour lists (CpInfo::neighbrs, built by calcNeighbours,
 // dataLL.cpp:1026-1075, and kept until the next reset, dataLL.impl.h:186-188) --
-// Entries are grouped in chunks of four per particle, chunk-major: chunk c of
-// particle t is the 16 bytes at nbl[(c * ns + t) * 4], so a wavefront reading
-// the next chunk of 64 consecutive particles reads 1 KiB of contiguous memory,
-// and the search writes whole chunks.  An entry is the store index of the
-// neighbour (a ghost's index for a periodic image).
-constexpr int kMaxNb = 256;  // a multiple of the chunk size
-constexpr int kChunk = 4;
-__device__ __forceinline__ size_t nbChunk(int c, int t, int ns) {
-  return ((size_t)c * ns + t) * kChunk;
+// Entries are grouped in eights per particle, group-major: entry k of particle
+// t is nbl[((k / 8) * ns + t) * 8 + k % 8], so a wavefront reading the next
+// four entries of 64 consecutive particles reads 64 16-byte pieces of one
+// 2 KiB block, and the search writes each 32-byte group with two 16-byte
+// stores of one lane, close in time (whole 32-byte sectors reach memory).  An
+// entry is the store index of the neighbour (a ghost's index for a periodic
+// image).
+constexpr int kMaxNb = 256;  // a multiple of the group size
+constexpr int kChunk = 4;    // entries per store
+constexpr int kGroup = 8;    // entries per 32-byte group
+__device__ __forceinline__ size_t nbEntry(int k, int t, int ns) {
+  return ((size_t)(k >> 3) * ns + t) * kGroup + (k & 7);
 }
 // Pointer into the constant address space (scalar loads for uniform indices).
 template <class T>
@@ -184,7 +187,7 @@ __global__ void __launch_bounds__(256) k_lists(const double4* __restrict__ pos,
       b1 = b0;
       b0 = e;
       if ((c & (kChunk - 1)) == kChunk - 1 && c < kMaxNb)
-        *reinterpret_cast<int4*>(nbl + nbChunk(c >> 2, t, ns)) = make_int4(b3, b2, b1, b0);
+        *reinterpret_cast<int4*>(nbl + nbEntry(c & ~3, t, ns)) = make_int4(b3, b2, b1, b0);
       c++;
     };
     // candidates [J0, J1), admitted for a lane when j lies in [lo, lo + wd) and
@@ -270,7 +273,7 @@ __global__ void __launch_bounds__(256) k_lists(const double4* __restrict__ pos,
       }
       const int rem = c & (kChunk - 1);  // entries of the last, partial chunk
       if (rem)
-        *reinterpret_cast<int4*>(nbl + nbChunk(c >> 2, t, ns)) =
+        *reinterpret_cast<int4*>(nbl + nbEntry(c & ~3, t, ns)) =
             rem == 1 ? make_int4(b0, 0, 0, 0)
                      : (rem == 2 ? make_int4(b1, b0, 0, 0) : make_int4(b2, b1, b0, 0));
       nbcnt[t] = c;
@@ -386,7 +389,7 @@ __global__ void __launch_bounds__(256) k_lists_t(const double4* __restrict__ pos
     b1 = b0;
     b0 = e;
     if ((c & (kChunk - 1)) == kChunk - 1 && c < kMaxNb)
-      *reinterpret_cast<int4*>(nbl + nbChunk(c >> 2, t, ns)) = make_int4(b3, b2, b1, b0);
+      *reinterpret_cast<int4*>(nbl + nbEntry(c & ~3, t, ns)) = make_int4(b3, b2, b1, b0);
     c++;
   };
   // candidates [J0, J1) in batches of NB: bit k of acc / may for candidate
@@ -448,7 +451,7 @@ __global__ void __launch_bounds__(256) k_lists_t(const double4* __restrict__ pos
   }
   const int rem = c & (kChunk - 1);  // entries of the last, partial chunk
   if (rem)
-    *reinterpret_cast<int4*>(nbl + nbChunk(c >> 2, t, ns)) =
+    *reinterpret_cast<int4*>(nbl + nbEntry(c & ~3, t, ns)) =
         rem == 1 ? make_int4(b0, 0, 0, 0)
                  : (rem == 2 ? make_int4(b1, b0, 0, 0) : make_int4(b2, b1, b0, 0));
   nbcnt[t] = c;
@@ -467,7 +470,7 @@ __global__ void k_list_pairs(Store S, int ns, int n, const int* __restrict__ nbl
   const int ia = S.orig[a];
   long long w = off[a];
   for (int k = 0; k < cnt; k++, w++) {
-    const int e = nbl[nbChunk(k >> 2, a, ns) + (k & 3)];
+    const int e = nbl[nbEntry(k, a, ns)];
     oa[w] = ia;
     ob[w] = S.orig[e];
     oimg[w] = e >= n ? gcode[e - n] : 13;
@@ -482,7 +485,7 @@ __global__ void k_list_pairs(Store S, int ns, int n, const int* __restrict__ nbl
 #define SPHDEM_FETCH_ENTRIES(U)                                                    \
   int e[U];                                                                        \
   _Pragma("unroll") for (int v = 0; v < U / kChunk; v++) {                         \
-    const int4 ch = *reinterpret_cast<const int4*>(nbl + nbChunk((k0 >> 2) + v, a, ns)); \
+    const int4 ch = *reinterpret_cast<const int4*>(nbl + nbEntry(k0 + 4 * v, a, ns)); \
     const int ce[4] = {ch.x, ch.y, ch.z, ch.w};                                      \
     _Pragma("unroll") for (int i = 0; i < kChunk; i++) {                           \
       const int u = v * kChunk + i;                                                \
