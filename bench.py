@@ -75,7 +75,9 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=None,
                    help="timed steps (default 100; the adjoint's rollout 32)")
-    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=None,
+                   help="untimed steps before the timed ones (default 5; the ensemble 40: "
+                        "its first tens of steps in a process run slower)")
     p.add_argument("--workload", choices=("sediment", "large", "ensemble", "adjoint"),
                    default="large",
                    help="large (the headline): 160^3 fluid + 65,536 DEM in the closed "
@@ -350,9 +352,12 @@ def run_forms(a):
 
 def ensemble(a):
     """Independent boxes (sd.ensemble_cases) advanced together by
-    sphdem_run_steps_many, then the same boxes one after another (sphdem_run_steps
-    per box), both timed over `steps` steps after `warmup` steps together.
-    particle-steps/s counts fluid + DEM of every box."""
+    sphdem_run_steps_many (every step of a box launched with its reset on the
+    device and nothing read back until the box's last step), then the same
+    boxes replayed from captured HIP graphs (sphdem_run_steps_many_graph), then
+    one after another (sphdem_run_steps per box), each timed over `steps` steps
+    after `warmup` steps together.  particle-steps/s counts fluid + DEM of
+    every box."""
     prec = sd.FP64 if a.precision == "fp64" else sd.MIXED
     cases = sd.ensemble_cases(nbox=a.boxes, seed=1005)
     sims = [sd.Sim(c, device=0, precision=prec) for c in cases]
@@ -360,20 +365,22 @@ def ensemble(a):
     fluid = sum(int(np.sum(c.iam == sd.SPH)) for c in cases)
     dem = sum(int(np.sum(c.iam == sd.DEM)) for c in cases)
     wall = sum(int(np.sum(c.iam == sd.SPH_BOUNDARY)) for c in cases)
-    sd.run_many(sims, a.warmup, workers=a.workers)
-    for s in sims:
-        s.sync()
+
+    def together(steps, graph=False):
+        t0 = time.perf_counter()
+        sd.run_many(sims, steps, workers=a.workers, graph=graph)
+        for s in sims:
+            s.sync()
+        return time.perf_counter() - t0
+
+    together(a.warmup)  # each box's first step runs as launches with host resets
     untuned = None
     shapes = {}
     if not a.no_tune:
         # the default shape first, then the autotuner: one representative box
         # per power-of-two fluid-count bucket is tuned (sphdem_tune_tile_shape,
         # state restored) and its shape given to the bucket
-        t0 = time.perf_counter()
-        sd.run_many(sims, a.steps, workers=a.workers)
-        for s in sims:
-            s.sync()
-        untuned = time.perf_counter() - t0
+        untuned = together(a.steps)
         buckets = {}
         for s, c in zip(sims, cases):
             buckets.setdefault(int(math.log2(int(np.sum(c.iam == sd.SPH)))), []).append(s)
@@ -383,11 +390,12 @@ def ensemble(a):
                 s.set_tile_shape(seg, blk)
             shapes[f"fluid 2^{k}"] = {"boxes": len(group), "seg_particles": seg,
                                       "walker_block": blk, "tuned_ms_per_step": ms}
-    t0 = time.perf_counter()
-    sd.run_many(sims, a.steps, workers=a.workers)
-    for s in sims:
-        s.sync()
-    el = time.perf_counter() - t0
+        together(1)  # the first steps with the tuned shapes
+    el = together(a.steps)
+    together(1, graph=True)  # one step as launches, then each box's capture
+    graphed = together(a.steps, graph=True)
+    together(1, graph="ensemble")  # the graph over all boxes is built
+    one_graph = together(a.steps, graph="ensemble")
     t1 = time.perf_counter()
     for s in sims:
         s.run(a.steps)
@@ -427,6 +435,16 @@ def ensemble(a):
             "what": "the same boxes one after another (sphdem_run_steps per box)",
         },
         "speedup_vs_sequential": seq / el,
+        "graph": {
+            "value": counted * a.steps / graphed, "unit": "particle-steps/s",
+            "ms_per_step": 1e3 * graphed / a.steps,
+            "what": "the same boxes together, each box's step replayed from its captured "
+                    "HIP graph (sphdem_run_steps_many_graph)"},
+        "ensemble_graph": {
+            "value": counted * a.steps / one_graph, "unit": "particle-steps/s",
+            "ms_per_step": 1e3 * one_graph / a.steps,
+            "what": "the same boxes together, one graph holding every box's captured step "
+                    "launched once per ensemble step (sphdem_run_steps_ensemble_graph)"},
         "tile_shapes": shapes or None,
         "default_shape": None if untuned is None else {
             "value": counted * a.steps / untuned, "unit": "particle-steps/s",
@@ -684,6 +702,8 @@ def main():
         return
     if a.steps is None:
         a.steps = 32 if a.workload == "adjoint" else 100
+    if a.warmup is None:
+        a.warmup = 40 if a.workload == "ensemble" else 5
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(the path shards by space, see the design notes)", file=sys.stderr)

@@ -213,9 +213,17 @@ int sphdem_run_steps(sphdem_ctx* ctx, int nsteps, int hookset);
 /* Ensemble of independent boxes: advances each of the nbox contexts by nsteps
  * (as sphdem_run_steps on each) with `workers` host threads (0: min(nbox, 8)),
  * each thread stepping its share of the boxes round robin, every box on its
- * own stream, so the small boxes' kernels and host round trips overlap on the
- * device (during the call each box runs all its passes on its main stream:
- * the boxes already overlap one another).  The reference runs such ensembles as separate processes, one case
+ * own stream, so the small boxes' kernels overlap on the device (during the
+ * call each box runs all its passes on its main stream: the boxes already
+ * overlap one another).  A box that has run a step gets its steps launched
+ * with the resets in their device form (the ghost count stays on the device,
+ * and a device check stops the step whose reset needs the host: the cell grid
+ * would change, particles were deleted, more ghosts than the launched
+ * capacity, an error bit), one step of each of a thread's boxes in turn, and
+ * nothing read back until the thread has launched all of them; a stopped box
+ * is restored to the step it stopped at (from a device copy of the state kept
+ * each step: one more particle store per context) and finishes as
+ * sphdem_run_steps.  The reference runs such ensembles as separate processes, one case
  * directory per run (the run.cpp loop once per box); this replaces that loop
  * over boxes.  Every box ends exactly as sphdem_run_steps alone would leave it.
  * The contexts must be distinct and must not be used by the caller during the
@@ -224,6 +232,35 @@ int sphdem_run_steps(sphdem_ctx* ctx, int nsteps, int hookset);
  * had not failed may have advanced by fewer than nsteps steps. */
 int sphdem_run_steps_many(sphdem_ctx* const* ctxs, int nbox, int nsteps, int hookset,
                           int workers, int* failed_box);
+
+/* The same steps replayed from a captured HIP graph.  No counterpart in the
+ * reference (a CPU loop has no launch cost); results are those of
+ * sphdem_run_steps bit for bit.  One step (hooks, start, middle, end) is
+ * captured after a step run as launches, with resets that read nothing back to
+ * the host: the ghost count stays on the device, bounded by the captured
+ * ghost entries, and a device check stops a replay whose reset needs the host
+ * (the cell grid would change, particles were deleted, more ghosts than
+ * captured, an error bit).  Each replay first keeps a device copy of the state
+ * it starts from; a stopped replay's step is restored and run as launches, and
+ * the step is captured again.  Steps with a density reinitialisation run as
+ * launches.  Memory: one more copy of the particle store per context.  The
+ * graph is replayed on the context's stream with no host round trip between
+ * replays; the call waits once at its end. */
+int sphdem_run_steps_graph(sphdem_ctx* ctx, int nsteps, int hookset);
+/* sphdem_run_steps_many with each box's steps replayed from its graph: the
+ * worker threads capture where needed, then queue all replays of their boxes
+ * on the boxes' streams, then wait for each box once.  Same arguments and
+ * results as sphdem_run_steps_many. */
+int sphdem_run_steps_many_graph(sphdem_ctx* const* ctxs, int nbox, int nsteps, int hookset,
+                                int workers, int* failed_box);
+/* sphdem_run_steps_many_graph with one graph over the whole ensemble: every
+ * box's captured step a child node of one graph, launched once per ensemble
+ * step on one stream (when every box replays the same number of steps and all
+ * are on one device; otherwise as sphdem_run_steps_many_graph).  The graph is
+ * kept for the next call with the same captures.  Same arguments and results
+ * as sphdem_run_steps_many.  No counterpart in the reference. */
+int sphdem_run_steps_ensemble_graph(sphdem_ctx* const* ctxs, int nbox, int nsteps, int hookset,
+                                    int workers, int* failed_box);
 
 /* ---- parity and per-pass entry points -------------------------------------- */
 /* CdataLL::reset() (dataLL.cpp:140-236): periodic wrap, out-of-range deletion,

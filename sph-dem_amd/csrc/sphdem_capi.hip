@@ -14,6 +14,7 @@
 #include <utility>
 #include <atomic>
 #include <chrono>
+#include <mutex>
 #include <cmath>
 #include <thread>
 #include <vector>
@@ -24,6 +25,20 @@
 #include "sphdem_adjoint.hip"
 
 using namespace sphdem;
+
+// The host-side fields a step changes: kept around a capture (which runs a
+// step's host code without running the step) and set after its replays.
+struct HostFields {
+  long long sphStep;
+  int ng, ns, nd, K, lgen, countsClean, gcntZeroAt;
+  unsigned gstale;
+  bool listsValid, segsValid, gridValid;
+  double ssr;
+  Grid gs, gc;
+  GhostBand band;
+  Store cur, alt;
+  void* storeMem[2];
+};
 
 struct sphdem_ctx {
   sphdem_params P;
@@ -61,9 +76,9 @@ struct sphdem_ctx {
   uint2* lrec = nullptr;
   int* lflag = nullptr;
   int lgen = 0;
-  // DEM cells with a sphere within one cell, marked with the reset's number
+  // DEM cells with a sphere within one cell, marked with the number in nearGen
   int* demNear = nullptr;
-  int demNearCap = 0, demNearGen = 0;
+  int demNearCap = 0;
   bool listsValid = false;
   double2* hinv = nullptr;  // {1/h, 1/h^5} per SPH-group particle
   long long* okey = nullptr;  // order key of each entry, in scatter order
@@ -104,6 +119,23 @@ struct sphdem_ctx {
   std::vector<hipEvent_t> evPool;
   size_t evUsed = 0;
   char err[512] = {0};
+  // steps replayed from a captured graph (sphdem_run_steps_graph)
+  bool graphMode = false;  // capturing: the resets read nothing back
+  int graphG = 0;          // ghost entries a captured reset processes
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graphExec = nullptr;
+  std::vector<unsigned char> graphKey;  // what the captured launches depend on
+  long long graphSerial = 0;            // which capture graph holds
+  int* gate = nullptr;      // device: stop flag, the replay that set it, replays started
+  int* gateHost = nullptr;  // pinned: the gate and the live ghost count
+  Store keep{};             // the state a replayed step started from
+  void* keepMem = nullptr;
+  DevGlobals* dgKeep = nullptr;
+  int* nearGen = nullptr;     // device: the number of the DEM-cell marks
+  long long allocEpoch = 0;   // counts buffer (re)allocations
+  long long readyEpoch = -1;  // allocEpoch when a step as launches last ended
+  long long deferredStep0 = 0;  // sphStep when the deferred steps began
+  HostFields graphAfter{};    // the host fields after the captured step
 };
 
 namespace {
@@ -264,11 +296,15 @@ int allocEntries(sphdem_ctx* ctx, int64_t ghostCap) {
   ctx->okey = okey;
   ctx->lrec = lrec;
   ctx->ghostCap = ghostCap;
+  ctx->allocEpoch++;
   HIPC(hipMalloc(&ctx->gsrc, sizeof(int) * ghostCap));
   HIPC(hipMalloc(&ctx->gord, sizeof(int) * ghostCap));
   HIPC(hipMalloc(&ctx->gcell, sizeof(int) * ghostCap));
   HIPC(hipMalloc(&ctx->gorig, sizeof(int) * ghostCap));
   HIPC(hipMalloc(&ctx->gcode, sizeof(int) * ghostCap));
+  // valid entries everywhere: a captured step's passes read up to its bound
+  HIPC(hipMemsetAsync(ctx->gorig, 0, sizeof(int) * ghostCap, ctx->stream));
+  HIPC(hipMemsetAsync(ctx->gcode, 0, sizeof(int) * ghostCap, ctx->stream));
   return 0;
 }
 
@@ -315,6 +351,7 @@ int ensureKeys(sphdem_ctx* ctx, int K) {
     ctx->countsClean = 0;
     HIPC(hipMalloc(&ctx->counts, sizeof(int) * ctx->Kcap));
     HIPC(hipMalloc(&ctx->start, sizeof(int) * ctx->Kcap));
+    ctx->allocEpoch++;
   }
   return 0;
 }
@@ -328,6 +365,7 @@ int scanInts(sphdem_ctx* ctx, const int* in, int* out, int m, int* copyLast = nu
     if (ctx->tiles) HIPC(hipFree(ctx->tiles));
     ctx->tileCap = tiles + 1 + (tiles + 1) / 4;
     HIPC(hipMalloc(&ctx->tiles, sizeof(int) * ctx->tileCap));
+    ctx->allocEpoch++;
   }
   hipLaunchKernelGGL(k_scan_tile, dim3(tiles), dim3(SCAN_THREADS), 0, ctx->stream, in, out,
                      ctx->tiles, m, tiles == 1 ? copyLast : (int*)nullptr);
@@ -354,6 +392,7 @@ int buildSegments(sphdem_ctx* ctx) {
     ctx->segColCap = (ncol + 1) + (ncol + 1) / 4;
     HIPC(hipMalloc(&ctx->segCnt, sizeof(int) * ctx->segColCap));
     HIPC(hipMalloc(&ctx->segOff, sizeof(int) * ctx->segColCap));
+    ctx->allocEpoch++;
   }
   // a column holds at most ceil(n_col/64) + (nonempty cells) segments
   const int need = ctx->ns + ncol + 1;
@@ -361,6 +400,7 @@ int buildSegments(sphdem_ctx* ctx) {
     if (ctx->segs) HIPC(hipFree(ctx->segs));
     ctx->segCap = need + need / 4;
     HIPC(hipMalloc(&ctx->segs, sizeof(Segment) * ctx->segCap));
+    ctx->allocEpoch++;
   }
   // one wavefront per column; the count kernel's waves cover ncol + 1 (the
   // last one writes the scan's closing zero)
@@ -389,7 +429,8 @@ void markStale(sphdem_ctx* ctx, unsigned groups) { ctx->gstale |= groups; }
 int refreshGhosts(sphdem_ctx* ctx, unsigned need) {
   if (ctx->ng == 0 || !(ctx->gstale & need)) return 0;
   hipLaunchKernelGGL(k_ghost_refresh, dim3(blocks(ctx->ng)), dim3(TPB), 0, ctx->stream, ctx->cur,
-                     ctx->n, ctx->ng, ctx->gorig, ctx->gcode, ctx->box);
+                     ctx->n, ctx->ng, ctx->gorig, ctx->gcode, ctx->box,
+                     ctx->graphMode ? &ctx->dg->ghostTotal : (const int*)nullptr);
   LAUNCHED();
   ctx->gstale = 0;
   return 0;
@@ -411,7 +452,8 @@ int launchLists(sphdem_ctx* ctx) {
     const ListQuant lq = listQuant(ctx->gs);
     const int gen = ctx->lgen++ & 1;
     hipLaunchKernelGGL(k_lrec, dim3(blocks(ctx->ns + ctx->ng)), dim3(TPB), 0, ctx->stream,
-                       ctx->cur.posh, ctx->ns, ctx->n, ctx->ng, lq, ctx->lrec, ctx->lflag, gen);
+                       ctx->cur.posh, ctx->ns, ctx->n, ctx->ng, lq, ctx->lrec, ctx->lflag, gen,
+                       ctx->graphMode ? &ctx->dg->ghostTotal : (const int*)nullptr);
     LAUNCHED();
     const int per = (ctx->ns + 7) / 8;
     hipLaunchKernelGGL(k_lists_t<32>, dim3(8 * ((per + TPB - 1) / TPB)), dim3(TPB), 0,
@@ -591,6 +633,14 @@ int checkFlags(sphdem_ctx* ctx) {
 int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false) {
   const sphdem_params& P = ctx->P;
   const int n = ctx->n;
+  // a captured reset reads nothing back: it needs the steady state of a
+  // reset on the same grid (buffers allocated, counts cleared, the ghost
+  // count kept on the device and bounded by graphG) and leaves the checks
+  // the host would make to k_reset_gate
+  const bool dev = ctx->graphMode;
+  if (dev && (n == 0 || ctx->gcntZeroAt != n || !ctx->gridValid || ctx->countsClean < ctx->K + 1))
+    return fail(ctx, SPHDEM_E_STATE, "a captured reset needs the state a reset on the same grid "
+                                     "leaves");
   ctx->segsValid = false;
   // hmax_bits is cleared by the previous reset's k_bin (or by the upload);
   // gcnt[n], the scans' closing zero, is written by nothing else
@@ -618,25 +668,34 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false) {
     LAUNCHED();
   }
   if (pre) CHECK(scanInts(ctx, ctx->gcnt, ctx->goff, n + 1, &ctx->dg->ghostTotal));
-  // into pinned memory: a direct copy, no staging through a runtime buffer
-  HIPC(hipMemcpyAsync(ctx->hostG, ctx->dg, sizeof(DevGlobals), hipMemcpyDeviceToHost,
-                      ctx->stream));
-  const DevGlobals& hg = *ctx->hostG;
-  HIPC(hipStreamSynchronize(ctx->stream));
-  if (pre) ngPre = hg.ghostTotal;
-  const double oldSsr = ctx->ssr;
-  CHECK(flagError(ctx, hg.errflags));
-  ctx->ns = ctx->ns0 - (int)hg.ndel_sph;
-  ctx->nd = ctx->nd0 - (int)hg.ndel_dem;
-  double hmax;
-  unsigned long long hb = hg.hmax_bits;
-  memcpy(&hmax, &hb, sizeof(hmax));
-  if (ctx->ns + ctx->nd == 0) hmax = -100000;  // calcDomainLimits on an empty container
-  if (!keepSsr && ((hmax > ctx->ssr) || (hmax < 0.9 * ctx->ssr))) ctx->ssr = hmax;
-  setupGrids(ctx);
-  setupBand(ctx);
-  const bool sameGrid = ctx->gridValid && ctx->ssr == oldSsr;
-  ctx->gridValid = true;
+  bool sameGrid = true;
+  if (dev) {
+    hipLaunchKernelGGL(k_reset_gate, dim3(1), dim3(1), 0, ctx->stream, ctx->dg, ctx->gate,
+                       ctx->ssr, keepSsr ? 1 : 0,
+                       (unsigned long long)(ctx->ns0 - ctx->ns),
+                       (unsigned long long)(ctx->nd0 - ctx->nd), pre ? ctx->graphG : 0x7fffffff);
+    LAUNCHED();
+  } else {
+    // into pinned memory: a direct copy, no staging through a runtime buffer
+    HIPC(hipMemcpyAsync(ctx->hostG, ctx->dg, sizeof(DevGlobals), hipMemcpyDeviceToHost,
+                        ctx->stream));
+    const DevGlobals& hg = *ctx->hostG;
+    HIPC(hipStreamSynchronize(ctx->stream));
+    if (pre) ngPre = hg.ghostTotal;
+    const double oldSsr = ctx->ssr;
+    CHECK(flagError(ctx, hg.errflags));
+    ctx->ns = ctx->ns0 - (int)hg.ndel_sph;
+    ctx->nd = ctx->nd0 - (int)hg.ndel_dem;
+    double hmax;
+    unsigned long long hb = hg.hmax_bits;
+    memcpy(&hmax, &hb, sizeof(hmax));
+    if (ctx->ns + ctx->nd == 0) hmax = -100000;  // calcDomainLimits on an empty container
+    if (!keepSsr && ((hmax > ctx->ssr) || (hmax < 0.9 * ctx->ssr))) ctx->ssr = hmax;
+    setupGrids(ctx);
+    setupBand(ctx);
+    sameGrid = ctx->gridValid && ctx->ssr == oldSsr;
+    ctx->gridValid = true;
+  }
   for (const Grid* gr : {&ctx->gs, &ctx->gc})
     if (gr->dim[0] > 2048 || gr->dim[1] > 2048 || gr->dim[2] > 1024)
       return fail(ctx, SPHDEM_E_CAPACITY, "cell grid %d x %d x %d exceeds 2048 x 2048 x 1024",
@@ -657,7 +716,7 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false) {
   int ng = 0;
   if (ctx->gbase > 0 && n > 0) {
     if (pre && sameGrid) {
-      ng = ngPre;
+      ng = dev ? ctx->graphG : ngPre;
     } else {
       hipLaunchKernelGGL(k_ghost_count, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, ctx->cur, n,
                          ctx->gs, ctx->box, ctx->band, ctx->gcnt);
@@ -670,7 +729,8 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false) {
     if (ng > 0) {
       hipLaunchKernelGGL(k_ghost_make, dim3(blocks(n)), dim3(TPB), 0, ctx->stream, ctx->cur, n,
                          ctx->gs, ctx->box, ctx->band, ctx->goff, ctx->gbase, ctx->key, ctx->slot,
-                         ctx->counts, ctx->gsrc, ctx->gord, ctx->gcell);
+                         ctx->counts, ctx->gsrc, ctx->gord, ctx->gcell,
+                         dev ? ctx->graphG : (int)ctx->ghostCap);
       LAUNCHED();
     }
   }
@@ -681,27 +741,29 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false) {
   Side side(ctx, false);
   CHECK(buildSegments(ctx));
   side.back();
+  // a captured reset's ghost count is the device's (bounded by ng = graphG)
+  const int* ngp = dev ? &ctx->dg->ghostTotal : nullptr;
   hipLaunchKernelGGL(k_scatter, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ntot, n, ctx->key,
                      ctx->slot, ctx->start, ctx->tmp, ctx->okey, ctx->cur.orig, ctx->gsrc,
-                     ctx->gord, ctx->counts, ctx->K + 1);
+                     ctx->gord, ctx->counts, ctx->K + 1, ngp);
   LAUNCHED();
   ctx->countsClean = ctx->K + 1;
   {
     Timed tm(ctx, KID_RANK);
-    hipLaunchKernelGGL(k_rank, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ntot, ctx->key,
-                       ctx->start, ctx->tmp, ctx->okey, ctx->perm);
+    hipLaunchKernelGGL(k_rank, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ntot, n, ctx->key,
+                       ctx->start, ctx->tmp, ctx->okey, ctx->perm, ngp);
     LAUNCHED();
   }
   {
     Timed tm(ctx, KID_PERMUTE);
     hipLaunchKernelGGL(k_permute, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ntot, n,
                        ctx->perm, ctx->cur, ctx->alt, ctx->gsrc, ctx->gord, ctx->gcell, ctx->box,
-                       ctx->inv);
+                       ctx->inv, ngp);
     LAUNCHED();
   }
   if (ng > 0) {
     hipLaunchKernelGGL(k_ghost_link, dim3(blocks(ng)), dim3(TPB), 0, ctx->stream, n, ng,
-                       ctx->perm, ctx->inv, ctx->gsrc, ctx->gord, ctx->gorig, ctx->gcode);
+                       ctx->perm, ctx->inv, ctx->gsrc, ctx->gord, ctx->gorig, ctx->gcode, ngp);
     LAUNCHED();
   }
   ctx->ng = ng;
@@ -740,6 +802,7 @@ int packPosm(sphdem_ctx* ctx) {
     ctx->posmCap = 0;
     HIPC(hipMalloc(&ctx->posm, sizeof(double4) * need));
     ctx->posmCap = need;
+    ctx->allocEpoch++;
   }
   const int cnt = ctx->ns + ctx->ng;
   if (cnt > 0) {
@@ -763,6 +826,7 @@ int packMix(sphdem_ctx* ctx, unsigned groups) {
     HIPC(hipMalloc(&ctx->mixV, sizeof(float4) * need));
     HIPC(hipMalloc(&ctx->mixD, sizeof(float4) * need));
     ctx->mixCap = need;
+    ctx->allocEpoch++;
   }
   const Grid& g = ctx->gs;
   Quant Q;
@@ -912,25 +976,30 @@ int passCoupling(sphdem_ctx* ctx) {
                        ctx->P.wcon, ctx->P.coupling_radius);
     LAUNCHED();
   }
-  // the DEM cells with a sphere around them (no memset: a new mark each reset)
+  // the DEM cells with a sphere around them (no memset: the marks carry a
+  // number the device advances before each marking)
   const int cells = ctx->gc.dim[0] * ctx->gc.dim[1] * ctx->gc.dim[2];
   if (cells > ctx->demNearCap) {
+    if (ctx->graphMode)
+      return fail(ctx, SPHDEM_E_STATE, "a captured step cannot grow the DEM-cell marks");
     if (ctx->demNear) HIPC(hipFree(ctx->demNear));
     ctx->demNearCap = cells + cells / 4;
     HIPC(hipMalloc(&ctx->demNear, sizeof(int) * ctx->demNearCap));
+    ctx->allocEpoch++;
     HIPC(hipMemsetAsync(ctx->demNear, 0, sizeof(int) * ctx->demNearCap, ctx->stream));
-    ctx->demNearGen = 0;
+    HIPC(hipMemsetAsync(ctx->nearGen, 0, sizeof(int), ctx->stream));
   }
-  const int gen = ++ctx->demNearGen;
+  hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, ctx->stream, ctx->nearGen);
+  LAUNCHED();
   if (ctx->nd > 0) {
     hipLaunchKernelGGL(k_dem_near, dim3(blocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur,
-                       ctx->ns, ctx->nd, ctx->gc, ctx->demNear, gen);
+                       ctx->ns, ctx->nd, ctx->gc, ctx->demNear, ctx->nearGen);
     LAUNCHED();
   }
   Timed tm(ctx, KID_COUPLING);
   hipLaunchKernelGGL(k_coupling_sph, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur,
                      ctx->ns, ctx->nd, ctx->gc, ctx->start, ctx->box, ctx->rule, ctx->P,
-                     ctx->demNear, gen);
+                     ctx->demNear, ctx->nearGen);
   LAUNCHED();
   markStale(ctx, GS_DPO);
   return 0;
@@ -1106,6 +1175,7 @@ int hookSumDrag(sphdem_ctx* ctx) {
     if (ctx->partial) HIPC(hipFree(ctx->partial));
     ctx->partialCap = nb * 6;
     HIPC(hipMalloc(&ctx->partial, sizeof(double) * ctx->partialCap));
+    ctx->allocEpoch++;
   }
   hipLaunchKernelGGL(k_sum_drag, dim3(nb), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns + ctx->nd,
                      ctx->partial);
@@ -1125,13 +1195,20 @@ void freeAll(sphdem_ctx* ctx) {
                   ctx->gcell,       ctx->gorig,       ctx->gcode,   ctx->inv,
                   ctx->mixP,        ctx->mixV,        ctx->mixD,    ctx->okey,
                   ctx->posm,        ctx->pairCnt,     ctx->pairOff, ctx->lrec,
-                  ctx->lflag,       ctx->demNear};
+                  ctx->lflag,       ctx->demNear,     ctx->nearGen, ctx->gate,
+                  ctx->keepMem,     ctx->dgKeep};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : ctx->evPool) (void)hipEventDestroy(e);
   ctx->evPool.clear();
   if (ctx->hostG) (void)hipHostFree(ctx->hostG);
   ctx->hostG = nullptr;
+  if (ctx->gateHost) (void)hipHostFree(ctx->gateHost);
+  ctx->gateHost = nullptr;
+  if (ctx->graphExec) (void)hipGraphExecDestroy(ctx->graphExec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  ctx->graphExec = nullptr;
+  ctx->graph = nullptr;
   if (ctx->evFork) (void)hipEventDestroy(ctx->evFork);
   if (ctx->evJoin) (void)hipEventDestroy(ctx->evJoin);
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
@@ -1182,6 +1259,11 @@ int sphdem_create(const sphdem_params* params, int device, int64_t capacity, sph
     HIPC(hipMalloc(&ctx->errflag, sizeof(int)));
     HIPC(hipMalloc(&ctx->lflag, sizeof(int) * 2));
     HIPC(hipMemset(ctx->lflag, 0, sizeof(int) * 2));
+    HIPC(hipMalloc(&ctx->nearGen, sizeof(int)));
+    HIPC(hipMemset(ctx->nearGen, 0, sizeof(int)));
+    HIPC(hipMalloc(&ctx->gate, sizeof(int) * 4));
+    HIPC(hipMemset(ctx->gate, 0, sizeof(int) * 4));
+    HIPC(hipHostMalloc((void**)&ctx->gateHost, sizeof(int) * 4, hipHostMallocDefault));
     HIPC(hipMalloc(&ctx->dg, sizeof(DevGlobals)));
     HIPC(hipMemset(ctx->dg, 0, sizeof(DevGlobals)));
     HIPC(hipHostMalloc((void**)&ctx->hostG, sizeof(DevGlobals), hipHostMallocDefault));
@@ -1300,6 +1382,7 @@ int sphdem_output_vars(sphdem_ctx* ctx, sphdem_output* out) {
     if (ctx->partial) HIPC(hipFree(ctx->partial));
     ctx->partialCap = nb * OV_COUNT + OV_COUNT;
     HIPC(hipMalloc(&ctx->partial, sizeof(double) * ctx->partialCap));
+    ctx->allocEpoch++;
   }
   double* fin = ctx->partial + (size_t)nb * OV_COUNT;
   double h0[OV_COUNT], h1[OV_COUNT];
@@ -1690,14 +1773,10 @@ int sphdem_hook_sum_drag(sphdem_ctx* ctx) {
 namespace {
 // nsteps of run.cpp:102-118; the error bits of the last step's lists are left
 // to the caller's checkFlags (each reset reads those of the step before)
-int runSteps(sphdem_ctx* ctx, int nsteps, int hookset) {
-  CHECK(checkLoaded(ctx));
-  if (nsteps < 0 || hookset < SPHDEM_HOOKS_NONE || hookset > SPHDEM_HOOKS_SEDIMENT)
-    return fail(ctx, SPHDEM_E_ARG, "bad nsteps/hookset");
-  HIPC(hipSetDevice(ctx->device));
+int runOneStep(sphdem_ctx* ctx, int hookset) {
   const double g[3] = {0.0, 0.0, -9.81};
   const uint32_t sphMask = 1u << SPHDEM_SPH, demMask = 1u << SPHDEM_DEM;
-  for (int s = 0; s < nsteps; s++) {
+  {
     ctx->sphStep++;
     // start(), then the beforeMiddle gravity on sph (fused into k_start)
     if (hookset != SPHDEM_HOOKS_NONE)
@@ -1719,6 +1798,347 @@ int runSteps(sphdem_ctx* ctx, int nsteps, int hookset) {
     CHECK(stepEnd(ctx));
     if (hookset == SPHDEM_HOOKS_SEDIMENT) CHECK(hookFreeze(ctx, demMask, ctx->P.time_drop));
   }
+  // every buffer a step uses is allocated now (device resets allocate nothing)
+  if (!ctx->graphMode) ctx->readyEpoch = ctx->allocEpoch;
+  return 0;
+}
+int runSteps(sphdem_ctx* ctx, int nsteps, int hookset) {
+  CHECK(checkLoaded(ctx));
+  if (nsteps < 0 || hookset < SPHDEM_HOOKS_NONE || hookset > SPHDEM_HOOKS_SEDIMENT)
+    return fail(ctx, SPHDEM_E_ARG, "bad nsteps/hookset");
+  HIPC(hipSetDevice(ctx->device));
+  for (int s = 0; s < nsteps; s++) CHECK(runOneStep(ctx, hookset));
+  return 0;
+}
+
+// ---- steps replayed from a captured graph ----------------------------------------
+// One step (start, hooks, middle, end) is captured with the resets in their
+// device form: the host reads nothing back, the ghost count stays on the
+// device (bounded by graphG entries) and k_reset_gate makes the checks the
+// host would make.  Each replay first keeps a copy of the state it starts
+// from (k_snapshot).  A replay whose reset meets what only the host handles
+// (a grid change, deletions, more ghosts, an error bit) sets the gate; the
+// gate stays set, so later replays keep no copy, and after the replays the
+// host restores the copy and runs that step as launches.
+HostFields saveHost(const sphdem_ctx* c) {
+  HostFields h;
+  h.sphStep = c->sphStep;
+  h.ng = c->ng;
+  h.ns = c->ns;
+  h.nd = c->nd;
+  h.K = c->K;
+  h.lgen = c->lgen;
+  h.countsClean = c->countsClean;
+  h.gcntZeroAt = c->gcntZeroAt;
+  h.gstale = c->gstale;
+  h.listsValid = c->listsValid;
+  h.segsValid = c->segsValid;
+  h.gridValid = c->gridValid;
+  h.ssr = c->ssr;
+  h.gs = c->gs;
+  h.gc = c->gc;
+  h.band = c->band;
+  h.cur = c->cur;
+  h.alt = c->alt;
+  h.storeMem[0] = c->storeMem[0];
+  h.storeMem[1] = c->storeMem[1];
+  return h;
+}
+void loadHost(sphdem_ctx* c, const HostFields& h) {
+  c->sphStep = h.sphStep;
+  c->ng = h.ng;
+  c->ns = h.ns;
+  c->nd = h.nd;
+  c->K = h.K;
+  c->lgen = h.lgen;
+  c->countsClean = h.countsClean;
+  c->gcntZeroAt = h.gcntZeroAt;
+  c->gstale = h.gstale;
+  c->listsValid = h.listsValid;
+  c->segsValid = h.segsValid;
+  c->gridValid = h.gridValid;
+  c->ssr = h.ssr;
+  c->gs = h.gs;
+  c->gc = h.gc;
+  c->band = h.band;
+  c->cur = h.cur;
+  c->alt = h.alt;
+  c->storeMem[0] = h.storeMem[0];
+  c->storeMem[1] = h.storeMem[1];
+}
+// what the captured launches depend on: buffers, counts, grids, constants
+void putRaw(std::vector<unsigned char>& v, const void* x, size_t bytes) {
+  const unsigned char* b = static_cast<const unsigned char*>(x);
+  v.insert(v.end(), b, b + bytes);
+}
+#define putBytes(v, x) putRaw(v, &(x), sizeof(x))
+std::vector<unsigned char> graphKeyOf(const sphdem_ctx* c, int hookset) {
+  std::vector<unsigned char> v;
+  putBytes(v, c->allocEpoch);
+  putBytes(v, c->storeMem[0]);
+  putBytes(v, c->storeMem[1]);
+  putBytes(v, c->cur.posh);
+  const int ints[] = {c->n, c->ns, c->nd, c->ns0, c->nd0, c->K, c->gbase, hookset, c->tileSeg,
+                      c->walkTPB, c->reachShep, c->overlap ? 1 : 0};
+  putBytes(v, ints);
+  putBytes(v, c->ghostCap);
+  putBytes(v, c->ssr);
+  putBytes(v, c->gs);
+  putBytes(v, c->gc);
+  putBytes(v, c->band);
+  // field by field: these structs have padding
+  putBytes(v, c->box.periodic);
+  putBytes(v, c->box.Lp);
+  putBytes(v, c->box.Lm);
+  const CouplingRule& R = c->rule;
+  const double rd[] = {R.cmin[0], R.cmin[1], R.cmin[2], R.e, R.s, R.thr, R.rc2, R.skip2,
+                       R.gmin[0], R.gmin[1], R.gmin[2], R.gspace[0], R.gspace[1], R.gspace[2]};
+  putBytes(v, rd);
+  putBytes(v, R.nc);
+  putBytes(v, c->P);
+  return v;
+}
+#undef putBytes
+std::atomic<long long> captureSerial{0};
+void dropGraph(sphdem_ctx* c) {
+  if (c->graphExec) (void)hipGraphExecDestroy(c->graphExec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graphExec = nullptr;
+  c->graph = nullptr;
+  c->graphKey.clear();
+}
+bool reinitDue(const sphdem_ctx* c, long long step) {
+  return (step % c->P.reinit_every == 0) || (step == c->P.reinit_at);
+}
+int captureStep(sphdem_ctx* ctx, int hookset) {
+  dropGraph(ctx);
+  if (!ctx->keepMem) {
+    CHECK(allocStore(ctx, ctx->keep, ctx->keepMem, ctx->cap));
+    HIPC(hipMalloc(&ctx->dgKeep, sizeof(DevGlobals)));
+  }
+  const bool periodic = ctx->box.periodic[0] || ctx->box.periodic[1] || ctx->box.periodic[2];
+  ctx->graphG = periodic ? (int)std::min<int64_t>(ctx->ghostCap, (int64_t)ctx->ng + ctx->ng / 8 + 256)
+                         : 0;
+  const HostFields before = saveHost(ctx);
+  HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  ctx->graphMode = true;
+  hipLaunchKernelGGL(k_snapshot, dim3(blocks(ctx->n)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                     ctx->keep, ctx->n, ctx->dg, ctx->dgKeep, ctx->gate);
+  int rc = hipGetLastError() == hipSuccess ? 0 : fail(ctx, SPHDEM_E_HIP, "k_snapshot");
+  if (!rc) rc = runOneStep(ctx, hookset);
+  ctx->graphMode = false;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  const HostFields after = saveHost(ctx);
+  loadHost(ctx, before);
+  if (rc || e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc ? rc : fail(ctx, SPHDEM_E_HIP, "graph capture: %s", hipGetErrorString(e));
+  }
+  hipGraphExec_t x = nullptr;
+  if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess || !x) {
+    (void)hipGraphDestroy(g);
+    return fail(ctx, SPHDEM_E_HIP, "graph instantiation failed");
+  }
+  ctx->graph = g;
+  ctx->graphExec = x;
+  ctx->graphSerial = ++captureSerial;
+  ctx->graphAfter = after;
+  ctx->graphKey = graphKeyOf(ctx, hookset);
+  return 0;
+}
+bool graphReady(const sphdem_ctx* ctx, int hookset) {
+  return ctx->graphExec && ctx->graphKey == graphKeyOf(ctx, hookset);
+}
+// count replays on the context's stream, without waiting
+int launchReplays(sphdem_ctx* ctx, int count) {
+  HIPC(hipMemsetAsync(ctx->gate, 0, sizeof(int) * 4, ctx->stream));
+  for (int r = 0; r < count; r++) HIPC(hipGraphLaunch(ctx->graphExec, ctx->stream));
+  HIPC(hipMemcpyAsync(ctx->gateHost, ctx->gate, sizeof(int) * 3, hipMemcpyDeviceToHost,
+                      ctx->stream));
+  HIPC(hipMemcpyAsync(ctx->gateHost + 3, &ctx->dg->ghostTotal, sizeof(int),
+                      hipMemcpyDeviceToHost, ctx->stream));
+  return 0;
+}
+// after launchReplays: waits, then sets the host fields; *done = the replays
+// that completed, *stopped = whether the gate stopped one (its step not run)
+int finishReplays(sphdem_ctx* ctx, int count, int* done, bool* stopped) {
+  HIPC(hipStreamSynchronize(ctx->stream));
+  const long long step0 = ctx->sphStep;
+  *stopped = ctx->gateHost[0] != 0;
+  *done = *stopped ? ctx->gateHost[1] - 1 : count;
+  loadHost(ctx, ctx->graphAfter);
+  ctx->sphStep = step0 + *done;
+  if (!*stopped) {
+    if (ctx->gbase > 0) ctx->ng = ctx->gateHost[3];
+    return 0;
+  }
+  // back to the state the stopped step started from.  The buffers a reset
+  // rebuilds (cells, ghosts, lists) stay as the stopped step left them: the
+  // caller runs that step as launches next, whose resets rebuild them before
+  // anything reads them
+  const size_t n = (size_t)ctx->n;
+  for (int k = 0; k < kNumVec; k++)
+    HIPC(hipMemcpyAsync(storeArr(ctx->cur, k), storeArr(ctx->keep, k), sizeof(double4) * n,
+                        hipMemcpyDeviceToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(ctx->cur.iam, ctx->keep.iam, sizeof(int) * n, hipMemcpyDeviceToDevice,
+                      ctx->stream));
+  HIPC(hipMemcpyAsync(ctx->cur.orig, ctx->keep.orig, sizeof(int) * n, hipMemcpyDeviceToDevice,
+                      ctx->stream));
+  HIPC(hipMemcpyAsync(ctx->cur.cell, ctx->keep.cell, sizeof(int) * n, hipMemcpyDeviceToDevice,
+                      ctx->stream));
+  HIPC(hipMemcpyAsync(ctx->dg, ctx->dgKeep, sizeof(DevGlobals), hipMemcpyDeviceToDevice,
+                      ctx->stream));
+  ctx->listsValid = false;
+  ctx->ng = 0;
+  dropGraph(ctx);
+  return 0;
+}
+// Steps launched one after another with the resets in their device form,
+// nothing read back until finishReplays (the same gate, copy and recovery as
+// the replays; without a capture, parity of the stores and reinitialisation
+// steps need no special care).  The context must be in the state a reset
+// leaves (any step run as launches).
+bool deviceResetReady(const sphdem_ctx* c) {
+  return c->n > 0 && c->gridValid && c->gcntZeroAt == c->n && c->countsClean >= c->K + 1 &&
+         c->readyEpoch == c->allocEpoch;
+}
+// beginDeferred, then deferredStep per step (the steps of several contexts
+// may interleave), then endDeferred, then finishReplays
+int beginDeferred(sphdem_ctx* ctx) {
+  if (!ctx->keepMem) {
+    CHECK(allocStore(ctx, ctx->keep, ctx->keepMem, ctx->cap));
+    HIPC(hipMalloc(&ctx->dgKeep, sizeof(DevGlobals)));
+  }
+  const bool periodic = ctx->box.periodic[0] || ctx->box.periodic[1] || ctx->box.periodic[2];
+  ctx->graphG = periodic ? (int)std::min<int64_t>(ctx->ghostCap, (int64_t)ctx->ng + ctx->ng / 8 + 256)
+                         : 0;
+  HIPC(hipMemsetAsync(ctx->gate, 0, sizeof(int) * 4, ctx->stream));
+  ctx->deferredStep0 = ctx->sphStep;
+  return 0;
+}
+int deferredStep(sphdem_ctx* ctx, int hookset) {
+  ctx->graphMode = true;
+  hipLaunchKernelGGL(k_snapshot, dim3(blocks(ctx->n)), dim3(TPB), 0, ctx->stream, ctx->cur,
+                     ctx->keep, ctx->n, ctx->dg, ctx->dgKeep, ctx->gate);
+  int rc = hipGetLastError() == hipSuccess ? 0 : fail(ctx, SPHDEM_E_HIP, "k_snapshot");
+  if (!rc) rc = runOneStep(ctx, hookset);
+  ctx->graphMode = false;
+  return rc;
+}
+int endDeferred(sphdem_ctx* ctx) {
+  ctx->graphAfter = saveHost(ctx);
+  ctx->sphStep = ctx->deferredStep0;  // finishReplays counts the steps that completed
+  HIPC(hipMemcpyAsync(ctx->gateHost, ctx->gate, sizeof(int) * 3, hipMemcpyDeviceToHost,
+                      ctx->stream));
+  HIPC(hipMemcpyAsync(ctx->gateHost + 3, &ctx->dg->ghostTotal, sizeof(int),
+                      hipMemcpyDeviceToHost, ctx->stream));
+  return 0;
+}
+
+// One graph holding the captured step of every box of an ensemble as a child
+// node, replayed with one launch per ensemble step on one stream.  It is kept
+// for the next call that passes the same captures (each capture has a serial
+// number no later capture repeats, so a box destroyed or captured again
+// never matches) and rebuilt otherwise.
+struct EnsembleGraph {
+  std::vector<long long> key;  // the device, then each replaying box's capture serial
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t stream = nullptr;
+  int device = -1;
+};
+std::mutex ensembleMutex;
+EnsembleGraph ensembleGraph;
+
+// the replays of launchReplays for every box with count[b] > 0 (all the same
+// count, all on one device), queued on the ensemble graph's stream; each
+// box's gate copied back as launchReplays copies it.  The boxes' own streams
+// are drained first, and finishReplays' wait on a box's stream is then
+// preceded by the wait on the ensemble stream here.
+int launchEnsemble(sphdem_ctx* const* ctxs, int nbox, const int* count, int replays) {
+  std::lock_guard<std::mutex> lock(ensembleMutex);
+  EnsembleGraph& E = ensembleGraph;
+  const int device = ctxs[0]->device;
+  std::vector<long long> key{device};
+  for (int b = 0; b < nbox; b++)
+    if (count[b]) key.push_back(ctxs[b]->graphSerial);
+  sphdem_ctx* c0 = nullptr;
+  for (int b = 0; b < nbox && !c0; b++)
+    if (count[b]) c0 = ctxs[b];
+  sphdem_ctx* const ctx = c0;  // where HIPC reports
+  if (E.device != device || key != E.key) {
+    if (E.exec) (void)hipGraphExecDestroy(E.exec);
+    if (E.graph) (void)hipGraphDestroy(E.graph);
+    E.exec = nullptr;
+    E.graph = nullptr;
+    E.key.clear();
+    if (E.stream && E.device != device) {
+      (void)hipSetDevice(E.device);
+      (void)hipStreamDestroy(E.stream);
+      E.stream = nullptr;
+    }
+    if (hipSetDevice(device) != hipSuccess) return fail(c0, SPHDEM_E_HIP, "hipSetDevice");
+    E.device = device;
+    if (!E.stream && hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking) != hipSuccess)
+      return fail(c0, SPHDEM_E_HIP, "ensemble stream");
+    if (hipGraphCreate(&E.graph, 0) != hipSuccess) return fail(c0, SPHDEM_E_HIP, "hipGraphCreate");
+    for (int b = 0; b < nbox; b++) {
+      if (!count[b]) continue;
+      hipGraphNode_t node;
+      if (hipGraphAddChildGraphNode(&node, E.graph, nullptr, 0, ctxs[b]->graph) != hipSuccess)
+        return fail(ctxs[b], SPHDEM_E_HIP, "ensemble graph: child node");
+    }
+    if (hipGraphInstantiate(&E.exec, E.graph, nullptr, nullptr, 0) != hipSuccess || !E.exec) {
+      E.exec = nullptr;
+      return fail(c0, SPHDEM_E_HIP, "ensemble graph instantiation failed");
+    }
+    E.key = key;
+  }
+  HIPC(hipSetDevice(device));
+  for (int b = 0; b < nbox; b++) {
+    if (!count[b]) continue;
+    HIPC(hipStreamSynchronize(ctxs[b]->stream));
+    HIPC(hipMemsetAsync(ctxs[b]->gate, 0, sizeof(int) * 4, E.stream));
+  }
+  for (int r = 0; r < replays; r++) HIPC(hipGraphLaunch(E.exec, E.stream));
+  for (int b = 0; b < nbox; b++) {
+    if (!count[b]) continue;
+    sphdem_ctx* c = ctxs[b];
+    HIPC(hipMemcpyAsync(c->gateHost, c->gate, sizeof(int) * 3, hipMemcpyDeviceToHost, E.stream));
+    HIPC(hipMemcpyAsync(c->gateHost + 3, &c->dg->ghostTotal, sizeof(int), hipMemcpyDeviceToHost,
+                        E.stream));
+  }
+  HIPC(hipStreamSynchronize(E.stream));
+  return 0;
+}
+
+int graphSteps(sphdem_ctx* ctx, int nsteps, int hookset) {
+  int done = 0;
+  while (done < nsteps) {
+    if (reinitDue(ctx, ctx->sphStep + 1)) {
+      CHECK(runOneStep(ctx, hookset));
+      done++;
+      continue;
+    }
+    if (!graphReady(ctx, hookset)) {
+      CHECK(runOneStep(ctx, hookset));  // the host step a capture follows
+      done++;
+      if (done < nsteps && !reinitDue(ctx, ctx->sphStep + 1)) CHECK(captureStep(ctx, hookset));
+      continue;
+    }
+    int count = 0;
+    while (done + count < nsteps && !reinitDue(ctx, ctx->sphStep + 1 + count)) count++;
+    int ok = 0;
+    bool stopped = false;
+    CHECK(launchReplays(ctx, count));
+    CHECK(finishReplays(ctx, count, &ok, &stopped));
+    done += ok;
+    if (stopped) {
+      CHECK(runOneStep(ctx, hookset));
+      done++;
+    }
+  }
   return 0;
 }
 }  // namespace
@@ -1726,6 +2146,134 @@ int runSteps(sphdem_ctx* ctx, int nsteps, int hookset) {
 int sphdem_run_steps(sphdem_ctx* ctx, int nsteps, int hookset) {
   CHECK(runSteps(ctx, nsteps, hookset));
   return checkFlags(ctx);
+}
+
+int sphdem_run_steps_graph(sphdem_ctx* ctx, int nsteps, int hookset) {
+  CHECK(checkLoaded(ctx));
+  if (nsteps < 0 || hookset < SPHDEM_HOOKS_NONE || hookset > SPHDEM_HOOKS_SEDIMENT)
+    return fail(ctx, SPHDEM_E_ARG, "bad nsteps/hookset");
+  HIPC(hipSetDevice(ctx->device));
+  // a timing window brackets launches one by one: those steps run as launches
+  if (ctx->timedKernel >= 0) return sphdem_run_steps(ctx, nsteps, hookset);
+  const bool ov = ctx->overlap;
+  ctx->overlap = false;  // one stream: the captured step is one chain
+  const int rc = graphSteps(ctx, nsteps, hookset);
+  ctx->overlap = ov;
+  CHECK(rc);
+  return checkFlags(ctx);
+}
+
+namespace {
+int manyGraph(sphdem_ctx* const* ctxs, int nbox, int nsteps, int hookset, int workers,
+              int* failed_box, bool oneGraph) {
+  if (failed_box) *failed_box = -1;
+  if (nbox < 0 || nsteps < 0 || workers < 0 || (nbox > 0 && !ctxs)) return SPHDEM_E_ARG;
+  if (hookset < SPHDEM_HOOKS_NONE || hookset > SPHDEM_HOOKS_SEDIMENT) return SPHDEM_E_ARG;
+  if (nbox == 0 || nsteps == 0) return 0;
+  std::vector<sphdem_ctx*> order(ctxs, ctxs + nbox);
+  std::sort(order.begin(), order.end());
+  if (!order[0] || std::adjacent_find(order.begin(), order.end()) != order.end())
+    return SPHDEM_E_ARG;  // a null or a repeated context
+  for (int b = 0; b < nbox; b++)
+    if (!ctxs[b]->loaded) return fail(ctxs[b], SPHDEM_E_STATE, "no particles uploaded");
+  if (workers == 0) workers = 8;
+  workers = std::min(workers, nbox);
+  std::vector<bool> overlap(nbox);
+  for (int b = 0; b < nbox; b++) {
+    overlap[b] = ctxs[b]->overlap;
+    ctxs[b]->overlap = false;
+  }
+  std::atomic<int> err{0};
+  std::atomic<int> errBox{-1};
+  // workers take every workers-th box: a host step and a capture where a box
+  // has no usable graph.  Then the replays: with oneGraph, and every box
+  // replaying the same number of steps, one graph holding all the boxes'
+  // captures is launched once per step on one stream; otherwise each box's
+  // replays are queued on its own stream.  Then the waits (and, for a box
+  // whose gate stopped a replay, the rest of its steps through graphSteps)
+  std::vector<int> done(nbox, 0), count(nbox, 0);
+  auto fault = [&](int b, int rc) {
+    int none = 0;
+    if (err.compare_exchange_strong(none, rc)) errBox.store(b);
+  };
+  auto parallel = [&](auto&& body) {
+    std::vector<std::thread> pool;
+    pool.reserve(workers - 1);
+    auto run = [&](int w) {
+      for (int b = w; b < nbox && !err.load(); b += workers) {
+        const int rc = hipSetDevice(ctxs[b]->device) == hipSuccess ? body(b) : SPHDEM_E_HIP;
+        if (rc) return fault(b, rc);
+      }
+    };
+    for (int w = 1; w < workers; w++) pool.emplace_back(run, w);
+    run(0);
+    for (std::thread& t : pool) t.join();
+  };
+  parallel([&](int b) {
+    sphdem_ctx* c = ctxs[b];
+    bool plain = false;  // a reinit step in the window: this box runs step by step
+    for (int k = 1; k <= nsteps && !plain; k++) plain = reinitDue(c, c->sphStep + k);
+    int rc = 0;
+    if (plain) {
+      rc = graphSteps(c, nsteps, hookset);
+      done[b] = nsteps;
+    } else if (!graphReady(c, hookset)) {
+      rc = runOneStep(c, hookset);
+      done[b] = 1;
+      if (!rc && nsteps > 1) rc = captureStep(c, hookset);
+    }
+    if (!rc) count[b] = nsteps - done[b];
+    return rc;
+  });
+  int together = oneGraph ? -1 : 0;  // the replay count every replaying box shares
+  for (int b = 0; b < nbox && together != 0; b++) {
+    if (count[b] == 0) continue;
+    if (together == -1) together = count[b];
+    if (count[b] != together || ctxs[b]->device != ctxs[0]->device) together = 0;
+  }
+  if (!err.load() && together > 0) {
+    const int rc = launchEnsemble(ctxs, nbox, count.data(), together);
+    if (rc) fault(0, rc);
+  } else if (!err.load()) {
+    parallel([&](int b) { return count[b] ? launchReplays(ctxs[b], count[b]) : 0; });
+  }
+  if (!err.load())
+    parallel([&](int b) {
+      if (count[b] == 0) return 0;
+      sphdem_ctx* c = ctxs[b];
+      int ok = 0;
+      bool stopped = false;
+      int rc = finishReplays(c, count[b], &ok, &stopped);
+      if (!rc && stopped) rc = graphSteps(c, nsteps - done[b] - ok, hookset);
+      return rc;
+    });
+  for (int b = 0; b < nbox; b++) ctxs[b]->overlap = overlap[b];
+  for (int b = 0; b < nbox && !err.load(); b++) {
+    sphdem_ctx* ctx = ctxs[b];
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+      err.store(SPHDEM_E_HIP);
+      errBox.store(b);
+      break;
+    }
+    const int rc = checkFlags(ctx);
+    if (rc) {
+      err.store(rc);
+      errBox.store(b);
+    }
+  }
+  if (failed_box) *failed_box = errBox.load();
+  return err.load();
+}
+}  // namespace
+
+int sphdem_run_steps_many_graph(sphdem_ctx* const* ctxs, int nbox, int nsteps, int hookset,
+                                int workers, int* failed_box) {
+  return manyGraph(ctxs, nbox, nsteps, hookset, workers, failed_box, false);
+}
+
+int sphdem_run_steps_ensemble_graph(sphdem_ctx* const* ctxs, int nbox, int nsteps, int hookset,
+                                    int workers, int* failed_box) {
+  return manyGraph(ctxs, nbox, nsteps, hookset, workers, failed_box, true);
 }
 
 int sphdem_run_steps_many(sphdem_ctx* const* ctxs, int nbox, int nsteps, int hookset,
@@ -1748,17 +2296,57 @@ int sphdem_run_steps_many(sphdem_ctx* const* ctxs, int nbox, int nsteps, int hoo
   }
   std::atomic<int> err{0};
   std::atomic<int> errBox{-1};
+  // each worker takes every workers-th box.  A box whose resets can run on
+  // the device (any box that has run a step) gets its steps launched with
+  // nothing read back, the worker launching one step of each of its boxes in
+  // turn, and is waited for once at the end; a box that has not yet run a
+  // step runs its first one as launches.  A box whose device reset met what
+  // only the host handles is rolled back to the step it stopped at and
+  // finishes on the host.
   auto work = [&](int w) {
-    for (int s = 0; s < nsteps; s++)
-      for (int b = w; b < nbox; b += workers) {
-        if (err.load(std::memory_order_relaxed)) return;
-        const int rc = runSteps(ctxs[b], 1, hookset);
-        if (rc) {
-          int none = 0;
-          if (err.compare_exchange_strong(none, rc)) errBox.store(b);
-          return;
-        }
+    std::vector<int> mine, done, count;
+    for (int b = w; b < nbox; b += workers) mine.push_back(b);
+    done.assign(mine.size(), 0);
+    count.assign(mine.size(), 0);
+    auto fault = [&](int b, int rc) {
+      int none = 0;
+      if (err.compare_exchange_strong(none, rc)) errBox.store(b);
+    };
+    for (size_t i = 0; i < mine.size() && !err.load(); i++) {
+      sphdem_ctx* c = ctxs[mine[i]];
+      int rc = hipSetDevice(c->device) == hipSuccess ? 0 : SPHDEM_E_HIP;
+      if (!rc) rc = checkLoaded(c);
+      if (!rc && (hookset < SPHDEM_HOOKS_NONE || hookset > SPHDEM_HOOKS_SEDIMENT))
+        rc = fail(c, SPHDEM_E_ARG, "bad nsteps/hookset");
+      if (!rc && !deviceResetReady(c)) {
+        rc = runOneStep(c, hookset);
+        done[i] = 1;
       }
+      if (!rc && done[i] < nsteps) {
+        count[i] = nsteps - done[i];
+        rc = beginDeferred(c);
+      }
+      if (rc) return fault(mine[i], rc);
+    }
+    for (int r = 0; r < nsteps && !err.load(); r++)
+      for (size_t i = 0; i < mine.size() && !err.load(); i++) {
+        if (r >= count[i]) continue;
+        sphdem_ctx* c = ctxs[mine[i]];
+        int rc = hipSetDevice(c->device) == hipSuccess ? 0 : SPHDEM_E_HIP;
+        if (!rc) rc = deferredStep(c, hookset);
+        if (!rc && r + 1 == count[i]) rc = endDeferred(c);
+        if (rc) return fault(mine[i], rc);
+      }
+    for (size_t i = 0; i < mine.size() && !err.load(); i++) {
+      if (count[i] == 0) continue;
+      sphdem_ctx* c = ctxs[mine[i]];
+      int ok = 0;
+      bool stopped = false;
+      int rc = hipSetDevice(c->device) == hipSuccess ? 0 : SPHDEM_E_HIP;
+      if (!rc) rc = finishReplays(c, count[i], &ok, &stopped);
+      if (!rc && stopped) rc = runSteps(c, count[i] - ok, hookset);
+      if (rc) return fault(mine[i], rc);
+    }
   };
   std::vector<std::thread> pool;
   pool.reserve(workers - 1);

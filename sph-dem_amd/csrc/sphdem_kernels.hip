@@ -296,11 +296,13 @@ __global__ void k_ghost_count(Store S, int n, Grid g, Box B, GhostBand gb,
 // combination of shifts in receive order.  The 26 combinations are walked
 // wave-uniformly, so neighbouring lanes with the same combination (mostly the
 // same ghost cell: the store is sorted by cell) share one atomic per run.
+// Ghosts beyond `cap` entries are left out (counted nowhere): a reset that
+// cannot read the count back bounds them by the ghost entries it processes.
 __global__ void k_ghost_make(Store S, int n, Grid g, Box B, GhostBand gb,
                              const int* __restrict__ goff, int gbase, int* __restrict__ key,
                              int* __restrict__ slot, int* __restrict__ counts,
                              int* __restrict__ gsrc, int* __restrict__ gord,
-                             int* __restrict__ gcell) {
+                             int* __restrict__ gcell, int cap) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int cnt = i < n ? goff[i + 1] - goff[i] : 0;
   if (__ballot(cnt > 0) == 0ull) return;  // wave-uniform: no lane has ghosts
@@ -318,7 +320,8 @@ __global__ void k_ghost_make(Store S, int n, Grid g, Box B, GhostBand gb,
     if (ord == 13) continue;  // no shift: the particle itself
     const int sx = ord / 9 - 1, sy = (ord / 3) % 3 - 1, sz = ord % 3 - 1;
     const bool on = cnt > 0 && (sx == 0 || ((mx >> (sx > 0)) & 1)) &&
-                    (sy == 0 || ((my >> (sy > 0)) & 1)) && (sz == 0 || ((mz >> (sz > 0)) & 1));
+                    (sy == 0 || ((my >> (sy > 0)) & 1)) && (sz == 0 || ((mz >> (sz > 0)) & 1)) &&
+                    e - n < cap;
     int k = -1, X = 0, Y = 0, Z = 0;
     if (on) {
       X = sx == 0 ? cellX(rc) : cx[sx > 0];
@@ -421,15 +424,23 @@ __device__ __forceinline__ long long orderKey(int e, int n, const int* __restric
   if (e < n) return orig[e];
   return ((long long)gord[e - n] << 32) | (unsigned)orig[gsrc[e - n]];
 }
+// Entries [0, nreal + live ghosts) of a launch over ncap = nreal + ghost
+// capacity entries: the live ghost count is read from the device (ngp) when
+// the host has not read it back, else the launch count is exact.
+__device__ __forceinline__ int liveEntries(int ncap, int nreal, const int* __restrict__ ngp) {
+  return ngp ? nreal + min(*ngp, ncap - nreal) : ncap;
+}
 // Scatter of the entries into their cells' ranges (any order within a cell),
 // with each entry's order key beside it.  Also clears the cell counts
 // [0, k1) for the next reset (the scan has consumed them).
-__global__ void k_scatter(int n, int nreal, const int* __restrict__ key,
+__global__ void k_scatter(int ncap, int nreal, const int* __restrict__ key,
                           const int* __restrict__ slot, const int* __restrict__ start,
                           int* __restrict__ tmp, long long* __restrict__ okey,
                           const int* __restrict__ orig, const int* __restrict__ gsrc,
-                          const int* __restrict__ gord, int* __restrict__ counts, int k1) {
+                          const int* __restrict__ gord, int* __restrict__ counts, int k1,
+                          const int* __restrict__ ngp) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = liveEntries(ncap, nreal, ngp);
   if (i < n) {
     const int q = start[key[i]] + slot[i];
     tmp[q] = i;
@@ -439,11 +450,12 @@ __global__ void k_scatter(int n, int nreal, const int* __restrict__ key,
 }
 
 // Each entry's rank is the number of cell members with a smaller order key.
-__global__ void k_rank(int ntot, const int* __restrict__ key, const int* __restrict__ start,
-                       const int* __restrict__ tmp, const long long* __restrict__ okey,
-                       int* __restrict__ perm) {
+__global__ void k_rank(int ncap, int nreal, const int* __restrict__ key,
+                       const int* __restrict__ start, const int* __restrict__ tmp,
+                       const long long* __restrict__ okey, int* __restrict__ perm,
+                       const int* __restrict__ ngp) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= ntot) return;
+  if (p >= liveEntries(ncap, nreal, ngp)) return;
   const int i = tmp[p];
   const int k = key[i];
   const int s0 = start[k], s1 = start[k + 1];
@@ -462,11 +474,12 @@ __device__ __forceinline__ void ordShift(int ord, int& sx, int& sy, int& sz) {
   sy = (ord / 3) % 3 - 1;
   sz = ord % 3 - 1;
 }
-__global__ void k_permute(int ntot, int n, const int* __restrict__ perm, Store src, Store dst,
+__global__ void k_permute(int ncap, int n, const int* __restrict__ perm, Store src, Store dst,
                           const int* __restrict__ gsrc, const int* __restrict__ gord,
-                          const int* __restrict__ gcell, Box B, int* __restrict__ inv) {
+                          const int* __restrict__ gcell, Box B, int* __restrict__ inv,
+                          const int* __restrict__ ngp) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= ntot) return;
+  if (p >= liveEntries(ncap, n, ngp)) return;
   const int e = perm[p];
   int i = e;
   if (e < n) {
@@ -507,9 +520,9 @@ __global__ void k_permute(int ntot, int n, const int* __restrict__ perm, Store s
 __global__ void k_ghost_link(int n, int ng, const int* __restrict__ perm,
                              const int* __restrict__ inv, const int* __restrict__ gsrc,
                              const int* __restrict__ gord, int* __restrict__ gorig,
-                             int* __restrict__ gcode) {
+                             int* __restrict__ gcode, const int* __restrict__ ngp) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ng) return;
+  if (g >= liveEntries(ng, 0, ngp)) return;
   const int e = perm[n + g];
   gorig[g] = inv[gsrc[e - n]];
   int sx, sy, sz;
@@ -520,9 +533,10 @@ __global__ void k_ghost_link(int n, int ng, const int* __restrict__ perm,
 // pdr2, omega, porosity) refreshed from their origins: the values a
 // syncParticlesBetweenProcs would ship.
 __global__ void k_ghost_refresh(Store S, int n, int ng, const int* __restrict__ gorig,
-                                const int* __restrict__ gcode, Box B) {
+                                const int* __restrict__ gcode, Box B,
+                                const int* __restrict__ ngp) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ng) return;
+  if (g >= liveEntries(ng, 0, ngp)) return;
   const int o = gorig[g];
   const int code = gcode[g];
   const double4 q = S.posh[o];
@@ -530,6 +544,54 @@ __global__ void k_ghost_refresh(Store S, int n, int ng, const int* __restrict__ 
   S.posh[n + g] = make_double4(im.x, im.y, im.z, q.w);
   S.vhm[n + g] = S.vhm[o];
   S.dpo[n + g] = S.dpo[o];
+}
+
+// ---- a reset without host round trips (steps replayed from a graph) -------------
+// gate[0]: set once a replayed step's reset finds what only the host can
+// handle (its grid would change: hmax outside [0.9 ssr, ssr]; particles were
+// deleted; more ghosts than the step processes; an error flag); it stays set,
+// and gate[1] keeps gate[2], the replayed step's number, at that moment.
+__global__ void k_reset_gate(const DevGlobals* __restrict__ g, int* __restrict__ gate, double ssr,
+                             int keepSsr, unsigned long long ndelSph, unsigned long long ndelDem,
+                             int ghostCap) {
+  if (gate[0]) return;
+  const double hmax = __longlong_as_double((long long)g->hmax_bits);
+  bool stop = g->errflags != 0u || g->ndel_sph != ndelSph || g->ndel_dem != ndelDem ||
+              g->ghostTotal > ghostCap;
+  if (!keepSsr && (hmax > ssr || hmax < 0.9 * ssr)) stop = true;
+  if (stop) {
+    gate[0] = 1;
+    gate[1] = gate[2];
+  }
+}
+// The state a replayed step starts from (every stream of the real entries
+// and the device globals), kept until a later replay starts cleanly; the
+// first thread also counts the replays.  Nothing is copied once the gate is
+// set, so the copy holds the state before the step that set it.
+__global__ void k_snapshot(Store src, Store dst, int n, const DevGlobals* __restrict__ g,
+                           DevGlobals* __restrict__ gkeep, int* __restrict__ gate) {
+  if (gate[0]) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    *gkeep = *g;
+    gate[2]++;
+  }
+  if (i >= n) return;
+  dst.posh[i] = src.posh[i];
+  dst.vhm[i] = src.vhm[i];
+  dst.dpo[i] = src.dpo[i];
+  dst.vel[i] = src.vel[i];
+  dst.v0p[i] = src.v0p[i];
+  dst.vh0d[i] = src.vh0d[i];
+  dst.frc[i] = src.frc[i];
+  dst.fp[i] = src.fp[i];
+  dst.fv[i] = src.fv[i];
+  dst.fb[i] = src.fb[i];
+  dst.fdrag[i] = src.fdrag[i];
+  dst.fvel[i] = src.fvel[i];
+  dst.iam[i] = src.iam[i];
+  dst.orig[i] = src.orig[i];
+  dst.cell[i] = src.cell[i];
 }
 
 // ============================================================ SPH pair passes
@@ -799,9 +861,12 @@ __global__ void __launch_bounds__(256) k_shepsum(Store S, int ns, int nd, Grid g
 // DEM cells with a DEM particle within one cell (the 3x3x3 block around them):
 // near[c] == gen.  A coupling search centred on any other cell of the DEM grid
 // finds no candidate and is skipped.
-__global__ void k_dem_near(Store S, int ns, int nd, Grid gc, int* __restrict__ near, int gen) {
+__global__ void k_bump(int* __restrict__ p) { ++*p; }
+__global__ void k_dem_near(Store S, int ns, int nd, Grid gc, int* __restrict__ near,
+                           const int* __restrict__ genp) {
   const int j = ns + blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= ns + nd || S.iam[j] != T_DEM) return;
+  const int gen = *genp;
   const int c = S.cell[j];
   const int x = c & 2047, y = (c >> 11) & 2047, z = cellZ(c);
   for (int a = max(x - 1, 0); a <= min(x + 1, gc.dim[0] - 1); a++)
@@ -812,9 +877,10 @@ __global__ void k_dem_near(Store S, int ns, int nd, Grid gc, int* __restrict__ n
 
 __global__ void k_coupling_sph(Store S, int ns, int nd, Grid gc, const int* __restrict__ start,
                                Box B, CouplingRule R, sphdem_params P,
-                               const int* __restrict__ near, int gen) {
+                               const int* __restrict__ near, const int* __restrict__ genp) {
   const int a = xcdThread();
   if (a >= ns || !isSphT(S.iam[a])) return;
+  const int gen = *genp;
   const double4 pa = S.posh[a];
   const double Rc = P.coupling_radius;
   const double rdens = 1.0 / S.dpo[a].x;

@@ -320,10 +320,11 @@ __device__ __forceinline__ unsigned q16(double x, double o, double uinv) {
 // flag slot of generation gen collects non-finite input; the next
 // generation's slot is cleared for the next build.
 __global__ void k_lrec(const double4* __restrict__ posh, int ns, int n, int ng, ListQuant lq,
-                       uint2* __restrict__ rec, int* __restrict__ lflag, int gen) {
+                       uint2* __restrict__ rec, int* __restrict__ lflag, int gen,
+                       const int* __restrict__ ngp) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) lflag[(gen + 1) & 1] = 0;
-  if (i >= ns + ng) return;
+  if (i >= ns + (ngp ? min(*ngp, ng) : ng)) return;
   const int s = i < ns ? i : n + (i - ns);
   const double4 p = posh[s];
   const bool ok = isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(p.w) && p.w > 0.0;

@@ -629,6 +629,9 @@ def load_library(path: str = LIB_PATH):
     lib.sphdem_run_steps_many.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_int)]
+    lib.sphdem_run_steps_graph.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    lib.sphdem_run_steps_many_graph.argtypes = lib.sphdem_run_steps_many.argtypes
+    lib.sphdem_run_steps_ensemble_graph.argtypes = lib.sphdem_run_steps_many.argtypes
     lib.sphdem_get_cell_coords.argtypes = [vp, ip]
     lib.sphdem_get_pairs.argtypes = [vp, ctypes.c_int, ctypes.POINTER(i64), ip, ip, ip, i64]
     lib.sphdem_get_pairs_device.argtypes = [vp, ctypes.c_int, ctypes.POINTER(i64), vp, vp, vp,
@@ -647,9 +650,15 @@ def load_library(path: str = LIB_PATH):
     return lib
 
 
-def run_many(sims: list, nsteps: int, hookset: int | None = None, workers: int = 0):
-    """Advances independent boxes together (sphdem_run_steps_many): each Sim by
-    nsteps steps, `workers` host threads stepping their share of the boxes."""
+def run_many(sims: list, nsteps: int, hookset: int | None = None, workers: int = 0,
+             graph: bool | str = False):
+    """Advances independent boxes together (sphdem_run_steps_many; with
+    graph=True sphdem_run_steps_many_graph, each box replaying its own graph;
+    with graph="ensemble" sphdem_run_steps_ensemble_graph, one graph over all
+    boxes): each Sim by nsteps steps, `workers` host threads stepping their
+    share of the boxes."""
+    if graph not in (False, True, "ensemble"):
+        raise ValueError(f"graph must be False, True or 'ensemble', not {graph!r}")
     if not sims:
         return
     lib = load_library()
@@ -659,13 +668,14 @@ def run_many(sims: list, nsteps: int, hookset: int | None = None, workers: int =
             raise ValueError("boxes with different hook sets need an explicit hookset")
     arr = (ctypes.c_void_p * len(sims))(*[s.ctx.value for s in sims])
     failed = ctypes.c_int(-1)
-    rc = lib.sphdem_run_steps_many(arr, len(sims), nsteps, hookset, workers,
-                                   ctypes.byref(failed))
+    fn = (lib.sphdem_run_steps_ensemble_graph if graph == "ensemble" else
+          lib.sphdem_run_steps_many_graph if graph else lib.sphdem_run_steps_many)
+    rc = fn(arr, len(sims), nsteps, hookset, workers, ctypes.byref(failed))
     if rc != 0:
         msg = b""
         if failed.value >= 0:
             msg = lib.sphdem_last_error(sims[failed.value].ctx) or b""
-        raise SphDemError(f"sphdem_run_steps_many failed ({rc}) in box {failed.value}: "
+        raise SphDemError(f"{fn.__name__} failed ({rc}) in box {failed.value}: "
                           f"{msg.decode(errors='replace')}")
 
 
@@ -769,6 +779,11 @@ class Sim:
     def run(self, nsteps: int, hookset: int | None = None):
         hs = self.case.hookset if hookset is None else hookset
         self._check(self.lib.sphdem_run_steps(self.ctx, nsteps, hs))
+
+    def run_graph(self, nsteps: int, hookset: int | None = None):
+        """sphdem_run_steps_graph: the same steps replayed from a captured graph."""
+        hs = self.case.hookset if hookset is None else hookset
+        self._check(self.lib.sphdem_run_steps_graph(self.ctx, nsteps, hs))
 
     def hook_body_force(self, type_mask: int, accel):
         a = np.ascontiguousarray(accel, dtype=np.float64)
