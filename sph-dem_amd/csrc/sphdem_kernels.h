@@ -82,17 +82,18 @@ struct GhostBand {
   int lo[3], hi[3];
 };
 
-// Global thread index with the blocks of a 1-D grid regrouped by XCD.  The
+// Block index (xcdBlock) and global thread index (xcdThread) with the blocks
+// of a 1-D grid regrouped by XCD.  The
 // dispatcher deals workgroups round robin to the 8 XCDs (block b to XCD b % 8);
 // here XCD x runs the x-th contiguous eighth of the blocks, in order, so the
 // neighbour gathers of a cell-sorted particle range stay within that XCD's L2
 // instead of every XCD streaming the whole domain.  A bijection on the grid.
-__device__ __forceinline__ int xcdThread() {
+__device__ __forceinline__ int xcdBlock() {
   const int nb = gridDim.x, x = blockIdx.x & 7, r = blockIdx.x >> 3;
   const int q = nb >> 3, rem = nb & 7;
-  const int b = x * q + min(x, rem) + r;
-  return b * blockDim.x + threadIdx.x;
+  return x * q + min(x, rem) + r;
 }
+__device__ __forceinline__ int xcdThread() { return xcdBlock() * blockDim.x + threadIdx.x; }
 
 // Viscosity variant of calcViscForce (sph.h:579-631): at most one is set.
 enum ViscVariant { VISC_NONE = 0, VISC_MOR = 1, VISC_CLE = 2, VISC_MON = 3 };

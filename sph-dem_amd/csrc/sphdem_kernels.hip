@@ -805,7 +805,7 @@ __global__ void __launch_bounds__(256) k_shepsum(Store S, int ns, int nd, Grid g
                                                  double Rc) {
   __shared__ int lbuf[4][kCompactCap], lcode[4][kCompactCap];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int j = ns + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int j = ns + xcdBlock() * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (j >= ns + nd) return;  // uniform per wavefront
   const double4 pj = S.posh[j];
   double sum = 0.0;
@@ -934,7 +934,7 @@ __global__ void __launch_bounds__(256) k_fpi(Store S, int ns, int nd, Grid gs,
                                              sphdem_params P) {
   __shared__ int lbuf[4][kCompactCap], lcode[4][kCompactCap];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int j = ns + blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int j = ns + xcdBlock() * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (j >= ns + nd) return;  // uniform per wavefront
   const double4 pj = S.posh[j];
   const double4 vj = S.vhm[j];
