@@ -116,7 +116,7 @@ __device__ void blockAdd(double (&v)[NV], double* dst) {
 //     den = r^2 + 0.01 hav^2,  hav = (h_a + h_b)/2
 //   pressure:         B  = -m_b (pdr2_a Fa + pdr2_b Fb) ia,  fp_a += B dx
 // One thread per query particle: its sums stay in registers.
-__global__ void k_force_fwd(int32_t nq, const int32_t* __restrict__ query,
+__global__ void __launch_bounds__(kTPB) k_force_fwd(int32_t nq, const int32_t* __restrict__ query,
                             const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
                             const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                             sphdem_adj_consts C, double* __restrict__ fv,
@@ -168,7 +168,7 @@ __global__ void k_force_fwd(int32_t nq, const int32_t* __restrict__ query,
 // same r, dx and dv with their signs flipped, F taken on both h, divided by
 // omega_b), so every adjoint is written by one thread, without atomics.  The
 // viscosity adjoint is summed over the own terms only (each pair once).
-__global__ void k_force_bwd(int32_t nq, const int32_t* __restrict__ query,
+__global__ void __launch_bounds__(kTPB) k_force_bwd(int32_t nq, const int32_t* __restrict__ query,
                             const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
                             const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                             sphdem_adj_consts C, const double* __restrict__ gfv,
@@ -309,7 +309,7 @@ __device__ __forceinline__ void wendP(double c, double r, double h, double& P, d
   }
 }
 
-__global__ void k_omega_fwd(int32_t nq, const int32_t* __restrict__ query,
+__global__ void __launch_bounds__(kTPB) k_omega_fwd(int32_t nq, const int32_t* __restrict__ query,
                             const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
                             const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                             sphdem_adj_consts C, double* __restrict__ omega) {
@@ -337,7 +337,7 @@ __global__ void k_omega_fwd(int32_t nq, const int32_t* __restrict__ query,
 // which a appears, so every adjoint is written by one thread, without atomics.
 //   own:     Sbar_a = go_a h_a / (3 rho_a),  r_a += Sbar_a m_b dP(r, h_a)/dr dx/r
 //   partner: r_a   += Sbar_b m_a dP(r, h_b)/dr dx/r     (dx = r_a - r_b')
-__global__ void k_omega_bwd(int32_t nq, const int32_t* __restrict__ query,
+__global__ void __launch_bounds__(kTPB) k_omega_bwd(int32_t nq, const int32_t* __restrict__ query,
                             const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
                             const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                             sphdem_adj_consts C, const double* __restrict__ gomega,
@@ -400,7 +400,7 @@ __global__ void k_omega_bwd(int32_t nq, const int32_t* __restrict__ query,
 // ---------------------------------------------------------------- dddt
 // CsphIncompress::calcDddtDudt (sphIncompress.h:783-851) with gradW on h_a
 // and omega_a: dddt_a = sum_b m_b (vhat_a - vhat_b).dx F(r, h_a) / omega_a.
-__global__ void k_dddt_fwd(int32_t nq, const int32_t* __restrict__ query,
+__global__ void __launch_bounds__(kTPB) k_dddt_fwd(int32_t nq, const int32_t* __restrict__ query,
                            const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
                            const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                            sphdem_adj_consts C, double* __restrict__ dddt) {
@@ -428,7 +428,7 @@ __global__ void k_dddt_fwd(int32_t nq, const int32_t* __restrict__ query,
 // thread of a adds the terms of dddt_a and of every dddt_b in which a is the
 // partner (u = dv.dx and dx change sign between the two, F is taken on the
 // query's h, divided by the query's omega).
-__global__ void k_dddt_bwd(int32_t nq, const int32_t* __restrict__ query,
+__global__ void __launch_bounds__(kTPB) k_dddt_bwd(int32_t nq, const int32_t* __restrict__ query,
                            const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
                            const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                            sphdem_adj_consts C, const double* __restrict__ gdddt,
@@ -496,7 +496,7 @@ __global__ void k_dddt_bwd(int32_t nq, const int32_t* __restrict__ query,
 // sph.h:1303-1362.  The Shepard sums need every fluid partner of d before
 // the drag, so the forward is two launches, and so is the backward (the
 // adjoint of shep_d is complete only after the first).
-__global__ void k_cpl_fwd1(int32_t nq, const int32_t* __restrict__ query,
+__global__ void __launch_bounds__(kTPB) k_cpl_fwd1(int32_t nq, const int32_t* __restrict__ query,
                            const int64_t* __restrict__ off, const int32_t* __restrict__ pd,
                            const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                            sphdem_adj_consts C, double* __restrict__ por_sum,
@@ -519,7 +519,7 @@ __global__ void k_cpl_fwd1(int32_t nq, const int32_t* __restrict__ query,
   por_sum[a] += por;
 }
 
-__global__ void k_cpl_fwd2(int32_t nq, const int32_t* __restrict__ query,
+__global__ void __launch_bounds__(kTPB) k_cpl_fwd2(int32_t nq, const int32_t* __restrict__ query,
                            const int64_t* __restrict__ off, const int32_t* __restrict__ pd,
                            const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                            sphdem_adj_consts C, double* __restrict__ drag) {
@@ -550,7 +550,7 @@ __global__ void k_cpl_fwd2(int32_t nq, const int32_t* __restrict__ query,
 
 // first backward launch: the drag and porosity terms; shepbar[d] collects the
 // adjoint of the Shepard sums
-__global__ void k_cpl_bwd1(int32_t nq, const int32_t* __restrict__ query,
+__global__ void __launch_bounds__(kTPB) k_cpl_bwd1(int32_t nq, const int32_t* __restrict__ query,
                            const int64_t* __restrict__ off, const int32_t* __restrict__ pd,
                            const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                            sphdem_adj_consts C, const double* __restrict__ gpor,
@@ -595,7 +595,7 @@ __global__ void k_cpl_bwd1(int32_t nq, const int32_t* __restrict__ query,
 }
 
 // second backward launch: the Shepard sums
-__global__ void k_cpl_bwd2(int32_t nq, const int32_t* __restrict__ query,
+__global__ void __launch_bounds__(kTPB) k_cpl_bwd2(int32_t nq, const int32_t* __restrict__ query,
                            const int64_t* __restrict__ off, const int32_t* __restrict__ pd,
                            const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                            sphdem_adj_consts C, const double* __restrict__ shepbar,
@@ -675,7 +675,7 @@ __device__ __forceinline__ void fpiSums(const sphdem_adj_fields& in, const sphde
   }
 }
 
-__global__ void k_fpi_fwd(int32_t nq, const int32_t* __restrict__ query,
+__global__ void __launch_bounds__(kTPB) k_fpi_fwd(int32_t nq, const int32_t* __restrict__ query,
                           const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
                           const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                           sphdem_adj_consts C, double* __restrict__ fdrag) {
@@ -709,7 +709,7 @@ __global__ void k_fpi_fwd(int32_t nq, const int32_t* __restrict__ query,
   fdrag[3 * p + 2] += kv * (o.Gv.z + o.Gp.z) * iS + kd * sp * dvz;
 }
 
-__global__ void k_fpi_bwd(int32_t nq, const int32_t* __restrict__ query,
+__global__ void __launch_bounds__(kTPB) k_fpi_bwd(int32_t nq, const int32_t* __restrict__ query,
                           const int64_t* __restrict__ off, const int32_t* __restrict__ pb,
                           const int32_t* __restrict__ pimg, sphdem_adj_fields in,
                           sphdem_adj_consts C, const double* __restrict__ gfd,
@@ -824,7 +824,7 @@ __global__ void k_fpi_bwd(int32_t nq, const int32_t* __restrict__ query,
 // Pair (a, b): dx = r_a - r_b', r = |dx|, n = dx/r, overlap = 2R - r; when
 // overlap > 0: od = -(vhat_a - vhat_b).n, sN = k overlap + gamma od,
 // fc_a += sN n / m_a.
-__global__ void k_contacts_fwd(int64_t np, const int32_t* __restrict__ pa,
+__global__ void __launch_bounds__(kTPB) k_contacts_fwd(int64_t np, const int32_t* __restrict__ pa,
                                const int32_t* __restrict__ pb, const int32_t* __restrict__ pimg,
                                sphdem_adj_fields in, sphdem_adj_consts C,
                                double* __restrict__ fc) {
@@ -843,7 +843,7 @@ __global__ void k_contacts_fwd(int64_t np, const int32_t* __restrict__ pa,
   add3(fc, a, s * nx, s * ny, s * nz);
 }
 
-__global__ void k_contacts_bwd(int64_t np, const int32_t* __restrict__ pa,
+__global__ void __launch_bounds__(kTPB) k_contacts_bwd(int64_t np, const int32_t* __restrict__ pa,
                                const int32_t* __restrict__ pb, const int32_t* __restrict__ pimg,
                                sphdem_adj_fields in, sphdem_adj_consts C,
                                const double* __restrict__ gfc, sphdem_adj_grads g) {
@@ -926,7 +926,7 @@ int scratchOfCurrentDevice(double** out) {
 }
 
 // zero fill (a kernel, not a memset: the call stays a plain kernel sequence)
-__global__ void k_zero_doubles(double* __restrict__ p, int64_t n) {
+__global__ void __launch_bounds__(sphdem_adj::kTPB) k_zero_doubles(double* __restrict__ p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;
 }
