@@ -588,9 +588,10 @@ __global__ void __launch_bounds__(256) k_density_l(Store S, int ns, Box B,
 // ---- calcDddtDudt (sphIncompress.h:783-851) over the cached list ----------------
 // OMG (VAR_H only): calcOmega fused into the walk, as in k_force_l -- every
 // dddt term of a is linear in 1/omega_a, so the terms are summed without it
-// beside the omega sum and scaled once.
+// beside the omega sum and scaled once.  Three wavefronts per SIMD: 168 VGPRs
+// without spills (170 would allow two), 2 % faster.
 template <bool VARH, bool OMG = false>
-__global__ void __launch_bounds__(256) k_dddt_l(Store S, int ns, Box B,
+__global__ void __launch_bounds__(256, 3) k_dddt_l(Store S, int ns, Box B,
                                                 const int* __restrict__ nbl,
                                                 const int* __restrict__ nbcnt,
                                                 const double2* __restrict__ hinv, double wcon) {
