@@ -355,11 +355,9 @@ __device__ __forceinline__ short2v asShort2(unsigned v) { return __builtin_bit_c
 
 // Grid: 8 * ceil(ceil(ns / 8) / blockDim) workgroups; XCD blockIdx % 8 takes
 // the corresponding eighth of the targets (a spatially coherent slab for its
-// L2).  NB = 32 candidates per batch (the masks' width).  At least five
-// wavefronts per SIMD: the compiler then keeps 96 VGPRs (a few spills outside
-// the candidate loop), which measured 3 % faster than four at 112.
+// L2).  NB = 32 candidates per batch (the masks' width).
 template <int NB>
-__global__ void __launch_bounds__(256, 5) k_lists_t(const double4* __restrict__ pos,
+__global__ void __launch_bounds__(256) k_lists_t(const double4* __restrict__ pos,
                                                  const uint2* __restrict__ rec,
                                                  const int* __restrict__ cellc, Grid g, int gbase,
                                                  const int* __restrict__ start, int ns,
