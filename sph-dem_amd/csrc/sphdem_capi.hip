@@ -1015,9 +1015,8 @@ int passFpi(sphdem_ctx* ctx) {
 int passContacts(sphdem_ctx* ctx) {
   if (ctx->nd == 0) return 0;
   Timed tm(ctx, KID_CONTACTS);
-  // one wavefront per workgroup: the few DEM wavefronts (one thread per
-  // sphere, each walking 27 cells of dependent loads) spread over as many CUs
-  hipLaunchKernelGGL(k_contacts, dim3((ctx->nd + 63) / 64), dim3(64), 0, ctx->stream, ctx->cur,
+  // one wavefront per sphere, four per workgroup
+  hipLaunchKernelGGL(k_contacts, dim3(waveBlocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur,
                      ctx->ns, ctx->nd, ctx->gc, ctx->start, ctx->box, ctx->P);
   LAUNCHED();
   return 0;  // writes DEM fields only

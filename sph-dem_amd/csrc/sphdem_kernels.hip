@@ -1054,41 +1054,87 @@ __global__ void __launch_bounds__(256) k_fpi(Store S, int ns, int nd, Grid gs,
 }
 
 // calcDEMContacts (sph.h:1676-1706) with demNormal LINEAR (sph.h:1513-1515)
-__global__ void k_contacts(Store S, int ns, int nd, Grid gc, const int* __restrict__ start,
-                           Box B, sphdem_params P) {
-  const int j = ns + blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ns + nd) return;
+__global__ void __launch_bounds__(256) k_contacts(Store S, int ns, int nd, Grid gc,
+                                                  const int* __restrict__ start, Box B,
+                                                  sphdem_params P) {
+  // one wavefront per sphere: the lanes test the candidates of its cells 64 at
+  // a time (the cells' chain of dependent loads then runs once per column
+  // range, not once per candidate), the partners within reach are appended in
+  // candidate order, and lane 0 applies their contact forces one after
+  // another in that order -- the sums of the one-thread walk, bit for bit
+  __shared__ int lbuf[4][kCompactCap], lcode[4][kCompactCap];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = ns + xcdBlock() * (blockDim.x >> 6) + w;
+  if (j >= ns + nd) return;  // uniform per wavefront
   const double4 pj = S.posh[j];
   const double4 vj = S.vhm[j];
   double4 fp = S.fp[j], f = S.frc[j];
   bool any = false;
-  forCells(gc, start, B, pj.x, pj.y, pj.z, 1, [&](int b, int sx, int sy, int sz) {
-    if (b == j && (sx | sy | sz) == 0) return;
-    const double4 pb = S.posh[b];
+  auto ranges = [&](auto&& visit) {
+    forCellRanges(gc, start, B, pj.x, pj.y, pj.z, 1, [&](int j0, int j1, int sx, int sy, int sz) {
+      visit(j0, j1, imageCode(sx, sy, sz));
+    });
+  };
+  auto dist2 = [&](const double4& pb, int code) {
+    int sx, sy, sz;
+    imageShift(code, sx, sy, sz);
     const Img im = imagePos(pb, sx, sy, sz, B);
     const double dx = pj.x - im.x, dy = pj.y - im.y, dz = pj.z - im.z;
-    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
-    const double lim = 2.0 * dmax2(pj.w, pb.w);
-    if (!(l2 <= lim * lim)) return;
-    const double4 vb = S.vhm[b];
-    const double r = sqrt(l2);
-    const double dv0 = vj.x - vb.x, dv1 = vj.y - vb.y, dv2 = vj.z - vb.z;
-    const double n0 = dx / r, n1 = dy / r, n2 = dz / r;
-    const double overlap = 2.0 * P.dem_radius - r;
-    if (overlap > 0) {
-      const double od = -dot3(dv0, dv1, dv2, n0, n1, n2);
-      const double s = P.dem_k * overlap + P.dem_gamma * od;
-      const double f0 = s * n0 / vj.w, f1 = s * n1 / vj.w, f2 = s * n2 / vj.w;
-      fp.x += f0;
-      fp.y += f1;
-      fp.z += f2;
-      f.x += f0;
-      f.y += f1;
-      f.z += f2;
-      any = true;
+    return make_double4(dx, dy, dz, dot3(dx, dy, dz, dx, dy, dz));
+  };
+  int cnt = 0;  // wave-uniform
+  auto flush = [&](int m) {
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane == 0)
+      for (int k = 0; k < m; k++) {
+        const int b = lbuf[w][k];
+        const double4 d = dist2(S.posh[b], lcode[w][k]);
+        const double4 vb = S.vhm[b];
+        const double r = sqrt(d.w);
+        const double dv0 = vj.x - vb.x, dv1 = vj.y - vb.y, dv2 = vj.z - vb.z;
+        const double n0 = d.x / r, n1 = d.y / r, n2 = d.z / r;
+        const double overlap = 2.0 * P.dem_radius - r;
+        if (overlap > 0) {
+          const double od = -dot3(dv0, dv1, dv2, n0, n1, n2);
+          const double s = P.dem_k * overlap + P.dem_gamma * od;
+          const double f0 = s * n0 / vj.w, f1 = s * n1 / vj.w, f2 = s * n2 / vj.w;
+          fp.x += f0;
+          fp.y += f1;
+          fp.z += f2;
+          f.x += f0;
+          f.y += f1;
+          f.z += f2;
+          any = true;
+        }
+      }
+    __builtin_amdgcn_wave_barrier();
+  };
+  ranges([&](int j0, int j1, int code) {
+    for (int jj = j0; jj < j1; jj += 64) {
+      const int b = jj + lane;
+      bool ok = false;
+      if (b < j1 && !(b == j && code == 13)) {
+        const double4 pb = S.posh[b];
+        const double lim = 2.0 * dmax2(pj.w, pb.w);
+        ok = dist2(pb, code).w <= lim * lim;
+      }
+      const unsigned long long m = __ballot(ok);
+      if (ok) {
+        const int at = cnt + (int)__builtin_amdgcn_mbcnt_hi(
+                                 (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        lbuf[w][at] = b;
+        lcode[w][at] = code;
+      }
+      cnt += __popcll(m);
+      if (cnt >= 64) {
+        flush(cnt);
+        cnt = 0;
+      }
     }
   });
-  if (any) {
+  if (cnt > 0) flush(cnt);
+  if (lane == 0 && any) {
     S.fp[j] = fp;
     S.frc[j] = f;
   }
