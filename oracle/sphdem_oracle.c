@@ -53,12 +53,13 @@
  *   porosity, calcShepSum,
  *   calcDEMtoSPHDrag             : porosity/Shepard-sum consistency; Newton's
  *                                  third law of the drag pair
+ *   gradW's omega under VAR_H     : pa's for both kernel terms of a pair,
+ *                                  pinned by the invariant sum m_a omega_a
+ *                                  f_a = 0 on a box where omega varies
  *   parity unpinned (only the compiled reference could pin them; restated
  *   from the cited lines):
  *     calcNeighboursCoupling's SPH-side cell rule (checked only against a
- *     brute-force evaluation of the same rule); which particle's omega
- *     divides gradW under VAR_H (pa's for both sides: equal on a uniform
- *     lattice, so no closed form tells them apart); the order of the
+ *     brute-force evaluation of the same rule); the order of the
  *     reverse-sync additions of the Shepard scatter.
  */
 #include <math.h>

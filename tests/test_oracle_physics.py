@@ -297,3 +297,45 @@ def test_output_vars_of_a_rigidly_moving_compressed_lattice():
                                           + out["eViscB"] + out["eBForce"] + out["eFF"], rel=1e-14)
     o.pass_("density")
     assert o.output_vars()["aveDensFromMass"] == 0.0
+
+
+def test_var_h_pair_terms_are_divided_by_the_query_particles_omega():
+    """Under VAR_H_CORRECTION gradW divides both kernel terms of a pair by
+    pa.omega, the query particle's own omega (sph.h:331-332), not each by its
+    side's.  omega_a times the pressure and viscous force of a is then a sum of
+    terms antisymmetric in (a, b), so sum_a m_a omega_a (fp_a + fv_a) vanishes
+    to rounding, while sum_a m_a (fp_a + fv_a) does not where omega varies;
+    dividing each side by its own omega would conserve sum_a m_a f_a instead.
+    An open, jittered box with a density wave (h and omega follow it) and a
+    velocity field that shears and rotates separates the two."""
+    base = sd.sediment_case(nf=12, n_dem=0, jitter=True, seed=17)
+    sph = base.iam == sd.SPH
+    P = base.params.copy()
+    for i in range(3):
+        P.periodic[i] = 0
+        P.rmin[i], P.rmax[i] = 0.0, 1.0
+        P.proc_domain[2 * i], P.proc_domain[2 * i + 1] = -0.25, 1.25
+    P.flags = sd.F_VISC_MORRIS | sd.F_VAR_H_CORRECTION
+    r, mass, iam = base.r[sph], base.mass[sph], base.iam[sph]
+    x, y, z = r.T
+    dens = P.refd * (1.0 + 0.03 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * z) + 0.02 * y)
+    h = P.hfac * (mass / dens) ** (1.0 / 3.0)
+    vhat = np.stack([0.1 * z ** 2 - 0.05 * y, 0.05 * x + 0.02 * z,
+                     0.03 * np.sin(2 * np.pi * y)], axis=1)
+    c = sd.Case("varh_box", P, r, vhat.copy(), vhat.copy(), mass, h, dens, iam, sd.HOOKS_NONE)
+    o = oracle.Oracle(c)
+    o.pass_("omega")
+    om = o.get("omega")
+    assert om.std() > 1e-3 * om.mean()
+    o.pass_("eos")
+    o.pass_("force")
+    # the pressure and the viscous parts each on their own scale
+    for name in ("fp", "fv"):
+        f = o.get(name)
+        w = (mass * om)[:, None] * f
+        u = mass[:, None] * f
+        s_omega = np.abs(w.sum(axis=0)).max() / np.abs(w).sum(axis=0).max()
+        s_plain = np.abs(u.sum(axis=0)).max() / np.abs(u).sum(axis=0).max()
+        assert s_omega < 1e-12, (name, s_omega)
+        assert s_plain > 1e-6, (name, s_plain)
+    o.close()
