@@ -886,13 +886,23 @@ int computeHinv(sphdem_ctx* ctx) {
 // calcForce instantiated per flag combination (bit 0 VAR_H_CORRECTION, 1 VISC_ARTIFICIAL,
 // 2 LIQ_DEM, bits 3-4 the viscosity variant); OMG: the walkers with calcOmega fused
 // in (VAR_H only: the OMG entries without VAR_H are empty and never selected)
+// The mixed force walker runs faster at two wavefronts per SIMD than at the
+// four its registers allow (2.83 against 3.10 ms per launch at 160^3: its
+// neighbour gathers thrash the caches with more wavefronts resident).  A
+// dynamic LDS reservation that the kernel never touches holds it there:
+// eight wavefronts per 160 KB CU, whatever the walker's block size.
+unsigned forceLdsReserve(const sphdem_ctx* ctx) {
+  const unsigned blocksPerCU = 8u * 64u / (unsigned)ctx->walkTPB;
+  return std::min(65536u, (163840u / blocksPerCU) & ~1023u) - 1024u;
+}
 template <int F, bool OMG>
 void launchForce(sphdem_ctx* ctx) {
   constexpr bool V = (F & 1) != 0, A = (F & 2) != 0, L = (F & 4) != 0;
   constexpr int M = F >> 3;
   if constexpr (!OMG || V) {
     if (mixed(ctx))
-      hipLaunchKernelGGL((k_force_m<V, M, A, L, OMG>), dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0,
+      hipLaunchKernelGGL((k_force_m<V, M, A, L, OMG>), dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB),
+                         forceLdsReserve(ctx),
                          ctx->stream, ctx->cur, ctx->ns, ctx->nbl, ctx->nbcnt, ctx->mixP, ctx->mixV,
                          ctx->mixD, ctx->mixRes, ctx->P, ctx->dg);
     else
