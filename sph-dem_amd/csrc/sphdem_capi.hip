@@ -891,6 +891,7 @@ int computeHinv(sphdem_ctx* ctx) {
 // neighbour gathers thrash the caches with more wavefronts resident).  A
 // dynamic LDS reservation that the kernel never touches holds it there:
 // eight wavefronts per 160 KB CU, whatever the walker's block size.
+constexpr int kContactsLaneFrom = 16384;  // spheres
 unsigned forceLdsReserve(const sphdem_ctx* ctx) {
   const unsigned blocksPerCU = 8u * 64u / (unsigned)ctx->walkTPB;
   return std::min(65536u, (163840u / blocksPerCU) & ~1023u) - 1024u;
@@ -1025,9 +1026,17 @@ int passFpi(sphdem_ctx* ctx) {
 int passContacts(sphdem_ctx* ctx) {
   if (ctx->nd == 0) return 0;
   Timed tm(ctx, KID_CONTACTS);
-  // one wavefront per sphere, four per workgroup
-  hipLaunchKernelGGL(k_contacts, dim3(waveBlocks(ctx->nd)), dim3(TPB), 0, ctx->stream, ctx->cur,
-                     ctx->ns, ctx->nd, ctx->gc, ctx->start, ctx->box, ctx->P);
+  // one wavefront per sphere (four per workgroup) for the few spheres of a
+  // small box, where the one-thread walk is a chain of dependent loads the
+  // launch cannot hide; one thread per sphere from kContactsLaneFrom on,
+  // where it runs beside the dddt walk and fits the slots that walk leaves.
+  // The contact forces are summed in candidate order either way.
+  if (ctx->nd >= kContactsLaneFrom)
+    hipLaunchKernelGGL(k_contacts_lane, dim3((ctx->nd + 63) / 64), dim3(64), 0, ctx->stream,
+                       ctx->cur, ctx->ns, ctx->nd, ctx->gc, ctx->start, ctx->box, ctx->P);
+  else
+    hipLaunchKernelGGL(k_contacts, dim3(waveBlocks(ctx->nd)), dim3(TPB), 0, ctx->stream,
+                       ctx->cur, ctx->ns, ctx->nd, ctx->gc, ctx->start, ctx->box, ctx->P);
   LAUNCHED();
   return 0;  // writes DEM fields only
 }

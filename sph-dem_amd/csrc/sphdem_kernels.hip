@@ -1054,6 +1054,49 @@ __global__ void __launch_bounds__(256) k_fpi(Store S, int ns, int nd, Grid gs,
 }
 
 // calcDEMContacts (sph.h:1676-1706) with demNormal LINEAR (sph.h:1513-1515)
+// One thread per sphere walking its cells in order (the reference's loop as
+// written): with tens of thousands of spheres it shares the GPU with the
+// passes beside it better than a wavefront per sphere does.
+__global__ void k_contacts_lane(Store S, int ns, int nd, Grid gc, const int* __restrict__ start,
+                           Box B, sphdem_params P) {
+  const int j = ns + blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns + nd) return;
+  const double4 pj = S.posh[j];
+  const double4 vj = S.vhm[j];
+  double4 fp = S.fp[j], f = S.frc[j];
+  bool any = false;
+  forCells(gc, start, B, pj.x, pj.y, pj.z, 1, [&](int b, int sx, int sy, int sz) {
+    if (b == j && (sx | sy | sz) == 0) return;
+    const double4 pb = S.posh[b];
+    const Img im = imagePos(pb, sx, sy, sz, B);
+    const double dx = pj.x - im.x, dy = pj.y - im.y, dz = pj.z - im.z;
+    const double l2 = dot3(dx, dy, dz, dx, dy, dz);
+    const double lim = 2.0 * dmax2(pj.w, pb.w);
+    if (!(l2 <= lim * lim)) return;
+    const double4 vb = S.vhm[b];
+    const double r = sqrt(l2);
+    const double dv0 = vj.x - vb.x, dv1 = vj.y - vb.y, dv2 = vj.z - vb.z;
+    const double n0 = dx / r, n1 = dy / r, n2 = dz / r;
+    const double overlap = 2.0 * P.dem_radius - r;
+    if (overlap > 0) {
+      const double od = -dot3(dv0, dv1, dv2, n0, n1, n2);
+      const double s = P.dem_k * overlap + P.dem_gamma * od;
+      const double f0 = s * n0 / vj.w, f1 = s * n1 / vj.w, f2 = s * n2 / vj.w;
+      fp.x += f0;
+      fp.y += f1;
+      fp.z += f2;
+      f.x += f0;
+      f.y += f1;
+      f.z += f2;
+      any = true;
+    }
+  });
+  if (any) {
+    S.fp[j] = fp;
+    S.frc[j] = f;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_contacts(Store S, int ns, int nd, Grid gc,
                                                   const int* __restrict__ start, Box B,
                                                   sphdem_params P) {
