@@ -630,7 +630,9 @@ int checkFlags(sphdem_ctx* ctx) {
 }
 
 // midInit: middle()'s reset, which also starts the dt minimum at maxdt
-int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false) {
+// fold: the half-step before the reset, done by k_wrap and k_permute (WrapFold)
+int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false,
+            const WrapFold& fold = WrapFold{}) {
   const sphdem_params& P = ctx->P;
   const int n = ctx->n;
   // a captured reset reads nothing back: it needs the steady state of a
@@ -661,10 +663,13 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false) {
   const bool pre = periodic && ctx->gridValid && n > 0;
   int ngPre = 0;
   {
+    // a folded half-step moves more bytes per particle: more waves in flight
+    // (and at least one block, which advances the time)
+    const int wb = fold.mode ? std::max(1, std::min(blocks(n), 4096)) : reduceBlocks(n);
     Timed tm(ctx, KID_WRAP);
-    hipLaunchKernelGGL(k_wrap, dim3(reduceBlocks(n)), dim3(TPB), 0, ctx->stream, ctx->cur, n, dmin[0],
+    hipLaunchKernelGGL(k_wrap, dim3(wb), dim3(TPB), 0, ctx->stream, ctx->cur, n, dmin[0],
                        dmin[1], dmin[2], dsp[0], dsp[1], dsp[2], ctx->box, ctx->dg, ctx->gs,
-                       ctx->band, pre ? ctx->gcnt : (int*)nullptr, midInit ? 1 : 0);
+                       ctx->band, pre ? ctx->gcnt : (int*)nullptr, midInit ? 1 : 0, fold);
     LAUNCHED();
   }
   if (pre) CHECK(scanInts(ctx, ctx->gcnt, ctx->goff, n + 1, &ctx->dg->ghostTotal));
@@ -756,9 +761,14 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false) {
   }
   {
     Timed tm(ctx, KID_PERMUTE);
-    hipLaunchKernelGGL(k_permute, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ntot, n,
-                       ctx->perm, ctx->cur, ctx->alt, ctx->gsrc, ctx->gord, ctx->gcell, ctx->box,
-                       ctx->inv, ngp);
+    if (fold.mode == 1)
+      hipLaunchKernelGGL(k_permute<true>, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ntot, n,
+                         ctx->perm, ctx->cur, ctx->alt, ctx->gsrc, ctx->gord, ctx->gcell,
+                         ctx->box, ctx->inv, ngp, ctx->dg, fold);
+    else
+      hipLaunchKernelGGL(k_permute<false>, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ntot, n,
+                         ctx->perm, ctx->cur, ctx->alt, ctx->gsrc, ctx->gord, ctx->gcell,
+                         ctx->box, ctx->inv, ngp, ctx->dg, fold);
     LAUNCHED();
   }
   if (ng > 0) {
@@ -1063,6 +1073,10 @@ int passDensity(sphdem_ctx* ctx) {
 }
 
 // ---- half-steps
+// whether start() of step `step` re-initialises the density (sphIncompress.cpp:17-35)
+bool reinitDue(const sphdem_ctx* c, long long step) {
+  return (step % c->P.reinit_every == 0) || (step == c->P.reinit_at);
+}
 // bodyMask/accel: the beforeMiddle body-force hook, applied by k_start right
 // after initSumsMiddle zeroes f (nothing in between touches f)
 int stepStart(sphdem_ctx* ctx, uint32_t bodyMask = 0, const double* accel = nullptr) {
@@ -1076,14 +1090,15 @@ int stepStart(sphdem_ctx* ctx, uint32_t bodyMask = 0, const double* accel = null
     LAUNCHED();
     markStale(ctx, GS_ALL);
   }
-  if ((ctx->sphStep % ctx->P.reinit_every == 0) || (ctx->sphStep == ctx->P.reinit_at)) {
+  if (reinitDue(ctx, ctx->sphStep)) {
     CHECK(doReset(ctx));
     for (int i = 0; i < 10; i++) CHECK(passDensity(ctx));
   }
   return 0;
 }
-int stepMiddle(sphdem_ctx* ctx) {
-  CHECK(doReset(ctx, false, true));
+// fold: start() (WrapFold mode 1) when the step's start() has not run
+int stepMiddle(sphdem_ctx* ctx, const WrapFold& fold = WrapFold{}) {
+  CHECK(doReset(ctx, false, true, fold));
   bool hinvReady = false;
   {
     // On the side stream, beside the list build: the coupling pass (porosity,
@@ -1118,12 +1133,19 @@ int stepMiddle(sphdem_ctx* ctx) {
   markStale(ctx, GS_ALL);
   return 0;
 }
-int stepEnd(sphdem_ctx* ctx) {
-  hipLaunchKernelGGL(k_end_drift, dim3(blocks(std::max(ctx->ns, 1))), dim3(TPB), 0, ctx->stream,
-                     ctx->cur, ctx->ns, ctx->dg);
-  LAUNCHED();
-  markStale(ctx, GS_ALL);
-  CHECK(doReset(ctx));
+// foldDrift: the drift of r (and the time's half step) done by the reset's k_wrap
+int stepEnd(sphdem_ctx* ctx, bool foldDrift = false) {
+  WrapFold fold{};
+  if (foldDrift) {
+    fold.mode = 2;
+    fold.nlive = ctx->ns;
+  } else {
+    hipLaunchKernelGGL(k_end_drift, dim3(blocks(std::max(ctx->ns, 1))), dim3(TPB), 0, ctx->stream,
+                       ctx->cur, ctx->ns, ctx->dg);
+    LAUNCHED();
+    markStale(ctx, GS_ALL);
+  }
+  CHECK(doReset(ctx, false, false, fold));
   // calcOmega (VAR_H) is summed in the dddt walk: the dddt of a reads omega_a only
   const bool fuse = varH(ctx);
   if (liqDem(ctx)) {
@@ -1796,12 +1818,30 @@ int runOneStep(sphdem_ctx* ctx, int hookset) {
   const uint32_t sphMask = 1u << SPHDEM_SPH, demMask = 1u << SPHDEM_DEM;
   {
     ctx->sphStep++;
-    // start(), then the beforeMiddle gravity on sph (fused into k_start)
-    if (hookset != SPHDEM_HOOKS_NONE)
-      CHECK(stepStart(ctx, sphMask, g));
-    else
-      CHECK(stepStart(ctx));
-    CHECK(stepMiddle(ctx));
+    // start(), then the beforeMiddle gravity on sph (fused into k_start); on
+    // a step without a density re-initialisation nothing reads start()'s
+    // results before middle()'s reset, which does start() in its own passes
+    const bool body = hookset != SPHDEM_HOOKS_NONE;
+    if (reinitDue(ctx, ctx->sphStep)) {
+      if (body)
+        CHECK(stepStart(ctx, sphMask, g));
+      else
+        CHECK(stepStart(ctx));
+      CHECK(stepMiddle(ctx));
+    } else {
+      WrapFold fold{};
+      fold.mode = 1;
+      fold.nlive = ctx->ns + ctx->nd;
+      fold.flags = ctx->P.flags;
+      fold.hfac = ctx->P.hfac;
+      if (body) {
+        fold.bodyMask = sphMask;
+        fold.ax = g[0];
+        fold.ay = g[1];
+        fold.az = g[2];
+      }
+      CHECK(stepMiddle(ctx, fold));
+    }
     if (hookset == SPHDEM_HOOKS_SEDIMENT) {  // beforeEnd, sediment3D/customSim.cpp:97-107
       CHECK(hookSumDrag(ctx));
       // gravity, the floor contact and the freeze on dem, fused per particle
@@ -1813,7 +1853,7 @@ int runOneStep(sphdem_ctx* ctx, int hookset) {
         markStale(ctx, GS_ALL);
       }
     }
-    CHECK(stepEnd(ctx));
+    CHECK(stepEnd(ctx, true));
     if (hookset == SPHDEM_HOOKS_SEDIMENT) CHECK(hookFreeze(ctx, demMask, ctx->P.time_drop));
   }
   // every buffer a step uses is allocated now (device resets allocate nothing)
@@ -1924,9 +1964,6 @@ void dropGraph(sphdem_ctx* c) {
   c->graphExec = nullptr;
   c->graph = nullptr;
   c->graphKey.clear();
-}
-bool reinitDue(const sphdem_ctx* c, long long step) {
-  return (step % c->P.reinit_every == 0) || (step == c->P.reinit_at);
 }
 int captureStep(sphdem_ctx* ctx, int hookset) {
   dropGraph(ctx);

@@ -82,6 +82,19 @@ struct GhostBand {
   int lo[3], hi[3];
 };
 
+// The per-particle work of the half-step just before a reset, done by the
+// reset's own passes (k_wrap, k_permute<true>) instead of a launch of its
+// own: mode 0 nothing, 1 start(), 2 end()'s drift of r.  Zero-initialised:
+// mode 0.
+struct WrapFold {
+  int mode;
+  int nlive;          // the half-step's range [0, nlive) of the store
+  unsigned flags;     // sphdem_params::flags (CONST_H)
+  double hfac;        // sphdem_params::hfac
+  unsigned bodyMask;  // start(): the beforeMiddle body force's types and value
+  double ax, ay, az;
+};
+
 // Block index (xcdBlock) and global thread index (xcdThread) with the blocks
 // of a 1-D grid regrouped by XCD.  The
 // dispatcher deals workgroups round robin to the 8 XCDs (block b to XCD b % 8);

@@ -167,18 +167,87 @@ __device__ __forceinline__ int ghostCount(const double4& p, const Grid& g, const
   const int mz = ghostShiftsAxis(p.z, 2, g, B, gb, cc);
   return (1 + popc2(mx)) * (1 + popc2(my)) * (1 + popc2(mz)) - 1;
 }
+// ============================================================ integrator parts
+// start() per particle (sphIncompress.cpp:4-16; sphIncompress.h:620-704,
+// 744-763) in two parts, so that the reset that follows it in a step can do
+// them inside its own passes with the same arithmetic:
+//  - startDrift: r += dt vhat (sph, sphBoundary, dem), then driftRest's
+//    dens += dt dddt and h = HFAC (mass/dens)^(1/3) (sph, sphBoundary);
+//  - startKick (sph, sphBoundary): v0 = v and vhat0 = vhat, v and vhat
+//    kicked by dt f (sph only), then initSumsMiddle (f, fp.xyz, fv, fb = 0)
+//    and the beforeMiddle body force for the types in bodyMask.
+// Neither reads a field the other writes.
+__device__ __forceinline__ void startDrift(const Store& S, int a, int t, unsigned flags,
+                                           double hfac, double dt, double4& p) {
+  const double4 vh = S.vhm[a];
+  p.x += dt * vh.x;
+  p.y += dt * vh.y;
+  p.z += dt * vh.z;
+  if (isSphT(t)) {
+    double4 d = S.dpo[a];
+    d.x += dt * S.vh0d[a].w;
+    S.dpo[a] = d;
+    if (!(flags & SPHDEM_F_CONST_H)) p.w = hfac * pow_third(vh.w / d.x);
+  }
+}
+__device__ __forceinline__ void halfKick(double4& v, const double4& f, double dt) {
+  v.x += dt * f.x;
+  v.y += dt * f.y;
+  v.z += dt * f.z;
+}
+// src entry i to dst entry q (the same entry in place); MOVE: also copy the
+// fields the kick leaves alone
+template <bool MOVE>
+__device__ __forceinline__ void startKick(const Store& src, int i, const Store& dst, int q, int t,
+                                          double dt, unsigned bodyMask, double ax, double ay,
+                                          double az) {
+  if (t != T_SPHB) {
+    double4 v = src.vel[i];
+    double4 vh = src.vhm[i];
+    const double4 f = src.frc[i];
+    dst.v0p[q] = make_double4(v.x, v.y, v.z, src.v0p[i].w);
+    dst.vh0d[q] = make_double4(vh.x, vh.y, vh.z, src.vh0d[i].w);
+    halfKick(v, f, dt);
+    halfKick(vh, f, dt);
+    dst.vel[q] = v;
+    dst.vhm[q] = vh;
+  } else if (MOVE) {
+    dst.vel[q] = src.vel[i];
+    dst.vhm[q] = src.vhm[i];
+    dst.v0p[q] = src.v0p[i];
+    dst.vh0d[q] = src.vh0d[i];
+  }
+  const double4 z = make_double4(0.0, 0.0, 0.0, 0.0);
+  double4 f0 = z;
+  if ((bodyMask >> t) & 1u) {
+    if (ax != 0.0) f0.x += ax;
+    if (ay != 0.0) f0.y += ay;
+    if (az != 0.0) f0.z += az;
+  }
+  dst.frc[q] = f0;
+  dst.fp[q] = make_double4(0.0, 0.0, 0.0, src.fp[i].w);
+  dst.fv[q] = z;
+  dst.fb[q] = z;
+}
+
 // ============================================================ cell build
 // CdataLL::updateDomain's migration for one rank whose neighbours are itself
 // through periodic faces (dataLL.cpp:370-403, 459-497): wrap or delete, then
 // hmax (calcDomainLimits, dataLL.cpp:954-975).
 // With gcnt set, also each particle's ghost count on the current grid (used
 // when the reset keeps that grid).
+// F.mode 1: the reset right after start() does start()'s drift part first
+// (its kick part goes into k_permute<true>; a particle deleted here gets both
+// parts here); F.mode 2: the reset right after end()'s drift does that drift
+// first (driftR on sph, sphIncompress.cpp:170).  Both add dt/2 to the time.
 __global__ void k_wrap(Store S, int n, double dmin0, double dmin1, double dmin2_,
                        double dsp0, double dsp1, double dsp2, Box B, DevGlobals* g, Grid gs,
-                       GhostBand gb, int* __restrict__ gcnt, int initNewDt) {
+                       GhostBand gb, int* __restrict__ gcnt, int initNewDt, WrapFold F) {
   // middle() starts its dt minimum at maxdt (nothing reads it during the reset)
   if (initNewDt && blockIdx.x == 0 && threadIdx.x == 0)
     g->newDt_bits = (unsigned long long)__double_as_longlong(g->maxdt);
+  const double dt = F.mode ? g->dt / 2 : 0.0;
+  if (F.mode && blockIdx.x == 0 && threadIdx.x == 0) g->time += g->dt / 2;
   // grid-stride over the particles: a few hundred blocks, so few atomics on hmax
   double hv = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -186,6 +255,17 @@ __global__ void k_wrap(Store S, int n, double dmin0, double dmin1, double dmin2_
     const int t = S.iam[i];
     if (t != T_DELETED) {
       double4 p = S.posh[i];
+      bool moved = false;
+      if (F.mode == 1 && i < F.nlive && (isSphT(t) || t == T_DEM)) {
+        startDrift(S, i, t, F.flags, F.hfac, dt, p);
+        moved = true;
+      } else if (F.mode == 2 && i < F.nlive && isSphT(t)) {
+        const double4 vh = S.vhm[i];
+        p.x += dt * vh.x;
+        p.y += dt * vh.y;
+        p.z += dt * vh.z;
+        moved = true;
+      }
       const double dmin[3] = {dmin0, dmin1, dmin2_};
       const double dsp[3] = {dsp0, dsp1, dsp2};
       int oc[3];
@@ -199,6 +279,9 @@ __global__ void k_wrap(Store S, int n, double dmin0, double dmin1, double dmin2_
         }
       }
       if (out || nonper) {
+        if (moved) S.posh[i] = p;
+        if (F.mode == 1 && moved && isSphT(t))
+          startKick<false>(S, i, S, i, t, dt, F.bodyMask, F.ax, F.ay, F.az);
         S.iam[i] = T_DELETED;
         atomicAdd(t == T_DEM ? &g->ndel_dem : &g->ndel_sph, 1ull);
       } else {
@@ -207,8 +290,8 @@ __global__ void k_wrap(Store S, int n, double dmin0, double dmin1, double dmin2_
             if (oc[k] == 0) comp(p, k) = B.Lp[k] + comp(p, k);
             if (oc[k] == 2) comp(p, k) = B.Lm[k] + comp(p, k);
           }
-          S.posh[i] = p;
         }
+        if (notone || moved) S.posh[i] = p;
         hv = dmax2(hv, p.w);
         if (gcnt && isSphT(t)) ghosts = ghostCount(p, gs, B, gb);
       }
@@ -474,14 +557,20 @@ __device__ __forceinline__ void ordShift(int ord, int& sx, int& sy, int& sz) {
   sy = (ord / 3) % 3 - 1;
   sz = ord % 3 - 1;
 }
+// START: the reset right after start(), whose kick part (startKick) is done
+// here on the way (k_wrap did its drift part); a ghost takes its origin's
+// kicked vhat.
+template <bool START>
 __global__ void k_permute(int ncap, int n, const int* __restrict__ perm, Store src, Store dst,
                           const int* __restrict__ gsrc, const int* __restrict__ gord,
                           const int* __restrict__ gcell, Box B, int* __restrict__ inv,
-                          const int* __restrict__ ngp) {
+                          const int* __restrict__ ngp, const DevGlobals* __restrict__ g,
+                          WrapFold F) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= liveEntries(ncap, n, ngp)) return;
   const int e = perm[p];
   int i = e;
+  const double dt = START ? g->dt / 2 : 0.0;
   if (e < n) {
     inv[e] = p;
   } else {
@@ -490,27 +579,38 @@ __global__ void k_permute(int ncap, int n, const int* __restrict__ perm, Store s
     ordShift(gord[e - n], sx, sy, sz);
     const double4 q = src.posh[i];
     const Img im = imagePos(q, sx, sy, sz, B);
+    const int t = src.iam[i];
+    double4 vh = src.vhm[i];
+    if (START && isSphT(t) && t != T_SPHB) {
+      const double4 f = src.frc[i];
+      halfKick(vh, f, dt);
+    }
     dst.posh[p] = make_double4(im.x, im.y, im.z, q.w);
-    dst.vhm[p] = src.vhm[i];
+    dst.vhm[p] = vh;
     dst.dpo[p] = src.dpo[i];
-    dst.iam[p] = src.iam[i];
+    dst.iam[p] = t;
     dst.orig[p] = src.orig[i];
     dst.cell[p] = gcell[e - n];
     return;  // the other fields of a ghost are never read
   }
+  const int t = src.iam[i];
   dst.posh[p] = src.posh[i];
-  dst.vhm[p] = src.vhm[i];
   dst.dpo[p] = src.dpo[i];
-  dst.vel[p] = src.vel[i];
-  dst.v0p[p] = src.v0p[i];
-  dst.vh0d[p] = src.vh0d[i];
-  dst.frc[p] = src.frc[i];
-  dst.fp[p] = src.fp[i];
-  dst.fv[p] = src.fv[i];
-  dst.fb[p] = src.fb[i];
+  if (START && isSphT(t)) {
+    startKick<true>(src, i, dst, p, t, dt, F.bodyMask, F.ax, F.ay, F.az);
+  } else {
+    dst.vhm[p] = src.vhm[i];
+    dst.vel[p] = src.vel[i];
+    dst.v0p[p] = src.v0p[i];
+    dst.vh0d[p] = src.vh0d[i];
+    dst.frc[p] = src.frc[i];
+    dst.fp[p] = src.fp[i];
+    dst.fv[p] = src.fv[i];
+    dst.fb[p] = src.fb[i];
+  }
   dst.fdrag[p] = src.fdrag[i];
   dst.fvel[p] = src.fvel[i];
-  dst.iam[p] = src.iam[i];
+  dst.iam[p] = t;
   dst.orig[p] = src.orig[i];
   dst.cell[p] = src.cell[i];
 }
@@ -1196,52 +1296,11 @@ __global__ void k_start(Store S, int n, sphdem_params P, DevGlobals* g, unsigned
   if (a == 0) g->time += g->dt / 2;
   if (a >= n) return;
   const int t = S.iam[a];
-  if (isSphT(t)) {
-    double4 p = S.posh[a];
-    double4 vh = S.vhm[a];
-    p.x += dt * vh.x;
-    p.y += dt * vh.y;
-    p.z += dt * vh.z;
-    if (t != T_SPHB) {
-      double4 v = S.vel[a];
-      const double4 f = S.frc[a];
-      S.v0p[a] = make_double4(v.x, v.y, v.z, S.v0p[a].w);
-      S.vh0d[a] = make_double4(vh.x, vh.y, vh.z, S.vh0d[a].w);
-      v.x += dt * f.x;
-      v.y += dt * f.y;
-      v.z += dt * f.z;
-      vh.x += dt * f.x;
-      vh.y += dt * f.y;
-      vh.z += dt * f.z;
-      S.vel[a] = v;
-      S.vhm[a] = vh;
-    }
-    // driftRest
-    double4 d = S.dpo[a];
-    d.x += dt * S.vh0d[a].w;
-    S.dpo[a] = d;
-    if (!(P.flags & SPHDEM_F_CONST_H)) p.w = P.hfac * pow_third(vh.w / d.x);
-    S.posh[a] = p;
-    // initSumsMiddle, then the body force of the hook
-    const double4 z = make_double4(0.0, 0.0, 0.0, 0.0);
-    double4 f0 = z;
-    if ((bodyMask >> t) & 1u) {
-      if (ax != 0.0) f0.x += ax;
-      if (ay != 0.0) f0.y += ay;
-      if (az != 0.0) f0.z += az;
-    }
-    S.frc[a] = f0;
-    S.fp[a] = make_double4(0.0, 0.0, 0.0, S.fp[a].w);
-    S.fv[a] = z;
-    S.fb[a] = z;
-  } else if (t == T_DEM) {
-    double4 p = S.posh[a];
-    const double4 vh = S.vhm[a];
-    p.x += dt * vh.x;
-    p.y += dt * vh.y;
-    p.z += dt * vh.z;
-    S.posh[a] = p;
-  }
+  if (!isSphT(t) && t != T_DEM) return;
+  double4 p = S.posh[a];
+  startDrift(S, a, t, P.flags, P.hfac, dt, p);
+  S.posh[a] = p;
+  if (isSphT(t)) startKick<false>(S, a, S, a, t, dt, bodyMask, ax, ay, az);
 }
 
 

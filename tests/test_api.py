@@ -2,6 +2,8 @@
 between them against sphdem_run_steps (whose hooks are fused), parameter
 changes on a live context against the oracle given the same changes, and the
 per-kernel timing window."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -39,6 +41,35 @@ def test_phase_calls_with_hooks_equal_run_steps():
     assert (ga.time, ga.dt, ga.sphStep) == (gb.time, gb.dt, gb.sphStep)
     assert list(ga.custom) == list(gb.custom)
     for nm in FIELDS:
+        assert np.array_equal(a.get(nm), b.get(nm)), nm
+
+
+@pytest.mark.parametrize("hflags", ["var_h", "plain", "const_h"])
+def test_phase_calls_equal_run_steps_on_the_closed_box(hflags):
+    """The dam-break hook set (gravity on sph before middle) on the closed
+    SPH-DEM box, with VAR_H, without it, and with CONST_H: sphdem_run_steps does start() inside
+    middle()'s reset and end()'s drift inside end()'s reset; the phase calls
+    launch them apart.  Same state bit for bit, a density re-initialisation
+    step included."""
+    case = sd.dambreak_case(nf=12, with_dem=28, liq_dem_flags=True)
+    P = case.params.copy()
+    P.reinit_every = 3
+    P.flags &= ~(sd.F_VAR_H_CORRECTION | sd.F_CONST_H)
+    P.flags |= {"var_h": sd.F_VAR_H_CORRECTION, "plain": 0, "const_h": sd.F_CONST_H}[hflags]
+    case = dataclasses.replace(case, params=P)
+    a, b = sd.Sim(case), sd.Sim(case)
+    a.run(4, hookset=sd.HOOKS_DAMBREAK)
+    for _ in range(4):
+        g = b.globals()
+        g.sphStep += 1
+        b.set_globals(g)
+        b.start()
+        b.hook_body_force(1 << sd.SPH, G)
+        b.middle()
+        b.end()
+    ga, gb = a.globals(), b.globals()
+    assert (ga.time, ga.dt, ga.sphStep) == (gb.time, gb.dt, gb.sphStep)
+    for nm in FIELDS + ("vhat", "omega", "dddt", "shepSum"):
         assert np.array_equal(a.get(nm), b.get(nm)), nm
 
 
