@@ -207,7 +207,12 @@ int sphdem_hook_sum_drag(sphdem_ctx* ctx);
 #define SPHDEM_HOOKS_NONE 0
 #define SPHDEM_HOOKS_DAMBREAK 1  /* gravity on sph before middle()                  */
 #define SPHDEM_HOOKS_SEDIMENT 2  /* the sediment3D CcustomSim (customSim.cpp:85-113) */
-/* Runs nsteps of run.cpp:102-118: sphStep++, hooks, start/middle/end. */
+/* Runs nsteps of run.cpp:102-118: sphStep++, hooks, start/middle/end.  On a
+ * step without a density re-initialisation start()'s per-particle work runs
+ * inside middle()'s reset (its drift in the "wrap" kernel, its kicks in the
+ * "permute" kernel) and end()'s drift inside end()'s reset, with the same
+ * arithmetic as the phase calls: the timing window then counts no "start"
+ * launches. */
 int sphdem_run_steps(sphdem_ctx* ctx, int nsteps, int hookset);
 
 /* Ensemble of independent boxes: advances each of the nbox contexts by nsteps
