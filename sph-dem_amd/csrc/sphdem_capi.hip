@@ -896,16 +896,19 @@ int computeHinv(sphdem_ctx* ctx) {
 // calcForce instantiated per flag combination (bit 0 VAR_H_CORRECTION, 1 VISC_ARTIFICIAL,
 // 2 LIQ_DEM, bits 3-4 the viscosity variant); OMG: the walkers with calcOmega fused
 // in (VAR_H only: the OMG entries without VAR_H are empty and never selected)
-// The mixed force walker runs faster at two wavefronts per SIMD than at the
-// four its registers allow (2.83 against 3.10 ms per launch at 160^3: its
-// neighbour gathers thrash the caches with more wavefronts resident).  A
-// dynamic LDS reservation that the kernel never touches holds it there:
-// eight wavefronts per 160 KB CU, whatever the walker's block size.
-constexpr int kContactsLaneFrom = 16384;  // spheres
-unsigned forceLdsReserve(const sphdem_ctx* ctx) {
-  const unsigned blocksPerCU = 8u * 64u / (unsigned)ctx->walkTPB;
-  return std::min(65536u, (163840u / blocksPerCU) & ~1023u) - 1024u;
+// The mixed walkers run faster below the occupancy their registers allow
+// (four wavefronts per SIMD): the force walker at two (2.83 against 3.10 ms
+// per launch at 160^3), the dddt walker at three (1.71 against 1.73 ms); their
+// neighbour gathers thrash the caches with more wavefronts resident.  A
+// dynamic LDS reservation that the kernel never touches holds a walker at
+// `waves` wavefronts per SIMD, whatever its block size (64, 128 or 256
+// threads): blocks + 1 of them do not fit in a 160 KB CU, blocks do.
+unsigned walkerLds(const sphdem_ctx* ctx, unsigned waves) {
+  const unsigned perBlock = (unsigned)ctx->walkTPB / 64u;
+  const unsigned blocks = std::max(1u, 4u * waves / perBlock);
+  return std::min(65536u, (163840u / (blocks + 1u) / 1024u + 1u) * 1024u);
 }
+constexpr unsigned kForceWaves = 2, kDddtWaves = 3;
 template <int F, bool OMG>
 void launchForce(sphdem_ctx* ctx) {
   constexpr bool V = (F & 1) != 0, A = (F & 2) != 0, L = (F & 4) != 0;
@@ -913,7 +916,7 @@ void launchForce(sphdem_ctx* ctx) {
   if constexpr (!OMG || V) {
     if (mixed(ctx))
       hipLaunchKernelGGL((k_force_m<V, M, A, L, OMG>), dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB),
-                         forceLdsReserve(ctx),
+                         walkerLds(ctx, kForceWaves),
                          ctx->stream, ctx->cur, ctx->ns, ctx->nbl, ctx->nbcnt, ctx->mixP, ctx->mixV,
                          ctx->mixD, ctx->mixRes, ctx->P, ctx->dg);
     else
@@ -958,7 +961,8 @@ int passDddt(sphdem_ctx* ctx, bool withOmega = false) {
     Timed tm(ctx, KID_DDDT);
     CHECK(packMix(ctx, MIX_P | MIX_V));
     if (varH(ctx) && withOmega)
-      hipLaunchKernelGGL((k_dddt_m<true, true>), dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB), 0,
+      hipLaunchKernelGGL((k_dddt_m<true, true>), dim3(walkBlocks(ctx, ctx->ns)), dim3(ctx->walkTPB),
+                         walkerLds(ctx, kDddtWaves),
                          ctx->stream, ctx->cur, ctx->ns, ctx->nbl, ctx->nbcnt, ctx->mixP, ctx->mixV,
                          ctx->mixRes, ctx->P.wcon);
     else if (varH(ctx))
@@ -1033,6 +1037,7 @@ int passFpi(sphdem_ctx* ctx) {
   LAUNCHED();
   return 0;  // writes DEM fields only (no ghost copies them)
 }
+constexpr int kContactsLaneFrom = 16384;  // spheres
 int passContacts(sphdem_ctx* ctx) {
   if (ctx->nd == 0) return 0;
   Timed tm(ctx, KID_CONTACTS);

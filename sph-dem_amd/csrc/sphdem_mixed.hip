@@ -254,7 +254,7 @@ __global__ void __launch_bounds__(256) k_force_m(Store S, int ns, const int* __r
     double ax = 0.0, ay = 0.0, az = 0.0, px = 0.0, py = 0.0, pz = 0.0, vx = 0.0, vy = 0.0,
            vz = 0.0;
     const int n = nbcnt[a];
-    constexpr int U = 4;
+    constexpr int U = 8;
     for (int k0 = 0; k0 < n; k0 += U) {
       SPHDEM_FETCH_ENTRIES(U)
       uint4 q[U];
