@@ -56,11 +56,16 @@
  *   gradW's omega under VAR_H     : pa's for both kernel terms of a pair,
  *                                  pinned by the invariant sum m_a omega_a
  *                                  f_a = 0 on a box where omega varies
- *   parity unpinned (only the compiled reference could pin them; restated
- *   from the cited lines):
- *     calcNeighboursCoupling's SPH-side cell rule (checked only against a
- *     brute-force evaluation of the same rule); the order of the
- *     reverse-sync additions of the Shepard scatter.
+ *   calcNeighboursCoupling        : known answers worked by hand from the
+ *     (SPH side)                   reference's text (whole cells beyond Rc
+ *                                  towards negative offsets, skipped cells
+ *                                  within Rc at mixed-sign offsets, the
+ *                                  strict < at plus == t^2); a closest-corner
+ *                                  rule, swapped plus/minus, <= or a sign flip
+ *                                  each fail one; plus brute force
+ *   parity unpinned (only the compiled reference could pin it; restated
+ *   from the cited lines): the order of the reverse-sync additions of the
+ *   Shepard scatter (it moves only the last bits of the sums).
  */
 #include <math.h>
 #include <stdint.h>
