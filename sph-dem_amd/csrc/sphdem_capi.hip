@@ -32,7 +32,7 @@ struct HostFields {
   long long sphStep;
   int ng, ns, nd, K, lgen, countsClean, gcntZeroAt;
   unsigned gstale;
-  bool listsValid, segsValid, gridValid;
+  bool listsValid, segsValid, gridValid, recFresh, pvFresh;
   double ssr;
   Grid gs, gc;
   GhostBand band;
@@ -80,6 +80,10 @@ struct sphdem_ctx {
   int* demNear = nullptr;
   int demNearCap = 0;
   bool listsValid = false;
+  // the list search's records (recFresh) and the mixed form's P and V records
+  // (pvFresh) were written by the last reset's permutation and no position,
+  // h or vhat changed since
+  bool recFresh = false, pvFresh = false;
   double2* hinv = nullptr;  // {1/h, 1/h^5} per SPH-group particle
   long long* okey = nullptr;  // order key of each entry, in scatter order
   // mixed form: packed neighbour records per entry (allocated on first use)
@@ -425,7 +429,10 @@ int tileBlocks(const sphdem_ctx* ctx) {
 // A launch that may write a group of real SPH-group particles marks it stale;
 // a pass refreshes the ghosts when a group it reads of neighbours is stale.
 constexpr unsigned GS_POSH = 1, GS_VHM = 2, GS_DPO = 4, GS_ALL = 7;
-void markStale(sphdem_ctx* ctx, unsigned groups) { ctx->gstale |= groups; }
+void markStale(sphdem_ctx* ctx, unsigned groups) {
+  ctx->gstale |= groups;
+  if (groups & (GS_POSH | GS_VHM)) ctx->recFresh = ctx->pvFresh = false;
+}
 int refreshGhosts(sphdem_ctx* ctx, unsigned need) {
   if (ctx->ng == 0 || !(ctx->gstale & need)) return 0;
   hipLaunchKernelGGL(k_ghost_refresh, dim3(blocks(ctx->ng)), dim3(TPB), 0, ctx->stream, ctx->cur,
@@ -451,10 +458,13 @@ int launchLists(sphdem_ctx* ctx) {
     if (ctx->ns == 0) return 0;
     const ListQuant lq = listQuant(ctx->gs);
     const int gen = ctx->lgen++ & 1;
-    hipLaunchKernelGGL(k_lrec, dim3(blocks(ctx->ns + ctx->ng)), dim3(TPB), 0, ctx->stream,
-                       ctx->cur.posh, ctx->ns, ctx->n, ctx->ng, lq, ctx->lrec, ctx->lflag, gen,
-                       ctx->graphMode ? &ctx->dg->ghostTotal : (const int*)nullptr);
-    LAUNCHED();
+    if (!ctx->recFresh) {  // else the reset's permutation wrote them
+      hipLaunchKernelGGL(k_lrec, dim3(blocks(ctx->ns + ctx->ng)), dim3(TPB), 0, ctx->stream,
+                         ctx->cur.posh, ctx->ns, ctx->n, ctx->ng, lq, ctx->lrec, ctx->lflag, gen,
+                         ctx->graphMode ? &ctx->dg->ghostTotal : (const int*)nullptr);
+      LAUNCHED();
+    }
+    ctx->recFresh = false;
     const int per = (ctx->ns + 7) / 8;
     hipLaunchKernelGGL(k_lists_t<32>, dim3(8 * ((per + TPB - 1) / TPB)), dim3(TPB), 0,
                        ctx->stream, ctx->cur.posh, ctx->lrec, ctx->cur.cell, ctx->gs, ctx->gbase,
@@ -629,8 +639,12 @@ int checkFlags(sphdem_ctx* ctx) {
   return flagError(ctx, ctx->hostG->errflags);
 }
 
+bool mixed(const sphdem_ctx* c);
+Quant mixQuant(const Grid& g);
 // midInit: middle()'s reset, which also starts the dt minimum at maxdt
-// fold: the half-step before the reset, done by k_wrap and k_permute (WrapFold)
+// fold: the half-step before the reset, done by k_wrap and k_permute
+// (WrapFold); its permutation also writes the records the passes after a
+// step's reset pack from the store (recFresh, pvFresh)
 int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false,
             const WrapFold& fold = WrapFold{}) {
   const sphdem_params& P = ctx->P;
@@ -644,6 +658,7 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false,
     return fail(ctx, SPHDEM_E_STATE, "a captured reset needs the state a reset on the same grid "
                                      "leaves");
   ctx->segsValid = false;
+  ctx->recFresh = ctx->pvFresh = false;
   // hmax_bits is cleared by the previous reset's k_bin (or by the upload);
   // gcnt[n], the scans' closing zero, is written by nothing else
   if (n == 0) HIPC(hipMemsetAsync(&ctx->dg->hmax_bits, 0, sizeof(unsigned long long), ctx->stream));
@@ -759,16 +774,33 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false,
                        ctx->start, ctx->tmp, ctx->okey, ctx->perm, ngp);
     LAUNCHED();
   }
+  PermRecords R{};
+  if (fold.mode && ctx->ns > 0) {
+    R.ns = ctx->ns;
+    if (ctx->tileSeg == 0) {
+      R.lists = 1;
+      R.lq = listQuant(ctx->gs);
+      R.rec = ctx->lrec;
+      R.lflag = ctx->lflag;
+      R.gen = ctx->lgen & 1;  // the generation the next list build takes
+    }
+    if (mixed(ctx) && ctx->mixCap >= ctx->cap + ctx->ghostCap) {
+      R.mixed = 1;
+      R.Q = mixQuant(ctx->gs);
+      R.P = ctx->mixP;
+      R.V = ctx->mixV;
+    }
+  }
   {
     Timed tm(ctx, KID_PERMUTE);
     if (fold.mode == 1)
       hipLaunchKernelGGL(k_permute<true>, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ntot, n,
                          ctx->perm, ctx->cur, ctx->alt, ctx->gsrc, ctx->gord, ctx->gcell,
-                         ctx->box, ctx->inv, ngp, ctx->dg, fold);
+                         ctx->box, ctx->inv, ngp, ctx->dg, fold, R);
     else
       hipLaunchKernelGGL(k_permute<false>, dim3(blocks(ntot)), dim3(TPB), 0, ctx->stream, ntot, n,
                          ctx->perm, ctx->cur, ctx->alt, ctx->gsrc, ctx->gord, ctx->gcell,
-                         ctx->box, ctx->inv, ngp, ctx->dg, fold);
+                         ctx->box, ctx->inv, ngp, ctx->dg, fold, R);
     LAUNCHED();
   }
   if (ng > 0) {
@@ -781,6 +813,8 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false,
   std::swap(ctx->cur, ctx->alt);
   std::swap(ctx->storeMem[0], ctx->storeMem[1]);
   ctx->listsValid = false;  // updateDomain clears every neighbour list (dataLL.cpp:374)
+  ctx->recFresh = R.lists != 0;
+  ctx->pvFresh = R.mixed != 0;
   return side.join();
 }
 
@@ -822,6 +856,20 @@ int packPosm(sphdem_ctx* ctx) {
   }
   return 0;
 }
+// The mixed form's fixed-point frame on grid g: the lower corner of cell 0
+// and the power of two that spreads the grid's extent over 2^32 steps.
+Quant mixQuant(const Grid& g) {
+  Quant Q;
+  double extent = 0.0;
+  for (int d = 0; d < 3; d++) {
+    Q.o[d] = g.cmin[d] - g.off * g.edge;  // lower corner of cell 0
+    extent = std::max(extent, g.dim[d] * g.edge);
+  }
+  const int k = (int)std::ceil(std::log2(extent)) - 32;
+  Q.inv = std::ldexp(1.0, -k);
+  Q.res = std::ldexp(1.0f, k);
+  return Q;
+}
 // Packs the mixed form's neighbour records (groups MIX_P/V/D) of every
 // SPH-group entry and ghost, positions in fixed point over the current grid.
 int packMix(sphdem_ctx* ctx, unsigned groups) {
@@ -836,22 +884,15 @@ int packMix(sphdem_ctx* ctx, unsigned groups) {
     HIPC(hipMalloc(&ctx->mixV, sizeof(float4) * need));
     HIPC(hipMalloc(&ctx->mixD, sizeof(float4) * need));
     ctx->mixCap = need;
+    ctx->pvFresh = false;
     ctx->allocEpoch++;
   }
-  const Grid& g = ctx->gs;
-  Quant Q;
-  double extent = 0.0;
-  for (int d = 0; d < 3; d++) {
-    Q.o[d] = g.cmin[d] - g.off * g.edge;  // lower corner of cell 0
-    extent = std::max(extent, g.dim[d] * g.edge);
-  }
-  // unit: the power of two that spreads the grid's extent over 2^32 steps
-  const int k = (int)std::ceil(std::log2(extent)) - 32;
-  Q.inv = std::ldexp(1.0, -k);
-  Q.res = std::ldexp(1.0f, k);
+  const Quant Q = mixQuant(ctx->gs);
   ctx->mixRes = Q.res;
+  if (ctx->pvFresh) groups &= ~(MIX_P | MIX_V);  // the reset's permutation wrote them
+  ctx->pvFresh = false;
   const int cnt = ctx->ns + ctx->ng;
-  if (cnt > 0) {
+  if (cnt > 0 && groups) {
     hipLaunchKernelGGL(k_pack_mix, dim3(blocks(cnt)), dim3(TPB), 0, ctx->stream, ctx->cur,
                        ctx->ns, ctx->n, ctx->ng, Q, groups, ctx->mixP, ctx->mixV, ctx->mixD);
     LAUNCHED();
@@ -1895,6 +1936,8 @@ HostFields saveHost(const sphdem_ctx* c) {
   h.gcntZeroAt = c->gcntZeroAt;
   h.gstale = c->gstale;
   h.listsValid = c->listsValid;
+  h.recFresh = c->recFresh;
+  h.pvFresh = c->pvFresh;
   h.segsValid = c->segsValid;
   h.gridValid = c->gridValid;
   h.ssr = c->ssr;
@@ -1918,6 +1961,8 @@ void loadHost(sphdem_ctx* c, const HostFields& h) {
   c->gcntZeroAt = h.gcntZeroAt;
   c->gstale = h.gstale;
   c->listsValid = h.listsValid;
+  c->recFresh = h.recFresh;
+  c->pvFresh = h.pvFresh;
   c->segsValid = h.segsValid;
   c->gridValid = h.gridValid;
   c->ssr = h.ssr;

@@ -95,6 +95,69 @@ struct WrapFold {
   double ax, ay, az;
 };
 
+// The per-target list search's record of an SPH-group entry or ghost: its
+// position as 16-bit fixed point modulo 2^16 in the unit 1/uinv (a power of
+// two) from the origin o, and the bound code B16 = ceil(((2 h / u + kQBand)^2
+// + 1) / 1024) above z (the search's use of it is described with k_lists_t).
+constexpr double kQBand = 1.74;
+struct ListQuant {
+  double o[3];  // any common origin (the grid's cmin)
+  double uinv;  // 1/u, a power of two
+};
+__device__ __forceinline__ unsigned q16(double x, double o, double uinv) {
+  return (unsigned)(long long)floor((x - o) * uinv) & 0xffffu;
+}
+// false (and a zero record) for a non-finite position or a non-positive h
+__device__ __forceinline__ bool listRecord(const double4& p, const ListQuant& lq, uint2& r) {
+  const bool ok = isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(p.w) && p.w > 0.0;
+  if (!ok) {
+    r = make_uint2(0u, 0u);
+    return false;
+  }
+  const double b = 2.0 * p.w * lq.uinv + kQBand;
+  const double bq = fmin(b * b + 1.0, 1.0e9);
+  const unsigned b16 = (unsigned)fmin(ceil(bq * (1.0 / 1024.0)), 65535.0);
+  r = make_uint2(q16(p.x, lq.o[0], lq.uinv) | (q16(p.y, lq.o[1], lq.uinv) << 16),
+                 q16(p.z, lq.o[2], lq.uinv) | (b16 << 16));
+  return true;
+}
+
+// Fixed-point frame of the mixed form's packed positions: grid lower corner
+// and the unit.
+struct Quant {
+  double o[3];
+  double inv;  // 1 / unit (a power of two)
+  float res;   // the unit
+};
+__device__ __forceinline__ unsigned quant1(double x, double o, double inv) {
+  const double u = rint((x - o) * inv);
+  return (unsigned)fmin(fmax(u, 0.0), 4294967295.0);
+}
+// The mixed form's P record (fixed-point position, FP32 mass) and V record
+// (FP32 vhat and h) of position-and-h p and vhat-and-mass vh.
+__device__ __forceinline__ uint4 mixRecordP(const double4& p, const double4& vh, const Quant& Q) {
+  return make_uint4(quant1(p.x, Q.o[0], Q.inv), quant1(p.y, Q.o[1], Q.inv),
+                    quant1(p.z, Q.o[2], Q.inv), __float_as_uint((float)vh.w));
+}
+__device__ __forceinline__ float4 mixRecordV(const double4& p, const double4& vh) {
+  return make_float4((float)vh.x, (float)vh.y, (float)vh.z, (float)p.w);
+}
+
+// Records a folded reset's permutation writes on the way (WrapFold): the
+// SPH-group entries [0, ns) and the ghosts, from the values it stores.
+struct PermRecords {
+  int lists;  // the list search's records (rec; non-finite input to lflag[gen & 1])
+  int mixed;  // the mixed form's P and V records
+  int ns;
+  ListQuant lq;
+  uint2* rec;
+  int* lflag;
+  int gen;
+  Quant Q;
+  uint4* P;
+  float4* V;
+};
+
 // Block index (xcdBlock) and global thread index (xcdThread) with the blocks
 // of a 1-D grid regrouped by XCD.  The
 // dispatcher deals workgroups round robin to the 8 XCDs (block b to XCD b % 8);

@@ -557,16 +557,31 @@ __device__ __forceinline__ void ordShift(int ord, int& sx, int& sy, int& sz) {
   sy = (ord / 3) % 3 - 1;
   sz = ord % 3 - 1;
 }
+// The records of PermRecords for store entry p from its new position-and-h
+// q and vhat-and-mass vh.
+__device__ __forceinline__ void permRecords(const PermRecords& R, int p, const double4& q,
+                                            const double4& vh) {
+  if (R.lists) {
+    uint2 r;
+    if (!listRecord(q, R.lq, r)) atomicOr(&R.lflag[R.gen & 1], 1);
+    R.rec[p] = r;
+  }
+  if (R.mixed) {
+    R.P[p] = mixRecordP(q, vh, R.Q);
+    R.V[p] = mixRecordV(q, vh);
+  }
+}
 // START: the reset right after start(), whose kick part (startKick) is done
 // here on the way (k_wrap did its drift part); a ghost takes its origin's
-// kicked vhat.
+// kicked vhat.  R: records written on the way (a folded reset's).
 template <bool START>
 __global__ void k_permute(int ncap, int n, const int* __restrict__ perm, Store src, Store dst,
                           const int* __restrict__ gsrc, const int* __restrict__ gord,
                           const int* __restrict__ gcell, Box B, int* __restrict__ inv,
                           const int* __restrict__ ngp, const DevGlobals* __restrict__ g,
-                          WrapFold F) {
+                          WrapFold F, PermRecords R) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0 && R.lists) R.lflag[(R.gen + 1) & 1] = 0;  // the next generation's flag
   if (p >= liveEntries(ncap, n, ngp)) return;
   const int e = perm[p];
   int i = e;
@@ -585,17 +600,28 @@ __global__ void k_permute(int ncap, int n, const int* __restrict__ perm, Store s
       const double4 f = src.frc[i];
       halfKick(vh, f, dt);
     }
-    dst.posh[p] = make_double4(im.x, im.y, im.z, q.w);
+    const double4 qi = make_double4(im.x, im.y, im.z, q.w);
+    dst.posh[p] = qi;
     dst.vhm[p] = vh;
     dst.dpo[p] = src.dpo[i];
     dst.iam[p] = t;
     dst.orig[p] = src.orig[i];
     dst.cell[p] = gcell[e - n];
+    permRecords(R, p, qi, vh);
     return;  // the other fields of a ghost are never read
   }
   const int t = src.iam[i];
-  dst.posh[p] = src.posh[i];
+  const double4 q = src.posh[i];
+  dst.posh[p] = q;
   dst.dpo[p] = src.dpo[i];
+  if (p < R.ns) {
+    double4 vh = src.vhm[i];
+    if (START && isSphT(t) && t != T_SPHB) {
+      const double4 f = src.frc[i];
+      halfKick(vh, f, dt);
+    }
+    permRecords(R, p, q, vh);
+  }
   if (START && isSphT(t)) {
     startKick<true>(src, i, dst, p, t, dt, F.bodyMask, F.ax, F.ay, F.az);
   } else {

@@ -45,16 +45,7 @@ __device__ __forceinline__ float wendPdWh_m(float wcon, float q, float rh, float
   return i3 * pdKh - c3 * wendK_m(wcon, q);
 }
 
-// Fixed-point frame of the packed positions: grid lower corner and the unit.
-struct Quant {
-  double o[3];
-  double inv;  // 1 / unit (a power of two)
-  float res;   // the unit
-};
-__device__ __forceinline__ unsigned quant1(double x, double o, double inv) {
-  const double u = rint((x - o) * inv);
-  return (unsigned)fmin(fmax(u, 0.0), 4294967295.0);
-}
+// (Quant, quant1, mixRecordP and mixRecordV: sphdem_kernels.h)
 struct DxF {
   float x, y, z, r2;
 };
@@ -77,13 +68,8 @@ __global__ void k_pack_mix(Store S, int ns, int n, int ng, Quant Q, unsigned gro
   if (i >= ns + ng) return;
   const int e = i < ns ? i : n + (i - ns);
   const double4 p = S.posh[e];
-  if (groups & MIX_P)
-    P[e] = make_uint4(quant1(p.x, Q.o[0], Q.inv), quant1(p.y, Q.o[1], Q.inv),
-                      quant1(p.z, Q.o[2], Q.inv), __float_as_uint((float)S.vhm[e].w));
-  if (groups & MIX_V) {
-    const double4 v = S.vhm[e];
-    V[e] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)p.w);
-  }
+  if (groups & MIX_P) P[e] = mixRecordP(p, S.vhm[e], Q);
+  if (groups & MIX_V) V[e] = mixRecordV(p, S.vhm[e]);
   if (groups & MIX_D) {
     const double4 d = S.dpo[e];
     const double rh = 1.0 / p.w;

@@ -308,14 +308,7 @@ __global__ void __launch_bounds__(256) k_lists(const double4* __restrict__ pos,
 // is "maybe" whenever D <= 1024 * B16.  A non-finite position
 // or a non-positive h anywhere sets a flag that sends every candidate to the
 // FP64 test.
-constexpr double kQBand = 1.74;
-struct ListQuant {
-  double o[3];  // any common origin (the grid's cmin)
-  double uinv;  // 1/u, a power of two
-};
-__device__ __forceinline__ unsigned q16(double x, double o, double uinv) {
-  return (unsigned)(long long)floor((x - o) * uinv) & 0xffffu;
-}
+// (kQBand, ListQuant, q16 and listRecord: sphdem_kernels.h)
 // Records of the SPH-group entries [0, ns) and the ghosts [n, n + ng).  The
 // flag slot of generation gen collects non-finite input; the next
 // generation's slot is cleared for the next build.
@@ -326,18 +319,9 @@ __global__ void k_lrec(const double4* __restrict__ posh, int ns, int n, int ng, 
   if (i == 0) lflag[(gen + 1) & 1] = 0;
   if (i >= ns + (ngp ? min(*ngp, ng) : ng)) return;
   const int s = i < ns ? i : n + (i - ns);
-  const double4 p = posh[s];
-  const bool ok = isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(p.w) && p.w > 0.0;
-  if (!ok) {
-    atomicOr(&lflag[gen & 1], 1);
-    rec[s] = make_uint2(0u, 0u);
-    return;
-  }
-  const double b = 2.0 * p.w * lq.uinv + kQBand;
-  const double bq = fmin(b * b + 1.0, 1.0e9);
-  const unsigned b16 = (unsigned)fmin(ceil(bq * (1.0 / 1024.0)), 65535.0);
-  rec[s] = make_uint2(q16(p.x, lq.o[0], lq.uinv) | (q16(p.y, lq.o[1], lq.uinv) << 16),
-                      q16(p.z, lq.o[2], lq.uinv) | (b16 << 16));
+  uint2 r;
+  if (!listRecord(posh[s], lq, r)) atomicOr(&lflag[gen & 1], 1);
+  rec[s] = r;
 }
 
 // acc = 2 acc + (x >= D ? 1 : 0): a compare into VCC and an add with carry,
