@@ -32,7 +32,7 @@ struct HostFields {
   long long sphStep;
   int ng, ns, nd, K, lgen, countsClean, gcntZeroAt;
   unsigned gstale;
-  bool listsValid, segsValid, gridValid, recFresh, pvFresh;
+  bool listsValid, segsValid, gridValid, recFresh, pvFresh, dFresh;
   double ssr;
   Grid gs, gc;
   GhostBand band;
@@ -84,6 +84,9 @@ struct sphdem_ctx {
   // (pvFresh) were written by the last reset's permutation and no position,
   // h or vhat changed since
   bool recFresh = false, pvFresh = false;
+  // the mixed form's D records were written by the last EOS pass and no
+  // density, pdr2 or h changed since (a box without ghosts)
+  bool dFresh = false;
   double2* hinv = nullptr;  // {1/h, 1/h^5} per SPH-group particle
   long long* okey = nullptr;  // order key of each entry, in scatter order
   // mixed form: packed neighbour records per entry (allocated on first use)
@@ -432,6 +435,7 @@ constexpr unsigned GS_POSH = 1, GS_VHM = 2, GS_DPO = 4, GS_ALL = 7;
 void markStale(sphdem_ctx* ctx, unsigned groups) {
   ctx->gstale |= groups;
   if (groups & (GS_POSH | GS_VHM)) ctx->recFresh = ctx->pvFresh = false;
+  if (groups & (GS_POSH | GS_DPO)) ctx->dFresh = false;
 }
 int refreshGhosts(sphdem_ctx* ctx, unsigned need) {
   if (ctx->ng == 0 || !(ctx->gstale & need)) return 0;
@@ -658,7 +662,7 @@ int doReset(sphdem_ctx* ctx, bool keepSsr = false, bool midInit = false,
     return fail(ctx, SPHDEM_E_STATE, "a captured reset needs the state a reset on the same grid "
                                      "leaves");
   ctx->segsValid = false;
-  ctx->recFresh = ctx->pvFresh = false;
+  ctx->recFresh = ctx->pvFresh = ctx->dFresh = false;
   // hmax_bits is cleared by the previous reset's k_bin (or by the upload);
   // gcnt[n], the scans' closing zero, is written by nothing else
   if (n == 0) HIPC(hipMemsetAsync(&ctx->dg->hmax_bits, 0, sizeof(unsigned long long), ctx->stream));
@@ -884,13 +888,14 @@ int packMix(sphdem_ctx* ctx, unsigned groups) {
     HIPC(hipMalloc(&ctx->mixV, sizeof(float4) * need));
     HIPC(hipMalloc(&ctx->mixD, sizeof(float4) * need));
     ctx->mixCap = need;
-    ctx->pvFresh = false;
+    ctx->pvFresh = ctx->dFresh = false;
     ctx->allocEpoch++;
   }
   const Quant Q = mixQuant(ctx->gs);
   ctx->mixRes = Q.res;
   if (ctx->pvFresh) groups &= ~(MIX_P | MIX_V);  // the reset's permutation wrote them
-  ctx->pvFresh = false;
+  if (ctx->dFresh) groups &= ~MIX_D;              // the EOS pass wrote them
+  ctx->pvFresh = ctx->dFresh = false;
   const int cnt = ctx->ns + ctx->ng;
   if (cnt > 0 && groups) {
     hipLaunchKernelGGL(k_pack_mix, dim3(blocks(cnt)), dim3(TPB), 0, ctx->stream, ctx->cur,
@@ -918,10 +923,16 @@ int passOmega(sphdem_ctx* ctx) {
 }
 int passEos(sphdem_ctx* ctx) {
   Timed tm(ctx, KID_EOS);
+  // the mixed D records too, when there are no ghosts (whose records the
+  // packing takes after their refresh) and the record buffers exist
+  float4* D = mixed(ctx) && ctx->ng == 0 && ctx->mixCap >= ctx->cap + ctx->ghostCap
+                  ? ctx->mixD
+                  : nullptr;
   hipLaunchKernelGGL(k_eos, dim3(blocks(ctx->ns)), dim3(TPB), 0, ctx->stream, ctx->cur, ctx->ns,
-                     ctx->P);
+                     ctx->P, D);
   LAUNCHED();
   markStale(ctx, GS_DPO);
+  ctx->dFresh = D != nullptr;
   return 0;
 }
 // {1/h, 1/h^5} of every SPH-group particle, for the passes that need them of
@@ -1938,6 +1949,7 @@ HostFields saveHost(const sphdem_ctx* c) {
   h.listsValid = c->listsValid;
   h.recFresh = c->recFresh;
   h.pvFresh = c->pvFresh;
+  h.dFresh = c->dFresh;
   h.segsValid = c->segsValid;
   h.gridValid = c->gridValid;
   h.ssr = c->ssr;
@@ -1963,6 +1975,7 @@ void loadHost(sphdem_ctx* c, const HostFields& h) {
   c->listsValid = h.listsValid;
   c->recFresh = h.recFresh;
   c->pvFresh = h.pvFresh;
+  c->dFresh = h.dFresh;
   c->segsValid = h.segsValid;
   c->gridValid = h.gridValid;
   c->ssr = h.ssr;

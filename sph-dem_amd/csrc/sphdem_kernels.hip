@@ -726,7 +726,9 @@ __global__ void k_snapshot(Store src, Store dst, int n, const DevGlobals* __rest
 // cached lists were built with.
 
 // CsphIncompress::calcPressSpsoundPdr2 sphIncompress.h:863-896
-__global__ void k_eos(Store S, int ns, sphdem_params P) {
+// D (optional): also the mixed form's D record {dens, pdr2, 1/h, 1/h^5} of
+// each entry, as the mixed record packing would write it afterwards.
+__global__ void k_eos(Store S, int ns, sphdem_params P, float4* __restrict__ D) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= ns) return;
   if (!isSphT(S.iam[a])) return;
@@ -737,8 +739,13 @@ __global__ void k_eos(Store S, int ns, sphdem_params P) {
   else
     press = P.prb * (pow(d.x / P.refd, 7.0) - 1.0);
   // only pdr2 and press are stored (omega may be written beside this pass)
-  S.dpo[a].y = press / (d.x * d.x);
+  const double pdr2 = press / (d.x * d.x);
+  S.dpo[a].y = pdr2;
   S.v0p[a].w = press;
+  if (D) {
+    const double rh = 1.0 / S.posh[a].w;
+    D[a] = make_float4((float)d.x, (float)pdr2, (float)rh, (float)(rh * rh * rh * rh * rh));
+  }
 }
 
 // Nsph::accelCondition sph.h:566-568 over sph, sphBoundary and dem

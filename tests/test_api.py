@@ -16,12 +16,15 @@ FIELDS = ("r", "v", "vhat", "f", "fp", "fb", "fdrag", "dens", "h", "press", "por
 G = np.array([0.0, 0.0, -9.81])
 
 
-def test_phase_calls_with_hooks_equal_run_steps():
+@pytest.mark.parametrize("precision", [sd.FP64, sd.MIXED])
+def test_phase_calls_with_hooks_equal_run_steps(precision):
     """run.cpp's loop spelled out -- sphStep++, start, gravity on sph, middle,
     the sediment3D beforeEnd hooks, end, afterEnd freeze -- ends where
-    sphdem_run_steps with the sediment hook set ends, bit for bit."""
+    sphdem_run_steps with the sediment hook set ends, bit for bit (the steps
+    take the list and mixed records from the resets' permutation, the phase
+    calls pack them apart)."""
     case = sd.sediment_case(nf=12, n_dem=60)
-    a, b = sd.Sim(case), sd.Sim(case)
+    a, b = sd.Sim(case, precision=precision), sd.Sim(case, precision=precision)
     a.run(3)
     P = case.params
     for _ in range(3):
@@ -44,8 +47,9 @@ def test_phase_calls_with_hooks_equal_run_steps():
         assert np.array_equal(a.get(nm), b.get(nm)), nm
 
 
+@pytest.mark.parametrize("precision", [sd.FP64, sd.MIXED])
 @pytest.mark.parametrize("hflags", ["var_h", "plain", "const_h"])
-def test_phase_calls_equal_run_steps_on_the_closed_box(hflags):
+def test_phase_calls_equal_run_steps_on_the_closed_box(hflags, precision):
     """The dam-break hook set (gravity on sph before middle) on the closed
     SPH-DEM box, with VAR_H, without it, and with CONST_H: sphdem_run_steps does start() inside
     middle()'s reset and end()'s drift inside end()'s reset; the phase calls
@@ -57,7 +61,7 @@ def test_phase_calls_equal_run_steps_on_the_closed_box(hflags):
     P.flags &= ~(sd.F_VAR_H_CORRECTION | sd.F_CONST_H)
     P.flags |= {"var_h": sd.F_VAR_H_CORRECTION, "plain": 0, "const_h": sd.F_CONST_H}[hflags]
     case = dataclasses.replace(case, params=P)
-    a, b = sd.Sim(case), sd.Sim(case)
+    a, b = sd.Sim(case, precision=precision), sd.Sim(case, precision=precision)
     a.run(4, hookset=sd.HOOKS_DAMBREAK)
     for _ in range(4):
         g = b.globals()
