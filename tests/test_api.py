@@ -77,6 +77,43 @@ def test_phase_calls_equal_run_steps_on_the_closed_box(hflags, precision):
         assert np.array_equal(a.get(nm), b.get(nm)), nm
 
 
+def test_phase_calls_equal_run_steps_when_the_reset_deletes():
+    """A particle moved above the domain is deleted by middle()'s reset.
+    sphdem_run_steps does start() inside that reset, so the particle gets
+    start()'s drift and kick there before it is marked deleted; the phase
+    calls launch start() first.  Same state bit for bit, the deleted
+    particle's fields included."""
+    case = sd.sediment_case(nf=12, n_dem=60)
+    a, b = sd.Sim(case), sd.Sim(case)
+    a.run(2)
+    b.run(2)
+    for s in (a, b):
+        r = s.get("r")
+        k = int(np.nonzero(case.iam == sd.SPH)[0][5])
+        r[k, 2] = case.params.proc_domain[5] + 0.5 * case.params.h  # leaves in start()'s drift too
+        s.set("r", r)
+    a.run(2)
+    P = case.params
+    for _ in range(2):
+        g = b.globals()
+        g.sphStep += 1
+        b.set_globals(g)
+        b.start()
+        b.hook_body_force(1 << sd.SPH, G)
+        b.middle()
+        b.hook_sum_drag()
+        b.hook_body_force(1 << sd.DEM, G)
+        b.hook_dem_floor()
+        b.hook_freeze(1 << sd.DEM, P.time_drop)
+        b.end()
+        b.hook_freeze(1 << sd.DEM, P.time_drop)
+    assert a.num_deleted() == b.num_deleted() == 1
+    ga, gb = a.globals(), b.globals()
+    assert (ga.time, ga.dt, ga.sphStep) == (gb.time, gb.dt, gb.sphStep)
+    for nm in FIELDS + ("vhat", "dddt", "iam"):
+        assert np.array_equal(a.get(nm), b.get(nm)), nm
+
+
 def test_parameter_change_mid_run_matches_the_oracle():
     """sphdem_set_params between steps (drag law, stiffness, viscosity) takes
     effect from the next step as in the oracle given the same change."""
