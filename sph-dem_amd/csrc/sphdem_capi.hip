@@ -510,6 +510,8 @@ void setupGrids(sphdem_ctx* ctx) {
     gc.dim[0] = gc.dim[1] = gc.dim[2] = 1;
   }
   gc.base = Cs;
+  gs.inv = 1.0 / gs.edge;  // the kernels' cell indices (cellTrunc)
+  gc.inv = 1.0 / gc.edge;
   const int Cc = gc.dim[0] * gc.dim[1] * gc.dim[2];
   // keys: SPH cells, DEM cells, one for deleted particles, then (periodic
   // boxes) the SPH cells again for the ghosts
@@ -536,6 +538,7 @@ void setupStatic(sphdem_ctx* ctx) {
     const double Rc = P.coupling_radius, DSR = P.dem_search_radius;
     for (int d = 0; d < 3; d++) R.cmin[d] = P.proc_domain[2 * d];
     R.e = 2 * DSR;
+    R.einv = 1.0 / R.e;
     R.s = 2 * Rc / DSR;
     const double t = 0.5 * Rc / DSR;
     R.thr = t * t;

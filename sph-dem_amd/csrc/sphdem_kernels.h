@@ -193,6 +193,7 @@ struct Grid {
   double off;
   int dim[3];
   int base;
+  double inv;  // 1 / edge (0: every cell index takes the division)
 };
 
 struct Box {

@@ -23,6 +23,19 @@ __device__ __forceinline__ double imgc(double r, int s, double Lp, double Lm) {
 __device__ __forceinline__ int cellOf(double x, double cmin, double edge, double off) {
   return (int)((x - cmin) / edge + off);
 }
+// (int)((x - o) / e + add), the reference's cell arithmetic, with the
+// quotient first taken as the product by inv (1/e rounded): that differs from
+// the quotient by a few units in the last place, so both truncate alike
+// unless the value lies within 1e-9 of an integer, and only then is the
+// division done.  The same cell as the division for every x.
+__device__ __forceinline__ int cellTrunc(double x, double o, double e, double inv, double add) {
+  const double t = (x - o) * inv + add;
+  if (fabs(t - rint(t)) > 1e-9) return (int)t;
+  return (int)((x - o) / e + add);
+}
+__device__ __forceinline__ int cellOf(double x, const Grid& g, int d) {
+  return cellTrunc(x, g.cmin[d], g.edge, g.inv, g.off);
+}
 __device__ __forceinline__ double dmin2(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double dmax2(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ unsigned long long dbits(double x) {
@@ -87,19 +100,19 @@ __device__ __forceinline__ void forCells(const Grid& g, const int* __restrict__ 
   for (int ix = 0; ix < nsx; ix++) {
     const int sx = ix == 0 ? 0 : (ix == 1 ? -1 : 1);
     const double tx = sx == 0 ? q[0] : (sx == 1 ? q[0] - B.Lp[0] : q[0] - B.Lm[0]);
-    const int cx = cellOf(tx, g.cmin[0], g.edge, g.off);
+    const int cx = cellOf(tx, g, 0);
     const int x0 = max(cx - reach, 0), x1 = min(cx + reach, g.dim[0] - 1);
     if (x0 > x1) continue;
     for (int iy = 0; iy < nsy; iy++) {
       const int sy = iy == 0 ? 0 : (iy == 1 ? -1 : 1);
       const double ty = sy == 0 ? q[1] : (sy == 1 ? q[1] - B.Lp[1] : q[1] - B.Lm[1]);
-      const int cy = cellOf(ty, g.cmin[1], g.edge, g.off);
+      const int cy = cellOf(ty, g, 1);
       const int y0 = max(cy - reach, 0), y1 = min(cy + reach, g.dim[1] - 1);
       if (y0 > y1) continue;
       for (int iz = 0; iz < nsz; iz++) {
         const int sz = iz == 0 ? 0 : (iz == 1 ? -1 : 1);
         const double tz = sz == 0 ? q[2] : (sz == 1 ? q[2] - B.Lp[2] : q[2] - B.Lm[2]);
-        const int cz = cellOf(tz, g.cmin[2], g.edge, g.off);
+        const int cz = cellOf(tz, g, 2);
         const int z0 = max(cz - reach, 0), z1 = min(cz + reach, g.dim[2] - 1);
         if (z0 > z1 || !keep(cx, cy, cz)) continue;
         for (int a = x0; a <= x1; a++)
@@ -149,8 +162,8 @@ __device__ __forceinline__ int ghostShiftsAxis(double x, int d, const Grid& g, c
   int m = 0;
   if (!B.periodic[d]) return 0;
   const int lo = gb.lo[d], hi = gb.hi[d];
-  const int cm = cellOf(B.Lm[d] + x, g.cmin[d], g.edge, g.off);
-  const int cp = cellOf(B.Lp[d] + x, g.cmin[d], g.edge, g.off);
+  const int cm = cellOf(B.Lm[d] + x, g, d);
+  const int cp = cellOf(B.Lp[d] + x, g, d);
   if (cm >= lo && cm <= hi) m |= 1;
   if (cp >= lo && cp <= hi) m |= 2;
   cell[0] = cm;
@@ -343,7 +356,7 @@ __global__ void k_bin(Store S, int n, Grid gs, Grid gc, int keyDeleted, int* __r
       const Grid& gr = (t == T_DEM) ? gc : gs;
       int c[3];
       for (int d = 0; d < 3; d++) {
-        c[d] = cellOf(comp(const_cast<double4&>(p), d), gr.cmin[d], gr.edge, gr.off);
+        c[d] = cellOf(comp(const_cast<double4&>(p), d), gr, d);
         if (c[d] < 0 || c[d] >= gr.dim[d]) {
           atomicOr(&g->errflags, 1u);
           c[d] = c[d] < 0 ? 0 : gr.dim[d] - 1;
@@ -796,19 +809,19 @@ __device__ __forceinline__ void forCellRanges(const Grid& g, const int* __restri
   for (int ix = 0; ix < nsx; ix++) {
     const int sx = ix == 0 ? 0 : (ix == 1 ? -1 : 1);
     const double tx = sx == 0 ? q[0] : (sx == 1 ? q[0] - B.Lp[0] : q[0] - B.Lm[0]);
-    const int cx = cellOf(tx, g.cmin[0], g.edge, g.off);
+    const int cx = cellOf(tx, g, 0);
     const int x0 = max(cx - reach, 0), x1 = min(cx + reach, g.dim[0] - 1);
     if (x0 > x1) continue;
     for (int iy = 0; iy < nsy; iy++) {
       const int sy = iy == 0 ? 0 : (iy == 1 ? -1 : 1);
       const double ty = sy == 0 ? q[1] : (sy == 1 ? q[1] - B.Lp[1] : q[1] - B.Lm[1]);
-      const int cy = cellOf(ty, g.cmin[1], g.edge, g.off);
+      const int cy = cellOf(ty, g, 1);
       const int y0 = max(cy - reach, 0), y1 = min(cy + reach, g.dim[1] - 1);
       if (y0 > y1) continue;
       for (int iz = 0; iz < nsz; iz++) {
         const int sz = iz == 0 ? 0 : (iz == 1 ? -1 : 1);
         const double tz = sz == 0 ? q[2] : (sz == 1 ? q[2] - B.Lp[2] : q[2] - B.Lm[2]);
-        const int cz = cellOf(tz, g.cmin[2], g.edge, g.off);
+        const int cz = cellOf(tz, g, 2);
         const int z0 = max(cz - reach, 0), z1 = min(cz + reach, g.dim[2] - 1);
         if (z0 > z1) continue;
         for (int a = x0; a <= x1; a++)
@@ -888,6 +901,7 @@ __device__ __forceinline__ void imageShift(int code, int& sx, int& sy, int& sz) 
 struct CouplingRule {
   double cmin[3];
   double e;       // 2*DEM_SEARCH_RADIUS
+  double einv;    // 1/e (0: every cell index takes the division)
   double s;       // 2*LIQ_DEM_COUPLING_RADIUS/DEM_SEARCH_RADIUS
   double thr;     // pow(0.5*Rc/DSR, 2)
   double rc2;     // pow(Rc, 2)
@@ -896,7 +910,12 @@ struct CouplingRule {
   // DEM ghost band (dataLL.cpp:337-364): images exist only within Rc of a face
   double gmin[3], gspace[3];
 };
+// the fast product as cellTrunc; the division, where needed, in the
+// reference's own order ((x - cmin)/e + s) - 1, which rounds differently from
+// (x - cmin)/e + (s - 1) exactly at the cell boundaries where it is taken
 __device__ __forceinline__ int demCell(const CouplingRule& R, double x, int d) {
+  const double t = (x - R.cmin[d]) * R.einv + (R.s - 1);
+  if (fabs(t - rint(t)) > 1e-9) return (int)t;
   return (int)((x - R.cmin[d]) / R.e + R.s - 1);
 }
 // accepted(l2) with the whole-cell (addAll) branch; `full` keeps addAll pairs
@@ -1758,7 +1777,7 @@ __global__ void k_cell_coords(Store S, int n, Grid gs, CouplingRule R, int* __re
     if (t == T_DEM)
       c = demCell(R, comp(p, d), d);
     else
-      c = cellOf(comp(p, d), gs.cmin[d], gs.edge, gs.off);
+      c = cellOf(comp(p, d), gs, d);
     out[3 * o + d] = c;
   }
 }
